@@ -1,0 +1,192 @@
+/*
+ * sph2grid.h -- C ABI of libsph2grid, the MI355X (gfx950) implementation of
+ * SPHtoGrid.jl's kernel-weighted particle deposition.
+ *
+ * The reference is a pure-Julia call graph with no FFI of its own; the entry
+ * points below replace its deposition engines at the call sites a Julia
+ * `ccall` would bind (paths are relative to the SPHtoGrid.jl source tree):
+ *
+ *   s2g_deposit          <- cic_mapping_2D  (cic_interpolation/cic_2D.jl:103-244),
+ *                           called at cic_interpolation/cic_interpolation.jl:152-155
+ *                           (serial) and :191-196 (per worker)
+ *                        <- cic_mapping_3D  (cic_interpolation/cic_3D.jl:110-209),
+ *                           called at cic_interpolation.jl:219-221 and :248-252
+ *   s2g_reduce           <- reduce_image_2D / reduce_image_3D
+ *                           (cic_interpolation/reduce_image.jl:8-31, 39-55)
+ *   s2g_map              <- the deposit + reduce tail of sphMapping
+ *                           (cic_interpolation.jl:148-272)
+ *   s2g_healpix_deposit  <- the particle loop of healpix_map
+ *                           (healpix_interpolation/main.jl:146-213)
+ *   s2g_tsc3d            <- TSCInterpolation(..., average=true)
+ *                           (tsc_interpolation/tsc_interpolation.jl:131-133,
+ *                           commented out in the reference)
+ *   s2g_deposit_batch    <- many sphMapping calls (no single reference entry;
+ *                           the closest is distributed_mapping/cic.jl:24-110)
+ *
+ * Conventions
+ *   - All arrays are plain pointers with explicit sizes; no ownership is
+ *     transferred and no pointer is retained after a call returns.
+ *   - Particle positions are 3 x n interleaved (Julia's 3 x N column-major
+ *     `Pos`), already centred and filtered by the caller exactly as
+ *     center_particles / filter_particles_in_image do
+ *     (cic_interpolation/filter_shift.jl:6-67).
+ *   - bin_q is nq x n with the quantity index fastest (Julia's Nq x N matrix),
+ *     or n values when nq == 1.
+ *   - A flat image is npix^dims x (nq+1) doubles, column-major, deposit order
+ *     (2D index i*npix + j, 3D index i*npix*npix + j*npix + k, with i, j, k
+ *     the x, y, z pixel indices); the last column holds the weights.
+ *   - Return value: S2G_OK (0) or a negative S2G_E* code; the message is
+ *     available from s2g_last_error().  No C++ exception crosses the ABI.
+ *   - A context is not thread safe; use one per host thread.
+ *   - *_device variants take device pointers and a hipStream_t (as void*),
+ *     enqueue asynchronously and return without synchronising.
+ */
+#ifndef SPH2GRID_H
+#define SPH2GRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S2G_OK 0
+#define S2G_EINVAL (-1)
+#define S2G_EHIP (-2)
+#define S2G_ENOMEM (-3)
+#define S2G_EUNSUPPORTED (-4)
+
+/* SPHKernels.jl kernel types (the `kernel::AbstractSPHKernel` argument of
+ * sphMapping, cic_interpolation.jl:40).  Their normalisation and h^-dim
+ * factor cancel in every deposit (pw = w V wk A / sum(wk A)), so kernel_dim
+ * (Cubic(2) vs Cubic(3)) does not change results; it is validated only. */
+enum s2g_kernel {
+    S2G_KERNEL_CUBIC = 0,
+    S2G_KERNEL_QUINTIC = 1,
+    S2G_KERNEL_WENDLAND_C2 = 2,
+    S2G_KERNEL_WENDLAND_C4 = 3,
+    S2G_KERNEL_WENDLAND_C6 = 4,
+    S2G_KERNEL_WENDLAND_C8 = 5
+};
+
+/* element type of every floating-point input array of one call */
+enum s2g_dtype { S2G_F32 = 0, S2G_F64 = 1 };
+
+/* flags */
+#define S2G_FLAG_CALC_MEAN 0x1u  /* calc_mean keyword (cic_2D.jl:110, main.jl:99) */
+#define S2G_FLAG_EXACT_NORM 0x2u /* always sum the footprint for the per-particle
+                                    normalisation (no closed-form shortcut for
+                                    large unclipped footprints) */
+#define S2G_FLAG_PERIODIC 0x4u   /* TSC only: wrap cells around the grid */
+#define S2G_FLAG_REDUCE 0x8u     /* s2g_map / s2g_reduce: reduce_image=true */
+
+typedef struct s2g_ctx s2g_ctx;
+
+/* The two fields of mappingParameters the deposit reads
+ * (shared/parameters.jl:14-27): Npixels[1] (all three are equal,
+ * parameters.jl:105) and len2pix of the centred parameters
+ * (parameters.jl:115, re-derived by filter_shift.jl:29-31). */
+typedef struct s2g_params {
+    int64_t npixels;
+    double len2pix;
+} s2g_params;
+
+/* optional per-call statistics */
+typedef struct s2g_stats {
+    double ms_device;         /* device time of the deposit (HIP events) */
+    double ms_h2d, ms_d2h;    /* host<->device copies of the host entry points */
+    int64_t n_particles;      /* particles passed in */
+    int64_t n_skipped;        /* zero quantity without calc_mean, empty footprint */
+    int64_t n_small;          /* particles deposited by the tile scatter path */
+    int64_t n_large;          /* particles deposited by the tile gather path */
+    int64_t n_large_exact;    /* ... of which normalised by an explicit sum */
+    int64_t n_pairs;          /* (tile, particle) pairs of the gather path */
+    double sum_footprint;     /* sum over deposited particles of the footprint size */
+} s2g_stats;
+
+/* Context: owns the device buffers (grown monotonically) and one stream on
+ * `device`. */
+int s2g_create(s2g_ctx **out, int device);
+void s2g_destroy(s2g_ctx *ctx);
+const char *s2g_last_error(const s2g_ctx *ctx);
+/* library version string, e.g. "sph2grid 0.1.0 gfx950" */
+const char *s2g_version(void);
+
+/* Deposit (cic_mapping_2D for dims == 2, cic_mapping_3D for dims == 3).
+ * Host pointers.  image_out receives the flat unreduced image
+ * (npix^dims x (nq+1), or x 2 in 3D where nq must be 1). */
+int s2g_deposit(s2g_ctx *ctx, int dims, int dtype, const void *pos,
+                const void *hsml, const void *m, const void *rho,
+                const void *bin_q, int nq, const void *weights, int64_t n,
+                const s2g_params *params, int kernel, int kernel_dim,
+                uint32_t flags, double *image_out, s2g_stats *stats);
+
+/* Same on device pointers; d_image_out must hold the flat image. */
+int s2g_deposit_device(s2g_ctx *ctx, int dims, int dtype, const void *d_pos,
+                       const void *d_hsml, const void *d_m, const void *d_rho,
+                       const void *d_bin_q, int nq, const void *d_weights,
+                       int64_t n, const s2g_params *params, int kernel,
+                       int kernel_dim, uint32_t flags, double *d_image_out,
+                       void *stream, s2g_stats *stats);
+
+/* reduce_image_2D / reduce_image_3D on a flat image (host pointers).
+ * 2D output: Julia [x,y,q] column-major, out[q*N*N + y*N + x], divided by
+ * the weight column iff (flags & S2G_FLAG_REDUCE) and weight > 0.
+ * 3D output: [z,y,x] in deposit order, divided iff the value is > 0; without
+ * S2G_FLAG_REDUCE the weights are taken as 1 (cic_interpolation.jl:230-232). */
+int s2g_reduce(s2g_ctx *ctx, int dims, int64_t npix, int nq, uint32_t flags,
+               const double *image, double *out);
+int s2g_reduce_device(s2g_ctx *ctx, int dims, int64_t npix, int nq,
+                      uint32_t flags, const double *d_image, double *d_out,
+                      void *stream);
+
+/* Deposit + reduce in one call, returning only the reduced map (what
+ * sphMapping returns when return_both_maps == false). */
+int s2g_map(s2g_ctx *ctx, int dims, int dtype, const void *pos,
+            const void *hsml, const void *m, const void *rho,
+            const void *bin_q, int nq, const void *weights, int64_t n,
+            const s2g_params *params, int kernel, int kernel_dim,
+            uint32_t flags, double *map_out, s2g_stats *stats);
+
+/* HEALPix all-sky deposit (the particle loop of healpix_map,
+ * healpix_interpolation/main.jl:146-213) over particles already centred on
+ * the observer, shell-filtered and ordered by the caller
+ * (filter_particles.jl:17-54).  Outputs two RING-ordered maps of 12*nside^2
+ * doubles: sum of q*pw and sum of pw (unreduced, as the reference returns). */
+int s2g_healpix_deposit(s2g_ctx *ctx, int dtype, const void *pos,
+                        const void *hsml, const void *m, const void *rho,
+                        const void *bin_q, const void *weights, int64_t n,
+                        int64_t nside, int kernel, int kernel_dim,
+                        uint32_t flags, double *map_out, double *weight_out,
+                        s2g_stats *stats);
+
+/* Triangular-shaped-cloud deposit onto an npix^3 grid (positions centred as
+ * for the 3D deposit, grid units x*len2pix + npix/2, cell c centred at c+0.5).
+ * grid_out: 2 x npix^3 doubles in 3D deposit order, plane 0 = sum q*W,
+ * plane 1 = sum W.  S2G_FLAG_PERIODIC wraps, otherwise outside cells drop. */
+int s2g_tsc3d(s2g_ctx *ctx, int dtype, const void *pos, const void *bin_q,
+              int64_t n, const s2g_params *params, uint32_t flags,
+              double *grid_out, s2g_stats *stats);
+
+/* One map of a batch (device pointers). */
+typedef struct s2g_map_desc {
+    int32_t dims;      /* 2 (3 reserved) */
+    int32_t nq;
+    int64_t n;
+    s2g_params params;
+    int32_t kernel;
+    uint32_t flags;    /* S2G_FLAG_CALC_MEAN / S2G_FLAG_REDUCE */
+    const void *d_pos, *d_hsml, *d_m, *d_rho, *d_bin_q, *d_weights;
+    double *d_out;     /* flat image, or reduced map with S2G_FLAG_REDUCE */
+} s2g_map_desc;
+
+/* Many independent maps (all with the same dtype) in one device pass.  The
+ * launches are recorded once into a hipGraph keyed by the batch shape and
+ * replayed on later calls with the same shape. */
+int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
+                      int nmaps, void *stream, s2g_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH2GRID_H */

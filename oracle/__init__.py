@@ -1,0 +1,414 @@
+"""FP64 CPU oracle for SPHtoGrid.jl's deposition path (ctypes binding).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the timed
+CPU baseline.  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg import this module.  The product package (sph2grid)
+never imports it.
+
+The C library (``sph2grid_oracle.c``) restates, function by function:
+
+* ``shared/parameters.jl:44-125``           -> :func:`mapping_parameters`
+* ``cic_interpolation/filter_shift.jl``     -> :func:`center_particles`,
+  :func:`filter_particles`
+* ``cic_interpolation/cic_2D.jl:11-244``    -> :func:`cic_mapping_2D`
+* ``cic_interpolation/cic_3D.jl:13-209``    -> :func:`cic_mapping_3D`
+* ``cic_interpolation/reduce_image.jl``     -> :func:`reduce_image_2D`,
+  :func:`reduce_image_3D`
+* ``parallel/domain_decomp.jl:7-17`` + the ``parallel=true`` fan-out of
+  ``cic_interpolation.jl:171-215`` -> ``*_parallel``
+* ``healpix_interpolation/*.jl``            -> :func:`healpix_map`
+* SPHKernels.jl shapes (external, not vendored) -> :func:`kernel_value`
+* Healpix.jl RING geometry (external)       -> ``hp_*``
+* TSC (reference fully commented out)       -> :func:`tsc_3d` (textbook)
+
+Pinning (what checks each function against something other than itself) is
+listed in tests/test_oracle_pins.py.  The pieces that only an import of the
+Julia reference could pin -- the exact SPHKernels coefficient spelling
+(WendlandC8 in particular), Healpix.jl's disc-edge membership, and the
+reference's ``@fastmath`` ulp-level rounding -- are "parity unpinned" at the
+ulp level; every other function is pinned by a known-answer test, a closed
+form or brute force.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsph2grid_oracle.so")
+_lib = None
+_lock = threading.Lock()
+
+KERNELS = {"cubic": 0, "quintic": 1, "wendlandc2": 2, "wendlandc4": 3,
+           "wendlandc6": 4, "wendlandc8": 5}
+
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+_i64 = C.c_int64
+
+
+class Params(C.Structure):
+    """Mirror of ``orc_params`` (shared/parameters.jl:14-27)."""
+    _fields_ = [("x_lim", C.c_double * 2), ("y_lim", C.c_double * 2),
+                ("z_lim", C.c_double * 2), ("center", C.c_double * 3),
+                ("halfsize", C.c_double * 3), ("len2pix", C.c_double),
+                ("pixelSideLength", C.c_double), ("Npixels", C.c_int64 * 3),
+                ("boxsize", C.c_double), ("periodic", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, no fast-math)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "libsph2grid_oracle.so"],
+                   check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(_LIB_PATH):
+                build()
+            L = C.CDLL(_LIB_PATH)
+            L.orc_kernel_shape.restype = C.c_double
+            L.orc_kernel_shape.argtypes = [C.c_int, C.c_double]
+            L.orc_kernel_norm.restype = C.c_double
+            L.orc_kernel_norm.argtypes = [C.c_int, C.c_int]
+            L.orc_kernel_value.restype = C.c_double
+            L.orc_kernel_value.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+            L.orc_mapping_parameters.restype = C.c_int
+            L.orc_mapping_parameters.argtypes = [
+                C.POINTER(Params), _d, _d, _d, _d, C.c_double, C.c_double,
+                C.c_double, C.c_double, _i64, C.c_double]
+            L.orc_center_particles.restype = None
+            L.orc_center_particles.argtypes = [C.c_void_p, C.c_int, _i64,
+                                               C.POINTER(Params), C.POINTER(Params)]
+            L.orc_filter_particles.restype = None
+            L.orc_filter_particles.argtypes = [C.c_void_p, C.c_int, _i64,
+                                               C.POINTER(Params), C.c_void_p]
+            L.orc_cic_mapping_2D.restype = C.c_int
+            L.orc_cic_mapping_2D.argtypes = [_d, _d, _d, _d, _d, C.c_int, _d, _i64,
+                                             _i64, C.c_double, C.c_int, C.c_int,
+                                             C.c_int, _d]
+            L.orc_cic_mapping_3D.restype = C.c_int
+            L.orc_cic_mapping_3D.argtypes = [_d, _d, _d, _d, _d, _d, _i64, _i64,
+                                             C.c_double, C.c_int, C.c_int, C.c_int,
+                                             _d, _d, _d]
+            L.orc_cic_mapping_2D_parallel.restype = C.c_int
+            L.orc_cic_mapping_2D_parallel.argtypes = [C.c_int] + \
+                L.orc_cic_mapping_2D.argtypes
+            L.orc_cic_mapping_3D_parallel.restype = C.c_int
+            L.orc_cic_mapping_3D_parallel.argtypes = [
+                C.c_int, _d, _d, _d, _d, _d, _d, _i64, _i64, C.c_double, C.c_int,
+                C.c_int, C.c_int, _d]
+            L.orc_domain_decomposition.restype = None
+            L.orc_domain_decomposition.argtypes = [_i64, C.c_int, C.c_int,
+                                                   C.POINTER(_i64), C.POINTER(_i64)]
+            L.orc_reduce_image_2D.restype = None
+            L.orc_reduce_image_2D.argtypes = [_d, _i64, C.c_int, C.c_int, _d]
+            L.orc_reduce_image_3D.restype = None
+            L.orc_reduce_image_3D.argtypes = [_d, _i64, _d]
+            L.orc_hp_npix.restype = _i64
+            L.orc_hp_npix.argtypes = [_i64]
+            L.orc_hp_pix2ang_ring.restype = None
+            L.orc_hp_pix2ang_ring.argtypes = [_i64, _i64, _d, _d]
+            L.orc_hp_ang2pix_ring.restype = _i64
+            L.orc_hp_ang2pix_ring.argtypes = [_i64, C.c_double, C.c_double]
+            L.orc_hp_pix2vec_ring.restype = None
+            L.orc_hp_pix2vec_ring.argtypes = [_i64, _i64, _d]
+            L.orc_hp_vec2ang.restype = None
+            L.orc_hp_vec2ang.argtypes = [_d, _d, _d]
+            L.orc_hp_ring_above.restype = _i64
+            L.orc_hp_ring_above.argtypes = [_i64, C.c_double]
+            L.orc_hp_query_disc_ring.restype = _i64
+            L.orc_hp_query_disc_ring.argtypes = [_i64, C.c_double, C.c_double,
+                                                 C.c_double, C.POINTER(_i64), _i64]
+            L.orc_healpix_map.restype = C.c_int
+            L.orc_healpix_map.argtypes = [_d, _d, _d, _d, _d, _d, _i64, _d, _d, _i64,
+                                          C.c_int, C.c_int, C.c_int, _d, _d]
+            L.orc_healpix_deposit_parallel.restype = C.c_int
+            L.orc_healpix_deposit_parallel.argtypes = [
+                C.c_int, _d, _d, _d, _d, _d, _d, _i64, _i64, C.c_int, C.c_int,
+                C.c_int, _d, _d]
+            L.orc_tsc_3d.restype = C.c_int
+            L.orc_tsc_3d.argtypes = [_d, _d, _i64, _i64, C.c_double, C.c_int, _d]
+            _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_d)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _kid(kernel):
+    return KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+
+
+# ---------------------------------------------------------------- kernels
+def kernel_shape(kernel, u: float) -> float:
+    return lib().orc_kernel_shape(_kid(kernel), float(u))
+
+
+def kernel_norm(kernel, dim: int) -> float:
+    return lib().orc_kernel_norm(_kid(kernel), int(dim))
+
+
+def kernel_value(kernel, dim: int, u: float, h_inv: float) -> float:
+    return lib().orc_kernel_value(_kid(kernel), int(dim), float(u), float(h_inv))
+
+
+# ---------------------------------------------------------------- parameters
+class OracleParameterError(Exception):
+    pass
+
+
+_PARAM_ERRORS = {
+    1: "Giving a center position requires extent in x, y and z direction.",
+    2: "Please specify pixelSideLength or number of pixels!",
+}
+
+
+def mapping_parameters(x_lim=(-1.0, -1.0), y_lim=(-1.0, -1.0), z_lim=(-1.0, -1.0),
+                       center=(-1.0, -1.0, -1.0), x_size=-1.0, y_size=-1.0,
+                       z_size=-1.0, pixelSideLength=-1.0, Npixels=0,
+                       boxsize=-1.0) -> Params:
+    out = Params()
+    arrs = [np.asarray(v, dtype=np.float64).copy() for v in (x_lim, y_lim, z_lim, center)]
+    rc = lib().orc_mapping_parameters(C.byref(out), *[_p(a) for a in arrs],
+                                      float(x_size), float(y_size), float(z_size),
+                                      float(pixelSideLength), int(Npixels),
+                                      float(boxsize))
+    if rc:
+        raise OracleParameterError(_PARAM_ERRORS[rc])
+    return out
+
+
+def center_particles(pos: np.ndarray, par: Params) -> Params:
+    """In place on ``pos`` (shape (n,3) C-order == Julia 3 x n column-major)."""
+    assert pos.flags.c_contiguous and pos.dtype in (np.float32, np.float64)
+    out = Params()
+    lib().orc_center_particles(pos.ctypes.data, int(pos.dtype == np.float32),
+                               pos.shape[0], C.byref(par), C.byref(out))
+    return out
+
+
+def filter_particles(pos: np.ndarray, par: Params) -> np.ndarray:
+    assert pos.flags.c_contiguous and pos.dtype in (np.float32, np.float64)
+    keep = np.zeros(pos.shape[0], dtype=np.uint8)
+    lib().orc_filter_particles(pos.ctypes.data, int(pos.dtype == np.float32),
+                               pos.shape[0], C.byref(par), keep.ctypes.data)
+    return keep.astype(bool)
+
+
+# ---------------------------------------------------------------- deposits
+def _bin_q_layout(bin_q, n):
+    b = np.asarray(bin_q)
+    if b.ndim == 1:
+        return _f64(b), 1
+    # Julia Nq x N column-major == numpy (N, Nq) C-order
+    assert b.shape[0] == n
+    return _f64(b), b.shape[1]
+
+
+def cic_mapping_2D(pos, hsml, m, rho, bin_q, weights, npix, len2pix, kernel,
+                   kernel_dim=2, calc_mean=True, nworkers=0) -> np.ndarray:
+    """Returns the flat unreduced image, shape (npix^2, nq+1), Fortran order."""
+    pos = _f64(pos)
+    n = pos.shape[0]
+    hsml, m, rho, weights = (_f64(a) for a in (hsml, m, rho, weights))
+    bq, nq = _bin_q_layout(bin_q, n)
+    img = np.zeros(npix * npix * (nq + 1), dtype=np.float64)
+    args = (_p(pos), _p(hsml), _p(m), _p(rho), _p(bq), nq, _p(weights), n, npix,
+            float(len2pix), _kid(kernel), int(kernel_dim), int(bool(calc_mean)),
+            _p(img))
+    rc = (lib().orc_cic_mapping_2D_parallel(int(nworkers), *args) if nworkers
+          else lib().orc_cic_mapping_2D(*args))
+    if rc:
+        raise MemoryError("oracle 2D mapping failed")
+    return img.reshape((npix * npix, nq + 1), order="F")
+
+
+def cic_mapping_3D(pos, hsml, m, rho, bin_q, weights, npix, len2pix, kernel,
+                   kernel_dim=3, calc_mean=False, nworkers=0, diagnostics=False):
+    pos = _f64(pos)
+    n = pos.shape[0]
+    hsml, m, rho, weights, bq = (_f64(a) for a in (hsml, m, rho, weights, bin_q))
+    img = np.zeros(npix ** 3 * 2, dtype=np.float64)
+    gm = C.c_double(0.0)
+    pm = C.c_double(0.0)
+    if nworkers:
+        rc = lib().orc_cic_mapping_3D_parallel(
+            int(nworkers), _p(pos), _p(hsml), _p(m), _p(rho), _p(bq), _p(weights), n,
+            npix, float(len2pix), _kid(kernel), int(kernel_dim), int(bool(calc_mean)),
+            _p(img))
+    else:
+        rc = lib().orc_cic_mapping_3D(
+            _p(pos), _p(hsml), _p(m), _p(rho), _p(bq), _p(weights), n, npix,
+            float(len2pix), _kid(kernel), int(kernel_dim), int(bool(calc_mean)),
+            _p(img), C.byref(gm), C.byref(pm))
+    if rc:
+        raise MemoryError("oracle 3D mapping failed")
+    img = img.reshape((npix ** 3, 2), order="F")
+    if diagnostics:
+        return img, gm.value, pm.value
+    return img
+
+
+def reduce_image_2D(image, npix, reduce_image=True) -> np.ndarray:
+    """Julia im_plot (npix, npix, nq) [x, y, q], returned Fortran-ordered."""
+    img = np.asfortranarray(image, dtype=np.float64)
+    nq = img.shape[1] - 1
+    out = np.zeros(npix * npix * nq, dtype=np.float64)
+    lib().orc_reduce_image_2D(img.ctypes.data_as(_d), npix, nq, int(bool(reduce_image)),
+                              _p(out))
+    return out.reshape((npix, npix, nq), order="F")
+
+
+def reduce_image_3D(image, npix) -> np.ndarray:
+    img = np.asfortranarray(image, dtype=np.float64)
+    out = np.zeros(npix ** 3, dtype=np.float64)
+    lib().orc_reduce_image_3D(img.ctypes.data_as(_d), npix, _p(out))
+    return out.reshape((npix, npix, npix), order="F")
+
+
+def domain_decomposition(n, nworkers):
+    out = []
+    for i in range(nworkers):
+        s = _i64(0)
+        ln = _i64(0)
+        lib().orc_domain_decomposition(n, nworkers, i, C.byref(s), C.byref(ln))
+        out.append((s.value, ln.value))
+    return out
+
+
+# ---------------------------------------------------------------- full sphMapping
+def sph_mapping(Pos, HSML, M, Rho, Bin_Quant, Weights, par: Params, kernel,
+                kernel_dim=2, reduce_image=True, return_both_maps=False,
+                dimensions=2, calc_mean=False, nworkers=0):
+    """cic_interpolation.jl:35-273 (no RM / stokes / Unitful).  Mutates Pos."""
+    Pos_, par2 = Pos, center_particles(Pos, par)
+    keep = filter_particles(Pos_, par2)
+    x = Pos_[keep].astype(np.float64)
+    f = lambda a: np.asarray(a)[keep].astype(np.float64)
+    hsml, m, rho, w = f(HSML), f(M), f(Rho), f(Weights)
+    bq = np.asarray(Bin_Quant)
+    bq = bq[keep].astype(np.float64)
+    npix = int(par.Npixels[0])
+    if dimensions == 2:
+        img = cic_mapping_2D(x, hsml, m, rho, bq, w, npix, par2.len2pix, kernel,
+                             kernel_dim, calc_mean, nworkers)
+        if return_both_maps:
+            return img
+        return reduce_image_2D(img, npix, reduce_image)
+    img = cic_mapping_3D(x, hsml, m, rho, bq, w, npix, par2.len2pix, kernel,
+                         kernel_dim, False, nworkers)
+    if not reduce_image:
+        img[:, 1] = 1.0
+    return reduce_image_3D(img, npix)
+
+
+# ---------------------------------------------------------------- HEALPix
+def hp_npix(nside):
+    return 12 * nside * nside
+
+
+def hp_pix2ang_ring(nside, pix):
+    t = C.c_double()
+    p = C.c_double()
+    lib().orc_hp_pix2ang_ring(nside, pix, C.byref(t), C.byref(p))
+    return t.value, p.value
+
+
+def hp_ang2pix_ring(nside, theta, phi):
+    return lib().orc_hp_ang2pix_ring(nside, float(theta), float(phi))
+
+
+def hp_pix2vec_ring(nside, pix):
+    v = np.zeros(3)
+    lib().orc_hp_pix2vec_ring(nside, pix, _p(v))
+    return v
+
+
+def hp_vec2ang(v):
+    v = _f64(v)
+    t = C.c_double()
+    p = C.c_double()
+    lib().orc_hp_vec2ang(_p(v), C.byref(t), C.byref(p))
+    return t.value, p.value
+
+
+def hp_ring_above(nside, z):
+    return lib().orc_hp_ring_above(nside, float(z))
+
+
+def hp_query_disc_ring(nside, theta, phi, radius):
+    cap = 1024
+    while True:
+        out = np.zeros(cap, dtype=np.int64)
+        cnt = lib().orc_hp_query_disc_ring(nside, float(theta), float(phi), float(radius),
+                                           out.ctypes.data_as(C.POINTER(_i64)), cap)
+        if cnt <= cap:
+            return out[:cnt].copy()
+        cap = int(cnt)
+
+
+class OracleBoundsError(IndexError):
+    pass
+
+
+def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, center=(0.0, 0.0, 0.0),
+                radius_limits=(0.0, np.inf), Nside=1024, kernel="wendlandc6",
+                kernel_dim=2, calc_mean=True):
+    """healpix_interpolation/main.jl:92-227.  Mutates Pos (Float64, (n,3))."""
+    assert Pos.dtype == np.float64 and Pos.flags.c_contiguous
+    n = Pos.shape[0]
+    arrs = [_f64(a) for a in (Hsml, M, Rho, Bin_q, Weights)]
+    c = _f64(center)
+    rl = _f64(radius_limits)
+    npix = hp_npix(Nside)
+    mp = np.zeros(npix)
+    wm = np.zeros(npix)
+    rc = lib().orc_healpix_map(_p(Pos), *[_p(a) for a in arrs], n, _p(c), _p(rl),
+                               int(Nside), _kid(kernel), int(kernel_dim),
+                               int(bool(calc_mean)), _p(mp), _p(wm))
+    if rc == -1:
+        raise OracleBoundsError("BoundsError in filter_sort_particles")
+    if rc:
+        raise MemoryError("oracle healpix failed")
+    return mp, wm
+
+
+def healpix_deposit_parallel(pos, hsml, m, rho, bin_q, weights, nside, kernel,
+                             kernel_dim=2, calc_mean=True, nworkers=1):
+    """Deposit of already centred particles, threaded (CPU baseline only)."""
+    pos = _f64(pos)
+    arrs = [_f64(a) for a in (hsml, m, rho, bin_q, weights)]
+    npix = hp_npix(nside)
+    mp = np.zeros(npix)
+    wm = np.zeros(npix)
+    rc = lib().orc_healpix_deposit_parallel(int(nworkers), _p(pos), *[_p(a) for a in arrs],
+                                            pos.shape[0], int(nside), _kid(kernel),
+                                            int(kernel_dim), int(bool(calc_mean)),
+                                            _p(mp), _p(wm))
+    if rc:
+        raise MemoryError("oracle healpix deposit failed")
+    return mp, wm
+
+
+# ---------------------------------------------------------------- TSC
+def tsc_3d(pos, bin_q, npix, len2pix, periodic=False):
+    pos = _f64(pos)
+    bq = _f64(bin_q)
+    g = np.zeros(2 * npix ** 3)
+    lib().orc_tsc_3d(_p(pos), _p(bq), pos.shape[0], npix, float(len2pix), int(periodic),
+                     _p(g))
+    return g.reshape((npix ** 3, 2), order="F")

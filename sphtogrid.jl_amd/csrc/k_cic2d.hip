@@ -1,0 +1,769 @@
+// k_cic2d.hip -- the 2D Smac pixel-overlap deposit (cic_interpolation/cic_2D.jl)
+// on gfx950.
+//
+// Pipeline of one map (all on one stream):
+//   k2d_setup        per particle, FP64: pixel coordinates, integer footprint,
+//                    zero-quantity skip, fallback decision, class
+//   k2d_partition    stable (particle-order) lists of small / large / exact
+//   k2d_norm         explicit FP64 normalisation sum of large clipped footprints
+//   k2d_pair_count   counting sort of (tile, particle) pairs: per-chunk counts
+//   (scan)           tile-major exclusive scan -> deterministic list offsets
+//   k2d_pair_fill    one wave per chunk fills the per-tile lists in particle order
+//   k2d_items        split every tile list into work items of <= G pairs
+//   k2d_gather       per-tile gather, FP64 register accumulation per pixel
+//   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
+//   k2d_small_*      scatter of small footprints (atomics on top of the image)
+//   k2d_reduce       reduce_image_2D: divide by the weight plane + transpose
+#include "s2g_internal.h"
+#include "k_cic2d.h"
+
+namespace s2g {
+
+// ---------------------------------------------------------------------------
+// setup: get_quantities_2D (cic_2D.jl:80-91), get_xyz (cic_shared.jl:85-100),
+// pix_index_min_max (cic_shared.jl:46-52) and the fallback test of
+// calculate_weights (cic_2D.jl:49-69), all in FP64 without FMA contraction.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
+                                                 const T *__restrict__ hsml,
+                                                 const T *__restrict__ m,
+                                                 const T *__restrict__ rho,
+                                                 const T *__restrict__ binq, int nq,
+                                                 const T *__restrict__ w, int64_t n,
+                                                 int64_t npix, double len2pix,
+                                                 int kernel, uint32_t flags,
+                                                 Rec2 *__restrict__ rec)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    Rec2 r;
+    r.x = r.y = r.h = r.s = 0.0;
+    r.i0 = r.j0 = 0;
+    r.i1 = r.j1 = -1;
+    r.cls = kClsSkip;
+    r.fb = 0;
+
+    // cic_2D.jl:155-168: skip if every quantity is zero and !calc_mean
+    bool allzero = true;
+    for (int q = 0; q < nq; ++q)
+        if (ld(binq, p * nq + q) != 0.0)
+            allzero = false;
+    if (allzero && !(flags & S2G_FLAG_CALC_MEAN)) {
+        rec[p] = r;
+        return;
+    }
+
+    const double h = mul_rn(ld(hsml, p), len2pix);
+    const double area = mul_rn(2.0 * h, 2.0 * h);
+    const double l3 = mul_rn(mul_rn(len2pix, len2pix), len2pix);
+    const double rh = mul_rn(ld(rho, p), 1.0 / l3);
+    const double dz = (ld(m, p) / rh) / area;
+    const double wl = ld(w, p);
+    const double half = 0.5 * (double)npix;
+    const double x = add_rn(mul_rn(ld(pos, 3 * p + 0), len2pix), half);
+    const double y = add_rn(mul_rn(ld(pos, 3 * p + 1), len2pix), half);
+
+    const double lim = (double)npix + 1.0;
+    double xa = floor(sub_rn(x, h)), xb = floor(add_rn(x, h));
+    double ya = floor(sub_rn(y, h)), yb = floor(add_rn(y, h));
+    if (!(xa == xa) || !(xb == xb) || !(ya == ya) || !(yb == yb) || !(h > 0.0)) {
+        rec[p] = r;  // NaN input or h <= 0: nothing is deposited
+        return;
+    }
+    xa = fmin(fmax(xa, -1.0), lim);
+    xb = fmin(fmax(xb, -1.0), lim);
+    ya = fmin(fmax(ya, -1.0), lim);
+    yb = fmin(fmax(yb, -1.0), lim);
+    const int64_t a0 = (int64_t)xa, a1 = (int64_t)xb, b0 = (int64_t)ya, b1 = (int64_t)yb;
+    const int64_t i0 = a0 > 0 ? a0 : 0, i1 = a1 < npix - 1 ? a1 : npix - 1;
+    const int64_t j0 = b0 > 0 ? b0 : 0, j1 = b1 < npix - 1 ? b1 : npix - 1;
+    if (i0 > i1 || j0 > j1) {
+        rec[p] = r;
+        return;
+    }
+
+    // Fallback iff no footprint pixel centre has u < 1 (u from the
+    // reference's FP64 expression).  u is monotone in |x - i - 1/2| and
+    // |y - j - 1/2|, so the minimum sits at the nearest column and row; three
+    // candidates per axis cover ties and clamping.
+    const double h_inv = 1.0 / h;
+    double bx = 1e300, by = 1e300;
+    {
+        const double fx = floor(x), fy = floor(y);
+        for (int d = -1; d <= 1; ++d) {
+            double ci = fmin(fmax(fx + d, (double)i0), (double)i1);
+            double cj = fmin(fmax(fy + d, (double)j0), (double)j1);
+            bx = fmin(bx, fabs(coff1(x, ci)));
+            by = fmin(by, fabs(coff1(y, cj)));
+        }
+    }
+    const double umin = mul_rn(sqrt(add_rn(mul_rn(bx, bx), mul_rn(by, by))), h_inv);
+    r.fb = !(umin < 1.0);
+
+    r.x = x;
+    r.y = y;
+    r.h = h;
+    r.i0 = (int32_t)i0;
+    r.i1 = (int32_t)i1;
+    r.j0 = (int32_t)j0;
+    r.j1 = (int32_t)j1;
+    const double num = mul_rn(mul_rn(area, wl), dz);
+    const bool small = (i1 - i0 + 1) <= kSmallSide2D && (j1 - j0 + 1) <= kSmallSide2D;
+    if (small) {
+        r.cls = kClsSmall;
+        r.s = num;
+    } else {
+        const bool clipped = a0 < 0 || a1 > npix - 1 || b0 < 0 || b1 > npix - 1;
+        if (!(flags & S2G_FLAG_EXACT_NORM) && !clipped && !r.fb &&
+            h >= closed_form_min_h(kernel)) {
+            r.cls = kClsLargeAn;
+            r.s = num / (mul_rn(h, h) * shape_disc_integral(kernel));
+        } else {
+            r.cls = kClsLargeEx;
+            r.s = num;
+        }
+    }
+    rec[p] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Stable partition into three particle-order lists.
+//   pass 1: per-block class counts          (k2d_part_count)
+//   scan  : exclusive over [class][block]
+//   pass 2: ballot ranks inside the block    (k2d_part_scatter)
+// list 0 = small, list 1 = large (An + Ex), list 2 = exact (Ex only)
+// ---------------------------------------------------------------------------
+constexpr int kPartBlock = 256;
+
+__device__ __forceinline__ void part_flags(int cls, int &f0, int &f1, int &f2)
+{
+    f0 = cls == kClsSmall;
+    f1 = cls == kClsLargeAn || cls == kClsLargeEx;
+    f2 = cls == kClsLargeEx;
+}
+
+__global__ __launch_bounds__(kPartBlock) void k2d_part_count(const Rec2 *__restrict__ rec,
+                                                             int64_t n,
+                                                             int64_t *__restrict__ counts)
+{
+    __shared__ int s[3][kPartBlock / kWave];
+    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
+    int cls = p < n ? rec[p].cls : kClsSkip;
+    int f0, f1, f2;
+    part_flags(cls, f0, f1, f2);
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    uint64_t b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
+    if (ln == 0) {
+        s[0][wv] = __popcll(b0);
+        s[1][wv] = __popcll(b1);
+        s[2][wv] = __popcll(b2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int t = 0;
+        for (int k = 0; k < kPartBlock / kWave; ++k)
+            t += s[threadIdx.x][k];
+        counts[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 3)
+        counts[3 * (int64_t)gridDim.x] = 0;  // becomes the grand total after the scan
+}
+
+__global__ __launch_bounds__(kPartBlock) void k2d_part_scatter(
+    const Rec2 *__restrict__ rec, int64_t n, const int64_t *__restrict__ offs,
+    int32_t *__restrict__ l0, int32_t *__restrict__ l1, int32_t *__restrict__ l2)
+{
+    __shared__ int s[3][kPartBlock / kWave];
+    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
+    int cls = p < n ? rec[p].cls : kClsSkip;
+    int f[3];
+    part_flags(cls, f[0], f[1], f[2]);
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const uint64_t below = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+    uint64_t b[3] = {__ballot(f[0]), __ballot(f[1]), __ballot(f[2])};
+    if (ln == 0)
+        for (int c = 0; c < 3; ++c)
+            s[c][wv] = __popcll(b[c]);
+    __syncthreads();
+    int32_t *lists[3] = {l0, l1, l2};
+    for (int c = 0; c < 3; ++c) {
+        if (!f[c])
+            continue;
+        int64_t pos = offs[(int64_t)c * gridDim.x + blockIdx.x];
+        for (int k = 0; k < wv; ++k)
+            pos += s[c][k];
+        pos += __popcll(b[c] & below);
+        lists[c][pos] = (int32_t)p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pixel weight factor shared by the normalisation and the gather so that the
+// sum and the deposited values are computed identically:
+//   f = shape(u) * ay * ax   (FP32, in [0, 1])
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ float pix_factor(float xd2, float yd2, float hinv, float ax,
+                                            float ay)
+{
+    float u = __builtin_amdgcn_sqrtf(xd2 + yd2) * hinv;
+    float W = shape_of<K, float>(u);
+    return (W * ay) * ax;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit normalisation for large footprints that are clipped by the image
+// border (or below the closed-form threshold): one block per particle,
+// deterministic FP64 sum (fixed per-thread order + fixed tree).
+// ---------------------------------------------------------------------------
+constexpr int kNormThreads = 256;
+constexpr int kNormColsPerLane = 16;  // 1024 columns per chunk
+
+template <int K>
+__global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
+                                                         Rec2 *__restrict__ rec)
+{
+    __shared__ double red[kNormThreads / kWave];
+    const int p = list[blockIdx.x];
+    Rec2 r = rec[p];
+    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    double acc = 0.0;
+    for (int jc = r.j0; jc <= r.j1; jc += kWave * kNormColsPerLane) {
+        float yd2[kNormColsPerLane], ay[kNormColsPerLane];
+#pragma unroll
+        for (int c = 0; c < kNormColsPerLane; ++c) {
+            int j = jc + ln + c * kWave;
+            double fj = (double)j;
+            float yd = (float)coff1(r.y, fj);
+            yd2[c] = yd * yd;
+            ay[c] = (j <= r.j1) ? (float)overlap1(r.y, r.h, fj) : 0.0f;
+        }
+        for (int i = r.i0 + wv; i <= r.i1; i += kNormThreads / kWave) {
+            double fi = (double)i;
+            float xd = (float)coff1(r.x, fi);
+            float xd2 = xd * xd;
+            float ax = (float)overlap1(r.x, r.h, fi);
+            double rowacc = 0.0;
+#pragma unroll
+            for (int c = 0; c < kNormColsPerLane; ++c)
+                rowacc += (double)pix_factor<K>(xd2, yd2[c], hinv, ax, ay[c]);
+            acc += rowacc;
+        }
+    }
+    acc = wave_sum(acc);
+    if (ln == 0)
+        red[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int k = 0; k < kNormThreads / kWave; ++k)
+            sum += red[k];
+        rec[p].s = sum != 0.0 ? r.s / sum : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (tile, particle) pair lists: a counting sort with deterministic order.
+// Chunk c holds large-list entries [c*CH, (c+1)*CH).  cnt is tile-major
+// [tile][chunk] so one exclusive scan yields every list offset.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tile_rect(const Rec2 &r, int &ta0, int &ta1, int &tb0,
+                                          int &tb1)
+{
+    ta0 = r.i0 / kGT;
+    ta1 = r.i1 / kGT;
+    tb0 = r.j0 / kGT;
+    tb1 = r.j1 / kGT;
+}
+
+__global__ __launch_bounds__(256) void k2d_pair_count(const int32_t *__restrict__ large,
+                                                      int64_t nl, const Rec2 *__restrict__ rec,
+                                                      int ntj, int ntiles, int nchunks,
+                                                      int64_t *__restrict__ cnt)
+{
+    extern __shared__ int hist[];
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        hist[t] = 0;
+    __syncthreads();
+    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
+    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
+    for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+        Rec2 r = rec[large[l]];
+        int ta0, ta1, tb0, tb1;
+        tile_rect(r, ta0, ta1, tb0, tb1);
+        for (int a = ta0; a <= ta1; ++a)
+            for (int b = tb0; b <= tb1; ++b)
+                atomicAdd(&hist[a * ntj + b], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        cnt[(int64_t)t * nchunks + blockIdx.x] = hist[t];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt[(int64_t)ntiles * nchunks] = 0;  // becomes the pair total after the scan
+}
+
+// one wave per chunk: sequential over the chunk's particles, lanes over the
+// particle's tiles (distinct per lane), so every list is in particle order
+__global__ __launch_bounds__(kWave) void k2d_pair_fill(const int32_t *__restrict__ large,
+                                                       int64_t nl, const Rec2 *__restrict__ rec,
+                                                       int ntj, int ntiles, int nchunks,
+                                                       const int64_t *__restrict__ off,
+                                                       int32_t *__restrict__ pairs)
+{
+    extern __shared__ int cursor[];
+    for (int t = threadIdx.x; t < ntiles; t += kWave)
+        cursor[t] = (int)off[(int64_t)t * nchunks + blockIdx.x];
+    __syncthreads();
+    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
+    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
+    for (int64_t l = l0; l < l1; ++l) {
+        Rec2 r = rec[large[l]];
+        int ta0, ta1, tb0, tb1;
+        tile_rect(r, ta0, ta1, tb0, tb1);
+        const int wb = tb1 - tb0 + 1;
+        const int nt = (ta1 - ta0 + 1) * wb;
+        for (int k = threadIdx.x; k < nt; k += kWave) {
+            int a = ta0 + k / wb, b = tb0 + k % wb;
+            int slot = atomicAdd(&cursor[a * ntj + b], 1);
+            pairs[slot] = (int32_t)l;
+        }
+    }
+}
+
+// items: tile t's list [start, start+count) is split into ceil(count/G) items
+__global__ void k2d_item_count(const int64_t *__restrict__ off, int ntiles, int nchunks,
+                               int64_t npairs, int64_t G, int64_t *__restrict__ nitems)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == ntiles)
+        nitems[t] = 0;  // becomes the item total after the scan
+    if (t >= ntiles)
+        return;
+    int64_t s = off[(int64_t)t * nchunks];
+    int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    nitems[t] = (e - s + G - 1) / G;
+}
+
+__global__ void k2d_item_fill(const int64_t *__restrict__ off, int ntiles, int nchunks,
+                              int64_t npairs, int64_t G, const int64_t *__restrict__ ibase,
+                              int4 *__restrict__ items, int64_t *__restrict__ tile_items)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    int64_t s = off[(int64_t)t * nchunks];
+    int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    int64_t b = ibase[t];
+    int64_t k = 0;
+    for (int64_t q = s; q < e; q += G, ++k) {
+        int64_t len = e - q < G ? e - q : G;
+        items[b + k] = make_int4(t, (int)q, (int)len, 0);
+    }
+    tile_items[2 * t] = b;
+    tile_items[2 * t + 1] = k;
+}
+
+// ---------------------------------------------------------------------------
+// Tile gather.  Block = 4 waves = one 64x64 tile; wave w owns rows
+// [16w, 16w+16) of the tile, lane = column.  Per particle the column terms
+// are per lane and the row terms are computed by lanes 0..15 and broadcast
+// with v_readlane (wave-uniform SGPR operands); per pixel the work is one
+// sqrt, the kernel polynomial and NP FP64 FMAs into registers.
+// ---------------------------------------------------------------------------
+constexpr int kGBatch = 64;
+
+template <typename T, int K, int NP>
+__global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items,
+                                                  const int32_t *__restrict__ pairs,
+                                                  const int32_t *__restrict__ large,
+                                                  const Rec2 *__restrict__ rec,
+                                                  const T *__restrict__ binq, int nq,
+                                                  int plane0, int ntj,
+                                                  double *__restrict__ partial)
+{
+    __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch], s_s[kGBatch];
+    __shared__ float s_hinv[kGBatch];
+    __shared__ int s_i0[kGBatch], s_i1[kGBatch], s_j0[kGBatch], s_j1[kGBatch];
+    __shared__ double s_mult[NP][kGBatch];
+
+    const int4 it = items[blockIdx.x];
+    const int tile = it.x;
+    const int ti0 = (tile / ntj) * kGT, tj0 = (tile % ntj) * kGT;
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const int rowbase = ti0 + wv * kGRows;
+    const int j = tj0 + ln;
+    const double fj = (double)j;
+    const double frow = (double)(rowbase + (ln & (kGRows - 1)));
+
+    double acc[kGRows][NP];
+#pragma unroll
+    for (int r = 0; r < kGRows; ++r)
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            acc[r][k] = 0.0;
+
+    for (int base = 0; base < it.z; base += kGBatch) {
+        const int cnt = it.z - base < kGBatch ? it.z - base : kGBatch;
+        if (threadIdx.x < cnt) {
+            const int l = pairs[it.y + base + threadIdx.x];
+            const int p = large[l];
+            const Rec2 r = rec[p];
+            s_x[threadIdx.x] = r.x;
+            s_y[threadIdx.x] = r.y;
+            s_h[threadIdx.x] = r.h;
+            s_s[threadIdx.x] = r.s;
+            s_hinv[threadIdx.x] = r.fb ? 0.0f : (float)(1.0 / r.h);
+            s_i0[threadIdx.x] = r.i0;
+            s_i1[threadIdx.x] = r.i1;
+            s_j0[threadIdx.x] = r.j0;
+            s_j1[threadIdx.x] = r.j1;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                const int P = plane0 + k;
+                s_mult[k][threadIdx.x] =
+                    P == nq ? 1.0 : ld(binq, (int64_t)p * nq + (P < nq ? P : 0));
+            }
+        }
+        __syncthreads();
+        for (int b = 0; b < cnt; ++b) {
+            const int lo = s_i0[b] - rowbase > 0 ? s_i0[b] - rowbase : 0;
+            const int hi = s_i1[b] - rowbase < kGRows - 1 ? s_i1[b] - rowbase : kGRows - 1;
+            if (lo > hi)
+                continue;
+            const double x = s_x[b], y = s_y[b], h = s_h[b];
+            const float hinv = s_hinv[b];
+            const double sc = s_s[b];
+            // column (lane) terms
+            const float yd = (float)coff1(y, fj);
+            const float yd2 = yd * yd;
+            const float ay =
+                (j >= s_j0[b] && j <= s_j1[b]) ? (float)overlap1(y, h, fj) : 0.0f;
+            // row terms, row = rowbase + (lane & 15)
+            const float xd = (float)coff1(x, frow);
+            const float xd2 = xd * xd;
+            const float ax = (float)overlap1(x, h, frow);
+            double mult[NP];
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                mult[k] = s_mult[k][b];
+#pragma unroll
+            for (int r = 0; r < kGRows; ++r) {
+                if (r < lo || r > hi)
+                    continue;
+                const float xd2r =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), r));
+                const float axr =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), r));
+                const float f = pix_factor<K>(xd2r, yd2, hinv, axr, ay);
+                const double pw = (double)f * sc;
+#pragma unroll
+                for (int k = 0; k < NP; ++k)
+                    acc[r][k] = fma(mult[k], pw, acc[r][k]);
+            }
+        }
+        __syncthreads();
+    }
+    double *dst = partial + (int64_t)blockIdx.x * NP * kGT * kGT;
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+#pragma unroll
+        for (int r = 0; r < kGRows; ++r)
+            dst[(k * kGT + wv * kGRows + r) * kGT + ln] = acc[r][k];
+}
+
+// fixed-order sum of a tile's item partials -> image planes (plain stores)
+template <int NP>
+__global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ partial,
+                                                    const int64_t *__restrict__ tile_items,
+                                                    int ntj, int64_t npix, int plane0,
+                                                    double *__restrict__ image)
+{
+    const int t = blockIdx.x;
+    const int ti0 = (t / ntj) * kGT, tj0 = (t % ntj) * kGT;
+    const int64_t b = tile_items[2 * t], k = tile_items[2 * t + 1];
+    const int64_t nd = npix * npix;
+    for (int px = threadIdx.x; px < kGT * kGT; px += 256) {
+        const int i = ti0 + px / kGT, j = tj0 + px % kGT;
+        if (i >= npix || j >= npix)
+            continue;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            double s = 0.0;
+            for (int64_t it = 0; it < k; ++it)
+                s += partial[((b + it) * NP + q) * kGT * kGT + px];
+            image[(int64_t)(plane0 + q) * nd + (int64_t)i * npix + j] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small footprints (box side <= 34): one wave per particle, FP64 weights,
+// wave-reduced normalisation, FP64 atomics into the image.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list,
+                                                        int64_t ns,
+                                                        const Rec2 *__restrict__ rec,
+                                                        const T *__restrict__ binq, int nq,
+                                                        int64_t npix,
+                                                        double *__restrict__ image)
+{
+    const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int ln = threadIdx.x % kWave;
+    if (gw >= ns)
+        return;
+    const int p = list[gw];
+    const Rec2 r = rec[p];
+    const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
+    const int fp = iw * jw;
+    const double hinv = r.fb ? 0.0 : 1.0 / r.h;
+    const float invjw = 1.0f / (float)jw;
+    double sum = 0.0;
+    for (int f = ln; f < fp; f += kWave) {
+        const int di = (int)(((float)f + 0.5f) * invjw);
+        const int dj = f - di * jw;
+        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj);
+        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj);
+        const double A = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj);
+        const double u = sqrt(xd * xd + yd * yd) * hinv;
+        sum += shape_of<K, double>(u) * A;
+    }
+    sum = wave_sum(sum);
+    if (!(sum != 0.0))
+        return;
+    const double sc = r.s / sum;
+    const int64_t nd = npix * npix;
+    double *wimg = image + (int64_t)nq * nd;
+    for (int f = ln; f < fp; f += kWave) {
+        const int di = (int)(((float)f + 0.5f) * invjw);
+        const int dj = f - di * jw;
+        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj);
+        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj);
+        const double A = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj);
+        const double u = sqrt(xd * xd + yd * yd) * hinv;
+        const double pw = shape_of<K, double>(u) * A * sc;
+        if (pw != 0.0) {
+            const int64_t idx = (int64_t)(r.i0 + di) * npix + (r.j0 + dj);
+            unsafeAtomicAdd(&wimg[idx], pw);
+            for (int q = 0; q < nq; ++q)
+                unsafeAtomicAdd(&image[(int64_t)q * nd + idx],
+                                ld(binq, (int64_t)p * nq + q) * pw);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduce_image_2D (reduce_image.jl:8-31): out[q][y][x] = img[q][x*N+y] / w.
+// 64x64 tiles transposed through LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k2d_reduce(const double *__restrict__ image,
+                                                  int64_t npix, int nq, int reduce,
+                                                  double *__restrict__ out)
+{
+    __shared__ double tile[64][65];
+    const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;  // i = x (slow), j = y (fast)
+    const int64_t nd = npix * npix;
+    const double *w = image + (int64_t)nq * nd;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+    // weights of the 16 output elements this thread writes, transposed via LDS
+    for (int r = ty; r < 64; r += 4) {
+        int i = ti + r, j = tj + tx;
+        tile[r][tx] = (i < npix && j < npix) ? w[(int64_t)i * npix + j] : 0.0;
+    }
+    __syncthreads();
+    double wt[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        wt[k] = tile[tx][ty + 4 * k];
+    for (int q = 0; q < nq; ++q) {
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            int i = ti + r, j = tj + tx;
+            tile[r][tx] = (i < npix && j < npix) ? image[q * nd + (int64_t)i * npix + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int r = ty + 4 * k;
+            const int j = tj + r, i = ti + tx;  // output row j (y), column i (x)
+            if (i < npix && j < npix) {
+                double v = tile[tx][r];
+                if (reduce && wt[k] > 0.0)
+                    v /= wt[k];
+                out[q * nd + (int64_t)j * npix + i] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+static void launch_setup_t(const Map2DArgs &a, Rec2 *rec, hipStream_t s)
+{
+    if (a.n == 0)
+        return;
+    hipLaunchKernelGGL(k2d_setup<T>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s,
+                       (const T *)a.pos, (const T *)a.hsml, (const T *)a.m,
+                       (const T *)a.rho, (const T *)a.binq, a.nq, (const T *)a.w, a.n,
+                       a.npix, a.len2pix, a.kernel, a.flags, rec);
+}
+
+void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, hipStream_t s)
+{
+    if (a.dtype == S2G_F32)
+        launch_setup_t<float>(a, rec, s);
+    else
+        launch_setup_t<double>(a, rec, s);
+}
+
+void launch_2d_partition(const Rec2 *rec, int64_t n, int64_t *counts, int64_t *scan_tmp,
+                         int32_t *l0, int32_t *l1, int32_t *l2, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const unsigned nb = (unsigned)((n + kPartBlock - 1) / kPartBlock);
+    hipLaunchKernelGGL(k2d_part_count, dim3(nb), dim3(kPartBlock), 0, s, rec, n, counts);
+    exclusive_scan_i64(counts, 3 * (int64_t)nb + 1, scan_tmp, s);
+    hipLaunchKernelGGL(k2d_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, rec, n, counts,
+                       l0, l1, l2);
+}
+
+int64_t partition_blocks(int64_t n) { return (n + kPartBlock - 1) / kPartBlock; }
+
+template <int K>
+static void launch_norm_k(const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
+{
+    hipLaunchKernelGGL(k2d_norm<K>, dim3((unsigned)ne), dim3(kNormThreads), 0, s, list, rec);
+}
+
+void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
+{
+    if (ne == 0)
+        return;
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: launch_norm_k<S2G_KERNEL_CUBIC>(list, ne, rec, s); break;
+    case S2G_KERNEL_QUINTIC: launch_norm_k<S2G_KERNEL_QUINTIC>(list, ne, rec, s); break;
+    case S2G_KERNEL_WENDLAND_C2: launch_norm_k<S2G_KERNEL_WENDLAND_C2>(list, ne, rec, s); break;
+    case S2G_KERNEL_WENDLAND_C4: launch_norm_k<S2G_KERNEL_WENDLAND_C4>(list, ne, rec, s); break;
+    case S2G_KERNEL_WENDLAND_C6: launch_norm_k<S2G_KERNEL_WENDLAND_C6>(list, ne, rec, s); break;
+    default: launch_norm_k<S2G_KERNEL_WENDLAND_C8>(list, ne, rec, s); break;
+    }
+}
+
+void launch_2d_pairs(const int32_t *large, int64_t nl, const Rec2 *rec, int ntj, int ntiles,
+                     int nchunks, int64_t *cnt, int64_t *scan_tmp, int32_t *pairs_or_null,
+                     bool fill, hipStream_t s)
+{
+    const size_t lds = sizeof(int) * (size_t)ntiles;
+    if (!fill) {
+        hipLaunchKernelGGL(k2d_pair_count, dim3((unsigned)nchunks), dim3(256), lds, s, large,
+                           nl, rec, ntj, ntiles, nchunks, cnt);
+        exclusive_scan_i64(cnt, (int64_t)ntiles * nchunks + 1, scan_tmp, s);
+    } else {
+        hipLaunchKernelGGL(k2d_pair_fill, dim3((unsigned)nchunks), dim3(kWave), lds, s, large,
+                           nl, rec, ntj, ntiles, nchunks, cnt, pairs_or_null);
+    }
+}
+
+void launch_2d_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs, int64_t G,
+                     int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null,
+                     int64_t *tile_items, bool fill, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
+    if (!fill) {
+        hipLaunchKernelGGL(k2d_item_count, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks,
+                           npairs, G, nitems);
+        exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
+    } else {
+        hipLaunchKernelGGL(k2d_item_fill, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks,
+                           npairs, G, nitems, items_or_null, tile_items);
+    }
+}
+
+template <typename T, int K, int NP>
+static void gather_knp(const GatherArgs &g, hipStream_t s)
+{
+    hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
+                       g.items, g.pairs, g.large, g.rec, (const T *)g.binq, g.nq, g.plane0,
+                       g.ntj, g.partial);
+    hipLaunchKernelGGL(k2d_finalize<NP>, dim3((unsigned)g.ntiles), dim3(256), 0, s, g.partial,
+                       g.tile_items, g.ntj, g.npix, g.plane0, g.image);
+}
+
+template <typename T, int K>
+static void gather_k(const GatherArgs &g, hipStream_t s)
+{
+    switch (g.np) {
+    case 1: gather_knp<T, K, 1>(g, s); break;
+    case 2: gather_knp<T, K, 2>(g, s); break;
+    case 3: gather_knp<T, K, 3>(g, s); break;
+    default: gather_knp<T, K, 4>(g, s); break;
+    }
+}
+
+template <typename T>
+static void gather_t(const GatherArgs &g, hipStream_t s)
+{
+    switch (g.kernel) {
+    case S2G_KERNEL_CUBIC: gather_k<T, S2G_KERNEL_CUBIC>(g, s); break;
+    case S2G_KERNEL_QUINTIC: gather_k<T, S2G_KERNEL_QUINTIC>(g, s); break;
+    case S2G_KERNEL_WENDLAND_C2: gather_k<T, S2G_KERNEL_WENDLAND_C2>(g, s); break;
+    case S2G_KERNEL_WENDLAND_C4: gather_k<T, S2G_KERNEL_WENDLAND_C4>(g, s); break;
+    case S2G_KERNEL_WENDLAND_C6: gather_k<T, S2G_KERNEL_WENDLAND_C6>(g, s); break;
+    default: gather_k<T, S2G_KERNEL_WENDLAND_C8>(g, s); break;
+    }
+}
+
+void launch_2d_gather(const GatherArgs &g, hipStream_t s)
+{
+    if (g.dtype == S2G_F32)
+        gather_t<float>(g, s);
+    else
+        gather_t<double>(g, s);
+}
+
+template <typename T, int K>
+static void small_tk(const SmallArgs &a, hipStream_t s)
+{
+    const int64_t threads = a.ns * kWave;
+    hipLaunchKernelGGL((k2d_small_atomic<T, K>), dim3((unsigned)((threads + 255) / 256)),
+                       dim3(256), 0, s, a.list, a.ns, a.rec, (const T *)a.binq, a.nq, a.npix,
+                       a.image);
+}
+
+template <typename T>
+static void small_t(const SmallArgs &a, hipStream_t s)
+{
+    switch (a.kernel) {
+    case S2G_KERNEL_CUBIC: small_tk<T, S2G_KERNEL_CUBIC>(a, s); break;
+    case S2G_KERNEL_QUINTIC: small_tk<T, S2G_KERNEL_QUINTIC>(a, s); break;
+    case S2G_KERNEL_WENDLAND_C2: small_tk<T, S2G_KERNEL_WENDLAND_C2>(a, s); break;
+    case S2G_KERNEL_WENDLAND_C4: small_tk<T, S2G_KERNEL_WENDLAND_C4>(a, s); break;
+    case S2G_KERNEL_WENDLAND_C6: small_tk<T, S2G_KERNEL_WENDLAND_C6>(a, s); break;
+    default: small_tk<T, S2G_KERNEL_WENDLAND_C8>(a, s); break;
+    }
+}
+
+void launch_2d_small(const SmallArgs &a, hipStream_t s)
+{
+    if (a.ns == 0)
+        return;
+    if (a.dtype == S2G_F32)
+        small_t<float>(a, s);
+    else
+        small_t<double>(a, s);
+}
+
+void launch_2d_reduce(const double *image, int64_t npix, int nq, int reduce, double *out,
+                      hipStream_t s)
+{
+    const unsigned nt = (unsigned)((npix + 63) / 64);
+    hipLaunchKernelGGL(k2d_reduce, dim3(nt, nt), dim3(256), 0, s, image, npix, nq, reduce, out);
+}
+
+}  // namespace s2g

@@ -1,0 +1,536 @@
+// s2g_api.hip -- the C ABI of libsph2grid (include/sph2grid.h): context,
+// staging of host arrays, orchestration of the device pipelines.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "k_cic2d.h"
+#include "s2g_internal.h"
+
+using namespace s2g;
+
+struct s2g_ctx {
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // staged host inputs
+    DevBuf in[7];
+    // 2D pipeline
+    DevBuf rec, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
+    // outputs
+    DevBuf image, out;
+    // pinned host scratch for small device->host reads
+    int64_t *h_small = nullptr;
+};
+
+namespace {
+
+const char *kVersion = "sph2grid 0.1.0 gfx950";
+
+int fail(s2g_ctx *c, int code, const std::string &m)
+{
+    if (c)
+        c->err = m;
+    return code;
+}
+
+template <typename F>
+int guarded(s2g_ctx *c, F &&f)
+{
+    if (!c)
+        return S2G_EINVAL;
+    try {
+        c->err.clear();
+        int dev = -1;
+        if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
+            S2G_HIP(hipSetDevice(c->device));
+        return f();
+    } catch (const Error &e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        c->err = e.what();
+        return S2G_EINVAL;
+    } catch (...) {
+        c->err = "unknown error";
+        return S2G_EINVAL;
+    }
+}
+
+size_t elem_size(int dtype) { return dtype == S2G_F32 ? 4 : 8; }
+
+void check_common(int dtype, int kernel, int kernel_dim, int64_t n, const s2g_params *p)
+{
+    if (dtype != S2G_F32 && dtype != S2G_F64)
+        throw Error(S2G_EINVAL, "dtype must be S2G_F32 or S2G_F64");
+    if (kernel < S2G_KERNEL_CUBIC || kernel > S2G_KERNEL_WENDLAND_C8)
+        throw Error(S2G_EUNSUPPORTED, "unsupported kernel id " + std::to_string(kernel));
+    if (kernel_dim != 2 && kernel_dim != 3)
+        throw Error(S2G_EINVAL, "kernel_dim must be 2 or 3");
+    if (n < 0 || n > (int64_t)INT32_MAX)
+        throw Error(S2G_EINVAL, "particle count out of range [0, 2^31)");
+    if (!p)
+        throw Error(S2G_EINVAL, "params is NULL");
+    if (p->npixels < 1 || p->npixels > 65536)
+        throw Error(S2G_EINVAL, "npixels out of range [1, 65536]");
+    if (!(p->len2pix > 0.0) || !std::isfinite(p->len2pix))
+        throw Error(S2G_EINVAL, "len2pix must be positive and finite");
+}
+
+// D2H of a few int64 values at device addresses (synchronises the stream)
+void read_i64(s2g_ctx *c, hipStream_t s, std::initializer_list<const int64_t *> src,
+              int64_t *dst)
+{
+    int k = 0;
+    for (const int64_t *p : src)
+        S2G_HIP(hipMemcpyAsync(&c->h_small[k++], p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < k; ++i)
+        dst[i] = c->h_small[i];
+}
+
+// ---------------------------------------------------------------------------
+// 2D deposit on device pointers
+// ---------------------------------------------------------------------------
+void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream_t s,
+                      s2g_stats *st)
+{
+    const int64_t n = a.n, npix = a.npix;
+    const int64_t nd = npix * npix;
+    const int nplanes = a.nq + 1;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+    }
+    S2G_HIP(hipEventRecord(c->ev0, s));
+    if (n == 0) {
+        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
+        S2G_HIP(hipEventRecord(c->ev1, s));
+        S2G_HIP(hipEventSynchronize(c->ev1));
+        return;
+    }
+    Rec2 *rec = (Rec2 *)c->rec.get(sizeof(Rec2) * n);
+    launch_2d_setup(a, rec, s);
+
+    const int64_t nb = partition_blocks(n);
+    int64_t *counts = (int64_t *)c->counts.get(sizeof(int64_t) * (3 * nb + 1));
+    int32_t *l0 = (int32_t *)c->lists[0].get(sizeof(int32_t) * n);
+    int32_t *l1 = (int32_t *)c->lists[1].get(sizeof(int32_t) * n);
+    int32_t *l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
+    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
+    launch_2d_partition(rec, n, counts, tmp, l0, l1, l2, s);
+    int64_t o[4];
+    read_i64(c, s, {counts, counts + nb, counts + 2 * nb, counts + 3 * nb}, o);
+    const int64_t ns = o[1] - o[0], nl = o[2] - o[1], ne = o[3] - o[2];
+    if (st) {
+        st->n_small = ns;
+        st->n_large = nl;
+        st->n_large_exact = ne;
+        st->n_skipped = n - ns - nl;
+    }
+
+    launch_2d_norm(a.kernel, l2, ne, rec, s);
+
+    const int ntj = (int)((npix + kGT - 1) / kGT);
+    const int ntiles = ntj * ntj;
+    if (nl > 0) {
+        if (ntiles > kMaxTiles2D)
+            throw Error(S2G_EUNSUPPORTED, "image too large for the tile gather (npix > 8192)");
+        const int nchunks = (int)((nl + kPairChunk - 1) / kPairChunk);
+        const int64_t ncnt = (int64_t)ntiles * nchunks + 1;
+        int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
+        tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
+        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp, nullptr, false, s);
+        int64_t npairs;
+        read_i64(c, s, {cnt + ncnt - 1}, &npairs);
+        if (npairs >= (int64_t)INT32_MAX)
+            throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
+        int32_t *pairs = (int32_t *)c->pairs.get(sizeof(int32_t) * (npairs ? npairs : 1));
+        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp, pairs, true, s);
+
+        // work items of <= G pairs: enough items to fill every CU several times
+        const int64_t target = (int64_t)c->ncu * 16;
+        int64_t G = (npairs + target - 1) / target;
+        if (G < 64)
+            G = 64;
+        int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
+        tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ntiles + 1));
+        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp, nullptr, nullptr, false, s);
+        int64_t nitems;
+        read_i64(c, s, {nit + ntiles}, &nitems);
+        int4 *items = (int4 *)c->items.get(sizeof(int4) * (nitems ? nitems : 1));
+        int64_t *tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
+        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp, items, tile_items, true, s);
+        if (st)
+            st->n_pairs = npairs;
+
+        for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
+            const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
+            GatherArgs g;
+            g.dtype = a.dtype;
+            g.kernel = a.kernel;
+            g.np = np;
+            g.plane0 = plane0;
+            g.nq = a.nq;
+            g.ntj = ntj;
+            g.ntiles = ntiles;
+            g.nitems = nitems;
+            g.npix = npix;
+            g.items = items;
+            g.pairs = pairs;
+            g.large = l1;
+            g.rec = rec;
+            g.binq = a.binq;
+            g.partial = (double *)c->partial.get(sizeof(double) * (nitems ? nitems : 1) * np *
+                                                 kGT * kGT);
+            g.tile_items = tile_items;
+            g.image = d_image;
+            if (nitems > 0) {
+                launch_2d_gather(g, s);
+            } else {
+                for (int q = 0; q < np; ++q)
+                    S2G_HIP(hipMemsetAsync(d_image + (int64_t)(plane0 + q) * nd, 0,
+                                           sizeof(double) * nd, s));
+            }
+        }
+    } else {
+        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
+    }
+
+    SmallArgs sa;
+    sa.dtype = a.dtype;
+    sa.kernel = a.kernel;
+    sa.nq = a.nq;
+    sa.ns = ns;
+    sa.npix = npix;
+    sa.list = l0;
+    sa.rec = rec;
+    sa.binq = a.binq;
+    sa.image = d_image;
+    launch_2d_small(sa, s);
+    S2G_HIP(hipGetLastError());
+    S2G_HIP(hipEventRecord(c->ev1, s));
+    S2G_HIP(hipEventSynchronize(c->ev1));
+    if (st) {
+        float ms = 0.f;
+        S2G_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->ms_device = ms;
+    }
+}
+
+Map2DArgs make_args(int dtype, const void *pos, const void *hsml, const void *m,
+                    const void *rho, const void *bin_q, int nq, const void *weights, int64_t n,
+                    const s2g_params *params, int kernel, uint32_t flags)
+{
+    Map2DArgs a;
+    a.dtype = dtype;
+    a.pos = pos;
+    a.hsml = hsml;
+    a.m = m;
+    a.rho = rho;
+    a.binq = bin_q;
+    a.w = weights;
+    a.nq = nq;
+    a.n = n;
+    a.npix = params->npixels;
+    a.len2pix = params->len2pix;
+    a.kernel = kernel;
+    a.flags = flags;
+    return a;
+}
+
+// copy host inputs into the context's staging buffers
+Map2DArgs stage_inputs(s2g_ctx *c, const Map2DArgs &h, hipStream_t s, s2g_stats *st)
+{
+    const size_t es = elem_size(h.dtype);
+    const int64_t n = h.n;
+    const void *src[6] = {h.pos, h.hsml, h.m, h.rho, h.binq, h.w};
+    const size_t cnt[6] = {(size_t)(3 * n), (size_t)n, (size_t)n, (size_t)n,
+                           (size_t)(n * h.nq), (size_t)n};
+    void *dst[6];
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    S2G_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < 6; ++k) {
+        if (n > 0 && !src[k])
+            throw Error(S2G_EINVAL, "input array pointer is NULL");
+        dst[k] = c->in[k].get(es * (cnt[k] ? cnt[k] : 1));
+        if (n > 0)
+            S2G_HIP(hipMemcpyAsync(dst[k], src[k], es * cnt[k], hipMemcpyHostToDevice, s));
+    }
+    S2G_HIP(hipEventRecord(e1, s));
+    S2G_HIP(hipEventSynchronize(e1));
+    if (st) {
+        float ms = 0.f;
+        S2G_HIP(hipEventElapsedTime(&ms, e0, e1));
+        st->ms_h2d = ms;
+    }
+    Map2DArgs d = h;
+    d.pos = dst[0];
+    d.hsml = dst[1];
+    d.m = dst[2];
+    d.rho = dst[3];
+    d.binq = dst[4];
+    d.w = dst[5];
+    return d;
+}
+
+void d2h(s2g_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t s, double *ms)
+{
+    S2G_HIP(hipEventRecord(c->ev0, s));
+    S2G_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipEventRecord(c->ev1, s));
+    S2G_HIP(hipEventSynchronize(c->ev1));
+    if (ms) {
+        float t = 0.f;
+        S2G_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        *ms = t;
+    }
+}
+
+void check_dims_nq(int dims, int nq)
+{
+    if (dims != 2 && dims != 3)
+        throw Error(S2G_EINVAL, "dims must be 2 or 3");
+    if (nq < 1 || nq > 64)
+        throw Error(S2G_EINVAL, "nq must be in [1, 64]");
+    if (dims == 3 && nq != 1)
+        throw Error(S2G_EINVAL, "the 3D deposit maps a scalar quantity (nq == 1)");
+}
+
+void deposit_device_any(s2g_ctx *c, int dims, const Map2DArgs &a, double *d_image,
+                        hipStream_t s, s2g_stats *st)
+{
+    if (dims == 2)
+        deposit2d_device(c, a, d_image, s, st);
+    else
+        throw Error(S2G_EUNSUPPORTED, "3D deposit not available in this build");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *s2g_version(void) { return kVersion; }
+
+int s2g_create(s2g_ctx **out, int device)
+{
+    if (!out)
+        return S2G_EINVAL;
+    *out = nullptr;
+    s2g_ctx *c = nullptr;
+    try {
+        c = new s2g_ctx();
+        c->device = device;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+            delete c;
+            return S2G_EHIP;
+        }
+        if (hipSetDevice(device) != hipSuccess) {
+            delete c;
+            return S2G_EHIP;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+            c->ncu = prop.multiProcessorCount;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+            hipHostMalloc((void **)&c->h_small, 64 * sizeof(int64_t), hipHostMallocDefault) !=
+                hipSuccess) {
+            s2g_destroy(c);
+            return S2G_EHIP;
+        }
+    } catch (...) {
+        delete c;
+        return S2G_ENOMEM;
+    }
+    *out = c;
+    return S2G_OK;
+}
+
+void s2g_destroy(s2g_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->in) b.release();
+    c->rec.release();
+    c->counts.release();
+    c->scan_tmp.release();
+    for (auto &b : c->lists) b.release();
+    c->cnt.release();
+    c->pairs.release();
+    c->nitems.release();
+    c->items.release();
+    c->tile_items.release();
+    c->partial.release();
+    c->image.release();
+    c->out.release();
+    if (c->h_small)
+        (void)hipHostFree(c->h_small);
+    if (c->ev0)
+        (void)hipEventDestroy(c->ev0);
+    if (c->ev1)
+        (void)hipEventDestroy(c->ev1);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *s2g_last_error(const s2g_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int s2g_deposit_device(s2g_ctx *c, int dims, int dtype, const void *d_pos, const void *d_hsml,
+                       const void *d_m, const void *d_rho, const void *d_bin_q, int nq,
+                       const void *d_weights, int64_t n, const s2g_params *params, int kernel,
+                       int kernel_dim, uint32_t flags, double *d_image_out, void *stream,
+                       s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        check_common(dtype, kernel, kernel_dim, n, params);
+        check_dims_nq(dims, nq);
+        if (!d_image_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        Map2DArgs a = make_args(dtype, d_pos, d_hsml, d_m, d_rho, d_bin_q, nq, d_weights, n,
+                                params, kernel, flags);
+        deposit_device_any(c, dims, a, d_image_out, s, stats);
+        return S2G_OK;
+    });
+}
+
+int s2g_deposit(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml,
+                const void *m, const void *rho, const void *bin_q, int nq, const void *weights,
+                int64_t n, const s2g_params *params, int kernel, int kernel_dim, uint32_t flags,
+                double *image_out, s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        check_common(dtype, kernel, kernel_dim, n, params);
+        check_dims_nq(dims, nq);
+        if (!image_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        const int64_t npix = params->npixels;
+        const int64_t nd = dims == 2 ? npix * npix : npix * npix * npix;
+        const int nplanes = nq + 1;
+        hipStream_t s = c->stream;
+        s2g_stats local;
+        s2g_stats *st = stats ? stats : &local;
+        Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, nq, weights, n, params,
+                                kernel, flags);
+        double h2d_ms = 0.0;
+        {
+            s2g_stats tmp;
+            h = stage_inputs(c, h, s, &tmp);
+            h2d_ms = tmp.ms_h2d;
+        }
+        double *img = (double *)c->image.get(sizeof(double) * nd * nplanes);
+        deposit_device_any(c, dims, h, img, s, st);
+        st->ms_h2d = h2d_ms;
+        d2h(c, image_out, img, sizeof(double) * nd * nplanes, s, &st->ms_d2h);
+        return S2G_OK;
+    });
+}
+
+int s2g_reduce_device(s2g_ctx *c, int dims, int64_t npix, int nq, uint32_t flags,
+                      const double *d_image, double *d_out, void *stream)
+{
+    return guarded(c, [&]() {
+        check_dims_nq(dims, nq);
+        if (npix < 1 || npix > 65536)
+            throw Error(S2G_EINVAL, "npixels out of range [1, 65536]");
+        if (!d_image || !d_out)
+            throw Error(S2G_EINVAL, "image pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        if (dims == 2)
+            launch_2d_reduce(d_image, npix, nq, (flags & S2G_FLAG_REDUCE) ? 1 : 0, d_out, s);
+        else
+            throw Error(S2G_EUNSUPPORTED, "3D reduce not available in this build");
+        S2G_HIP(hipGetLastError());
+        return S2G_OK;
+    });
+}
+
+int s2g_reduce(s2g_ctx *c, int dims, int64_t npix, int nq, uint32_t flags, const double *image,
+               double *out)
+{
+    return guarded(c, [&]() {
+        check_dims_nq(dims, nq);
+        if (npix < 1 || npix > 65536)
+            throw Error(S2G_EINVAL, "npixels out of range [1, 65536]");
+        if (!image || !out)
+            throw Error(S2G_EINVAL, "image pointer is NULL");
+        const int64_t nd = dims == 2 ? npix * npix : npix * npix * npix;
+        const int nin = dims == 2 ? nq + 1 : 2, nout = dims == 2 ? nq : 1;
+        hipStream_t s = c->stream;
+        double *img = (double *)c->image.get(sizeof(double) * nd * nin);
+        double *o = (double *)c->out.get(sizeof(double) * nd * nout);
+        S2G_HIP(hipMemcpyAsync(img, image, sizeof(double) * nd * nin, hipMemcpyHostToDevice, s));
+        int rc = s2g_reduce_device(c, dims, npix, nq, flags, img, o, s);
+        if (rc)
+            return rc;
+        d2h(c, out, o, sizeof(double) * nd * nout, s, nullptr);
+        return S2G_OK;
+    });
+}
+
+int s2g_map(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml, const void *m,
+            const void *rho, const void *bin_q, int nq, const void *weights, int64_t n,
+            const s2g_params *params, int kernel, int kernel_dim, uint32_t flags,
+            double *map_out, s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        check_common(dtype, kernel, kernel_dim, n, params);
+        check_dims_nq(dims, nq);
+        if (!map_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        const int64_t npix = params->npixels;
+        const int64_t nd = dims == 2 ? npix * npix : npix * npix * npix;
+        const int nout = dims == 2 ? nq : 1;
+        hipStream_t s = c->stream;
+        s2g_stats local;
+        s2g_stats *st = stats ? stats : &local;
+        Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, nq, weights, n, params,
+                                kernel, flags);
+        double h2d_ms = 0.0;
+        {
+            s2g_stats tmp;
+            h = stage_inputs(c, h, s, &tmp);
+            h2d_ms = tmp.ms_h2d;
+        }
+        double *img = (double *)c->image.get(sizeof(double) * nd * (nq + 1));
+        deposit_device_any(c, dims, h, img, s, st);
+        st->ms_h2d = h2d_ms;
+        double *o = (double *)c->out.get(sizeof(double) * nd * nout);
+        int rc = s2g_reduce_device(c, dims, npix, nq, flags, img, o, s);
+        if (rc)
+            return rc;
+        d2h(c, map_out, o, sizeof(double) * nd * nout, s, &st->ms_d2h);
+        return S2G_OK;
+    });
+}
+
+int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml,
+                        const void *m, const void *rho, const void *bin_q,
+                        const void *weights, int64_t n, int64_t nside, int kernel,
+                        int kernel_dim, uint32_t flags, double *map_out, double *weight_out,
+                        s2g_stats *stats)
+{
+    return fail(c, S2G_EUNSUPPORTED, "healpix deposit not available in this build");
+}
+
+int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t n,
+              const s2g_params *params, uint32_t flags, double *grid_out, s2g_stats *stats)
+{
+    return fail(c, S2G_EUNSUPPORTED, "tsc deposit not available in this build");
+}
+
+int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps, void *stream,
+                      s2g_stats *stats)
+{
+    return fail(c, S2G_EUNSUPPORTED, "batched deposit not available in this build");
+}
+
+}  // extern "C"

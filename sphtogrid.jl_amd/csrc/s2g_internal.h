@@ -1,0 +1,206 @@
+// s2g_internal.h -- shared device helpers and host-side declarations of
+// libsph2grid (gfx950, wave64).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sph2grid.h"
+
+namespace s2g {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Particle classes of the 2D/3D deposits
+// ---------------------------------------------------------------------------
+enum : int32_t {
+    kClsSkip = 0,      // zero quantity without calc_mean, or empty footprint
+    kClsSmall = 1,     // footprint box fits the scatter tile halo
+    kClsLargeAn = 2,   // gather path, closed-form normalisation
+    kClsLargeEx = 3,   // gather path, explicit normalisation sum
+};
+
+// Largest footprint side (pixels) of the scatter path.  A box of side <= 34
+// corresponds to hsml <= 16 pixels (side = floor(x+h) - floor(x-h) + 1).
+constexpr int kSmallSide2D = 34;
+// 2D gather tile: 64 x 64 pixels, 4 waves x 16 rows x 64 columns.
+constexpr int kGT = 64;
+constexpr int kGRows = 16;
+
+// Per-particle record of the 2D deposit, built once per call.
+struct Rec2 {
+    double x, y;     // pixel coordinates, cic_shared.jl:85-100
+    double h;        // hsml in pixels, cic_2D.jl:83
+    double s;        // area*w*dz (small / exact) or the final pixel scale (large)
+    int32_t i0, i1, j0, j1;  // footprint, cic_shared.jl:46-52
+    int32_t cls;
+    int32_t fb;      // fallback (no pixel centre inside the kernel), cic_2D.jl:51-69
+};
+
+// ---------------------------------------------------------------------------
+// Kernel shapes (SPHKernels.jl closed forms).  shape(u) for u < 1, 0 beyond.
+// The norm * h_inv^dim factor cancels in every deposit (pw ~ wk*A/sum(wk*A)).
+// ---------------------------------------------------------------------------
+template <int K, typename R>
+__device__ __forceinline__ R shape_of(R u)
+{
+    const R one = R(1);
+    R t = one - u;
+    t = t > R(0) ? t : R(0);
+    if constexpr (K == S2G_KERNEL_CUBIC) {
+        // u < 1/2: 1 + 6 (u-1) u^2 ; 1/2 <= u < 1: 2 (1-u)^3
+        R a = R(1) + R(6) * (u - one) * u * u;
+        R b = R(2) * t * t * t;
+        return u < R(0.5) ? a : b;
+    } else if constexpr (K == S2G_KERNEL_QUINTIC) {
+        R t2 = t * t;
+        R v = t2 * t2 * t;
+        R a = R(2.0 / 3.0) - u;
+        a = a > R(0) ? a : R(0);
+        R a2 = a * a;
+        R b = R(1.0 / 3.0) - u;
+        b = b > R(0) ? b : R(0);
+        R b2 = b * b;
+        return v - R(6) * (a2 * a2 * a) + R(15) * (b2 * b2 * b);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C2) {
+        R t2 = t * t;
+        return t2 * t2 * (R(1) + R(4) * u);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C4) {
+        R t2 = t * t;
+        R p = R(1) + u * (R(6) + u * R(35.0 / 3.0));
+        return t2 * t2 * t2 * p;
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C6) {
+        R t2 = t * t;
+        R t4 = t2 * t2;
+        R p = R(1) + u * (R(8) + u * (R(25) + u * R(32)));
+        return t4 * t4 * p;
+    } else {  // WENDLAND_C8
+        R t2 = t * t;
+        R t4 = t2 * t2;
+        R p = R(1) + u * (R(10) + u * (R(42) + u * (R(90) + u * R(429.0 / 5.0))));
+        return t4 * t4 * t2 * p;
+    }
+}
+
+// integral of shape(|v|) over the unit disc (= 1 / norm_2D), used for the
+// closed-form normalisation of large unclipped footprints
+__host__ __device__ inline double shape_disc_integral(int kernel)
+{
+    const double pi = 3.14159265358979323846;
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: return 7.0 * pi / 40.0;
+    case S2G_KERNEL_QUINTIC: return 478.0 * pi / 15309.0;
+    case S2G_KERNEL_WENDLAND_C2: return pi / 7.0;
+    case S2G_KERNEL_WENDLAND_C4: return pi / 9.0;
+    case S2G_KERNEL_WENDLAND_C6: return 7.0 * pi / 78.0;
+    default: return 3.0 * pi / 40.0;  // WENDLAND_C8
+    }
+}
+
+// Smallest hsml (pixels) at which the closed-form normalisation h^2 * I_k
+// matches the discrete footprint sum to better than ~1e-9 relative
+// (midpoint-rule error of a compactly supported C^k function; measured over
+// random sub-pixel offsets, see tests/test_norm_closed_form.py).
+__host__ __device__ inline double closed_form_min_h(int kernel)
+{
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: return 64.0;
+    case S2G_KERNEL_QUINTIC: return 32.0;
+    case S2G_KERNEL_WENDLAND_C2: return 64.0;
+    case S2G_KERNEL_WENDLAND_C4: return 32.0;
+    case S2G_KERNEL_WENDLAND_C6: return 24.0;
+    default: return 20.0;  // WENDLAND_C8
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 helpers that must not be contracted into FMAs (cic_shared.jl:85-100
+// rounds x*len2pix and then +0.5*Npix).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+
+// overlap of [x-h, x+h] with pixel [i, i+1] (cic_shared.jl:60-62)
+__device__ __forceinline__ double overlap1(double x, double h, double fi)
+{
+    double a = add_rn(x, h), b = fi + 1.0;
+    double c = sub_rn(x, h);
+    return sub_rn(fmin(a, b), fmax(c, fi));
+}
+// offset of x from the centre of pixel i (cic_shared.jl:70-76)
+__device__ __forceinline__ double coff1(double x, double fi)
+{
+    return sub_rn(sub_rn(x, fi), 0.5);
+}
+
+template <typename T>
+__device__ __forceinline__ double ld(const T *p, int64_t i)
+{
+    return (double)p[i];
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+#define S2G_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw ::s2g::Error(S2G_EHIP, std::string(#call) + ": " +         \
+                                             hipGetErrorString(e_));         \
+    } while (0)
+
+struct Error {
+    int code;
+    std::string msg;
+    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need)
+    {
+        if (need > bytes) {
+            if (p)
+                (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            size_t b = need < 256 ? 256 : need;
+            b = (b + 255) & ~size_t(255);
+            if (hipMalloc(&p, b) != hipSuccess) {
+                p = nullptr;
+                throw Error(S2G_ENOMEM, "device allocation of " + std::to_string(b) +
+                                            " bytes failed");
+            }
+            bytes = b;
+        }
+        return p;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// exclusive scan of n int64 values in place; returns nothing, device only.
+// tmp must hold scan_tmp_elems(n) int64.
+size_t scan_tmp_elems(int64_t n);
+void exclusive_scan_i64(int64_t *d, int64_t n, int64_t *tmp, hipStream_t s);
+
+}  // namespace s2g
