@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark of the deposition hot path on one MI355X.
+
+Default workload: the cluster-like particle set (2^24 particles, 80% in an
+r^-2 halo + 20% uniform background, hsml log-uniform over [1e-4, 1e-1] box
+lengths), WendlandC6, 2D 4096^2 map of T weighted by Rho with reduce_image.
+One step = one deposit (cic_mapping_2D equivalent) + reduce_image_2D over the
+whole particle set, inputs already resident in HBM.
+
+Prints ONE JSON line (stdout).  Everything else goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "sphtogrid.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBPS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+# flops per pixel-update of the 2D deposit (roofline convention): 10 + 2*Nq + K
+K_FLOPS = {"cubic": 6, "quintic": 16, "wendlandc2": 6, "wendlandc4": 9, "wendlandc6": 11,
+           "wendlandc8": 14}
+KIDS = {"cubic": 0, "quintic": 1, "wendlandc2": 2, "wendlandc4": 3, "wendlandc6": 4,
+        "wendlandc8": 5}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def footprints(x, h, npix, len2pix):
+    """Integer footprint sizes per particle (cic_shared.jl:46-52, FP64)."""
+    xs = x[:, 0].astype(np.float64) * len2pix + 0.5 * npix
+    ys = x[:, 1].astype(np.float64) * len2pix + 0.5 * npix
+    hp = h.astype(np.float64) * len2pix
+    i0 = np.maximum(np.floor(xs - hp), 0)
+    i1 = np.minimum(np.floor(xs + hp), npix - 1)
+    j0 = np.maximum(np.floor(ys - hp), 0)
+    j1 = np.minimum(np.floor(ys + hp), npix - 1)
+    wi = np.maximum(i1 - i0 + 1, 0)
+    wj = np.maximum(j1 - j0 + 1, 0)
+    return wi, wj
+
+
+def prepare(d, s2g):
+    """Host-side part of sphMapping: centre + filter (cic_interpolation.jl:59-129)."""
+    par = s2g.mappingParameters(x_lim=d["lims"][0], y_lim=d["lims"][1], z_lim=d["lims"][2],
+                                Npixels=d["Npixels"])
+    pos = d["Pos"].copy()
+    pos, par2 = s2g.center_particles(pos, par)
+    keep = s2g.filter_particles_in_image(pos, par2)
+    bq = d["Bin_Quant"]
+    out = dict(x=np.ascontiguousarray(pos[:, keep].T), hsml=d["HSML"][keep], m=d["M"][keep],
+               rho=d["Rho"][keep], w=d["Weights"][keep],
+               bq=(bq[keep] if bq.ndim == 1 else np.ascontiguousarray(bq[:, keep].T)),
+               npix=int(par2.Npixels[0]), len2pix=float(par2.len2pix),
+               kernel=d["kernel"], calc_mean=d["calc_mean"], reduce=d["reduce_image"])
+    out["nq"] = 1 if out["bq"].ndim == 1 else out["bq"].shape[1]
+    return out
+
+
+def cpu_baseline(p, target_s, threads):
+    """Oracle (FP64 C restatement, reference parallel=true layout: contiguous
+    batches, private images, summed in worker order) on a seeded particle
+    subsample; scaled to the full set by the ratio of footprint sums."""
+    import oracle
+    n = p["x"].shape[0]
+    wi, wj = footprints(p["x"], p["hsml"], p["npix"], p["len2pix"])
+    fp = wi * wj
+    fp_full = float(fp.sum())
+    rng = np.random.default_rng(12345)
+
+    def timed(fs_target):
+        frac = min(1.0, fs_target / max(fp_full, 1.0))
+        idx = np.sort(rng.choice(n, size=max(1, int(n * frac)), replace=False))
+        sub = {k: p[k][idx] for k in ("x", "hsml", "m", "rho", "w", "bq")}
+        t0 = time.perf_counter()
+        img = oracle.cic_mapping_2D(sub["x"], sub["hsml"], sub["m"], sub["rho"], sub["bq"],
+                                    sub["w"], p["npix"], p["len2pix"], p["kernel"], 2,
+                                    p["calc_mean"], nworkers=threads)
+        oracle.reduce_image_2D(img, p["npix"], p["reduce"])
+        dt = time.perf_counter() - t0
+        fs = float(fp[idx].sum())
+        log(f"cpu baseline sample: {idx.size} particles, {fs:.3g} footprint pixels, "
+            f"{dt:.2f} s")
+        return idx.size, fs, dt
+
+    # wall time = a + b * (footprint pixels): a is the per-call cost of the
+    # reference layout (one full-size image + scratch per worker, summed in
+    # worker order), b the per-pixel cost; fit on two samples, the second
+    # sized to ~target_s, then extrapolate to the full set
+    _, f1, t1 = timed(min(fp_full, 3e7))
+    _, f2, t2 = timed(min(fp_full, 1.5e8))
+    b = max((t2 - t1) / max(f2 - f1, 1.0), 1e-15)
+    a = max(t1 - b * f1, 0.0)
+    f3_target = min(fp_full, max(f2, (target_s - a) / b))
+    n3, f3, t3 = timed(f3_target)
+    b = max((t3 - t1) / max(f3 - f1, 1.0), 1e-15) if f3 > f1 else b
+    a = max(t1 - b * f1, 0.0)
+    t_full = a + b * fp_full
+    sample = (f"{n3} of {n} particles (seeded uniform subsample, {f3:.3g} of {fp_full:.3g} "
+              f"footprint pixels) in {t3:.1f} s on {threads} threads; full set "
+              f"extrapolated as a + b*footprint (a = {a:.2f} s per call, b = {b:.3g} s per "
+              f"pixel-update) = {t_full:.1f} s")
+    return {"value": n / t_full, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": sample}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cluster", choices=["cluster", "plain", "sedov"])
+    ap.add_argument("--n", type=int, default=0, help="override the particle count")
+    ap.add_argument("--npix", type=int, default=0)
+    ap.add_argument("--exact-norm", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        msg = ("bench.py runs on one GPU only: the deposit is measured on a single MI355X "
+               f"(--gpus {args.gpus} is not supported)")
+        log(msg)
+        print(msg)
+        sys.exit(2)
+
+    import torch
+
+    import sph2grid as s2g
+    from sph2grid import _lib, workloads
+
+    t0 = time.perf_counter()
+    gen = {"cluster": workloads.cluster, "plain": workloads.plain, "sedov": workloads.sedov}
+    kw = {}
+    if args.n:
+        kw["n"] = args.n
+    if args.npix:
+        kw["npix"] = args.npix
+    d = gen[args.workload](**kw)
+    p = prepare(d, s2g)
+    n = p["x"].shape[0]
+    log(f"workload {args.workload}: {n} particles, {p['npix']}^2, {p['kernel']}, "
+        f"nq={p['nq']}, generated in {time.perf_counter() - t0:.1f} s")
+
+    dev = torch.device("cuda:0")
+    tdt = torch.float32 if p["x"].dtype == np.float32 else torch.float64
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype=tdt)
+    dx, dh, dm, drho, dw, dbq = (T(p[k]) for k in ("x", "hsml", "m", "rho", "w", "bq"))
+    npix, nq = p["npix"], p["nq"]
+    nd = npix * npix
+    d_img = torch.empty(nd * (nq + 1), dtype=torch.float64, device=dev)
+    d_map = torch.empty(nd * nq, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    ctx = s2g.default_context()
+    params = _lib.Params(npixels=npix, len2pix=p["len2pix"])
+    flags = (_lib.S2G_FLAG_CALC_MEAN if p["calc_mean"] else 0) | \
+        (_lib.S2G_FLAG_EXACT_NORM if args.exact_norm else 0)
+    rflags = _lib.S2G_FLAG_REDUCE if p["reduce"] else 0
+    dtype_id = _lib.S2G_F32 if tdt == torch.float32 else _lib.S2G_F64
+    stream = torch.cuda.current_stream().cuda_stream
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    st = _lib.Stats()
+
+    def step():
+        rc = ctx.lib.s2g_deposit_device(ctx.handle, 2, dtype_id, vp(dx), vp(dh), vp(dm),
+                                        vp(drho), vp(dbq), nq, vp(dw), n, C.byref(params),
+                                        KIDS[p["kernel"]], 2, flags, vp(d_img),
+                                        C.c_void_p(stream), C.byref(st))
+        ctx.check(rc, "s2g_deposit_device")
+        rc = ctx.lib.s2g_reduce_device(ctx.handle, 2, npix, nq, rflags, vp(d_img), vp(d_map),
+                                       C.c_void_p(stream))
+        ctx.check(rc, "s2g_reduce_device")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stage = {k: 0.0 for k in ("ms_device", "ms_setup", "ms_norm", "ms_sort", "ms_gather",
+                              "ms_finalize", "ms_small")}
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        for k in stage:
+            stage[k] += getattr(st, k)
+        log(f"step {i}: device {st.ms_device:.1f} ms (gather {st.ms_gather:.1f} ms)")
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ms_step = (t2 - t1) * 1e3 / args.steps
+    for k in stage:
+        stage[k] /= args.steps
+    value = n / (ms_step / 1e3)
+
+    # full-size property checks: conservation of the weight plane
+    img = d_img.view(nq + 1, nd)
+    w_sum = float(img[nq].sum().item())
+    x64 = {k: p[k].astype(np.float64) for k in ("hsml", "m", "rho", "w")}
+    wi, wj = footprints(p["x"], p["hsml"], npix, p["len2pix"])
+    dep = (wi > 0) & (wj > 0)
+    if not p["calc_mean"]:
+        bq = p["bq"] if p["bq"].ndim == 1 else p["bq"].any(axis=1)
+        dep &= (bq != 0)
+    w_expect = float((x64["w"] * x64["m"] / x64["rho"])[dep].sum() * p["len2pix"] ** 3)
+    cons = abs(w_sum / w_expect - 1.0) if w_expect else 0.0
+
+    # roofline of the dominant kernel (the tile gather): algorithmic flops of
+    # the pixel-updates it performs / its measured duration
+    small = (wi <= 34) & (wj <= 34)
+    fp_large = float((wi * wj)[dep & ~small].sum())
+    fp_all = float((wi * wj)[dep].sum())
+    fk = 10 + 2 * nq + K_FLOPS[p["kernel"]]
+    gather_s = stage["ms_gather"] / 1e3
+    achieved = fp_large * fk / gather_s / 1e12 if gather_s > 0 else 0.0
+    s_in = p["x"].dtype.itemsize
+    b_alg = n * (6 + nq) * s_in + nd * nq * 8
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "kernel": "k2d_gather", "compute_unit": "FP32 vector ALU (no MFMA: "
+                "scatter/gather); peak = MI355X FP32 vector rate",
+                "alg_flops_per_launch": fp_large * fk,
+                "alg_pixel_updates_per_launch": fp_large,
+                "ms_per_launch": stage["ms_gather"]}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            aff = len(os.sched_getaffinity(0))
+        except AttributeError:
+            aff = os.cpu_count() or 1
+        threads = max(1, min(16, aff))
+        cpu = cpu_baseline(p, args.cpu_seconds, threads)
+
+    out = {
+        "metric": "particles/s (2D 4096^2 WendlandC6 map, cluster 2^24)"
+        if args.workload == "cluster" else f"particles/s ({args.workload})",
+        "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n} particles, {npix}^2 2D map, "
+                               f"{p['kernel']}, Nq={nq}, FP32 inputs in HBM, deposit + "
+                               f"reduce_image per step",
+                   "particles": n, "npix": npix, "kernel": p["kernel"], "nq": nq,
+                   "exact_norm": bool(args.exact_norm)},
+        "hbm_alg_GBps": b_alg / (ms_step / 1e3) / 1e9,
+        "pixel_updates_per_s": fp_all / (ms_step / 1e3),
+        "stage_ms": stage,
+        "counts": {"small": st.n_small, "large": st.n_large, "large_exact": st.n_large_exact,
+                   "pairs": st.n_pairs},
+        "conservation_rel_err": cons,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

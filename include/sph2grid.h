@@ -101,7 +101,15 @@ typedef struct s2g_stats {
     int64_t n_large;          /* particles deposited by the tile gather path */
     int64_t n_large_exact;    /* ... of which normalised by an explicit sum */
     int64_t n_pairs;          /* (tile, particle) pairs of the gather path */
-    double sum_footprint;     /* sum over deposited particles of the footprint size */
+    double sum_footprint;     /* reserved (0) */
+    /* device time per stage (HIP events on the deposit stream), ms */
+    double ms_setup;          /* per-particle setup + class partition */
+    double ms_norm;           /* explicit footprint normalisation sums */
+    double ms_sort;           /* pair counting sort + work items */
+    double ms_gather;         /* tile gather kernels only */
+    double ms_finalize;       /* fixed-order partial sums -> image */
+    double ms_small;          /* small-footprint scatter */
+    int64_t n_gather_launches;
 } s2g_stats;
 
 /* Context: owns the device buffers (grown monotonically) and one stream on

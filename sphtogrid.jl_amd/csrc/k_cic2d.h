@@ -58,6 +58,7 @@ void launch_2d_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs
                      int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null,
                      int64_t *tile_items, bool fill, hipStream_t s);
 void launch_2d_gather(const GatherArgs &g, hipStream_t s);
+void launch_2d_finalize(const GatherArgs &g, hipStream_t s);
 void launch_2d_small(const SmallArgs &a, hipStream_t s);
 void launch_2d_reduce(const double *image, int64_t npix, int nq, int reduce, double *out,
                       hipStream_t s);

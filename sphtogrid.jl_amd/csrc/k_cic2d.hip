@@ -191,7 +191,9 @@ __global__ __launch_bounds__(kPartBlock) void k2d_part_scatter(
     for (int c = 0; c < 3; ++c) {
         if (!f[c])
             continue;
-        int64_t pos = offs[(int64_t)c * gridDim.x + blockIdx.x];
+        // the scan runs over [class][block]: subtract the class base so the
+        // position is local to list c (each list holds at most n entries)
+        int64_t pos = offs[(int64_t)c * gridDim.x + blockIdx.x] - offs[(int64_t)c * gridDim.x];
         for (int k = 0; k < wv; ++k)
             pos += s[c][k];
         pos += __popcll(b[c] & below);
@@ -691,8 +693,6 @@ static void gather_knp(const GatherArgs &g, hipStream_t s)
     hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
                        g.items, g.pairs, g.large, g.rec, (const T *)g.binq, g.nq, g.plane0,
                        g.ntj, g.partial);
-    hipLaunchKernelGGL(k2d_finalize<NP>, dim3((unsigned)g.ntiles), dim3(256), 0, s, g.partial,
-                       g.tile_items, g.ntj, g.npix, g.plane0, g.image);
 }
 
 template <typename T, int K>
@@ -725,6 +725,17 @@ void launch_2d_gather(const GatherArgs &g, hipStream_t s)
         gather_t<float>(g, s);
     else
         gather_t<double>(g, s);
+}
+
+void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
+{
+    const dim3 grid((unsigned)g.ntiles), block(256);
+    switch (g.np) {
+    case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
+    case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
+    case 3: hipLaunchKernelGGL(k2d_finalize<3>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
+    default: hipLaunchKernelGGL(k2d_finalize<4>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
+    }
 }
 
 template <typename T, int K>

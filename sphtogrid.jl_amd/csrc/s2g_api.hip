@@ -14,6 +14,8 @@ struct s2g_ctx {
     int ncu = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int kNumStageEvents = 80;
+    hipEvent_t stage[kNumStageEvents] = {};
     std::string err;
     // staged host inputs
     DevBuf in[7];
@@ -91,6 +93,13 @@ void read_i64(s2g_ctx *c, hipStream_t s, std::initializer_list<const int64_t *> 
         dst[i] = c->h_small[i];
 }
 
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    S2G_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
 // ---------------------------------------------------------------------------
 // 2D deposit on device pointers
 // ---------------------------------------------------------------------------
@@ -104,23 +113,24 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
         std::memset(st, 0, sizeof(*st));
         st->n_particles = n;
     }
-    S2G_HIP(hipEventRecord(c->ev0, s));
+    hipEvent_t *E = c->stage;
+    S2G_HIP(hipEventRecord(E[0], s));
     if (n == 0) {
         S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
-        S2G_HIP(hipEventRecord(c->ev1, s));
-        S2G_HIP(hipEventSynchronize(c->ev1));
+        S2G_HIP(hipEventRecord(E[1], s));
+        S2G_HIP(hipEventSynchronize(E[1]));
         return;
     }
     Rec2 *rec = (Rec2 *)c->rec.get(sizeof(Rec2) * n);
-    launch_2d_setup(a, rec, s);
-
     const int64_t nb = partition_blocks(n);
     int64_t *counts = (int64_t *)c->counts.get(sizeof(int64_t) * (3 * nb + 1));
     int32_t *l0 = (int32_t *)c->lists[0].get(sizeof(int32_t) * n);
     int32_t *l1 = (int32_t *)c->lists[1].get(sizeof(int32_t) * n);
     int32_t *l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
     int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
+    launch_2d_setup(a, rec, s);
     launch_2d_partition(rec, n, counts, tmp, l0, l1, l2, s);
+    S2G_HIP(hipEventRecord(E[1], s));
     int64_t o[4];
     read_i64(c, s, {counts, counts + nb, counts + 2 * nb, counts + 3 * nb}, o);
     const int64_t ns = o[1] - o[0], nl = o[2] - o[1], ne = o[3] - o[2];
@@ -131,10 +141,14 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
         st->n_skipped = n - ns - nl;
     }
 
+    S2G_HIP(hipEventRecord(E[2], s));
     launch_2d_norm(a.kernel, l2, ne, rec, s);
+    S2G_HIP(hipEventRecord(E[3], s));
 
     const int ntj = (int)((npix + kGT - 1) / kGT);
     const int ntiles = ntj * ntj;
+    double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;
+    int64_t launches = 0;
     if (nl > 0) {
         if (ntiles > kMaxTiles2D)
             throw Error(S2G_EUNSUPPORTED, "image too large for the tile gather (npix > 8192)");
@@ -142,30 +156,36 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
         const int64_t ncnt = (int64_t)ntiles * nchunks + 1;
         int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
         tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
+        S2G_HIP(hipEventRecord(E[4], s));
         launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp, nullptr, false, s);
+        S2G_HIP(hipEventRecord(E[5], s));
         int64_t npairs;
         read_i64(c, s, {cnt + ncnt - 1}, &npairs);
         if (npairs >= (int64_t)INT32_MAX)
             throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
         int32_t *pairs = (int32_t *)c->pairs.get(sizeof(int32_t) * (npairs ? npairs : 1));
-        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp, pairs, true, s);
-
         // work items of <= G pairs: enough items to fill every CU several times
         const int64_t target = (int64_t)c->ncu * 16;
         int64_t G = (npairs + target - 1) / target;
         if (G < 64)
             G = 64;
         int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
-        tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ntiles + 1));
-        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp, nullptr, nullptr, false, s);
+        int64_t *tmp2 = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
+        S2G_HIP(hipEventRecord(E[6], s));
+        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp2, pairs, true, s);
+        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp2, nullptr, nullptr, false, s);
+        S2G_HIP(hipEventRecord(E[7], s));
         int64_t nitems;
         read_i64(c, s, {nit + ntiles}, &nitems);
         int4 *items = (int4 *)c->items.get(sizeof(int4) * (nitems ? nitems : 1));
         int64_t *tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
-        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp, items, tile_items, true, s);
+        S2G_HIP(hipEventRecord(E[8], s));
+        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp2, items, tile_items, true, s);
+        S2G_HIP(hipEventRecord(E[9], s));
         if (st)
             st->n_pairs = npairs;
 
+        int ev = 10;
         for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
             const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
             GatherArgs g;
@@ -187,13 +207,26 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
                                                  kGT * kGT);
             g.tile_items = tile_items;
             g.image = d_image;
+            const bool timed = ev + 2 < s2g_ctx::kNumStageEvents;
+            if (timed)
+                S2G_HIP(hipEventRecord(E[ev], s));
             if (nitems > 0) {
                 launch_2d_gather(g, s);
-            } else {
-                for (int q = 0; q < np; ++q)
-                    S2G_HIP(hipMemsetAsync(d_image + (int64_t)(plane0 + q) * nd, 0,
-                                           sizeof(double) * nd, s));
+                ++launches;
             }
+            if (timed)
+                S2G_HIP(hipEventRecord(E[ev + 1], s));
+            launch_2d_finalize(g, s);
+            if (timed)
+                S2G_HIP(hipEventRecord(E[ev + 2], s));
+            if (timed)
+                ev += 3;
+        }
+        S2G_HIP(hipEventSynchronize(E[ev - 1]));
+        ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]) + elapsed(E[8], E[9]);
+        for (int k = 10; k + 2 < ev + 1 && k < ev; k += 3) {
+            ms_gather += elapsed(E[k], E[k + 1]);
+            ms_fin += elapsed(E[k + 1], E[k + 2]);
         }
     } else {
         S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
@@ -209,14 +242,21 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
     sa.rec = rec;
     sa.binq = a.binq;
     sa.image = d_image;
+    const int eS = s2g_ctx::kNumStageEvents - 3;
+    S2G_HIP(hipEventRecord(E[eS], s));
     launch_2d_small(sa, s);
     S2G_HIP(hipGetLastError());
-    S2G_HIP(hipEventRecord(c->ev1, s));
-    S2G_HIP(hipEventSynchronize(c->ev1));
+    S2G_HIP(hipEventRecord(E[eS + 1], s));
+    S2G_HIP(hipEventSynchronize(E[eS + 1]));
     if (st) {
-        float ms = 0.f;
-        S2G_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        st->ms_device = ms;
+        st->ms_device = elapsed(E[0], E[eS + 1]);
+        st->ms_setup = elapsed(E[0], E[1]);
+        st->ms_norm = elapsed(E[2], E[3]);
+        st->ms_sort = ms_sort;
+        st->ms_gather = ms_gather;
+        st->ms_finalize = ms_fin;
+        st->ms_small = elapsed(E[eS], E[eS + 1]);
+        st->n_gather_launches = launches;
     }
 }
 
@@ -342,6 +382,13 @@ int s2g_create(s2g_ctx **out, int device)
             s2g_destroy(c);
             return S2G_EHIP;
         }
+        bool ok = true;
+        for (auto &e : c->stage)
+            ok = ok && hipEventCreate(&e) == hipSuccess;
+        if (!ok) {
+            s2g_destroy(c);
+            return S2G_EHIP;
+        }
     } catch (...) {
         delete c;
         return S2G_ENOMEM;
@@ -376,6 +423,9 @@ void s2g_destroy(s2g_ctx *c)
         (void)hipEventDestroy(c->ev0);
     if (c->ev1)
         (void)hipEventDestroy(c->ev1);
+    for (auto &e : c->stage)
+        if (e)
+            (void)hipEventDestroy(e);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

@@ -53,7 +53,10 @@ class Stats(C.Structure):
                 ("n_particles", C.c_int64), ("n_skipped", C.c_int64),
                 ("n_small", C.c_int64), ("n_large", C.c_int64),
                 ("n_large_exact", C.c_int64), ("n_pairs", C.c_int64),
-                ("sum_footprint", C.c_double)]
+                ("sum_footprint", C.c_double), ("ms_setup", C.c_double),
+                ("ms_norm", C.c_double), ("ms_sort", C.c_double), ("ms_gather", C.c_double),
+                ("ms_finalize", C.c_double), ("ms_small", C.c_double),
+                ("n_gather_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -68,7 +71,7 @@ class MapDesc(C.Structure):
 
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()  # default_context() -> Context() -> load() re-enters
 
 
 def build_library(jobs: int = 8) -> str:

@@ -140,7 +140,8 @@ def test_return_both_maps_and_reduce(s2g, orc):
                                calc_mean=True)
     _check_flat(flat, ref_flat, "sedov flat")
     # reducing the GPU flat image on the CPU oracle gives the GPU reduced map
-    np.testing.assert_array_equal(orc.reduce_image_2D(flat, 128, True), red)
+    # (two separate deposits: the atomic scatter order differs at the ulp level)
+    np.testing.assert_allclose(orc.reduce_image_2D(flat, 128, True), red, rtol=1e-12)
 
 
 def test_reduce_entry_point_bitexact(s2g, orc):

@@ -64,3 +64,29 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp", ".jl")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in txt and "sph2grid_oracle" not in txt, f
+
+
+def test_default_context_does_not_deadlock(s2g):
+    """default_context() creates the context while holding the module lock;
+    the nested load() must re-enter it (regression: a non-reentrant lock
+    blocked forever here)."""
+    import threading
+
+    import torch
+    from sph2grid import _lib
+    box = {}
+
+    def run():
+        try:
+            box["ctx"] = _lib.default_context()
+        except _lib.LibraryError as e:
+            box["err"] = e
+
+    t = threading.Thread(target=run, daemon=True)
+    t.start()
+    t.join(timeout=60)
+    assert not t.is_alive(), "default_context() deadlocked"
+    if torch.cuda.is_available():
+        assert "ctx" in box
+    else:
+        assert "err" in box
