@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--exact-norm", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         msg = ("bench.py runs on one GPU only: the deposit is measured on a single MI355X "
@@ -195,6 +197,13 @@ def main():
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / args.steps
+    if args.calib_copy:
+        # a plain copy of known size: reads 1 GiB and writes 1 GiB
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for k in stage:
         stage[k] /= args.steps
     value = n / (ms_step / 1e3)

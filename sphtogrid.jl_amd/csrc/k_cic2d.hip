@@ -221,50 +221,47 @@ __device__ __forceinline__ float pix_factor(float xd2, float yd2, float hinv, fl
 // deterministic FP64 sum (fixed per-thread order + fixed tree).
 // ---------------------------------------------------------------------------
 constexpr int kNormThreads = 256;
-constexpr int kNormColsPerLane = 16;  // 1024 columns per chunk
 
+// One wave per particle.  Rows are taken 64 at a time: lane l computes the
+// row terms of row i0+64c+l once, and the row loop broadcasts them with
+// v_readlane; lanes run over the columns.  Per-lane FP64 sums in a fixed
+// order and a fixed butterfly make the result bit-reproducible.
 template <int K>
 __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
-                                                         Rec2 *__restrict__ rec)
+                                                         int64_t ne, Rec2 *__restrict__ rec)
 {
-    __shared__ double red[kNormThreads / kWave];
-    const int p = list[blockIdx.x];
-    Rec2 r = rec[p];
+    const int64_t gw = ((int64_t)blockIdx.x * kNormThreads + threadIdx.x) / kWave;
+    const int ln = threadIdx.x % kWave;
+    if (gw >= ne)
+        return;
+    const int p = list[gw];
+    const Rec2 r = rec[p];
     const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     double acc = 0.0;
-    for (int jc = r.j0; jc <= r.j1; jc += kWave * kNormColsPerLane) {
-        float yd2[kNormColsPerLane], ay[kNormColsPerLane];
-#pragma unroll
-        for (int c = 0; c < kNormColsPerLane; ++c) {
-            int j = jc + ln + c * kWave;
-            double fj = (double)j;
-            float yd = (float)coff1(r.y, fj);
-            yd2[c] = yd * yd;
-            ay[c] = (j <= r.j1) ? (float)overlap1(r.y, r.h, fj) : 0.0f;
-        }
-        for (int i = r.i0 + wv; i <= r.i1; i += kNormThreads / kWave) {
-            double fi = (double)i;
-            float xd = (float)coff1(r.x, fi);
-            float xd2 = xd * xd;
-            float ax = (float)overlap1(r.x, r.h, fi);
-            double rowacc = 0.0;
-#pragma unroll
-            for (int c = 0; c < kNormColsPerLane; ++c)
-                rowacc += (double)pix_factor<K>(xd2, yd2[c], hinv, ax, ay[c]);
-            acc += rowacc;
+    for (int ic = r.i0; ic <= r.i1; ic += kWave) {
+        const double fi = (double)(ic + ln);
+        const float xd = (float)coff1(r.x, fi);
+        const float xd2 = xd * xd;
+        const float ax = fmaxf((float)overlap1(r.x, r.h, fi), 0.0f);
+        const int nrow = r.i1 - ic + 1 < kWave ? r.i1 - ic + 1 : kWave;
+        for (int jc = r.j0; jc <= r.j1; jc += kWave) {
+            const int j = jc + ln;
+            const double fj = (double)j;
+            const float yd = (float)coff1(r.y, fj);
+            const float yd2 = yd * yd;
+            const float ay = (j <= r.j1) ? fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) : 0.0f;
+            for (int rr = 0; rr < nrow; ++rr) {
+                const float xd2r =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), rr));
+                const float axr =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), rr));
+                acc += (double)pix_factor<K>(xd2r, yd2, hinv, axr, ay);
+            }
         }
     }
     acc = wave_sum(acc);
     if (ln == 0)
-        red[wv] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sum = 0.0;
-        for (int k = 0; k < kNormThreads / kWave; ++k)
-            sum += red[k];
-        rec[p].s = sum != 0.0 ? r.s / sum : 0.0;
-    }
+        rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -369,13 +366,17 @@ __global__ void k2d_item_fill(const int64_t *__restrict__ off, int ntiles, int n
 }
 
 // ---------------------------------------------------------------------------
-// Tile gather.  Block = 4 waves = one 64x64 tile; wave w owns rows
-// [16w, 16w+16) of the tile, lane = column.  Per particle the column terms
-// are per lane and the row terms are computed by lanes 0..15 and broadcast
-// with v_readlane (wave-uniform SGPR operands); per pixel the work is one
-// sqrt, the kernel polynomial and NP FP64 FMAs into registers.
+// Tile gather.  Block = 4 waves = one 64x64 tile.  Wave w owns the 16-column
+// strip [16w, 16w+16) of the tile; lane l works on column 16w + (l & 15) and
+// rows 4k + (l >> 4), k = 0..15, so each lane accumulates 16 pixels in FP64
+// registers.  Per batch of particles the row terms (x offset^2, x overlap) of
+// all 64 tile rows are tabulated once in LDS; per particle each lane computes
+// its column terms, and waves skip particles that miss their strip or row
+// groups (wave-uniform scalar tests).  Per pixel: one sqrt, the kernel
+// polynomial, one FP32 product chain and NP FP64 FMAs (the plane multiplier
+// already carries the particle's scale).
 // ---------------------------------------------------------------------------
-constexpr int kGBatch = 64;
+constexpr int kGBatch = 32;
 
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items,
@@ -386,7 +387,8 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
                                                   int plane0, int ntj,
                                                   double *__restrict__ partial)
 {
-    __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch], s_s[kGBatch];
+    __shared__ float2 s_row[kGBatch][kGT];
+    __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch];
     __shared__ float s_hinv[kGBatch];
     __shared__ int s_i0[kGBatch], s_i1[kGBatch], s_j0[kGBatch], s_j1[kGBatch];
     __shared__ double s_mult[NP][kGBatch];
@@ -395,17 +397,17 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
     const int tile = it.x;
     const int ti0 = (tile / ntj) * kGT, tj0 = (tile % ntj) * kGT;
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
-    const int rowbase = ti0 + wv * kGRows;
-    const int j = tj0 + ln;
+    const int rg = ln >> 4;
+    const int sj0 = tj0 + 16 * wv;  // first column of this wave's strip
+    const int j = sj0 + (ln & 15);
     const double fj = (double)j;
-    const double frow = (double)(rowbase + (ln & (kGRows - 1)));
 
     double acc[kGRows][NP];
 #pragma unroll
-    for (int r = 0; r < kGRows; ++r)
+    for (int k = 0; k < kGRows; ++k)
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
-            acc[r][k] = 0.0;
+        for (int q = 0; q < NP; ++q)
+            acc[k][q] = 0.0;
 
     for (int base = 0; base < it.z; base += kGBatch) {
         const int cnt = it.z - base < kGBatch ? it.z - base : kGBatch;
@@ -416,64 +418,72 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             s_x[threadIdx.x] = r.x;
             s_y[threadIdx.x] = r.y;
             s_h[threadIdx.x] = r.h;
-            s_s[threadIdx.x] = r.s;
             s_hinv[threadIdx.x] = r.fb ? 0.0f : (float)(1.0 / r.h);
             s_i0[threadIdx.x] = r.i0;
             s_i1[threadIdx.x] = r.i1;
             s_j0[threadIdx.x] = r.j0;
             s_j1[threadIdx.x] = r.j1;
 #pragma unroll
-            for (int k = 0; k < NP; ++k) {
-                const int P = plane0 + k;
-                s_mult[k][threadIdx.x] =
+            for (int q = 0; q < NP; ++q) {
+                const int P = plane0 + q;
+                const double mult =
                     P == nq ? 1.0 : ld(binq, (int64_t)p * nq + (P < nq ? P : 0));
+                s_mult[q][threadIdx.x] = mult * r.s;
             }
         }
         __syncthreads();
+        // row table of the batch: (x offset^2, x overlap) for the 64 tile rows
+        for (int e = threadIdx.x; e < cnt * kGT; e += 256) {
+            const int b = e / kGT, rr = e % kGT;
+            const double fi = (double)(ti0 + rr);
+            const float xd = (float)coff1(s_x[b], fi);
+            const float ax = fmaxf((float)overlap1(s_x[b], s_h[b], fi), 0.0f);
+            s_row[b][rr] = make_float2(xd * xd, ax);
+        }
+        __syncthreads();
         for (int b = 0; b < cnt; ++b) {
-            const int lo = s_i0[b] - rowbase > 0 ? s_i0[b] - rowbase : 0;
-            const int hi = s_i1[b] - rowbase < kGRows - 1 ? s_i1[b] - rowbase : kGRows - 1;
-            if (lo > hi)
+            const int j0 = __builtin_amdgcn_readfirstlane(s_j0[b]);
+            const int j1 = __builtin_amdgcn_readfirstlane(s_j1[b]);
+            if (sj0 + 15 < j0 || sj0 > j1)
                 continue;
-            const double x = s_x[b], y = s_y[b], h = s_h[b];
+            const int i0 = __builtin_amdgcn_readfirstlane(s_i0[b]) - ti0;
+            const int i1 = __builtin_amdgcn_readfirstlane(s_i1[b]) - ti0;
+            const int klo = i0 > 0 ? (i0 >> 2) : 0;
+            const int khi = (i1 >> 2) < kGRows - 1 ? (i1 >> 2) : kGRows - 1;
+            if (klo > khi)
+                continue;
+            const double y = s_y[b], h = s_h[b];
             const float hinv = s_hinv[b];
-            const double sc = s_s[b];
-            // column (lane) terms
             const float yd = (float)coff1(y, fj);
             const float yd2 = yd * yd;
-            const float ay =
-                (j >= s_j0[b] && j <= s_j1[b]) ? (float)overlap1(y, h, fj) : 0.0f;
-            // row terms, row = rowbase + (lane & 15)
-            const float xd = (float)coff1(x, frow);
-            const float xd2 = xd * xd;
-            const float ax = (float)overlap1(x, h, frow);
+            const float ay = fmaxf((float)overlap1(y, h, fj), 0.0f);
+            // the lane's column overlap is common to its 16 pixels: fold it
+            // into the FP64 plane multipliers
+            const double ayd = (double)ay;
             double mult[NP];
 #pragma unroll
-            for (int k = 0; k < NP; ++k)
-                mult[k] = s_mult[k][b];
+            for (int q = 0; q < NP; ++q)
+                mult[q] = s_mult[q][b] * ayd;
 #pragma unroll
-            for (int r = 0; r < kGRows; ++r) {
-                if (r < lo || r > hi)
+            for (int k = 0; k < kGRows; ++k) {
+                if (k < klo || k > khi)
                     continue;
-                const float xd2r =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), r));
-                const float axr =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), r));
-                const float f = pix_factor<K>(xd2r, yd2, hinv, axr, ay);
-                const double pw = (double)f * sc;
+                const float2 rvk = s_row[b][4 * k + rg];
+                const float u = __builtin_amdgcn_sqrtf(rvk.x + yd2) * hinv;
+                const double f = (double)(shape_of<K, float>(u) * rvk.y);
 #pragma unroll
-                for (int k = 0; k < NP; ++k)
-                    acc[r][k] = fma(mult[k], pw, acc[r][k]);
+                for (int q = 0; q < NP; ++q)
+                    acc[k][q] = fma(f, mult[q], acc[k][q]);
             }
         }
         __syncthreads();
     }
     double *dst = partial + (int64_t)blockIdx.x * NP * kGT * kGT;
 #pragma unroll
-    for (int k = 0; k < NP; ++k)
+    for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int r = 0; r < kGRows; ++r)
-            dst[(k * kGT + wv * kGRows + r) * kGT + ln] = acc[r][k];
+        for (int k = 0; k < kGRows; ++k)
+            dst[(q * kGT + 4 * k + rg) * kGT + 16 * wv + (ln & 15)] = acc[k][q];
 }
 
 // fixed-order sum of a tile's item partials -> image planes (plain stores)
@@ -640,7 +650,9 @@ int64_t partition_blocks(int64_t n) { return (n + kPartBlock - 1) / kPartBlock; 
 template <int K>
 static void launch_norm_k(const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
 {
-    hipLaunchKernelGGL(k2d_norm<K>, dim3((unsigned)ne), dim3(kNormThreads), 0, s, list, rec);
+    const int64_t blocks = (ne * kWave + kNormThreads - 1) / kNormThreads;
+    hipLaunchKernelGGL(k2d_norm<K>, dim3((unsigned)blocks), dim3(kNormThreads), 0, s, list, ne,
+                       rec);
 }
 
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)

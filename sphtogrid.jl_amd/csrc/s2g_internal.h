@@ -108,11 +108,11 @@ __host__ __device__ inline double closed_form_min_h(int kernel)
 {
     switch (kernel) {
     case S2G_KERNEL_CUBIC: return 64.0;
-    case S2G_KERNEL_QUINTIC: return 32.0;
+    case S2G_KERNEL_QUINTIC: return 24.0;
     case S2G_KERNEL_WENDLAND_C2: return 64.0;
-    case S2G_KERNEL_WENDLAND_C4: return 32.0;
-    case S2G_KERNEL_WENDLAND_C6: return 24.0;
-    default: return 20.0;  // WENDLAND_C8
+    case S2G_KERNEL_WENDLAND_C4: return 28.0;
+    case S2G_KERNEL_WENDLAND_C6: return 16.0;
+    default: return 12.0;  // WENDLAND_C8
     }
 }
 
