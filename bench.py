@@ -114,6 +114,22 @@ def cpu_baseline(p, target_s, threads):
             "sample": sample}
 
 
+def measured_traffic(args, n, npix):
+    """HBM bytes per launch of k2d_gather for the default cluster workload, as
+    measured by the two rocprofv3 --pmc passes summarised under profiles/
+    (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy).  None otherwise."""
+    if args.workload != "cluster" or n != 1 << 24 or npix != 4096 or args.exact_norm:
+        return None, None
+    path = os.path.join(ROOT, "profiles", "k2d_gather.cluster_2e24_4096.traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch k2d_gather:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,8 +246,10 @@ def main():
     achieved = fp_large * fk / gather_s / 1e12 if gather_s > 0 else 0.0
     s_in = p["x"].dtype.itemsize
     b_alg = n * (6 + nq) * s_in + nd * nq * 8
+    traffic, traffic_src = measured_traffic(args, n, npix)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "kernel": "k2d_gather", "compute_unit": "FP32 vector ALU (no MFMA: "
                 "scatter/gather); peak = MI355X FP32 vector rate",
                 "alg_flops_per_launch": fp_large * fk,
