@@ -14,7 +14,6 @@
 //   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
 //   k2d_small_*      scatter of small footprints (atomics on top of the image)
 //   k2d_reduce       reduce_image_2D: divide by the weight plane + transpose
-#include "s2g_internal.h"
 #include "k_cic2d.h"
 
 namespace s2g {
@@ -33,7 +32,8 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
                                                  const T *__restrict__ w, int64_t n,
                                                  int64_t npix, double len2pix,
                                                  int kernel, uint32_t flags,
-                                                 Rec2 *__restrict__ rec)
+                                                 Rec2 *__restrict__ rec,
+                                                 int32_t *__restrict__ cls)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n)
@@ -52,6 +52,7 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
             allzero = false;
     if (allzero && !(flags & S2G_FLAG_CALC_MEAN)) {
         rec[p] = r;
+        cls[p] = r.cls;
         return;
     }
 
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
     double ya = floor(sub_rn(y, h)), yb = floor(add_rn(y, h));
     if (!(xa == xa) || !(xb == xb) || !(ya == ya) || !(yb == yb) || !(h > 0.0)) {
         rec[p] = r;  // NaN input or h <= 0: nothing is deposited
+        cls[p] = r.cls;
         return;
     }
     xa = fmin(fmax(xa, -1.0), lim);
@@ -81,6 +83,7 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
     const int64_t j0 = b0 > 0 ? b0 : 0, j1 = b1 < npix - 1 ? b1 : npix - 1;
     if (i0 > i1 || j0 > j1) {
         rec[p] = r;
+        cls[p] = r.cls;
         return;
     }
 
@@ -126,79 +129,7 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
         }
     }
     rec[p] = r;
-}
-
-// ---------------------------------------------------------------------------
-// Stable partition into three particle-order lists.
-//   pass 1: per-block class counts          (k2d_part_count)
-//   scan  : exclusive over [class][block]
-//   pass 2: ballot ranks inside the block    (k2d_part_scatter)
-// list 0 = small, list 1 = large (An + Ex), list 2 = exact (Ex only)
-// ---------------------------------------------------------------------------
-constexpr int kPartBlock = 256;
-
-__device__ __forceinline__ void part_flags(int cls, int &f0, int &f1, int &f2)
-{
-    f0 = cls == kClsSmall;
-    f1 = cls == kClsLargeAn || cls == kClsLargeEx;
-    f2 = cls == kClsLargeEx;
-}
-
-__global__ __launch_bounds__(kPartBlock) void k2d_part_count(const Rec2 *__restrict__ rec,
-                                                             int64_t n,
-                                                             int64_t *__restrict__ counts)
-{
-    __shared__ int s[3][kPartBlock / kWave];
-    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
-    int cls = p < n ? rec[p].cls : kClsSkip;
-    int f0, f1, f2;
-    part_flags(cls, f0, f1, f2);
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
-    uint64_t b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
-    if (ln == 0) {
-        s[0][wv] = __popcll(b0);
-        s[1][wv] = __popcll(b1);
-        s[2][wv] = __popcll(b2);
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        int t = 0;
-        for (int k = 0; k < kPartBlock / kWave; ++k)
-            t += s[threadIdx.x][k];
-        counts[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 3)
-        counts[3 * (int64_t)gridDim.x] = 0;  // becomes the grand total after the scan
-}
-
-__global__ __launch_bounds__(kPartBlock) void k2d_part_scatter(
-    const Rec2 *__restrict__ rec, int64_t n, const int64_t *__restrict__ offs,
-    int32_t *__restrict__ l0, int32_t *__restrict__ l1, int32_t *__restrict__ l2)
-{
-    __shared__ int s[3][kPartBlock / kWave];
-    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
-    int cls = p < n ? rec[p].cls : kClsSkip;
-    int f[3];
-    part_flags(cls, f[0], f[1], f[2]);
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
-    const uint64_t below = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
-    uint64_t b[3] = {__ballot(f[0]), __ballot(f[1]), __ballot(f[2])};
-    if (ln == 0)
-        for (int c = 0; c < 3; ++c)
-            s[c][wv] = __popcll(b[c]);
-    __syncthreads();
-    int32_t *lists[3] = {l0, l1, l2};
-    for (int c = 0; c < 3; ++c) {
-        if (!f[c])
-            continue;
-        // the scan runs over [class][block]: subtract the class base so the
-        // position is local to list c (each list holds at most n entries)
-        int64_t pos = offs[(int64_t)c * gridDim.x + blockIdx.x] - offs[(int64_t)c * gridDim.x];
-        for (int k = 0; k < wv; ++k)
-            pos += s[c][k];
-        pos += __popcll(b[c] & below);
-        lists[c][pos] = (int32_t)p;
-    }
+    cls[p] = r.cls;
 }
 
 // ---------------------------------------------------------------------------
@@ -262,107 +193,6 @@ __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restri
     acc = wave_sum(acc);
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
-}
-
-// ---------------------------------------------------------------------------
-// (tile, particle) pair lists: a counting sort with deterministic order.
-// Chunk c holds large-list entries [c*CH, (c+1)*CH).  cnt is tile-major
-// [tile][chunk] so one exclusive scan yields every list offset.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void tile_rect(const Rec2 &r, int &ta0, int &ta1, int &tb0,
-                                          int &tb1)
-{
-    ta0 = r.i0 / kGT;
-    ta1 = r.i1 / kGT;
-    tb0 = r.j0 / kGT;
-    tb1 = r.j1 / kGT;
-}
-
-__global__ __launch_bounds__(256) void k2d_pair_count(const int32_t *__restrict__ large,
-                                                      int64_t nl, const Rec2 *__restrict__ rec,
-                                                      int ntj, int ntiles, int nchunks,
-                                                      int64_t *__restrict__ cnt)
-{
-    extern __shared__ int hist[];
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-        hist[t] = 0;
-    __syncthreads();
-    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
-    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
-    for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
-        Rec2 r = rec[large[l]];
-        int ta0, ta1, tb0, tb1;
-        tile_rect(r, ta0, ta1, tb0, tb1);
-        for (int a = ta0; a <= ta1; ++a)
-            for (int b = tb0; b <= tb1; ++b)
-                atomicAdd(&hist[a * ntj + b], 1);
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-        cnt[(int64_t)t * nchunks + blockIdx.x] = hist[t];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        cnt[(int64_t)ntiles * nchunks] = 0;  // becomes the pair total after the scan
-}
-
-// one wave per chunk: sequential over the chunk's particles, lanes over the
-// particle's tiles (distinct per lane), so every list is in particle order
-__global__ __launch_bounds__(kWave) void k2d_pair_fill(const int32_t *__restrict__ large,
-                                                       int64_t nl, const Rec2 *__restrict__ rec,
-                                                       int ntj, int ntiles, int nchunks,
-                                                       const int64_t *__restrict__ off,
-                                                       int32_t *__restrict__ pairs)
-{
-    extern __shared__ int cursor[];
-    for (int t = threadIdx.x; t < ntiles; t += kWave)
-        cursor[t] = (int)off[(int64_t)t * nchunks + blockIdx.x];
-    __syncthreads();
-    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
-    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
-    for (int64_t l = l0; l < l1; ++l) {
-        Rec2 r = rec[large[l]];
-        int ta0, ta1, tb0, tb1;
-        tile_rect(r, ta0, ta1, tb0, tb1);
-        const int wb = tb1 - tb0 + 1;
-        const int nt = (ta1 - ta0 + 1) * wb;
-        for (int k = threadIdx.x; k < nt; k += kWave) {
-            int a = ta0 + k / wb, b = tb0 + k % wb;
-            int slot = atomicAdd(&cursor[a * ntj + b], 1);
-            pairs[slot] = (int32_t)l;
-        }
-    }
-}
-
-// items: tile t's list [start, start+count) is split into ceil(count/G) items
-__global__ void k2d_item_count(const int64_t *__restrict__ off, int ntiles, int nchunks,
-                               int64_t npairs, int64_t G, int64_t *__restrict__ nitems)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == ntiles)
-        nitems[t] = 0;  // becomes the item total after the scan
-    if (t >= ntiles)
-        return;
-    int64_t s = off[(int64_t)t * nchunks];
-    int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
-    nitems[t] = (e - s + G - 1) / G;
-}
-
-__global__ void k2d_item_fill(const int64_t *__restrict__ off, int ntiles, int nchunks,
-                              int64_t npairs, int64_t G, const int64_t *__restrict__ ibase,
-                              int4 *__restrict__ items, int64_t *__restrict__ tile_items)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles)
-        return;
-    int64_t s = off[(int64_t)t * nchunks];
-    int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
-    int64_t b = ibase[t];
-    int64_t k = 0;
-    for (int64_t q = s; q < e; q += G, ++k) {
-        int64_t len = e - q < G ? e - q : G;
-        items[b + k] = make_int4(t, (int)q, (int)len, 0);
-    }
-    tile_items[2 * t] = b;
-    tile_items[2 * t + 1] = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -615,37 +445,23 @@ __global__ __launch_bounds__(256) void k2d_reduce(const double *__restrict__ ima
 // host-side launchers
 // ---------------------------------------------------------------------------
 template <typename T>
-static void launch_setup_t(const Map2DArgs &a, Rec2 *rec, hipStream_t s)
+static void launch_setup_t(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s)
 {
     if (a.n == 0)
         return;
     hipLaunchKernelGGL(k2d_setup<T>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s,
                        (const T *)a.pos, (const T *)a.hsml, (const T *)a.m,
                        (const T *)a.rho, (const T *)a.binq, a.nq, (const T *)a.w, a.n,
-                       a.npix, a.len2pix, a.kernel, a.flags, rec);
+                       a.npix, a.len2pix, a.kernel, a.flags, rec, cls);
 }
 
-void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, hipStream_t s)
+void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s)
 {
     if (a.dtype == S2G_F32)
-        launch_setup_t<float>(a, rec, s);
+        launch_setup_t<float>(a, rec, cls, s);
     else
-        launch_setup_t<double>(a, rec, s);
+        launch_setup_t<double>(a, rec, cls, s);
 }
-
-void launch_2d_partition(const Rec2 *rec, int64_t n, int64_t *counts, int64_t *scan_tmp,
-                         int32_t *l0, int32_t *l1, int32_t *l2, hipStream_t s)
-{
-    if (n == 0)
-        return;
-    const unsigned nb = (unsigned)((n + kPartBlock - 1) / kPartBlock);
-    hipLaunchKernelGGL(k2d_part_count, dim3(nb), dim3(kPartBlock), 0, s, rec, n, counts);
-    exclusive_scan_i64(counts, 3 * (int64_t)nb + 1, scan_tmp, s);
-    hipLaunchKernelGGL(k2d_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, rec, n, counts,
-                       l0, l1, l2);
-}
-
-int64_t partition_blocks(int64_t n) { return (n + kPartBlock - 1) / kPartBlock; }
 
 template <int K>
 static void launch_norm_k(const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
@@ -666,36 +482,6 @@ void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipS
     case S2G_KERNEL_WENDLAND_C4: launch_norm_k<S2G_KERNEL_WENDLAND_C4>(list, ne, rec, s); break;
     case S2G_KERNEL_WENDLAND_C6: launch_norm_k<S2G_KERNEL_WENDLAND_C6>(list, ne, rec, s); break;
     default: launch_norm_k<S2G_KERNEL_WENDLAND_C8>(list, ne, rec, s); break;
-    }
-}
-
-void launch_2d_pairs(const int32_t *large, int64_t nl, const Rec2 *rec, int ntj, int ntiles,
-                     int nchunks, int64_t *cnt, int64_t *scan_tmp, int32_t *pairs_or_null,
-                     bool fill, hipStream_t s)
-{
-    const size_t lds = sizeof(int) * (size_t)ntiles;
-    if (!fill) {
-        hipLaunchKernelGGL(k2d_pair_count, dim3((unsigned)nchunks), dim3(256), lds, s, large,
-                           nl, rec, ntj, ntiles, nchunks, cnt);
-        exclusive_scan_i64(cnt, (int64_t)ntiles * nchunks + 1, scan_tmp, s);
-    } else {
-        hipLaunchKernelGGL(k2d_pair_fill, dim3((unsigned)nchunks), dim3(kWave), lds, s, large,
-                           nl, rec, ntj, ntiles, nchunks, cnt, pairs_or_null);
-    }
-}
-
-void launch_2d_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs, int64_t G,
-                     int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null,
-                     int64_t *tile_items, bool fill, hipStream_t s)
-{
-    const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
-    if (!fill) {
-        hipLaunchKernelGGL(k2d_item_count, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks,
-                           npairs, G, nitems);
-        exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
-    } else {
-        hipLaunchKernelGGL(k2d_item_fill, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks,
-                           npairs, G, nitems, items_or_null, tile_items);
     }
 }
 

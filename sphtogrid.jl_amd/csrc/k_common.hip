@@ -1,0 +1,254 @@
+// k_common.hip -- dimension-independent stages of the deposits:
+//   * stable (particle-order) partition of the particle classes into lists
+//   * the (tile, particle) counting sort into per-tile lists
+//   * splitting tile lists into work items
+// All offsets come from deterministic exclusive scans, so the lists are
+// bit-reproducible.
+#include "k_common.h"
+
+namespace s2g {
+
+// ---------------------------------------------------------------------------
+// Stable partition into three particle-order lists from a class array.
+//   pass 1: per-block class counts          (k_part_count)
+//   scan  : exclusive over [class][block] (+ a sentinel: the grand total)
+//   pass 2: ballot ranks inside the block    (k_part_scatter)
+// list 0 = small, list 1 = large (An + Ex), list 2 = exact (Ex only)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void part_flags(int cls, int &f0, int &f1, int &f2)
+{
+    f0 = cls == kClsSmall;
+    f1 = cls == kClsLargeAn || cls == kClsLargeEx;
+    f2 = cls == kClsLargeEx;
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_part_count(const int32_t *__restrict__ cls,
+                                                           int64_t n,
+                                                           int64_t *__restrict__ counts)
+{
+    __shared__ int s[3][kPartBlock / kWave];
+    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
+    const int c = p < n ? cls[p] : kClsSkip;
+    int f0, f1, f2;
+    part_flags(c, f0, f1, f2);
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const uint64_t b0 = __ballot(f0), b1 = __ballot(f1), b2 = __ballot(f2);
+    if (ln == 0) {
+        s[0][wv] = __popcll(b0);
+        s[1][wv] = __popcll(b1);
+        s[2][wv] = __popcll(b2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int t = 0;
+        for (int k = 0; k < kPartBlock / kWave; ++k)
+            t += s[threadIdx.x][k];
+        counts[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 3)
+        counts[3 * (int64_t)gridDim.x] = 0;  // becomes the grand total after the scan
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_part_scatter(
+    const int32_t *__restrict__ cls, int64_t n, const int64_t *__restrict__ offs,
+    int32_t *__restrict__ l0, int32_t *__restrict__ l1, int32_t *__restrict__ l2)
+{
+    __shared__ int s[3][kPartBlock / kWave];
+    const int64_t p = (int64_t)blockIdx.x * kPartBlock + threadIdx.x;
+    const int c = p < n ? cls[p] : kClsSkip;
+    int f[3];
+    part_flags(c, f[0], f[1], f[2]);
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const uint64_t below = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+    const uint64_t b[3] = {__ballot(f[0]), __ballot(f[1]), __ballot(f[2])};
+    if (ln == 0)
+        for (int k = 0; k < 3; ++k)
+            s[k][wv] = __popcll(b[k]);
+    __syncthreads();
+    int32_t *lists[3] = {l0, l1, l2};
+    for (int k = 0; k < 3; ++k) {
+        if (!f[k])
+            continue;
+        // the scan runs over [class][block]: subtract the class base so the
+        // position is local to list k (each list holds at most n entries)
+        int64_t pos = offs[(int64_t)k * gridDim.x + blockIdx.x] - offs[(int64_t)k * gridDim.x];
+        for (int w = 0; w < wv; ++w)
+            pos += s[k][w];
+        pos += __popcll(b[k] & below);
+        lists[k][pos] = (int32_t)p;
+    }
+}
+
+int64_t partition_blocks(int64_t n) { return (n + kPartBlock - 1) / kPartBlock; }
+
+void launch_partition(const int32_t *cls, int64_t n, int64_t *counts, int64_t *scan_tmp,
+                      int32_t *l0, int32_t *l1, int32_t *l2, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const unsigned nb = (unsigned)partition_blocks(n);
+    hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(kPartBlock), 0, s, cls, n, counts);
+    exclusive_scan_i64(counts, 3 * (int64_t)nb + 1, scan_tmp, s);
+    hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(kPartBlock), 0, s, cls, n, counts, l0,
+                       l1, l2);
+}
+
+// ---------------------------------------------------------------------------
+// (tile, particle) pair lists: a counting sort with deterministic order.
+// Chunk c holds large-list entries [c*CH, (c+1)*CH).  cnt is tile-major
+// [tile][chunk] so one exclusive scan yields every list offset.  Tiles are
+// numbered t = (a * nt + b) * ntc + c over the tile grid (ntc = 1 in 2D).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tile_rect(const Rec2 &r, int &a0, int &a1, int &b0, int &b1,
+                                          int &c0, int &c1)
+{
+    a0 = r.i0 / kGT;
+    a1 = r.i1 / kGT;
+    b0 = r.j0 / kGT;
+    b1 = r.j1 / kGT;
+    c0 = c1 = 0;
+}
+
+__device__ __forceinline__ void tile_rect(const Rec3 &r, int &a0, int &a1, int &b0, int &b1,
+                                          int &c0, int &c1)
+{
+    a0 = r.i0 / kBT;
+    a1 = r.i1 / kBT;
+    b0 = r.j0 / kBT;
+    b1 = r.j1 / kBT;
+    c0 = r.k0 / kBT;
+    c1 = r.k1 / kBT;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ large,
+                                                    int64_t nl, const R *__restrict__ rec,
+                                                    int nt, int ntc, int ntiles, int nchunks,
+                                                    int64_t *__restrict__ cnt)
+{
+    extern __shared__ int hist[];
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        hist[t] = 0;
+    __syncthreads();
+    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
+    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
+    for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+        const R r = rec[large[l]];
+        int a0, a1, b0, b1, c0, c1;
+        tile_rect(r, a0, a1, b0, b1, c0, c1);
+        for (int a = a0; a <= a1; ++a)
+            for (int b = b0; b <= b1; ++b)
+                for (int c = c0; c <= c1; ++c)
+                    atomicAdd(&hist[(a * nt + b) * ntc + c], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+        cnt[(int64_t)t * nchunks + blockIdx.x] = hist[t];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt[(int64_t)ntiles * nchunks] = 0;  // becomes the pair total after the scan
+}
+
+// one wave per chunk: sequential over the chunk's particles, lanes over the
+// particle's tiles (distinct per lane), so every list is in particle order
+template <typename R>
+__global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__ large,
+                                                     int64_t nl, const R *__restrict__ rec,
+                                                     int nt, int ntc, int ntiles, int nchunks,
+                                                     const int64_t *__restrict__ off,
+                                                     int32_t *__restrict__ pairs)
+{
+    extern __shared__ int cursor[];
+    for (int t = threadIdx.x; t < ntiles; t += kWave)
+        cursor[t] = (int)off[(int64_t)t * nchunks + blockIdx.x];
+    __syncthreads();
+    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
+    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
+    for (int64_t l = l0; l < l1; ++l) {
+        const R r = rec[large[l]];
+        int a0, a1, b0, b1, c0, c1;
+        tile_rect(r, a0, a1, b0, b1, c0, c1);
+        const int wb = b1 - b0 + 1, wc = c1 - c0 + 1;
+        const int ntr = (a1 - a0 + 1) * wb * wc;
+        for (int k = threadIdx.x; k < ntr; k += kWave) {
+            const int a = a0 + k / (wb * wc);
+            const int rem = k % (wb * wc);
+            const int b = b0 + rem / wc, c = c0 + rem % wc;
+            const int slot = atomicAdd(&cursor[(a * nt + b) * ntc + c], 1);
+            pairs[slot] = (int32_t)l;
+        }
+    }
+}
+
+template <typename R>
+void launch_pairs(const int32_t *large, int64_t nl, const R *rec, int nt, int ntc, int ntiles,
+                  int nchunks, int64_t *cnt, int64_t *scan_tmp, int32_t *pairs_or_null,
+                  bool fill, hipStream_t s)
+{
+    const size_t lds = sizeof(int) * (size_t)ntiles;
+    if (!fill) {
+        hipLaunchKernelGGL(k_pair_count<R>, dim3((unsigned)nchunks), dim3(256), lds, s, large,
+                           nl, rec, nt, ntc, ntiles, nchunks, cnt);
+        exclusive_scan_i64(cnt, (int64_t)ntiles * nchunks + 1, scan_tmp, s);
+    } else {
+        hipLaunchKernelGGL(k_pair_fill<R>, dim3((unsigned)nchunks), dim3(kWave), lds, s, large,
+                           nl, rec, nt, ntc, ntiles, nchunks, cnt, pairs_or_null);
+    }
+}
+
+template void launch_pairs<Rec2>(const int32_t *, int64_t, const Rec2 *, int, int, int, int,
+                                 int64_t *, int64_t *, int32_t *, bool, hipStream_t);
+template void launch_pairs<Rec3>(const int32_t *, int64_t, const Rec3 *, int, int, int, int,
+                                 int64_t *, int64_t *, int32_t *, bool, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Work items: tile t's list [start, start+count) is split into ceil(count/G)
+// items (tile, first pair, count).
+// ---------------------------------------------------------------------------
+__global__ void k_item_count(const int64_t *__restrict__ off, int ntiles, int nchunks,
+                             int64_t npairs, int64_t G, int64_t *__restrict__ nitems)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == ntiles)
+        nitems[t] = 0;  // becomes the item total after the scan
+    if (t >= ntiles)
+        return;
+    const int64_t s = off[(int64_t)t * nchunks];
+    const int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    nitems[t] = (e - s + G - 1) / G;
+}
+
+__global__ void k_item_fill(const int64_t *__restrict__ off, int ntiles, int nchunks,
+                            int64_t npairs, int64_t G, const int64_t *__restrict__ ibase,
+                            int4 *__restrict__ items, int64_t *__restrict__ tile_items)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    const int64_t s = off[(int64_t)t * nchunks];
+    const int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    const int64_t b = ibase[t];
+    int64_t k = 0;
+    for (int64_t q = s; q < e; q += G, ++k) {
+        const int64_t len = e - q < G ? e - q : G;
+        items[b + k] = make_int4(t, (int)q, (int)len, 0);
+    }
+    tile_items[2 * t] = b;
+    tile_items[2 * t + 1] = k;
+}
+
+void launch_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs, int64_t G,
+                  int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items,
+                  bool fill, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
+    if (!fill) {
+        hipLaunchKernelGGL(k_item_count, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks, npairs,
+                           G, nitems);
+        exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
+    } else {
+        hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks, npairs,
+                           G, nitems, items_or_null, tile_items);
+    }
+}
+
+}  // namespace s2g

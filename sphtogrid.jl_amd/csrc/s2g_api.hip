@@ -4,7 +4,7 @@
 #include <cstring>
 #include <string>
 
-#include "k_cic2d.h"
+#include "k_cic3d.h"
 #include "s2g_internal.h"
 
 using namespace s2g;
@@ -20,7 +20,7 @@ struct s2g_ctx {
     // staged host inputs
     DevBuf in[7];
     // 2D pipeline
-    DevBuf rec, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
+    DevBuf rec, cls, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
     // outputs
     DevBuf image, out;
     // pinned host scratch for small device->host reads
@@ -101,6 +101,92 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 }
 
 // ---------------------------------------------------------------------------
+// Shared large-particle stage: per-tile pair lists and work items.
+// ---------------------------------------------------------------------------
+struct TilePlan {
+    int64_t npairs = 0, nitems = 0;
+    int32_t *pairs = nullptr;
+    int4 *items = nullptr;
+    int64_t *tile_items = nullptr;
+};
+
+template <typename R>
+TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, int64_t nl, const R *rec, int nt,
+                          int ntc, int ntiles, hipStream_t s, double *ms_sort)
+{
+    hipEvent_t *E = c->stage;
+    TilePlan tp;
+    if (ntiles > kMaxTiles)
+        throw Error(S2G_EUNSUPPORTED, "grid too large for the tile lists (" +
+                                          std::to_string(ntiles) + " tiles > " +
+                                          std::to_string(kMaxTiles) + ")");
+    const int nchunks = (int)((nl + kPairChunk - 1) / kPairChunk);
+    const int64_t ncnt = (int64_t)ntiles * nchunks + 1;
+    int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
+    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
+    S2G_HIP(hipEventRecord(E[4], s));
+    launch_pairs<R>(large, nl, rec, nt, ntc, ntiles, nchunks, cnt, tmp, nullptr, false, s);
+    S2G_HIP(hipEventRecord(E[5], s));
+    read_i64(c, s, {cnt + ncnt - 1}, &tp.npairs);
+    if (tp.npairs >= (int64_t)INT32_MAX)
+        throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
+    tp.pairs = (int32_t *)c->pairs.get(sizeof(int32_t) * (tp.npairs ? tp.npairs : 1));
+    // work items of <= G pairs: enough items to fill every CU several times
+    const int64_t target = (int64_t)c->ncu * 16;
+    int64_t G = (tp.npairs + target - 1) / target;
+    if (G < 64)
+        G = 64;
+    int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
+    S2G_HIP(hipEventRecord(E[6], s));
+    launch_pairs<R>(large, nl, rec, nt, ntc, ntiles, nchunks, cnt, tmp, tp.pairs, true, s);
+    launch_items(cnt, ntiles, nchunks, tp.npairs, G, nit, tmp, nullptr, nullptr, false, s);
+    S2G_HIP(hipEventRecord(E[7], s));
+    read_i64(c, s, {nit + ntiles}, &tp.nitems);
+    tp.items = (int4 *)c->items.get(sizeof(int4) * (tp.nitems ? tp.nitems : 1));
+    tp.tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
+    S2G_HIP(hipEventRecord(E[8], s));
+    launch_items(cnt, ntiles, nchunks, tp.npairs, G, nit, tmp, tp.items, tp.tile_items, true, s);
+    S2G_HIP(hipEventRecord(E[9], s));
+    S2G_HIP(hipEventSynchronize(E[9]));
+    *ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]) + elapsed(E[8], E[9]);
+    return tp;
+}
+
+struct ClassLists {
+    int64_t ns = 0, nl = 0, ne = 0;
+    int32_t *l0 = nullptr, *l1 = nullptr, *l2 = nullptr;
+};
+
+ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, hipStream_t s)
+{
+    ClassLists cl;
+    const int64_t nb = partition_blocks(n);
+    int64_t *counts = (int64_t *)c->counts.get(sizeof(int64_t) * (3 * nb + 1));
+    cl.l0 = (int32_t *)c->lists[0].get(sizeof(int32_t) * n);
+    cl.l1 = (int32_t *)c->lists[1].get(sizeof(int32_t) * n);
+    cl.l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
+    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
+    launch_partition(cls, n, counts, tmp, cl.l0, cl.l1, cl.l2, s);
+    S2G_HIP(hipEventRecord(c->stage[1], s));
+    int64_t o[4];
+    read_i64(c, s, {counts, counts + nb, counts + 2 * nb, counts + 3 * nb}, o);
+    cl.ns = o[1] - o[0];
+    cl.nl = o[2] - o[1];
+    cl.ne = o[3] - o[2];
+    return cl;
+}
+
+void fill_class_stats(s2g_stats *st, int64_t n, const ClassLists &cl)
+{
+    if (!st)
+        return;
+    st->n_small = cl.ns;
+    st->n_large = cl.nl;
+    st->n_large_exact = cl.ne;
+    st->n_skipped = n - cl.ns - cl.nl;
+}
+
+// ---------------------------------------------------------------------------
 // 2D deposit on device pointers
 // ---------------------------------------------------------------------------
 void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream_t s,
@@ -122,69 +208,24 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
         return;
     }
     Rec2 *rec = (Rec2 *)c->rec.get(sizeof(Rec2) * n);
-    const int64_t nb = partition_blocks(n);
-    int64_t *counts = (int64_t *)c->counts.get(sizeof(int64_t) * (3 * nb + 1));
-    int32_t *l0 = (int32_t *)c->lists[0].get(sizeof(int32_t) * n);
-    int32_t *l1 = (int32_t *)c->lists[1].get(sizeof(int32_t) * n);
-    int32_t *l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
-    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
-    launch_2d_setup(a, rec, s);
-    launch_2d_partition(rec, n, counts, tmp, l0, l1, l2, s);
-    S2G_HIP(hipEventRecord(E[1], s));
-    int64_t o[4];
-    read_i64(c, s, {counts, counts + nb, counts + 2 * nb, counts + 3 * nb}, o);
-    const int64_t ns = o[1] - o[0], nl = o[2] - o[1], ne = o[3] - o[2];
-    if (st) {
-        st->n_small = ns;
-        st->n_large = nl;
-        st->n_large_exact = ne;
-        st->n_skipped = n - ns - nl;
-    }
+    int32_t *cls = (int32_t *)c->cls.get(sizeof(int32_t) * n);
+    launch_2d_setup(a, rec, cls, s);
+    const ClassLists cl = partition_classes(c, cls, n, s);
+    fill_class_stats(st, n, cl);
 
     S2G_HIP(hipEventRecord(E[2], s));
-    launch_2d_norm(a.kernel, l2, ne, rec, s);
+    launch_2d_norm(a.kernel, cl.l2, cl.ne, rec, s);
     S2G_HIP(hipEventRecord(E[3], s));
 
     const int ntj = (int)((npix + kGT - 1) / kGT);
     const int ntiles = ntj * ntj;
     double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;
     int64_t launches = 0;
-    if (nl > 0) {
-        if (ntiles > kMaxTiles2D)
-            throw Error(S2G_EUNSUPPORTED, "image too large for the tile gather (npix > 8192)");
-        const int nchunks = (int)((nl + kPairChunk - 1) / kPairChunk);
-        const int64_t ncnt = (int64_t)ntiles * nchunks + 1;
-        int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
-        tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
-        S2G_HIP(hipEventRecord(E[4], s));
-        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp, nullptr, false, s);
-        S2G_HIP(hipEventRecord(E[5], s));
-        int64_t npairs;
-        read_i64(c, s, {cnt + ncnt - 1}, &npairs);
-        if (npairs >= (int64_t)INT32_MAX)
-            throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
-        int32_t *pairs = (int32_t *)c->pairs.get(sizeof(int32_t) * (npairs ? npairs : 1));
-        // work items of <= G pairs: enough items to fill every CU several times
-        const int64_t target = (int64_t)c->ncu * 16;
-        int64_t G = (npairs + target - 1) / target;
-        if (G < 64)
-            G = 64;
-        int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
-        int64_t *tmp2 = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
-        S2G_HIP(hipEventRecord(E[6], s));
-        launch_2d_pairs(l1, nl, rec, ntj, ntiles, nchunks, cnt, tmp2, pairs, true, s);
-        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp2, nullptr, nullptr, false, s);
-        S2G_HIP(hipEventRecord(E[7], s));
-        int64_t nitems;
-        read_i64(c, s, {nit + ntiles}, &nitems);
-        int4 *items = (int4 *)c->items.get(sizeof(int4) * (nitems ? nitems : 1));
-        int64_t *tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
-        S2G_HIP(hipEventRecord(E[8], s));
-        launch_2d_items(cnt, ntiles, nchunks, npairs, G, nit, tmp2, items, tile_items, true, s);
-        S2G_HIP(hipEventRecord(E[9], s));
+    if (cl.nl > 0) {
+        const TilePlan tp = build_tile_lists<Rec2>(c, cl.l1, cl.nl, rec, ntj, 1, ntiles, s,
+                                                   &ms_sort);
         if (st)
-            st->n_pairs = npairs;
-
+            st->n_pairs = tp.npairs;
         int ev = 10;
         for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
             const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
@@ -196,35 +237,34 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
             g.nq = a.nq;
             g.ntj = ntj;
             g.ntiles = ntiles;
-            g.nitems = nitems;
+            g.nitems = tp.nitems;
             g.npix = npix;
-            g.items = items;
-            g.pairs = pairs;
-            g.large = l1;
+            g.items = tp.items;
+            g.pairs = tp.pairs;
+            g.large = cl.l1;
             g.rec = rec;
             g.binq = a.binq;
-            g.partial = (double *)c->partial.get(sizeof(double) * (nitems ? nitems : 1) * np *
-                                                 kGT * kGT);
-            g.tile_items = tile_items;
+            g.partial = (double *)c->partial.get(sizeof(double) * (tp.nitems ? tp.nitems : 1) *
+                                                 np * kGT * kGT);
+            g.tile_items = tp.tile_items;
             g.image = d_image;
-            const bool timed = ev + 2 < s2g_ctx::kNumStageEvents;
+            const bool timed = ev + 2 < s2g_ctx::kNumStageEvents - 3;
             if (timed)
                 S2G_HIP(hipEventRecord(E[ev], s));
-            if (nitems > 0) {
+            if (tp.nitems > 0) {
                 launch_2d_gather(g, s);
                 ++launches;
             }
             if (timed)
                 S2G_HIP(hipEventRecord(E[ev + 1], s));
             launch_2d_finalize(g, s);
-            if (timed)
+            if (timed) {
                 S2G_HIP(hipEventRecord(E[ev + 2], s));
-            if (timed)
                 ev += 3;
+            }
         }
         S2G_HIP(hipEventSynchronize(E[ev - 1]));
-        ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]) + elapsed(E[8], E[9]);
-        for (int k = 10; k + 2 < ev + 1 && k < ev; k += 3) {
+        for (int k = 10; k < ev; k += 3) {
             ms_gather += elapsed(E[k], E[k + 1]);
             ms_fin += elapsed(E[k + 1], E[k + 2]);
         }
@@ -236,15 +276,114 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
     sa.dtype = a.dtype;
     sa.kernel = a.kernel;
     sa.nq = a.nq;
-    sa.ns = ns;
+    sa.ns = cl.ns;
     sa.npix = npix;
-    sa.list = l0;
+    sa.list = cl.l0;
     sa.rec = rec;
     sa.binq = a.binq;
     sa.image = d_image;
     const int eS = s2g_ctx::kNumStageEvents - 3;
     S2G_HIP(hipEventRecord(E[eS], s));
     launch_2d_small(sa, s);
+    S2G_HIP(hipGetLastError());
+    S2G_HIP(hipEventRecord(E[eS + 1], s));
+    S2G_HIP(hipEventSynchronize(E[eS + 1]));
+    if (st) {
+        st->ms_device = elapsed(E[0], E[eS + 1]);
+        st->ms_setup = elapsed(E[0], E[1]);
+        st->ms_norm = elapsed(E[2], E[3]);
+        st->ms_sort = ms_sort;
+        st->ms_gather = ms_gather;
+        st->ms_finalize = ms_fin;
+        st->ms_small = elapsed(E[eS], E[eS + 1]);
+        st->n_gather_launches = launches;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3D deposit on device pointers (image: npix^3 x 2, plane 0 = q, 1 = weights)
+// ---------------------------------------------------------------------------
+void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream_t s,
+                      s2g_stats *st)
+{
+    const int64_t n = a.n, npix = a.npix;
+    const int64_t nd = npix * npix * npix;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+    }
+    hipEvent_t *E = c->stage;
+    S2G_HIP(hipEventRecord(E[0], s));
+    if (n == 0) {
+        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * 2, s));
+        S2G_HIP(hipEventRecord(E[1], s));
+        S2G_HIP(hipEventSynchronize(E[1]));
+        return;
+    }
+    Rec3 *rec = (Rec3 *)c->rec.get(sizeof(Rec3) * n);
+    int32_t *cls = (int32_t *)c->cls.get(sizeof(int32_t) * n);
+    launch_3d_setup(a, rec, cls, s);
+    const ClassLists cl = partition_classes(c, cls, n, s);
+    fill_class_stats(st, n, cl);
+
+    S2G_HIP(hipEventRecord(E[2], s));
+    launch_3d_norm(a.kernel, cl.l2, cl.ne, rec, s);
+    S2G_HIP(hipEventRecord(E[3], s));
+
+    const int nb = (int)((npix + kBT - 1) / kBT);
+    const int ntiles = nb * nb * nb;
+    double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;
+    int64_t launches = 0;
+    if (cl.nl > 0) {
+        const TilePlan tp = build_tile_lists<Rec3>(c, cl.l1, cl.nl, rec, nb, nb, ntiles, s,
+                                                   &ms_sort);
+        if (st)
+            st->n_pairs = tp.npairs;
+        Gather3Args g;
+        g.dtype = a.dtype;
+        g.kernel = a.kernel;
+        g.np = 2;
+        g.plane0 = 0;
+        g.nq = 1;
+        g.nb = nb;
+        g.ntiles = ntiles;
+        g.nitems = tp.nitems;
+        g.npix = npix;
+        g.items = tp.items;
+        g.pairs = tp.pairs;
+        g.large = cl.l1;
+        g.rec = rec;
+        g.binq = a.binq;
+        g.partial = (double *)c->partial.get(sizeof(double) * (tp.nitems ? tp.nitems : 1) * 2 *
+                                             kBT * kBT * kBT);
+        g.tile_items = tp.tile_items;
+        g.image = d_image;
+        S2G_HIP(hipEventRecord(E[10], s));
+        if (tp.nitems > 0) {
+            launch_3d_gather(g, s);
+            ++launches;
+        }
+        S2G_HIP(hipEventRecord(E[11], s));
+        launch_3d_finalize(g, s);
+        S2G_HIP(hipEventRecord(E[12], s));
+        S2G_HIP(hipEventSynchronize(E[12]));
+        ms_gather = elapsed(E[10], E[11]);
+        ms_fin = elapsed(E[11], E[12]);
+    } else {
+        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * 2, s));
+    }
+    Small3Args sa;
+    sa.dtype = a.dtype;
+    sa.kernel = a.kernel;
+    sa.ns = cl.ns;
+    sa.npix = npix;
+    sa.list = cl.l0;
+    sa.rec = rec;
+    sa.binq = a.binq;
+    sa.image = d_image;
+    const int eS = s2g_ctx::kNumStageEvents - 3;
+    S2G_HIP(hipEventRecord(E[eS], s));
+    launch_3d_small(sa, s);
     S2G_HIP(hipGetLastError());
     S2G_HIP(hipEventRecord(E[eS + 1], s));
     S2G_HIP(hipEventSynchronize(E[eS + 1]));
@@ -345,7 +484,7 @@ void deposit_device_any(s2g_ctx *c, int dims, const Map2DArgs &a, double *d_imag
     if (dims == 2)
         deposit2d_device(c, a, d_image, s, st);
     else
-        throw Error(S2G_EUNSUPPORTED, "3D deposit not available in this build");
+        deposit3d_device(c, a, d_image, s, st);
 }
 
 }  // namespace
@@ -406,6 +545,7 @@ void s2g_destroy(s2g_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     for (auto &b : c->in) b.release();
     c->rec.release();
+    c->cls.release();
     c->counts.release();
     c->scan_tmp.release();
     for (auto &b : c->lists) b.release();
@@ -497,7 +637,7 @@ int s2g_reduce_device(s2g_ctx *c, int dims, int64_t npix, int nq, uint32_t flags
         if (dims == 2)
             launch_2d_reduce(d_image, npix, nq, (flags & S2G_FLAG_REDUCE) ? 1 : 0, d_out, s);
         else
-            throw Error(S2G_EUNSUPPORTED, "3D reduce not available in this build");
+            launch_3d_reduce(d_image, npix, (flags & S2G_FLAG_REDUCE) ? 1 : 0, d_out, s);
         S2G_HIP(hipGetLastError());
         return S2G_OK;
     });

@@ -30,6 +30,10 @@ constexpr int kSmallSide2D = 34;
 constexpr int kGT = 64;
 constexpr int kGRows = 16;
 
+// 3D: footprint box side <= 8 voxels goes to the scatter path; bricks of 16^3.
+constexpr int kSmallSide3D = 8;
+constexpr int kBT = 16;
+
 // Per-particle record of the 2D deposit, built once per call.
 struct Rec2 {
     double x, y;     // pixel coordinates, cic_shared.jl:85-100
@@ -38,6 +42,16 @@ struct Rec2 {
     int32_t i0, i1, j0, j1;  // footprint, cic_shared.jl:46-52
     int32_t cls;
     int32_t fb;      // fallback (no pixel centre inside the kernel), cic_2D.jl:51-69
+};
+
+// Per-particle record of the 3D deposit (cic_3D.jl:87-97, 150-155).
+struct Rec3 {
+    double x, y, z;  // voxel coordinates
+    double h;        // hsml in voxels
+    double s;        // vol*w*len2pix (small / exact) or the final voxel scale (large)
+    int32_t i0, i1, j0, j1, k0, k1;
+    int32_t cls;
+    int32_t fb;
 };
 
 // ---------------------------------------------------------------------------
@@ -113,6 +127,33 @@ __host__ __device__ inline double closed_form_min_h(int kernel)
     case S2G_KERNEL_WENDLAND_C4: return 28.0;
     case S2G_KERNEL_WENDLAND_C6: return 16.0;
     default: return 12.0;  // WENDLAND_C8
+    }
+}
+
+// integral of shape(|v|) over the unit ball (= 1 / norm_3D)
+__host__ __device__ inline double shape_ball_integral(int kernel)
+{
+    const double pi = 3.14159265358979323846;
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: return pi / 8.0;
+    case S2G_KERNEL_QUINTIC: return 40.0 * pi / 2187.0;
+    case S2G_KERNEL_WENDLAND_C2: return 2.0 * pi / 21.0;
+    case S2G_KERNEL_WENDLAND_C4: return 32.0 * pi / 495.0;
+    case S2G_KERNEL_WENDLAND_C6: return 64.0 * pi / 1365.0;
+    default: return 64.0 * pi / 1785.0;  // WENDLAND_C8
+    }
+}
+
+// 3D counterpart of closed_form_min_h (voxels; same 2e-9 bound)
+__host__ __device__ inline double closed_form_min_h3(int kernel)
+{
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: return 40.0;
+    case S2G_KERNEL_QUINTIC: return 24.0;
+    case S2G_KERNEL_WENDLAND_C2: return 32.0;
+    case S2G_KERNEL_WENDLAND_C4: return 20.0;
+    case S2G_KERNEL_WENDLAND_C6: return 14.0;
+    default: return 11.0;  // WENDLAND_C8
     }
 }
 
