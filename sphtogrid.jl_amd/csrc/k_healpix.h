@@ -1,0 +1,37 @@
+// k_healpix.h -- launchers of the all-sky (HEALPix RING) deposit kernels.
+#pragma once
+
+#include "s2g_internal.h"
+
+namespace s2g {
+
+// per-particle record of the all-sky deposit
+struct HpRec {
+    double z0, phi0, theta0, sth0, xa;  // direction (cos, phi, colatitude, sin) and 1/sin
+    double theta;                       // projected hsml asin(h/dx), main.jl:182
+    double s;                           // area*w*dz, then the final pixel scale
+    int64_t cpix;                       // RING pixel containing the particle
+    int64_t unit0;                      // first work unit
+    int32_t ra, rb;                     // ring range incl. polar caps fully inside
+    int32_t irmin, irmax;               // rings with a partial phi interval
+    int32_t fb, ok;
+};
+
+struct HpArgs {
+    int dtype, kernel;
+    const void *pos, *hsml, *m, *rho, *binq, *w;
+    int64_t n, nside;
+    uint32_t flags;
+};
+
+// nunits: n+1 int64 (last = 0, the total after an exclusive scan)
+void launch_hp_setup(const HpArgs &a, HpRec *rec, int64_t *nunits, hipStream_t s);
+void launch_hp_units(const int64_t *ubase, int64_t n, HpRec *rec, int2 *units, hipStream_t s);
+void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
+                    double4 *part, hipStream_t s);
+void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRec *rec,
+                      hipStream_t s);
+void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpRec *rec,
+                       double *map, double *wmap, hipStream_t s);
+
+}  // namespace s2g

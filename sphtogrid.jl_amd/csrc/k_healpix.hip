@@ -1,0 +1,515 @@
+// k_healpix.hip -- the all-sky deposit of healpix_map
+// (healpix_interpolation/main.jl:146-213) on gfx950.
+//
+//   k_hp_setup      per particle, FP64: skips (main.jl:160-179), projected
+//                   hsml (:182), area and depth (:56-63, 193), direction, the
+//                   RING-scheme disc query's ring range and polar caps, the
+//                   pixel containing the centre (constributing_pixels.jl:7-22)
+//   k_hp_units      (particle, group of rings) work units from a scan
+//   k_hp_norm       per unit: partial sums of W*A and A (FP64, fixed order)
+//   k_hp_finish     per particle: ordered sum of its units -> scale, fallback
+//   k_hp_deposit    per unit: W*A*scale into the two maps (FP64 atomics)
+//
+// Pixel distances use the haversine form: the ring's colatitude difference
+// and the pixel's azimuth difference are formed in FP64, the remaining
+// trigonometry in FP32.  This gives the great-circle distance to FP32
+// relative accuracy even at pixel-scale angles.
+#include "k_healpix.h"
+
+namespace s2g {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kHpRingsPerUnit = 4;
+
+// ---------------------------------------------------------------------------
+// RING-scheme geometry (Gorski et al. 2005), FP64
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hp_ring_info(int64_t nside, int64_t ring, int64_t &start,
+                                             int64_t &nr, double &shift)
+{
+    const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    if (ring < nside) {
+        shift = 0.5;
+        nr = 4 * ring;
+        start = 2 * ring * (ring - 1);
+    } else if (ring < 3 * nside) {
+        shift = ((ring - nside) & 1) == 0 ? 0.5 : 0.0;
+        nr = 4 * nside;
+        start = ncap + (ring - nside) * 4 * nside;
+    } else {
+        const int64_t r = 4 * nside - ring;
+        shift = 0.5;
+        nr = 4 * r;
+        start = npix - 2 * r * (r + 1);
+    }
+}
+
+__device__ __forceinline__ double hp_ring_z(int64_t nside, int64_t ring)
+{
+    const double fact2 = 4.0 / (double)(12 * nside * nside);
+    if (ring < nside)
+        return 1.0 - (double)(ring * ring) * fact2;
+    if (ring <= 3 * nside)
+        return (double)(2 * nside - ring) * (2.0 / (3.0 * (double)nside));
+    const int64_t r = 4 * nside - ring;
+    return -1.0 + (double)(r * r) * fact2;
+}
+
+__device__ __forceinline__ int64_t hp_ring_above(int64_t nside, double z)
+{
+    const double az = fabs(z);
+    if (az <= 2.0 / 3.0)
+        return (int64_t)((double)nside * (2.0 - 1.5 * z));
+    const int64_t ir = (int64_t)((double)nside * sqrt(3.0 * (1.0 - az)));
+    return (z > 0) ? ir : 4 * nside - ir - 1;
+}
+
+__device__ __forceinline__ int64_t hp_ang2pix(int64_t nside, double z, double phi)
+{
+    const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    const double za = fabs(z);
+    double tt = fmod(phi, 2.0 * kPi);
+    if (tt < 0.0)
+        tt += 2.0 * kPi;
+    tt = tt / (0.5 * kPi);
+    if (za <= 2.0 / 3.0) {
+        const double t1 = (double)nside * (0.5 + tt);
+        const double t2 = (double)nside * z * 0.75;
+        const int64_t jp = (int64_t)(t1 - t2), jm = (int64_t)(t1 + t2);
+        const int64_t ir = nside + 1 + jp - jm;
+        const int64_t kshift = 1 - (ir & 1);
+        int64_t ip = (jp + jm - nside + kshift + 1) / 2;
+        ip = ((ip % (4 * nside)) + 4 * nside) % (4 * nside);
+        return ncap + (ir - 1) * 4 * nside + ip;
+    }
+    const double tp = tt - (double)(int64_t)tt;
+    const double tmp = (double)nside * sqrt(3.0 * (1.0 - za));
+    const int64_t jp = (int64_t)(tp * tmp), jm = (int64_t)((1.0 - tp) * tmp);
+    const int64_t ir = jp + jm + 1;
+    int64_t ip = (int64_t)(tt * (double)ir);
+    ip = ((ip % (4 * ir)) + 4 * ir) % (4 * ir);
+    return (z > 0) ? 2 * ir * (ir - 1) + ip : npix - 2 * ir * (ir + 1) + ip;
+}
+
+// ring index (1-based) and ring-local index of a RING pixel
+__device__ __forceinline__ void hp_pix2ring(int64_t nside, int64_t pix, int64_t &ring,
+                                            int64_t &k)
+{
+    const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    if (pix < ncap) {
+        int64_t r = (int64_t)((1.0 + sqrt(1.0 + 2.0 * (double)pix)) * 0.5);
+        while (2 * r * (r - 1) > pix) --r;
+        while (2 * (r + 1) * r <= pix) ++r;
+        ring = r;
+        k = pix - 2 * r * (r - 1);
+    } else if (pix < npix - ncap) {
+        const int64_t ip = pix - ncap;
+        ring = ip / (4 * nside) + nside;
+        k = ip % (4 * nside);
+    } else {
+        const int64_t ip = npix - pix;  // in [1, ncap]
+        int64_t r = (int64_t)((1.0 + sqrt(2.0 * (double)ip - 1.0)) * 0.5);
+        while (2 * r * (r + 1) < ip) ++r;
+        while (r > 1 && 2 * (r - 1) * r >= ip) --r;
+        ring = 4 * nside - r;
+        k = pix - (npix - 2 * r * (r + 1));
+    }
+}
+
+// The pixel interval of ring iz inside the disc: [lo, hi] ring-local, where
+// lo may be negative (wrap: pixels lo+nr .. nr-1 and 0 .. hi).  Returns
+// false if the ring contributes no pixel.  Mirrors the per-ring step of the
+// disc query (non-inclusive: pixel centres within the radius).
+__device__ __forceinline__ bool hp_ring_interval(int64_t nside, int64_t iz, double z0,
+                                                 double xa, double cosr, double phi0,
+                                                 int64_t &lo, int64_t &hi)
+{
+    const double z = hp_ring_z(nside, iz);
+    const double x = (cosr - z * z0) * xa;
+    const double ysq = 1.0 - z * z - x * x;
+    double dphi;
+    if (ysq <= 0.0)
+        dphi = (x < 0.0) ? kPi : 0.0;
+    else
+        dphi = atan2(sqrt(ysq), x);
+    if (!(dphi > 0.0))
+        return false;
+    int64_t start, nr;
+    double shift;
+    hp_ring_info(nside, iz, start, nr, shift);
+    const double inv2pi = 1.0 / (2.0 * kPi);
+    int64_t a = (int64_t)floor((double)nr * inv2pi * (phi0 - dphi) - shift) + 1;
+    int64_t b = (int64_t)floor((double)nr * inv2pi * (phi0 + dphi) - shift);
+    if (a > b)
+        return false;
+    if (b >= nr) {
+        a -= nr;
+        b -= nr;
+    }
+    lo = a;
+    hi = b;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
+                                                  const T *__restrict__ hsml,
+                                                  const T *__restrict__ m,
+                                                  const T *__restrict__ rho,
+                                                  const T *__restrict__ binq,
+                                                  const T *__restrict__ w, int64_t n,
+                                                  int64_t nside, uint32_t flags,
+                                                  HpRec *__restrict__ rec,
+                                                  int64_t *__restrict__ nunits)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    HpRec r;
+    r.z0 = r.phi0 = r.theta0 = r.sth0 = r.xa = r.theta = r.s = 0.0;
+    r.ra = 1;
+    r.rb = 0;
+    r.irmin = 1;
+    r.irmax = 0;
+    r.cpix = -1;
+    r.fb = 0;
+    r.ok = 0;
+    r.unit0 = 0;
+    const double bq = ld(binq, p);
+    const double h = ld(hsml, p);
+    const double px = ld(pos, 3 * p), py = ld(pos, 3 * p + 1), pz = ld(pos, 3 * p + 2);
+    // shared.jl:1-10: sqrt((0 + x^2) + y^2) + z^2)
+    const double dx = sqrt(add_rn(add_rn(mul_rn(px, px), mul_rn(py, py)), mul_rn(pz, pz)));
+    bool skip = (!(flags & S2G_FLAG_CALC_MEAN) && bq == 0.0) || dx < h || !(dx == dx) ||
+                !(h == h);
+    if (!skip) {
+        const double npix = (double)(12 * nside * nside);
+        const double ang_pix = sqrt(4.0 * kPi / npix);  // main.jl:144
+        const double theta = asin(h / dx);               // main.jl:182
+        const double dz0 = 2.0 * h;                       // main.jl:59-60
+        const double area = (ld(m, p) / ld(rho, p)) / dz0;
+        const double ap = ang_pix * dx;
+        const double dz = dz0 / (ap * ap);  // main.jl:193
+        r.s = mul_rn(mul_rn(area, ld(w, p)), dz);
+        r.theta = theta;
+        // vec2ang
+        const double th0 = atan2(sqrt(add_rn(mul_rn(px, px), mul_rn(py, py))), pz);
+        double ph0 = atan2(py, px);
+        if (ph0 < 0.0)
+            ph0 += 2.0 * kPi;
+        r.theta0 = th0;
+        r.phi0 = ph0;
+        r.z0 = cos(th0);
+        r.sth0 = sin(th0);
+        r.xa = 1.0 / sqrt((1.0 - r.z0) * (1.0 + r.z0));
+        // ring range of the disc query
+        const double rlat1 = th0 - theta;
+        const int64_t irmin = hp_ring_above(nside, cos(rlat1)) + 1;
+        const double rlat2 = th0 + theta;
+        const int64_t irmax = hp_ring_above(nside, cos(rlat2));
+        r.irmin = (int32_t)irmin;
+        r.irmax = (int32_t)irmax;
+        r.ra = (rlat1 <= 0.0 && irmin > 1) ? 1 : (int32_t)irmin;
+        r.rb = (rlat2 >= kPi && irmax + 1 < 4 * nside) ? (int32_t)(4 * nside - 1) : (int32_t)irmax;
+        r.cpix = hp_ang2pix(nside, r.z0, ph0);
+        r.ok = 1;
+    }
+    rec[p] = r;
+    const int64_t nrings = r.ok ? (int64_t)(r.rb - r.ra + 1) : 0;
+    // one extra unit when only the centre pixel contributes
+    int64_t u = nrings > 0 ? (nrings + kHpRingsPerUnit - 1) / kHpRingsPerUnit : (r.ok ? 1 : 0);
+    nunits[p] = u;
+    if (p == n - 1)
+        nunits[n] = 0;  // becomes the unit total after the scan
+}
+
+__global__ void k_hp_units(const int64_t *__restrict__ ubase, int64_t n, HpRec *__restrict__ rec,
+                           int2 *__restrict__ units)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t b = ubase[p], e = ubase[p + 1];
+    rec[p].unit0 = b;
+    for (int64_t k = b; k < e; ++k)
+        units[k] = make_int2((int)p, (int)(k - b));
+}
+
+// ---------------------------------------------------------------------------
+// per-pixel factors
+// ---------------------------------------------------------------------------
+struct RingTerms {
+    float s1, s2;      // sin^2(dtheta/2), sin(theta_ring) sin(theta0)
+    double dphi_pix;   // 2 pi / nr
+    double phase;      // shift * dphi_pix - phi0
+    int64_t start;
+    int64_t nr;
+};
+
+__device__ __forceinline__ RingTerms ring_terms(int64_t nside, int64_t iz, const HpRec &r)
+{
+    RingTerms t;
+    double shift;
+    hp_ring_info(nside, iz, t.start, t.nr, shift);
+    const double z = hp_ring_z(nside, iz);
+    const double th = acos(z);
+    const double sh = sin(0.5 * (th - r.theta0));
+    t.s1 = (float)(sh * sh);
+    t.s2 = (float)(sqrt((1.0 - z) * (1.0 + z)) * r.sth0);
+    t.dphi_pix = 2.0 * kPi / (double)t.nr;
+    t.phase = shift * t.dphi_pix - r.phi0;
+    return t;
+}
+
+// sin(x) for |x| <= pi/2 from its Taylor series to x^15 (relative error
+// below 1e-9 on the interval)
+__device__ __forceinline__ float sin_half_pi(float x)
+{
+    const float x2 = x * x;
+    float p = -1.0f / 1307674368000.0f;
+    p = p * x2 + 1.0f / 6227020800.0f;
+    p = p * x2 - 1.0f / 39916800.0f;
+    p = p * x2 + 1.0f / 362880.0f;
+    p = p * x2 - 1.0f / 5040.0f;
+    p = p * x2 + 1.0f / 120.0f;
+    p = p * x2 - 1.0f / 6.0f;
+    return x + x * x2 * p;
+}
+
+// angular distance (radians) from the particle to pixel k of the ring
+__device__ __forceinline__ float hp_dist(const RingTerms &t, int64_t k)
+{
+    double dphi = (double)k * t.dphi_pix + t.phase;  // phi_k - phi0
+    dphi = dphi > kPi ? dphi - 2.0 * kPi : dphi;
+    dphi = dphi < -kPi ? dphi + 2.0 * kPi : dphi;
+    const float sx = sin_half_pi((float)(0.5 * dphi));
+    const float hav = fminf(t.s1 + t.s2 * sx * sx, 1.0f);
+    return 2.0f * asinf(sqrtf(hav));
+}
+
+// W (shape, or 1 in the fallback) and the area fraction (pixel_weights.jl:6-8,
+// without the constant 1/(ang_pix*dx)^2 that cancels)
+template <int K>
+__device__ __forceinline__ void hp_weights(float d, float theta, float inv_theta, float ang,
+                                           float inv_ang, float &W, float &A)
+{
+    const float u = d * inv_theta;
+    W = shape_of<K, float>(u);
+    const float a = fabsf(theta - (d - 0.5f * ang));
+    A = fmaxf(0.0f, fminf(ang, a)) * inv_ang;
+}
+
+// Iterate the pixels of unit `ui` of particle r: rings [ra + 4*ui, ...].
+// F(pix, W, A) is called per pixel on every lane that holds a pixel.
+template <int K, typename F>
+__device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, int ui, int ln,
+                                               float ang, F &&f)
+{
+    const float theta = (float)r.theta, inv_theta = (float)(1.0 / r.theta);
+    const float inv_ang = 1.0f / ang;
+    const double cosr = cos(r.theta);
+    const int64_t r0 = (int64_t)r.ra + (int64_t)ui * kHpRingsPerUnit;
+    const int64_t r1 = r0 + kHpRingsPerUnit - 1 < r.rb ? r0 + kHpRingsPerUnit - 1 : r.rb;
+    bool covered_cp = false;
+    int64_t cring = 0, ck = 0;
+    hp_pix2ring(nside, r.cpix, cring, ck);
+    for (int64_t iz = r0; iz <= r1; ++iz) {
+        int64_t lo, hi;
+        bool any;
+        if (iz < r.irmin || iz > r.irmax) {
+            // a ring fully inside a polar cap
+            int64_t st, nr;
+            double sh;
+            hp_ring_info(nside, iz, st, nr, sh);
+            lo = 0;
+            hi = nr - 1;
+            any = true;
+        } else {
+            any = hp_ring_interval(nside, iz, r.z0, r.xa, cosr, r.phi0, lo, hi);
+        }
+        if (!any)
+            continue;
+        const RingTerms t = ring_terms(nside, iz, r);
+        if (iz == cring) {
+            const int64_t kk = ck;
+            if ((lo >= 0 && kk >= lo && kk <= hi) ||
+                (lo < 0 && (kk <= hi || kk >= lo + t.nr)))
+                covered_cp = true;
+        }
+        const int64_t cnt = hi - lo + 1;
+        for (int64_t c = ln; c < cnt; c += kWave) {
+            int64_t k = lo + c;
+            if (k < 0)
+                k += t.nr;
+            const float d = hp_dist(t, k);
+            float W, A;
+            hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+            f(t.start + k, W, A);
+        }
+    }
+    // constributing_pixels.jl:16-19: the pixel containing the particle is
+    // added if the disc query did not return it (handled by the unit that
+    // owns its ring, or by unit 0 if its ring is outside [ra, rb])
+    const bool own = (cring >= r0 && cring <= r1) ||
+                     (ui == 0 && (cring < r.ra || cring > r.rb || r.rb < r.ra));
+    if (own && !covered_cp && ln == 0) {
+        const RingTerms t = ring_terms(nside, cring, r);
+        const float d = hp_dist(t, ck);
+        float W, A;
+        hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+        f(r.cpix, W, A);
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units, int64_t nu,
+                                                 const HpRec *__restrict__ rec, int64_t nside,
+                                                 double4 *__restrict__ part)
+{
+    const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
+    const int ln = threadIdx.x % kWave;
+    if (gw >= nu)
+        return;
+    const int2 u = units[gw];
+    const HpRec r = rec[u.x];
+    const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
+    double swa = 0.0, sa = 0.0, nin = 0.0;
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, [&](int64_t, float W, float A) {
+        swa += (double)(W * A);
+        sa += (double)A;
+        nin += W > 0.0f ? 1.0 : 0.0;
+    });
+    swa = wave_sum(swa);
+    sa = wave_sum(sa);
+    nin = wave_sum(nin);
+    if (ln == 0)
+        part[gw] = make_double4(swa, sa, nin, 0.0);
+}
+
+// ordered sum of a particle's unit partials -> final scale and fallback
+__global__ void k_hp_finish(const int64_t *__restrict__ ubase, int64_t n,
+                            const double4 *__restrict__ part, HpRec *__restrict__ rec)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    HpRec r = rec[p];
+    if (!r.ok)
+        return;
+    double swa = 0.0, sa = 0.0;
+    for (int64_t k = ubase[p]; k < ubase[p + 1]; ++k) {
+        swa += part[k].x;
+        sa += part[k].y;
+    }
+    // pixel_weights.jl:120-137: fall back to pure area weights when no pixel
+    // centre lies inside the kernel
+    if (swa != 0.0) {
+        r.fb = 0;
+        r.s = r.s / swa;
+    } else {
+        r.fb = 1;
+        r.s = sa != 0.0 ? r.s / sa : 0.0;
+    }
+    rec[p] = r;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu,
+                                                    const HpRec *__restrict__ rec,
+                                                    const T *__restrict__ binq, int64_t nside,
+                                                    double *__restrict__ map,
+                                                    double *__restrict__ wmap)
+{
+    const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
+    const int ln = threadIdx.x % kWave;
+    if (gw >= nu)
+        return;
+    const int2 u = units[gw];
+    const HpRec r = rec[u.x];
+    if (r.s == 0.0)
+        return;
+    const double bq = ld(binq, u.x);
+    const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
+    const bool fb = r.fb != 0;
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, [&](int64_t pix, float W, float A) {
+        const double pw = (double)((fb ? 1.0f : W) * A) * r.s;
+        if (pw != 0.0) {
+            unsafeAtomicAdd(&wmap[pix], pw);
+            unsafeAtomicAdd(&map[pix], bq * pw);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define S2G_HP_KERNEL_SWITCH(kernel, CALL)                                      \
+    switch (kernel) {                                                           \
+    case S2G_KERNEL_CUBIC: { constexpr int KK = S2G_KERNEL_CUBIC; CALL; } break; \
+    case S2G_KERNEL_QUINTIC: { constexpr int KK = S2G_KERNEL_QUINTIC; CALL; } break; \
+    case S2G_KERNEL_WENDLAND_C2: { constexpr int KK = S2G_KERNEL_WENDLAND_C2; CALL; } break; \
+    case S2G_KERNEL_WENDLAND_C4: { constexpr int KK = S2G_KERNEL_WENDLAND_C4; CALL; } break; \
+    case S2G_KERNEL_WENDLAND_C6: { constexpr int KK = S2G_KERNEL_WENDLAND_C6; CALL; } break; \
+    default: { constexpr int KK = S2G_KERNEL_WENDLAND_C8; CALL; } break;       \
+    }
+
+void launch_hp_setup(const HpArgs &a, HpRec *rec, int64_t *nunits, hipStream_t s)
+{
+    const dim3 g((unsigned)((a.n + 255) / 256)), blk(256);
+    if (a.dtype == S2G_F32)
+        hipLaunchKernelGGL(k_hp_setup<float>, g, blk, 0, s, (const float *)a.pos,
+                           (const float *)a.hsml, (const float *)a.m, (const float *)a.rho,
+                           (const float *)a.binq, (const float *)a.w, a.n, a.nside, a.flags,
+                           rec, nunits);
+    else
+        hipLaunchKernelGGL(k_hp_setup<double>, g, blk, 0, s, (const double *)a.pos,
+                           (const double *)a.hsml, (const double *)a.m, (const double *)a.rho,
+                           (const double *)a.binq, (const double *)a.w, a.n, a.nside, a.flags,
+                           rec, nunits);
+}
+
+void launch_hp_units(const int64_t *ubase, int64_t n, HpRec *rec, int2 *units, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_units, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ubase, n,
+                       rec, units);
+}
+
+void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
+                    double4 *part, hipStream_t s)
+{
+    if (nu == 0)
+        return;
+    const dim3 g((unsigned)((nu * kWave + 255) / 256)), blk(256);
+    S2G_HP_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k_hp_norm<KK>, g, blk, 0, s, units, nu, rec,
+                                                    nside, part));
+}
+
+void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRec *rec,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ubase, n,
+                       part, rec);
+}
+
+void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpRec *rec,
+                       double *map, double *wmap, hipStream_t s)
+{
+    if (nu == 0)
+        return;
+    const dim3 g((unsigned)((nu * kWave + 255) / 256)), blk(256);
+    if (a.dtype == S2G_F32) {
+        S2G_HP_KERNEL_SWITCH(a.kernel,
+                             hipLaunchKernelGGL((k_hp_deposit<float, KK>), g, blk, 0, s, units, nu,
+                                                rec, (const float *)a.binq, a.nside, map, wmap));
+    } else {
+        S2G_HP_KERNEL_SWITCH(a.kernel,
+                             hipLaunchKernelGGL((k_hp_deposit<double, KK>), g, blk, 0, s, units,
+                                                nu, rec, (const double *)a.binq, a.nside, map,
+                                                wmap));
+    }
+}
+
+}  // namespace s2g

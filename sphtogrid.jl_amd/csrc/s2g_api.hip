@@ -5,6 +5,7 @@
 #include <string>
 
 #include "k_cic3d.h"
+#include "k_healpix.h"
 #include "s2g_internal.h"
 
 using namespace s2g;
@@ -21,6 +22,8 @@ struct s2g_ctx {
     DevBuf in[7];
     // 2D pipeline
     DevBuf rec, cls, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
+    // all-sky deposit
+    DevBuf hp_rec, hp_nunits, hp_units, hp_part;
     // outputs
     DevBuf image, out;
     // pinned host scratch for small device->host reads
@@ -555,6 +558,10 @@ void s2g_destroy(s2g_ctx *c)
     c->items.release();
     c->tile_items.release();
     c->partial.release();
+    c->hp_rec.release();
+    c->hp_nunits.release();
+    c->hp_units.release();
+    c->hp_part.release();
     c->image.release();
     c->out.release();
     if (c->h_small)
@@ -708,7 +715,78 @@ int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml
                         int kernel_dim, uint32_t flags, double *map_out, double *weight_out,
                         s2g_stats *stats)
 {
-    return fail(c, S2G_EUNSUPPORTED, "healpix deposit not available in this build");
+    return guarded(c, [&]() {
+        s2g_params dummy{1, 1.0};
+        check_common(dtype, kernel, kernel_dim, n, &dummy);
+        if (nside < 1 || nside > 8192)
+            throw Error(S2G_EINVAL, "nside out of range [1, 8192]");
+        if (!map_out || !weight_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        hipStream_t s = c->stream;
+        s2g_stats local;
+        s2g_stats *st = stats ? stats : &local;
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+        Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, 1, weights, n, &dummy, kernel,
+                                flags);
+        {
+            s2g_stats tmp;
+            h = stage_inputs(c, h, s, &tmp);
+            st->ms_h2d = tmp.ms_h2d;
+        }
+        const int64_t npx = 12 * nside * nside;
+        double *maps = (double *)c->image.get(sizeof(double) * 2 * npx);
+        hipEvent_t *E = c->stage;
+        S2G_HIP(hipEventRecord(E[0], s));
+        S2G_HIP(hipMemsetAsync(maps, 0, sizeof(double) * 2 * npx, s));
+        if (n > 0) {
+            HpArgs a;
+            a.dtype = dtype;
+            a.kernel = kernel;
+            a.pos = h.pos;
+            a.hsml = h.hsml;
+            a.m = h.m;
+            a.rho = h.rho;
+            a.binq = h.binq;
+            a.w = h.w;
+            a.n = n;
+            a.nside = nside;
+            a.flags = flags;
+            HpRec *rec = (HpRec *)c->hp_rec.get(sizeof(HpRec) * n);
+            int64_t *nun = (int64_t *)c->hp_nunits.get(sizeof(int64_t) * (n + 1));
+            int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(n + 1));
+            launch_hp_setup(a, rec, nun, s);
+            exclusive_scan_i64(nun, n + 1, tmp, s);
+            S2G_HIP(hipEventRecord(E[1], s));
+            int64_t nu;
+            read_i64(c, s, {nun + n}, &nu);
+            if (nu >= (int64_t)INT32_MAX)
+                throw Error(S2G_EUNSUPPORTED, "too many all-sky work units");
+            int2 *units = (int2 *)c->hp_units.get(sizeof(int2) * (nu ? nu : 1));
+            double4 *part = (double4 *)c->hp_part.get(sizeof(double4) * (nu ? nu : 1));
+            launch_hp_units(nun, n, rec, units, s);
+            S2G_HIP(hipEventRecord(E[2], s));
+            launch_hp_norm(kernel, units, nu, rec, nside, part, s);
+            launch_hp_finish(nun, n, part, rec, s);
+            S2G_HIP(hipEventRecord(E[3], s));
+            launch_hp_deposit(a, units, nu, rec, maps + npx, maps, s);
+            S2G_HIP(hipGetLastError());
+            st->n_pairs = nu;
+        }
+        S2G_HIP(hipEventRecord(E[4], s));
+        S2G_HIP(hipEventSynchronize(E[4]));
+        st->ms_device = elapsed(E[0], E[4]);
+        if (n > 0) {
+            st->ms_setup = elapsed(E[0], E[1]);
+            st->ms_norm = elapsed(E[2], E[3]);
+            st->ms_small = elapsed(E[3], E[4]);
+        }
+        double t1 = 0.0, t2 = 0.0;
+        d2h(c, map_out, maps + npx, sizeof(double) * npx, s, &t1);
+        d2h(c, weight_out, maps, sizeof(double) * npx, s, &t2);
+        st->ms_d2h = t1 + t2;
+        return S2G_OK;
+    });
 }
 
 int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t n,

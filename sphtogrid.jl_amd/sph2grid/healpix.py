@@ -1,0 +1,100 @@
+"""``healpix_map`` -- host side of the all-sky deposit, mirroring
+``healpix_interpolation/main.jl:92-227`` and ``filter_particles.jl:6-54``.
+
+Returns two RING-ordered Float64 arrays of 12*Nside^2 pixels (0-based pixel
+index): the quantity map and the weight map, unreduced, as the reference
+returns its two ``HealpixMap``s.  The particle loop runs on the GPU through
+``s2g_healpix_deposit``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from .kernels import kernel_args
+
+
+class BoundsError(IndexError):
+    """Julia's BoundsError, raised where the reference raises it."""
+
+
+def find_in_shell(dx, radius_limits):
+    """filter_particles.jl:6-8."""
+    return (radius_limits[0] <= dx) & (dx <= radius_limits[1])
+
+
+def filter_sort_particles(Pos, Hsml, M, Rho, Bin_q, Weights, center, radius_limits,
+                          calc_mean):
+    """filter_particles.jl:17-54.  Subtracts ``center`` from ``Pos`` IN PLACE.
+
+    The selection mirrors the reference exactly: ``sorted[sel]`` applies the
+    shell mask, which is in the original particle order, to the radially
+    sorted order; with ``calc_mean=False`` the reference only runs when no
+    particle is removed and every ``Bin_q > 0`` (otherwise it raises a
+    BoundsError, as here).  When the mask keeps every particle the selection
+    is a permutation and the radial sort is skipped: the deposit does not
+    depend on particle order beyond floating-point summation order.
+    """
+    for d in range(3):
+        Pos[d] -= center[d]
+    dx = np.sqrt(Pos[0] ** 2 + Pos[1] ** 2 + Pos[2] ** 2)
+    sel = find_in_shell(dx, radius_limits)
+    n = sel.shape[0]
+    if not calc_mean:
+        k = int(sel.sum())
+        if k != n:
+            raise BoundsError("BoundsError: sel[Bin_q[sel] .> 0.0] (filter_particles.jl:28-30)")
+        if not np.all(np.asarray(Bin_q) > 0.0):
+            raise BoundsError("BoundsError: sorted[sel] (filter_particles.jl:36)")
+    if sel.all():
+        idx = slice(None)
+    else:
+        order = np.argsort(dx, kind="stable")[::-1]
+        idx = order[sel]
+    return (Pos[:, idx], np.asarray(Hsml)[idx], np.asarray(M)[idx], np.asarray(Rho)[idx],
+            np.asarray(Bin_q)[idx], np.asarray(Weights)[idx])
+
+
+def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
+                radius_limits=(0.0, np.inf), Nside: int = 1024, kernel,
+                show_progress: bool = True, output_from_all_workers: bool = False,
+                calc_mean: bool = True, context=None, stats: dict | None = None):
+    """main.jl:92-227 on the GPU.  ``Pos`` (3 x N, Float64) is mutated."""
+    Pos = np.asarray(Pos)
+    arrays = [Pos] + [np.asarray(a) for a in (Hsml, M, Rho, Bin_q, Weights)]
+    if any(a.dtype != np.float64 for a in arrays):
+        # calculate_weights is Float64-only (pixel_weights.jl:87-91)
+        raise TypeError("healpix_map takes Float64 arrays (MethodError in the reference "
+                        "for other element types)")
+    if Pos.ndim != 2 or Pos.shape[0] != 3:
+        raise ValueError("Pos must be a 3 x N array")
+    kid, kdim = kernel_args(kernel)
+    npix = 12 * int(Nside) ** 2
+    allsky = np.zeros(npix)
+    weight = np.zeros(npix)
+    # main.jl:114-121
+    if not calc_mean and np.sum(Bin_q) == 0:
+        return allsky, weight
+    t0 = time.perf_counter()
+    pos, hsml, m, rho, bq, w = filter_sort_particles(Pos, Hsml, M, Rho, Bin_q, Weights,
+                                                     np.asarray(center, dtype=np.float64),
+                                                     radius_limits, calc_mean)
+    n = hsml.shape[0]
+    x_il = np.ascontiguousarray(pos.T)
+    cv = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    hsml, m, rho, bq, w = (cv(a) for a in (hsml, m, rho, bq, w))
+    ctx = context or _lib.default_context()
+    flags = _lib.S2G_FLAG_CALC_MEAN if calc_mean else 0
+    st = _lib.Stats()
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = ctx.lib.s2g_healpix_deposit(ctx.handle, _lib.S2G_F64, ptr(x_il), ptr(hsml), ptr(m),
+                                     ptr(rho), ptr(bq), ptr(w), n, int(Nside), kid, kdim,
+                                     flags, ptr(allsky), ptr(weight), C.byref(st))
+    ctx.check(rc, "s2g_healpix_deposit")
+    if stats is not None:
+        stats.update(st.as_dict())
+        stats["wall_s"] = time.perf_counter() - t0
+    return allsky, weight
