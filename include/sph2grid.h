@@ -175,6 +175,10 @@ int s2g_healpix_deposit(s2g_ctx *ctx, int dtype, const void *pos,
 int s2g_tsc3d(s2g_ctx *ctx, int dtype, const void *pos, const void *bin_q,
               int64_t n, const s2g_params *params, uint32_t flags,
               double *grid_out, s2g_stats *stats);
+/* Same on device pointers (asynchronous on `stream`). */
+int s2g_tsc3d_device(s2g_ctx *ctx, int dtype, const void *d_pos,
+                     const void *d_bin_q, int64_t n, const s2g_params *params,
+                     uint32_t flags, double *d_grid_out, void *stream);
 
 /* One map of a batch (device pointers). */
 typedef struct s2g_map_desc {

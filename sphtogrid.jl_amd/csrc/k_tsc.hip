@@ -1,0 +1,97 @@
+// k_tsc.hip -- triangular-shaped-cloud deposit onto an npix^3 grid (textbook
+// TSC, Hockney & Eastwood; the reference's TSC path is commented out,
+// tsc_interpolation/tsc_interpolation.jl:1-182).
+//
+// Grid units follow the 3D deposit (get_xyz, cic_shared.jl:85-100):
+// g = x*len2pix + npix/2, cell c has its centre at c + 1/2.  Per axis the
+// weights of the nearest cell and its two neighbours are
+//   W(d) = 3/4 - d^2          |d| < 1/2
+//        = (3/2 - |d|)^2 / 2  1/2 <= |d| < 3/2
+// and they sum to one.  Plane 0 accumulates q*W, plane 1 accumulates W.
+#include "k_tsc.h"
+
+namespace s2g {
+
+__device__ __forceinline__ double tsc_w(double d)
+{
+    const double a = fabs(d);
+    if (a < 0.5)
+        return 0.75 - a * a;
+    if (a < 1.5) {
+        const double b = 1.5 - a;
+        return 0.5 * b * b;
+    }
+    return 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T *__restrict__ binq,
+                                             int64_t n, int64_t npix, double len2pix,
+                                             int periodic, double *__restrict__ grid)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t nd = npix * npix * npix;
+    int64_t c[3];
+    double w[3][3];
+    bool bad = false;
+    for (int d = 0; d < 3; ++d) {
+        const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
+        const double fg = floor(g);
+        if (!(fg == fg) || fg < -2.0 || fg > (double)npix + 2.0) {
+            if (!periodic || !(fg == fg))
+                bad = true;
+        }
+        c[d] = bad ? 0 : (int64_t)fg;
+        for (int o = -1; o <= 1; ++o)
+            w[d][o + 1] = tsc_w(sub_rn(g, (double)(c[d] + o) + 0.5));
+    }
+    if (bad)
+        return;
+    const double q = ld(binq, p);
+    for (int a = -1; a <= 1; ++a) {
+        int64_t i = c[0] + a;
+        if (periodic)
+            i = ((i % npix) + npix) % npix;
+        else if (i < 0 || i >= npix)
+            continue;
+        for (int b = -1; b <= 1; ++b) {
+            int64_t j = c[1] + b;
+            if (periodic)
+                j = ((j % npix) + npix) % npix;
+            else if (j < 0 || j >= npix)
+                continue;
+            const double wab = w[0][a + 1] * w[1][b + 1];
+            for (int e = -1; e <= 1; ++e) {
+                int64_t k = c[2] + e;
+                if (periodic)
+                    k = ((k % npix) + npix) % npix;
+                else if (k < 0 || k >= npix)
+                    continue;
+                const double ww = wab * w[2][e + 1];
+                if (ww == 0.0)
+                    continue;
+                const int64_t idx = (i * npix + j) * npix + k;
+                unsafeAtomicAdd(&grid[idx], q * ww);
+                unsafeAtomicAdd(&grid[nd + idx], ww);
+            }
+        }
+    }
+}
+
+void launch_tsc(int dtype, const void *pos, const void *binq, int64_t n, int64_t npix,
+                double len2pix, int periodic, double *grid, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const dim3 g((unsigned)((n + 255) / 256)), blk(256);
+    if (dtype == S2G_F32)
+        hipLaunchKernelGGL(k_tsc<float>, g, blk, 0, s, (const float *)pos, (const float *)binq,
+                           n, npix, len2pix, periodic, grid);
+    else
+        hipLaunchKernelGGL(k_tsc<double>, g, blk, 0, s, (const double *)pos,
+                           (const double *)binq, n, npix, len2pix, periodic, grid);
+}
+
+}  // namespace s2g

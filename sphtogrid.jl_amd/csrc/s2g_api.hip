@@ -6,6 +6,7 @@
 
 #include "k_cic3d.h"
 #include "k_healpix.h"
+#include "k_tsc.h"
 #include "s2g_internal.h"
 
 using namespace s2g;
@@ -792,7 +793,64 @@ int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml
 int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t n,
               const s2g_params *params, uint32_t flags, double *grid_out, s2g_stats *stats)
 {
-    return fail(c, S2G_EUNSUPPORTED, "tsc deposit not available in this build");
+    return guarded(c, [&]() {
+        check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
+        if (params->npixels > 2048)
+            throw Error(S2G_EINVAL, "npixels out of range [1, 2048] for the TSC grid");
+        if (!grid_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        hipStream_t s = c->stream;
+        s2g_stats local;
+        s2g_stats *st = stats ? stats : &local;
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+        const size_t es = elem_size(dtype);
+        void *dpos = c->in[0].get(es * (size_t)(3 * n > 0 ? 3 * n : 1));
+        void *dq = c->in[4].get(es * (size_t)(n > 0 ? n : 1));
+        hipEvent_t *E = c->stage;
+        S2G_HIP(hipEventRecord(E[0], s));
+        if (n > 0) {
+            if (!pos || !bin_q)
+                throw Error(S2G_EINVAL, "input array pointer is NULL");
+            S2G_HIP(hipMemcpyAsync(dpos, pos, es * 3 * n, hipMemcpyHostToDevice, s));
+            S2G_HIP(hipMemcpyAsync(dq, bin_q, es * n, hipMemcpyHostToDevice, s));
+        }
+        const int64_t npix = params->npixels;
+        const int64_t nd = npix * npix * npix;
+        double *grid = (double *)c->image.get(sizeof(double) * 2 * nd);
+        S2G_HIP(hipEventRecord(E[1], s));
+        S2G_HIP(hipMemsetAsync(grid, 0, sizeof(double) * 2 * nd, s));
+        launch_tsc(dtype, dpos, dq, n, npix, params->len2pix,
+                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, grid, s);
+        S2G_HIP(hipGetLastError());
+        S2G_HIP(hipEventRecord(E[2], s));
+        S2G_HIP(hipEventSynchronize(E[2]));
+        st->ms_h2d = elapsed(E[0], E[1]);
+        st->ms_device = elapsed(E[1], E[2]);
+        st->n_small = n;
+        d2h(c, grid_out, grid, sizeof(double) * 2 * nd, s, &st->ms_d2h);
+        return S2G_OK;
+    });
+}
+
+int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin_q, int64_t n,
+                     const s2g_params *params, uint32_t flags, double *d_grid_out, void *stream)
+{
+    return guarded(c, [&]() {
+        check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
+        if (params->npixels > 2048)
+            throw Error(S2G_EINVAL, "npixels out of range [1, 2048] for the TSC grid");
+        if (!d_grid_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        const int64_t npix = params->npixels;
+        const int64_t nd = npix * npix * npix;
+        S2G_HIP(hipMemsetAsync(d_grid_out, 0, sizeof(double) * 2 * nd, s));
+        launch_tsc(dtype, d_pos, d_bin_q, n, npix, params->len2pix,
+                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, d_grid_out, s);
+        S2G_HIP(hipGetLastError());
+        return S2G_OK;
+    });
 }
 
 int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps, void *stream,

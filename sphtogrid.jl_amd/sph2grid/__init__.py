@@ -14,6 +14,7 @@ from .kernels import (AbstractSPHKernel, Cubic, Quintic, WendlandC2,  # noqa: F4
 from .mapping import (center_particles, deposit, filter_particles_in_image,  # noqa: F401
                       sphMapping)
 from .parameters import ErrorException, MappingParameters, mappingParameters  # noqa: F401
+from .tsc import tsc_mapping  # noqa: F401
 from .weights import (part_weight_emission, part_weight_one,  # noqa: F401
                       part_weight_physical, part_weight_spectroscopic,
                       part_weight_XrayBand)
@@ -25,4 +26,5 @@ __all__ = [
     "part_weight_XrayBand", "AbstractSPHKernel", "Cubic", "Quintic", "WendlandC2",
     "WendlandC4", "WendlandC6", "WendlandC8", "Context", "LibraryError",
     "default_context", "version", "healpix_map", "filter_sort_particles", "BoundsError",
+    "tsc_mapping",
 ]

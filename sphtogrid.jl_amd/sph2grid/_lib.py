@@ -32,7 +32,7 @@ S2G_FLAG_REDUCE = 0x8
 EXPORTS = (
     "s2g_create", "s2g_destroy", "s2g_last_error", "s2g_version",
     "s2g_deposit", "s2g_deposit_device", "s2g_reduce", "s2g_reduce_device",
-    "s2g_map", "s2g_healpix_deposit", "s2g_tsc3d", "s2g_deposit_batch",
+    "s2g_map", "s2g_healpix_deposit", "s2g_tsc3d", "s2g_tsc3d_device", "s2g_deposit_batch",
 )
 
 
@@ -121,6 +121,9 @@ def load(path: str | None = None):
         L.s2g_tsc3d.restype = C.c_int
         L.s2g_tsc3d.argtypes = [vp, C.c_int, vp, vp, i64, C.POINTER(Params), C.c_uint32, vp,
                                 C.POINTER(Stats)]
+        L.s2g_tsc3d_device.restype = C.c_int
+        L.s2g_tsc3d_device.argtypes = [vp, C.c_int, vp, vp, i64, C.POINTER(Params), C.c_uint32,
+                                       vp, vp]
         L.s2g_deposit_batch.restype = C.c_int
         L.s2g_deposit_batch.argtypes = [vp, C.c_int, C.POINTER(MapDesc), C.c_int, vp,
                                         C.POINTER(Stats)]
