@@ -114,6 +114,202 @@ def cpu_baseline(p, target_s, threads):
             "sample": sample}
 
 
+def _time_steps(fn, torch, steps=3, warmup=1):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def _errs(gpu, ref):
+    gpu = np.asarray(gpu, dtype=np.float64).ravel(order="F")
+    ref = np.asarray(ref, dtype=np.float64).ravel(order="F")
+    den = np.abs(ref).sum()
+    m = np.abs(ref) > 1e-3 * np.abs(ref).max() if ref.size else np.zeros(0, bool)
+    return {"rel_l1": float(np.abs(gpu - ref).sum() / den) if den else 0.0,
+            "max_pixel_rel": float((np.abs(gpu - ref)[m] / np.abs(ref)[m]).max()) if m.any() else 0.0}
+
+
+def extras(args, p, dev_arrays, torch, s2g, _lib):
+    """Other workloads and oracle comparisons (reported beside the headline)."""
+    import oracle
+    from sph2grid import workloads
+    out = {}
+    ctx = s2g.default_context()
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dev = torch.device("cuda:0")
+    st = _lib.Stats()
+
+    # --- cluster 3D: the same resident particles on a 256^3 grid (WendlandC6)
+    dx, dh, dm, drho, dw, dbq = dev_arrays
+    n = p["x"].shape[0]
+    N3 = 256
+    img3 = torch.empty(2 * N3 ** 3, dtype=torch.float64, device=dev)
+    map3 = torch.empty(N3 ** 3, dtype=torch.float64, device=dev)
+    par3 = _lib.Params(npixels=N3, len2pix=float(N3))
+    dt = _lib.S2G_F32 if p["x"].dtype == np.float32 else _lib.S2G_F64
+
+    def step3():
+        ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 3, dt, vp(dx), vp(dh), vp(dm), vp(drho),
+                                             vp(dbq), 1, vp(dw), n, C.byref(par3),
+                                             KIDS[p["kernel"]], 3, 0, vp(img3), stream,
+                                             C.byref(st)), "3D deposit")
+        ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 3, N3, 1, _lib.S2G_FLAG_REDUCE, vp(img3),
+                                            vp(map3), stream), "3D reduce")
+    ms3 = _time_steps(step3, torch)
+    w3 = float(img3.view(2, -1)[1].sum().item())
+    x64 = {k: p[k].astype(np.float64) for k in ("hsml", "m", "rho", "w")}
+    w3_exp = float((x64["w"] * x64["m"] / x64["rho"]).sum() * N3 ** 4)
+    out["cluster_3d_256"] = {"ms_per_step": ms3, "particles_per_s": n / (ms3 / 1e3),
+                             "stage_ms": {"gather": st.ms_gather, "norm": st.ms_norm,
+                                          "small": st.ms_small, "sort": st.ms_sort},
+                             "counts": {"small": st.n_small, "large": st.n_large,
+                                        "exact": st.n_large_exact, "pairs": st.n_pairs},
+                             "conservation_rel_err": abs(w3 / w3_exp - 1)}
+    log(f"extra cluster 3D: {ms3:.1f} ms")
+
+    # --- cluster 2D subsample against the FP64 oracle (same distribution, 2^18)
+    if not args.no_oracle:
+        d = workloads.cluster(n=1 << 18, npix=4096)
+        ps = prepare(d, s2g)
+        gpu = s2g.deposit(ps["x"], ps["hsml"], ps["m"], ps["rho"], ps["bq"], ps["w"], npix=4096,
+                          len2pix=ps["len2pix"], kernel=s2g.WendlandC6(2))
+        ref = oracle.cic_mapping_2D(ps["x"], ps["hsml"], ps["m"], ps["rho"], ps["bq"], ps["w"],
+                                    4096, ps["len2pix"], "wendlandc6", 2, False,
+                                    nworkers=min(16, os.cpu_count() or 1))
+        e = _errs(oracle.reduce_image_2D(gpu, 4096, True), oracle.reduce_image_2D(ref, 4096, True))
+        e["conservation_rel_err"] = abs(gpu[:, 1].sum() / ref[:, 1].sum() - 1)
+        out["cluster_2d_4096_subsample_2e18_vs_oracle"] = e
+        log(f"extra cluster 2D oracle check: {e}")
+
+    # --- plain workload (65,536 particles, 256^2, Cubic, Float64)
+    d = workloads.plain()
+    pp = prepare(d, s2g)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    arr = [T(pp[k]) for k in ("x", "hsml", "m", "rho", "w", "bq")]
+    img = torch.empty(256 * 256 * 2, dtype=torch.float64, device=dev)
+    mp = torch.empty(256 * 256, dtype=torch.float64, device=dev)
+    parp = _lib.Params(npixels=256, len2pix=pp["len2pix"])
+
+    def stepp():
+        ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F64, vp(arr[0]), vp(arr[1]),
+                                             vp(arr[2]), vp(arr[3]), vp(arr[5]), 1, vp(arr[4]),
+                                             pp["x"].shape[0], C.byref(parp), 0, 2, 0, vp(img),
+                                             stream, C.byref(st)), "plain")
+        ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, 256, 1, _lib.S2G_FLAG_REDUCE,
+                                            vp(img), vp(mp), stream), "plain reduce")
+    msp = _time_steps(stepp, torch, steps=10, warmup=2)
+    rec = {"ms_per_step": msp, "particles_per_s": pp["x"].shape[0] / (msp / 1e3)}
+    if not args.no_oracle:
+        t0 = time.perf_counter()
+        ref = oracle.cic_mapping_2D(pp["x"], pp["hsml"], pp["m"], pp["rho"], pp["bq"], pp["w"],
+                                    256, pp["len2pix"], "cubic", 2, False)
+        rec["oracle_serial_s"] = time.perf_counter() - t0
+        g = img.view(2, -1).T.cpu().numpy()
+        rec.update(_errs(oracle.reduce_image_2D(np.asfortranarray(g), 256, True),
+                         oracle.reduce_image_2D(ref, 256, True)))
+        rec["conservation_rel_err"] = abs(g[:, 1].sum() / ref[:, 1].sum() - 1)
+    out["plain_65536_256"] = rec
+    log(f"extra plain: {rec}")
+
+    # --- sedov-like (2^20 particles, 1024^2, WendlandC6, [rho; T], calc_mean)
+    d = workloads.sedov()
+    ps = prepare(d, s2g)
+    ns = ps["x"].shape[0]
+    arr = [T(ps[k]) for k in ("x", "hsml", "m", "rho", "w", "bq")]
+    img = torch.empty(1024 * 1024 * 3, dtype=torch.float64, device=dev)
+    mp = torch.empty(1024 * 1024 * 2, dtype=torch.float64, device=dev)
+    pars = _lib.Params(npixels=1024, len2pix=ps["len2pix"])
+
+    def steps():
+        ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F32, vp(arr[0]), vp(arr[1]),
+                                             vp(arr[2]), vp(arr[3]), vp(arr[5]), 2, vp(arr[4]),
+                                             ns, C.byref(pars), 4, 2, _lib.S2G_FLAG_CALC_MEAN,
+                                             vp(img), stream, C.byref(st)), "sedov")
+        ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, 1024, 2, _lib.S2G_FLAG_REDUCE,
+                                            vp(img), vp(mp), stream), "sedov reduce")
+    mss = _time_steps(steps, torch)
+    rec = {"ms_per_step": mss, "particles_per_s": ns / (mss / 1e3),
+           "stage_ms": {"gather": st.ms_gather, "small": st.ms_small, "sort": st.ms_sort}}
+    if not args.no_oracle:
+        ref = oracle.cic_mapping_2D(ps["x"], ps["hsml"], ps["m"], ps["rho"], ps["bq"], ps["w"],
+                                    1024, ps["len2pix"], "wendlandc6", 2, True,
+                                    nworkers=min(16, os.cpu_count() or 1))
+        g = img.view(3, -1).T.cpu().numpy()
+        rec.update(_errs(oracle.reduce_image_2D(np.asfortranarray(g), 1024, True),
+                         oracle.reduce_image_2D(ref, 1024, True)))
+        rec["conservation_rel_err"] = abs(g[:, 2].sum() / ref[:, 2].sum() - 1)
+    out["sedov_2e20_1024"] = rec
+    log(f"extra sedov: {rec}")
+
+    # --- all-sky: 2^22 particles, Nside 1024, WendlandC4: rho (Weights = 1)
+    #     and T (Weights = Rho), plus TSC 512^3 of the same particles
+    d = workloads.allsky()
+    na = d["HSML"].shape[0]
+    nside = d["Nside"]
+    npx = 12 * nside * nside
+    apos = T(np.ascontiguousarray(d["Pos"].T))
+    ah, am, arho, aT = (T(d[k]) for k in ("HSML", "M", "Rho", "T"))
+    ones = torch.ones(na, dtype=torch.float64, device=dev)
+    maps = torch.empty(4 * npx, dtype=torch.float64, device=dev)
+    st2 = _lib.Stats()
+
+    def stepa():
+        ctx.check(ctx.lib.s2g_healpix_deposit_device(
+            ctx.handle, _lib.S2G_F64, vp(apos), vp(ah), vp(am), vp(arho), vp(arho), vp(ones), na,
+            nside, 3, 2, _lib.S2G_FLAG_CALC_MEAN, C.c_void_p(maps.data_ptr()),
+            C.c_void_p(maps.data_ptr() + 8 * npx), stream, C.byref(st)), "allsky rho")
+        ctx.check(ctx.lib.s2g_healpix_deposit_device(
+            ctx.handle, _lib.S2G_F64, vp(apos), vp(ah), vp(am), vp(arho), vp(aT), vp(arho), na,
+            nside, 3, 2, _lib.S2G_FLAG_CALC_MEAN, C.c_void_p(maps.data_ptr() + 16 * npx),
+            C.c_void_p(maps.data_ptr() + 24 * npx), stream, C.byref(st2)), "allsky T")
+    msa = _time_steps(stepa, torch)
+    mp = maps.view(4, -1)
+    pos64 = d["Pos"].T
+    dxa = np.linalg.norm(pos64, axis=1)
+    ang = np.sqrt(4 * np.pi / npx)
+    keep = dxa >= d["HSML"]
+    wexp = float((d["M"] / d["Rho"] / (ang * dxa) ** 2)[keep].sum())
+    rec = {"ms_per_step_two_maps": msa, "particles_per_s": 2 * na / (msa / 1e3),
+           "stage_ms_T": {"norm": st2.ms_norm, "deposit": st2.ms_small, "setup": st2.ms_setup},
+           "units": st2.n_pairs,
+           "conservation_rel_err_rho_weights": abs(float(mp[1].sum().item()) / wexp - 1)}
+    if not args.no_oracle:
+        sub = np.random.default_rng(7).choice(na, 1 << 16, replace=False)
+        gm, gw = s2g.healpix_map(d["Pos"][:, sub].copy(), d["HSML"][sub], d["M"][sub],
+                                 d["Rho"][sub], d["T"][sub], d["Rho"][sub], Nside=nside,
+                                 kernel=s2g.WendlandC4(2), show_progress=False)
+        om, ow = oracle.healpix_deposit_parallel(pos64[sub], d["HSML"][sub], d["M"][sub],
+                                                 d["Rho"][sub], d["T"][sub], d["Rho"][sub], nside,
+                                                 "wendlandc4", 2, True,
+                                                 nworkers=min(16, os.cpu_count() or 1))
+        e = _errs(gm, om)
+        e["weights_rel_l1"] = _errs(gw, ow)["rel_l1"]
+        e["conservation_rel_err"] = abs(gw.sum() / ow.sum() - 1)
+        rec["subsample_2e16_vs_oracle"] = e
+    out["allsky_2e22_nside1024"] = rec
+    log(f"extra allsky: {rec}")
+
+    N5 = 512
+    grid = torch.empty(2 * N5 ** 3, dtype=torch.float64, device=dev)
+    part = _lib.Params(npixels=N5, len2pix=N5 / 2.0)
+
+    def stept():
+        ctx.check(ctx.lib.s2g_tsc3d_device(ctx.handle, _lib.S2G_F64, vp(apos), vp(aT), na,
+                                           C.byref(part), 0, vp(grid), stream), "tsc")
+    mst = _time_steps(stept, torch)
+    wsum = float(grid.view(2, -1)[1].sum().item())
+    out["tsc_2e22_512"] = {"ms_per_step": mst, "particles_per_s": na / (mst / 1e3),
+                           "weight_sum_over_n": wsum / na}
+    log(f"extra tsc: {out['tsc_2e22_512']}")
+    return out
+
+
 def measured_traffic(args, n, npix):
     """HBM bytes per launch of k2d_gather for the default cluster workload, as
     measured by the two rocprofv3 --pmc passes summarised under profiles/
@@ -141,6 +337,10 @@ def main():
     ap.add_argument("--exact-norm", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other workloads (3D, all-sky, TSC, plain, sedov)")
+    ap.add_argument("--no-oracle", action="store_true",
+                    help="skip the oracle comparisons inside the extras")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -288,6 +488,8 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
+    if not args.no_extras and args.workload == "cluster":
+        out["extra"] = extras(args, p, (dx, dh, dm, drho, dw, dbq), torch, s2g, _lib)
     print(json.dumps(out), flush=True)
 
 

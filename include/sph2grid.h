@@ -167,6 +167,14 @@ int s2g_healpix_deposit(s2g_ctx *ctx, int dtype, const void *pos,
                         int64_t nside, int kernel, int kernel_dim,
                         uint32_t flags, double *map_out, double *weight_out,
                         s2g_stats *stats);
+/* Same on device pointers (maps zeroed and filled on `stream`). */
+int s2g_healpix_deposit_device(s2g_ctx *ctx, int dtype, const void *d_pos,
+                               const void *d_hsml, const void *d_m,
+                               const void *d_rho, const void *d_bin_q,
+                               const void *d_weights, int64_t n, int64_t nside,
+                               int kernel, int kernel_dim, uint32_t flags,
+                               double *d_map_out, double *d_weight_out,
+                               void *stream, s2g_stats *stats);
 
 /* Triangular-shaped-cloud deposit onto an npix^3 grid (positions centred as
  * for the 3D deposit, grid units x*len2pix + npix/2, cell c centred at c+0.5).

@@ -710,6 +710,70 @@ int s2g_map(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml, 
     });
 }
 
+// all-sky deposit on device pointers into d_map / d_wmap (zeroed here)
+static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d_wmap,
+                           hipStream_t s, s2g_stats *st)
+{
+    const int64_t n = a.n, nside = a.nside;
+    const int64_t npx = 12 * nside * nside;
+    hipEvent_t *E = c->stage;
+    S2G_HIP(hipEventRecord(E[0], s));
+    S2G_HIP(hipMemsetAsync(d_map, 0, sizeof(double) * npx, s));
+    S2G_HIP(hipMemsetAsync(d_wmap, 0, sizeof(double) * npx, s));
+    if (n > 0) {
+        HpRec *rec = (HpRec *)c->hp_rec.get(sizeof(HpRec) * n);
+        int64_t *nun = (int64_t *)c->hp_nunits.get(sizeof(int64_t) * (n + 1));
+        int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(n + 1));
+        launch_hp_setup(a, rec, nun, s);
+        exclusive_scan_i64(nun, n + 1, tmp, s);
+        S2G_HIP(hipEventRecord(E[1], s));
+        int64_t nu;
+        read_i64(c, s, {nun + n}, &nu);
+        if (nu >= (int64_t)INT32_MAX)
+            throw Error(S2G_EUNSUPPORTED, "too many all-sky work units");
+        int2 *units = (int2 *)c->hp_units.get(sizeof(int2) * (nu ? nu : 1));
+        double4 *part = (double4 *)c->hp_part.get(sizeof(double4) * (nu ? nu : 1));
+        launch_hp_units(nun, n, rec, units, s);
+        S2G_HIP(hipEventRecord(E[2], s));
+        launch_hp_norm(a.kernel, units, nu, rec, nside, part, s);
+        launch_hp_finish(nun, n, part, rec, s);
+        S2G_HIP(hipEventRecord(E[3], s));
+        launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
+        S2G_HIP(hipGetLastError());
+        if (st)
+            st->n_pairs = nu;
+    }
+    S2G_HIP(hipEventRecord(E[4], s));
+    S2G_HIP(hipEventSynchronize(E[4]));
+    if (st) {
+        st->ms_device = elapsed(E[0], E[4]);
+        if (n > 0) {
+            st->ms_setup = elapsed(E[0], E[1]);
+            st->ms_norm = elapsed(E[2], E[3]);
+            st->ms_small = elapsed(E[3], E[4]);
+        }
+    }
+}
+
+static HpArgs hp_args(int dtype, int kernel, const void *pos, const void *hsml, const void *m,
+                      const void *rho, const void *bin_q, const void *w, int64_t n,
+                      int64_t nside, uint32_t flags)
+{
+    HpArgs a;
+    a.dtype = dtype;
+    a.kernel = kernel;
+    a.pos = pos;
+    a.hsml = hsml;
+    a.m = m;
+    a.rho = rho;
+    a.binq = bin_q;
+    a.w = w;
+    a.n = n;
+    a.nside = nside;
+    a.flags = flags;
+    return a;
+}
+
 int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml,
                         const void *m, const void *rho, const void *bin_q,
                         const void *weights, int64_t n, int64_t nside, int kernel,
@@ -730,62 +794,48 @@ int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml
         st->n_particles = n;
         Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, 1, weights, n, &dummy, kernel,
                                 flags);
+        double h2d = 0.0;
         {
             s2g_stats tmp;
             h = stage_inputs(c, h, s, &tmp);
-            st->ms_h2d = tmp.ms_h2d;
+            h2d = tmp.ms_h2d;
         }
         const int64_t npx = 12 * nside * nside;
         double *maps = (double *)c->image.get(sizeof(double) * 2 * npx);
-        hipEvent_t *E = c->stage;
-        S2G_HIP(hipEventRecord(E[0], s));
-        S2G_HIP(hipMemsetAsync(maps, 0, sizeof(double) * 2 * npx, s));
-        if (n > 0) {
-            HpArgs a;
-            a.dtype = dtype;
-            a.kernel = kernel;
-            a.pos = h.pos;
-            a.hsml = h.hsml;
-            a.m = h.m;
-            a.rho = h.rho;
-            a.binq = h.binq;
-            a.w = h.w;
-            a.n = n;
-            a.nside = nside;
-            a.flags = flags;
-            HpRec *rec = (HpRec *)c->hp_rec.get(sizeof(HpRec) * n);
-            int64_t *nun = (int64_t *)c->hp_nunits.get(sizeof(int64_t) * (n + 1));
-            int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(n + 1));
-            launch_hp_setup(a, rec, nun, s);
-            exclusive_scan_i64(nun, n + 1, tmp, s);
-            S2G_HIP(hipEventRecord(E[1], s));
-            int64_t nu;
-            read_i64(c, s, {nun + n}, &nu);
-            if (nu >= (int64_t)INT32_MAX)
-                throw Error(S2G_EUNSUPPORTED, "too many all-sky work units");
-            int2 *units = (int2 *)c->hp_units.get(sizeof(int2) * (nu ? nu : 1));
-            double4 *part = (double4 *)c->hp_part.get(sizeof(double4) * (nu ? nu : 1));
-            launch_hp_units(nun, n, rec, units, s);
-            S2G_HIP(hipEventRecord(E[2], s));
-            launch_hp_norm(kernel, units, nu, rec, nside, part, s);
-            launch_hp_finish(nun, n, part, rec, s);
-            S2G_HIP(hipEventRecord(E[3], s));
-            launch_hp_deposit(a, units, nu, rec, maps + npx, maps, s);
-            S2G_HIP(hipGetLastError());
-            st->n_pairs = nu;
-        }
-        S2G_HIP(hipEventRecord(E[4], s));
-        S2G_HIP(hipEventSynchronize(E[4]));
-        st->ms_device = elapsed(E[0], E[4]);
-        if (n > 0) {
-            st->ms_setup = elapsed(E[0], E[1]);
-            st->ms_norm = elapsed(E[2], E[3]);
-            st->ms_small = elapsed(E[3], E[4]);
-        }
+        healpix_device(c, hp_args(dtype, kernel, h.pos, h.hsml, h.m, h.rho, h.binq, h.w, n,
+                                  nside, flags),
+                       maps + npx, maps, s, st);
+        st->n_particles = n;
+        st->ms_h2d = h2d;
         double t1 = 0.0, t2 = 0.0;
         d2h(c, map_out, maps + npx, sizeof(double) * npx, s, &t1);
         d2h(c, weight_out, maps, sizeof(double) * npx, s, &t2);
         st->ms_d2h = t1 + t2;
+        return S2G_OK;
+    });
+}
+
+int s2g_healpix_deposit_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_hsml,
+                               const void *d_m, const void *d_rho, const void *d_bin_q,
+                               const void *d_weights, int64_t n, int64_t nside, int kernel,
+                               int kernel_dim, uint32_t flags, double *d_map_out,
+                               double *d_weight_out, void *stream, s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        s2g_params dummy{1, 1.0};
+        check_common(dtype, kernel, kernel_dim, n, &dummy);
+        if (nside < 1 || nside > 8192)
+            throw Error(S2G_EINVAL, "nside out of range [1, 8192]");
+        if (!d_map_out || !d_weight_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->n_particles = n;
+        }
+        healpix_device(c, hp_args(dtype, kernel, d_pos, d_hsml, d_m, d_rho, d_bin_q, d_weights, n,
+                                  nside, flags),
+                       d_map_out, d_weight_out, s, stats);
         return S2G_OK;
     });
 }

@@ -32,7 +32,7 @@ S2G_FLAG_REDUCE = 0x8
 EXPORTS = (
     "s2g_create", "s2g_destroy", "s2g_last_error", "s2g_version",
     "s2g_deposit", "s2g_deposit_device", "s2g_reduce", "s2g_reduce_device",
-    "s2g_map", "s2g_healpix_deposit", "s2g_tsc3d", "s2g_tsc3d_device", "s2g_deposit_batch",
+    "s2g_map", "s2g_healpix_deposit", "s2g_healpix_deposit_device", "s2g_tsc3d", "s2g_tsc3d_device", "s2g_deposit_batch",
 )
 
 
@@ -118,6 +118,10 @@ def load(path: str | None = None):
         L.s2g_healpix_deposit.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, i64, i64,
                                           C.c_int, C.c_int, C.c_uint32, vp, vp,
                                           C.POINTER(Stats)]
+        L.s2g_healpix_deposit_device.restype = C.c_int
+        L.s2g_healpix_deposit_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, i64, i64,
+                                                 C.c_int, C.c_int, C.c_uint32, vp, vp, vp,
+                                                 C.POINTER(Stats)]
         L.s2g_tsc3d.restype = C.c_int
         L.s2g_tsc3d.argtypes = [vp, C.c_int, vp, vp, i64, C.POINTER(Params), C.c_uint32, vp,
                                 C.POINTER(Stats)]
