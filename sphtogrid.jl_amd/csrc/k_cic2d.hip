@@ -158,13 +158,10 @@ constexpr int kNormThreads = 256;
 // v_readlane; lanes run over the columns.  Per-lane FP64 sums in a fixed
 // order and a fixed butterfly make the result bit-reproducible.
 template <int K>
-__global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
-                                                         int64_t ne, Rec2 *__restrict__ rec)
+__device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restrict__ list, int64_t ne,
+                                             Rec2 *__restrict__ rec)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * kNormThreads + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= ne)
-        return;
     const int p = list[gw];
     const Rec2 r = rec[p];
     const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
@@ -194,6 +191,17 @@ __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restri
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
 }
+
+template <int K>
+__global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
+                                                         int64_t ne, Rec2 *__restrict__ rec)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
+         gw += nw)
+        k2d_norm_one<K>(gw, list, ne, rec);
+}
+
 
 // ---------------------------------------------------------------------------
 // Tile gather.  Block = 4 waves = one 64x64 tile.  Wave w owns the 16-column
@@ -346,17 +354,10 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
 // wave-reduced normalisation, FP64 atomics into the image.
 // ---------------------------------------------------------------------------
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list,
-                                                        int64_t ns,
-                                                        const Rec2 *__restrict__ rec,
-                                                        const T *__restrict__ binq, int nq,
-                                                        int64_t npix,
-                                                        double *__restrict__ image)
+__device__ __forceinline__ void k2d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list, int64_t ns, const Rec2 *__restrict__ rec,
+        const T *__restrict__ binq, int nq, int64_t npix, double *__restrict__ image)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= ns)
-        return;
     const int p = list[gw];
     const Rec2 r = rec[p];
     const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
@@ -396,6 +397,16 @@ __global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restric
         }
     }
 }
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list, int64_t ns, const Rec2 *__restrict__ rec,
+        const T *__restrict__ binq, int nq, int64_t npix, double *__restrict__ image)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ns;
+         gw += nw)
+        k2d_small_atomic_one<T, K>(gw, list, ns, rec, binq, nq, npix, image);
+}
+
 
 // ---------------------------------------------------------------------------
 // reduce_image_2D (reduce_image.jl:8-31): out[q][y][x] = img[q][x*N+y] / w.
@@ -466,7 +477,7 @@ void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s)
 template <int K>
 static void launch_norm_k(const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
 {
-    const int64_t blocks = (ne * kWave + kNormThreads - 1) / kNormThreads;
+    const int64_t blocks = wave_blocks(ne, kNormThreads);
     hipLaunchKernelGGL(k2d_norm<K>, dim3((unsigned)blocks), dim3(kNormThreads), 0, s, list, ne,
                        rec);
 }
@@ -539,8 +550,7 @@ void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
 template <typename T, int K>
 static void small_tk(const SmallArgs &a, hipStream_t s)
 {
-    const int64_t threads = a.ns * kWave;
-    hipLaunchKernelGGL((k2d_small_atomic<T, K>), dim3((unsigned)((threads + 255) / 256)),
+    hipLaunchKernelGGL((k2d_small_atomic<T, K>), dim3((unsigned)wave_blocks(a.ns, 256)),
                        dim3(256), 0, s, a.list, a.ns, a.rec, (const T *)a.binq, a.nq, a.npix,
                        a.image);
 }

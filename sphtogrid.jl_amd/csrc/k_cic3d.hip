@@ -136,13 +136,9 @@ __device__ __forceinline__ float vox_factor(float r2xy, float zd2, float hinv, f
 // broadcast by v_readlane.  Fixed per-lane order + butterfly: reproducible.
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list, int64_t ne,
-                                                Rec3 *__restrict__ rec)
+__device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restrict__ list, int64_t ne, Rec3 *__restrict__ rec)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= ne)
-        return;
     const int p = list[gw];
     const Rec3 r = rec[p];
     const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
@@ -179,6 +175,15 @@ __global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
 }
+template <int K>
+__global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list, int64_t ne, Rec3 *__restrict__ rec)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
+         gw += nw)
+        k3d_norm_one<K>(gw, list, ne, rec);
+}
+
 
 // ---------------------------------------------------------------------------
 // Brick gather.  Block = 4 waves = one 16^3 brick (4096 voxels); voxel
@@ -325,17 +330,10 @@ __global__ __launch_bounds__(256) void k3d_finalize(const double *__restrict__ p
 // wave-reduced normalisation, FP64 atomics (cic_3D.jl:158-188).
 // ---------------------------------------------------------------------------
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k3d_small_atomic(const int32_t *__restrict__ list,
-                                                        int64_t ns,
-                                                        const Rec3 *__restrict__ rec,
-                                                        const T *__restrict__ binq,
-                                                        int64_t npix,
-                                                        double *__restrict__ image)
+__device__ __forceinline__ void k3d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list, int64_t ns, const Rec3 *__restrict__ rec,
+        const T *__restrict__ binq, int64_t npix, double *__restrict__ image)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= ns)
-        return;
     const int p = list[gw];
     const Rec3 r = rec[p];
     const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1, kw = r.k1 - r.k0 + 1;
@@ -377,6 +375,16 @@ __global__ __launch_bounds__(256) void k3d_small_atomic(const int32_t *__restric
         }
     }
 }
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k3d_small_atomic(const int32_t *__restrict__ list, int64_t ns, const Rec3 *__restrict__ rec,
+        const T *__restrict__ binq, int64_t npix, double *__restrict__ image)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ns;
+         gw += nw)
+        k3d_small_atomic_one<T, K>(gw, list, ns, rec, binq, npix, image);
+}
+
 
 // reduce_image_3D (reduce_image.jl:39-55): out = q / w where q > 0; without
 // reduce_image the weights are 1 (cic_interpolation.jl:230-232)
@@ -426,7 +434,7 @@ void launch_3d_norm(int kernel, const int32_t *list, int64_t ne, Rec3 *rec, hipS
 {
     if (ne == 0)
         return;
-    const dim3 g((unsigned)((ne * kWave + 255) / 256)), blk(256);
+    const dim3 g((unsigned)wave_blocks(ne, 256)), blk(256);
     S2G_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k3d_norm<KK>, g, blk, 0, s, list, ne, rec));
 }
 
@@ -465,7 +473,7 @@ void launch_3d_finalize(const Gather3Args &g, hipStream_t s)
 template <typename T>
 static void small3_t(const Small3Args &a, hipStream_t s)
 {
-    const dim3 g((unsigned)((a.ns * kWave + 255) / 256)), blk(256);
+    const dim3 g((unsigned)wave_blocks(a.ns, 256)), blk(256);
     S2G_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k3d_small_atomic<T, KK>), g, blk, 0, s,
                                                    a.list, a.ns, a.rec, (const T *)a.binq,
                                                    a.npix, a.image));

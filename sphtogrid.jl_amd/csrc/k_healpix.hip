@@ -19,7 +19,7 @@
 namespace s2g {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kHpRingsPerUnit = 4;
+constexpr int kHpRingsPerUnit = 64;  // one ring per lane in the interval phase
 
 // ---------------------------------------------------------------------------
 // RING-scheme geometry (Gorski et al. 2005), FP64
@@ -302,7 +302,11 @@ __device__ __forceinline__ void hp_weights(float d, float theta, float inv_theta
     A = fmaxf(0.0f, fminf(ang, a)) * inv_ang;
 }
 
-// Iterate the pixels of unit `ui` of particle r: rings [ra + 4*ui, ...].
+// Iterate the pixels of unit `ui` of particle r: rings [ra + 64*ui, ...].
+// Phase 1: lane l computes the pixel interval and the terms of ring
+// ra + 64*ui + l.  Phase 2: the pixels are spread over sub-groups of
+// G = nextpow2(longest interval) lanes, one ring per sub-group, 64/G rings at
+// a time, so short rings (small discs) still fill the wave.
 // F(pix, W, A) is called per pixel on every lane that holds a pixel.
 template <int K, typename F>
 __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, int ui, int ln,
@@ -313,41 +317,85 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
     const double cosr = cos(r.theta);
     const int64_t r0 = (int64_t)r.ra + (int64_t)ui * kHpRingsPerUnit;
     const int64_t r1 = r0 + kHpRingsPerUnit - 1 < r.rb ? r0 + kHpRingsPerUnit - 1 : r.rb;
-    bool covered_cp = false;
+    const int nrings = r1 >= r0 ? (int)(r1 - r0 + 1) : 0;
     int64_t cring = 0, ck = 0;
     hp_pix2ring(nside, r.cpix, cring, ck);
-    for (int64_t iz = r0; iz <= r1; ++iz) {
-        int64_t lo, hi;
+
+    // phase 1: one ring per lane
+    int lo = 0, len = 0, start = 0, nr = 1;
+    float s1 = 0.0f, s2 = 0.0f;
+    double dphi = 0.0, phase = 0.0;
+    bool cov = false;
+    if (ln < nrings) {
+        const int64_t iz = r0 + ln;
+        int64_t a = 0, b = -1;
         bool any;
-        if (iz < r.irmin || iz > r.irmax) {
-            // a ring fully inside a polar cap
-            int64_t st, nr;
+        if (iz < r.irmin || iz > r.irmax) {  // a ring fully inside a polar cap
+            int64_t st, n_r;
             double sh;
-            hp_ring_info(nside, iz, st, nr, sh);
-            lo = 0;
-            hi = nr - 1;
+            hp_ring_info(nside, iz, st, n_r, sh);
+            a = 0;
+            b = n_r - 1;
             any = true;
         } else {
-            any = hp_ring_interval(nside, iz, r.z0, r.xa, cosr, r.phi0, lo, hi);
+            any = hp_ring_interval(nside, iz, r.z0, r.xa, cosr, r.phi0, a, b);
         }
-        if (!any)
-            continue;
-        const RingTerms t = ring_terms(nside, iz, r);
-        if (iz == cring) {
-            const int64_t kk = ck;
-            if ((lo >= 0 && kk >= lo && kk <= hi) ||
-                (lo < 0 && (kk <= hi || kk >= lo + t.nr)))
-                covered_cp = true;
+        if (any) {
+            const RingTerms t = ring_terms(nside, iz, r);
+            lo = (int)a;
+            len = (int)(b - a + 1);
+            start = (int)t.start;
+            nr = (int)t.nr;
+            s1 = t.s1;
+            s2 = t.s2;
+            dphi = t.dphi_pix;
+            phase = t.phase;
+            if (iz == cring)
+                cov = (a >= 0 && ck >= a && ck <= b) || (a < 0 && (ck <= b || ck >= a + t.nr));
         }
-        const int64_t cnt = hi - lo + 1;
-        for (int64_t c = ln; c < cnt; c += kWave) {
-            int64_t k = lo + c;
-            if (k < 0)
-                k += t.nr;
-            const float d = hp_dist(t, k);
-            float W, A;
-            hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
-            f(t.start + k, W, A);
+    }
+    const bool covered_cp = __ballot(cov) != 0ull;
+    int maxlen = len;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(maxlen, o, kWave);
+        maxlen = v > maxlen ? v : maxlen;
+    }
+    // phase 2: sub-groups of G lanes per ring
+    int G = 1;
+    while (G < maxlen && G < kWave)
+        G <<= 1;
+    const int P = kWave / G;
+    const int sub = ln % G, slot = ln / G;
+    for (int g = 0; g < nrings; g += P) {
+        const int j = g + slot;
+        const int src = j < nrings ? j : 0;
+        const int jlo = __shfl(lo, src, kWave);
+        const int jlen = j < nrings ? __shfl(len, src, kWave) : 0;
+        const int jstart = __shfl(start, src, kWave);
+        const int jnr = __shfl(nr, src, kWave);
+        RingTerms t;
+        t.s1 = __shfl(s1, src, kWave);
+        t.s2 = __shfl(s2, src, kWave);
+        t.dphi_pix = __shfl(dphi, src, kWave);
+        t.phase = __shfl(phase, src, kWave);
+        t.start = jstart;
+        t.nr = jnr;
+        int glen = jlen;
+        for (int o = 32; o > 0; o >>= 1) {
+            const int v = __shfl_xor(glen, o, kWave);
+            glen = v > glen ? v : glen;
+        }
+        for (int c0 = 0; c0 < glen; c0 += G) {
+            const int c = c0 + sub;
+            if (c < jlen) {
+                int k = jlo + c;
+                if (k < 0)
+                    k += jnr;
+                const float d = hp_dist(t, k);
+                float W, A;
+                hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+                f((int64_t)jstart + k, W, A);
+            }
         }
     }
     // constributing_pixels.jl:16-19: the pixel containing the particle is
@@ -365,14 +413,10 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units, int64_t nu,
-                                                 const HpRec *__restrict__ rec, int64_t nside,
-                                                 double4 *__restrict__ part)
+__device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
+        int64_t nside, double4 *__restrict__ part)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= nu)
-        return;
     const int2 u = units[gw];
     const HpRec r = rec[u.x];
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
@@ -388,6 +432,16 @@ __global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units,
     if (ln == 0)
         part[gw] = make_double4(swa, sa, nin, 0.0);
 }
+template <int K>
+__global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
+        int64_t nside, double4 *__restrict__ part)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
+         gw += nw)
+        k_hp_norm_one<K>(gw, units, nu, rec, nside, part);
+}
+
 
 // ordered sum of a particle's unit partials -> final scale and fallback
 __global__ void k_hp_finish(const int64_t *__restrict__ ubase, int64_t n,
@@ -417,16 +471,11 @@ __global__ void k_hp_finish(const int64_t *__restrict__ ubase, int64_t n,
 }
 
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu,
-                                                    const HpRec *__restrict__ rec,
-                                                    const T *__restrict__ binq, int64_t nside,
-                                                    double *__restrict__ map,
-                                                    double *__restrict__ wmap)
+__device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
+        const T *__restrict__ binq, int64_t nside, double *__restrict__ map,
+        double *__restrict__ wmap)
 {
-    const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave;
     const int ln = threadIdx.x % kWave;
-    if (gw >= nu)
-        return;
     const int2 u = units[gw];
     const HpRec r = rec[u.x];
     if (r.s == 0.0)
@@ -442,6 +491,17 @@ __global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ uni
         }
     });
 }
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
+        const T *__restrict__ binq, int64_t nside, double *__restrict__ map,
+        double *__restrict__ wmap)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
+         gw += nw)
+        k_hp_deposit_one<T, K>(gw, units, nu, rec, binq, nside, map, wmap);
+}
+
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -482,7 +542,7 @@ void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec,
 {
     if (nu == 0)
         return;
-    const dim3 g((unsigned)((nu * kWave + 255) / 256)), blk(256);
+    const dim3 g((unsigned)wave_blocks(nu, 256)), blk(256);
     S2G_HP_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k_hp_norm<KK>, g, blk, 0, s, units, nu, rec,
                                                     nside, part));
 }
@@ -499,7 +559,7 @@ void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpR
 {
     if (nu == 0)
         return;
-    const dim3 g((unsigned)((nu * kWave + 255) / 256)), blk(256);
+    const dim3 g((unsigned)wave_blocks(nu, 256)), blk(256);
     if (a.dtype == S2G_F32) {
         S2G_HP_KERNEL_SWITCH(a.kernel,
                              hipLaunchKernelGGL((k_hp_deposit<float, KK>), g, blk, 0, s, units, nu,

@@ -184,6 +184,17 @@ __device__ __forceinline__ double ld(const T *p, int64_t i)
     return (double)p[i];
 }
 
+// Blocks for a one-wave-per-item kernel with a grid-stride loop.  The grid is
+// capped so that blocks * threads stays far below 2^32 work-items (the
+// dispatch grid size is 32-bit); larger counts loop.
+inline int64_t wave_blocks(int64_t items, int threads_per_block)
+{
+    const int64_t waves_per_block = threads_per_block / kWave;
+    int64_t b = (items + waves_per_block - 1) / waves_per_block;
+    const int64_t cap = 1 << 18;
+    return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int o = 32; o > 0; o >>= 1)
