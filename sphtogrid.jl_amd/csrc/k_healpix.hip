@@ -367,10 +367,14 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
     const int P = kWave / G;
     const int sub = ln % G, slot = ln / G;
     for (int g = 0; g < nrings; g += P) {
+        // every lane takes part in every shuffle: a cross-lane read placed
+        // under a lane condition reads 0 from source lanes that the
+        // condition switched off
         const int j = g + slot;
         const int src = j < nrings ? j : 0;
         const int jlo = __shfl(lo, src, kWave);
-        const int jlen = j < nrings ? __shfl(len, src, kWave) : 0;
+        const int jlen_src = __shfl(len, src, kWave);
+        const int jlen = j < nrings ? jlen_src : 0;
         const int jstart = __shfl(start, src, kWave);
         const int jnr = __shfl(nr, src, kWave);
         RingTerms t;
