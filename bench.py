@@ -307,7 +307,101 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
     out["tsc_2e22_512"] = {"ms_per_step": mst, "particles_per_s": na / (mst / 1e3),
                            "weight_sum_over_n": wsum / na}
     log(f"extra tsc: {out['tsc_2e22_512']}")
+    out["batched_64maps"] = batched_extra(args, torch, s2g, _lib)
     return out
+
+
+def batched_extra(args, torch, s2g, _lib):
+    """The batched workload: 64 sedov-like maps (WendlandC6, Nq = 3 with
+    Weights = Rho, calc_mean, reduce_image; N log-uniform [1e4, 4e6], sides
+    128-2048).  One step maps all 64: either one s2g_deposit_batch call, or
+    64 x (s2g_deposit_device + s2g_reduce_device), the reference's
+    one-sphMapping-per-map loop on device-resident inputs."""
+    import oracle
+    from sph2grid import workloads
+    ctx = s2g.default_context()
+    dev = torch.device("cuda:0")
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    t0 = time.perf_counter()
+    maps = workloads.batched(nmaps=64)
+    flags = _lib.S2G_FLAG_CALC_MEAN | _lib.S2G_FLAG_REDUCE
+    kid = KIDS["wendlandc6"]
+    res, fps = [], []
+    for d in maps:
+        pp = prepare(d, s2g)
+        npix, nq = pp["npix"], pp["nq"]
+        nd = npix * npix
+        wi, wj = footprints(pp["x"], pp["hsml"], npix, pp["len2pix"])
+        fps.append(float((wi * wj).sum()))
+        res.append(dict(pp=pp, t=[T(pp[k]) for k in ("x", "hsml", "m", "rho", "bq", "w")],
+                        n=pp["x"].shape[0], npix=npix, nq=nq,
+                        par=_lib.Params(npixels=npix, len2pix=pp["len2pix"]),
+                        img=torch.empty(nd * (nq + 1), dtype=torch.float64, device=dev),
+                        o_plain=torch.empty(nd * nq, dtype=torch.float64, device=dev),
+                        o_batch=torch.empty(nd * nq, dtype=torch.float64, device=dev)))
+    log(f"batched workload ready in {time.perf_counter() - t0:.1f} s")
+    ntot = sum(r["n"] for r in res)
+
+    def step_plain():
+        for r in res:
+            x, h, m, rho, bq, w = r["t"]
+            ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F32, vp(x), vp(h), vp(m),
+                                                 vp(rho), vp(bq), r["nq"], vp(w), r["n"],
+                                                 C.byref(r["par"]), kid, 2, flags, vp(r["img"]),
+                                                 stream, None), "deposit")
+            ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, r["npix"], r["nq"], flags,
+                                                vp(r["img"]), vp(r["o_plain"]), stream), "reduce")
+
+    descs = (_lib.MapDesc * len(res))()
+    for k, r in enumerate(res):
+        x, h, m, rho, bq, w = r["t"]
+        descs[k] = _lib.MapDesc(dims=2, nq=r["nq"], n=r["n"], params=r["par"], kernel=kid,
+                                flags=flags, d_pos=x.data_ptr(), d_hsml=h.data_ptr(),
+                                d_m=m.data_ptr(), d_rho=rho.data_ptr(), d_bin_q=bq.data_ptr(),
+                                d_weights=w.data_ptr(), d_out=r["o_batch"].data_ptr())
+    st = _lib.Stats()
+
+    def step_batch():
+        ctx.check(ctx.lib.s2g_deposit_batch(ctx.handle, _lib.S2G_F32, descs, len(res), stream,
+                                            C.byref(st)), "batch")
+
+    ms_plain = _time_steps(step_plain, torch, steps=3, warmup=1)
+    ms_batch = _time_steps(step_batch, torch, steps=3, warmup=1)
+    diff = 0.0
+    for r in res:
+        a, b = r["o_batch"], r["o_plain"]
+        diff = max(diff, float((a - b).abs().max().item()) / max(float(b.abs().max().item()),
+                                                                  1e-300))
+    rec = {"maps": len(res), "particles": ntot, "sum_footprint": sum(fps),
+           "plain_ms_per_step": ms_plain, "batched_ms_per_step": ms_batch,
+           "plain_maps_per_s": len(res) / (ms_plain / 1e3),
+           "batched_maps_per_s": len(res) / (ms_batch / 1e3),
+           "batched_particles_per_s": ntot / (ms_batch / 1e3),
+           "speedup_batched_vs_plain": ms_plain / ms_batch,
+           "batched_stage_ms": {"setup": st.ms_setup, "norm": st.ms_norm, "sort": st.ms_sort,
+                                "gather": st.ms_gather, "finalize": st.ms_finalize,
+                                "small": st.ms_small, "device": st.ms_device},
+           "counts": {"small": st.n_small, "large": st.n_large, "pairs": st.n_pairs},
+           "max_rel_diff_batched_vs_plain": diff}
+    if not args.no_oracle:
+        # the three maps with the fewest footprint pixels against the oracle
+        errs = {}
+        for k in np.argsort(fps)[:3]:
+            r = res[int(k)]
+            pp = r["pp"]
+            ref = oracle.cic_mapping_2D(pp["x"], pp["hsml"], pp["m"], pp["rho"], pp["bq"], pp["w"],
+                                        r["npix"], pp["len2pix"], "wendlandc6", 2, True,
+                                        nworkers=min(16, os.cpu_count() or 1))
+            ref = oracle.reduce_image_2D(ref, r["npix"], True)
+            got = r["o_batch"].cpu().numpy().reshape((r["npix"], r["npix"], r["nq"]), order="F")
+            errs[f"map{int(k)}_n{r['n']}_npix{r['npix']}"] = _errs(got, ref)
+        rec["oracle_checks"] = errs
+    log(f"extra batched: {rec}")
+    del res
+    torch.cuda.empty_cache()
+    return rec
 
 
 def measured_traffic(args, n, npix):

@@ -5,7 +5,25 @@
 
 namespace s2g {
 
+// One map of a 2D deposit pass.  A pass deposits any number of maps that
+// share the input dtype, the kernel and nq; their particles are concatenated
+// ([pbase, pbase + n) in pass order) and so are their tile grids.  The device
+// copy of this table is read by every stage.
+struct Map2D {
+    const void *pos, *hsml, *m, *rho, *binq, *w;
+    double *image;    // nq + 1 planes of npix^2, deposit order (cic_2D.jl:120-123)
+    double *out;      // reduced map (reduce_image_2D layout) or null
+    double len2pix;
+    int64_t npix;
+    int64_t n;
+    int64_t pbase;    // first particle of the map in the pass
+    int32_t tbase;    // first gather tile of the map in the pass
+    int32_t ntj;      // gather tiles per side
+    uint32_t flags;
+    int32_t pad;
+};
 
+// host-side arguments of one map (Map2D without the pass offsets)
 struct Map2DArgs {
     int dtype;
     const void *pos, *hsml, *m, *rho, *binq, *w;
@@ -17,32 +35,27 @@ struct Map2DArgs {
 };
 
 struct GatherArgs {
-    int dtype, kernel, np, plane0, nq, ntj;
-    int64_t ntiles, nitems, npix;
+    int dtype, kernel, np, plane0, nq, nmaps;
+    int64_t ntiles, nitems;
     const int4 *items;
     const int32_t *pairs, *large;
     const Rec2 *rec;
-    const void *binq;
+    const Map2D *maps;
     double *partial;
-    const int64_t *tile_items;
-    double *image;
+    const int64_t *tile_items;  // null: no large particle in the pass
 };
 
-struct SmallArgs {
-    int dtype, kernel, nq;
-    int64_t ns, npix;
-    const int32_t *list;
-    const Rec2 *rec;
-    const void *binq;
-    double *image;
-};
-
-// setup writes the per-particle records and the class array
-void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s);
+// setup over all particles of the pass: records and the class array
+void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int nq, int kernel,
+                     Rec2 *rec, int32_t *cls, hipStream_t s);
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s);
 void launch_2d_gather(const GatherArgs &g, hipStream_t s);
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s);
-void launch_2d_small(const SmallArgs &a, hipStream_t s);
+void launch_2d_small(int dtype, int kernel, int nq, const int32_t *list, int64_t ns,
+                     const Rec2 *rec, const Map2D *maps, hipStream_t s);
+// reduce_image_2D of every map of the pass that has an output
+void launch_2d_reduce_maps(const Map2D *maps, int nmaps, int ntiles, int nq, hipStream_t s);
+// reduce_image_2D of one flat image
 void launch_2d_reduce(const double *image, int64_t npix, int nq, int reduce, double *out,
                       hipStream_t s);
 

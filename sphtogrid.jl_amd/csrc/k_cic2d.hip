@@ -1,19 +1,21 @@
 // k_cic2d.hip -- the 2D Smac pixel-overlap deposit (cic_interpolation/cic_2D.jl)
 // on gfx950.
 //
-// Pipeline of one map (all on one stream):
+// Pipeline of one pass (all on one stream).  A pass deposits one map or a
+// batch of maps; the maps' particles and tiles are concatenated and every
+// stage reads the per-map table (Map2D) where it needs map data.
 //   k2d_setup        per particle, FP64: pixel coordinates, integer footprint,
 //                    zero-quantity skip, fallback decision, class
 //   k2d_partition    stable (particle-order) lists of small / large / exact
 //   k2d_norm         explicit FP64 normalisation sum of large clipped footprints
-//   k2d_pair_count   counting sort of (tile, particle) pairs: per-chunk counts
+//   k_pair_count     counting sort of (tile, particle) pairs: per-chunk counts
 //   (scan)           tile-major exclusive scan -> deterministic list offsets
-//   k2d_pair_fill    one wave per chunk fills the per-tile lists in particle order
-//   k2d_items        split every tile list into work items of <= G pairs
+//   k_pair_fill      one wave per chunk fills the per-tile lists in particle order
+//   k_item_*         split every tile list into work items of <= G pairs
 //   k2d_gather       per-tile gather, FP64 register accumulation per pixel
 //   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
 //   k2d_small_*      scatter of small footprints (atomics on top of the image)
-//   k2d_reduce       reduce_image_2D: divide by the weight plane + transpose
+//   k2d_reduce_maps  reduce_image_2D: divide by the weight plane + transpose
 #include "k_cic2d.h"
 
 namespace s2g {
@@ -24,31 +26,34 @@ namespace s2g {
 // calculate_weights (cic_2D.jl:49-69), all in FP64 without FMA contraction.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
-                                                 const T *__restrict__ hsml,
-                                                 const T *__restrict__ m,
-                                                 const T *__restrict__ rho,
-                                                 const T *__restrict__ binq, int nq,
-                                                 const T *__restrict__ w, int64_t n,
-                                                 int64_t npix, double len2pix,
-                                                 int kernel, uint32_t flags,
+__global__ __launch_bounds__(256) void k2d_setup(const Map2D *__restrict__ maps, int nmaps,
+                                                 int64_t ntot, int nq, int kernel,
                                                  Rec2 *__restrict__ rec,
                                                  int32_t *__restrict__ cls)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n)
+    if (p >= ntot)
         return;
+    const int mi = find_entry(maps, nmaps, p, [](const Map2D &e) { return e.pbase; });
+    const Map2D &md = maps[mi];
+    const int64_t pl = p - md.pbase;
+    const T *pos = (const T *)md.pos, *hsml = (const T *)md.hsml, *m = (const T *)md.m;
+    const T *rho = (const T *)md.rho, *binq = (const T *)md.binq, *w = (const T *)md.w;
+    const int64_t npix = md.npix;
+    const double len2pix = md.len2pix;
+    const uint32_t flags = md.flags;
     Rec2 r;
     r.x = r.y = r.h = r.s = 0.0;
     r.i0 = r.j0 = 0;
     r.i1 = r.j1 = -1;
     r.cls = kClsSkip;
     r.fb = 0;
+    r.map = mi;
 
     // cic_2D.jl:155-168: skip if every quantity is zero and !calc_mean
     bool allzero = true;
     for (int q = 0; q < nq; ++q)
-        if (ld(binq, p * nq + q) != 0.0)
+        if (ld(binq, pl * nq + q) != 0.0)
             allzero = false;
     if (allzero && !(flags & S2G_FLAG_CALC_MEAN)) {
         rec[p] = r;
@@ -56,15 +61,15 @@ __global__ __launch_bounds__(256) void k2d_setup(const T *__restrict__ pos,
         return;
     }
 
-    const double h = mul_rn(ld(hsml, p), len2pix);
+    const double h = mul_rn(ld(hsml, pl), len2pix);
     const double area = mul_rn(2.0 * h, 2.0 * h);
     const double l3 = mul_rn(mul_rn(len2pix, len2pix), len2pix);
-    const double rh = mul_rn(ld(rho, p), 1.0 / l3);
-    const double dz = (ld(m, p) / rh) / area;
-    const double wl = ld(w, p);
+    const double rh = mul_rn(ld(rho, pl), 1.0 / l3);
+    const double dz = (ld(m, pl) / rh) / area;
+    const double wl = ld(w, pl);
     const double half = 0.5 * (double)npix;
-    const double x = add_rn(mul_rn(ld(pos, 3 * p + 0), len2pix), half);
-    const double y = add_rn(mul_rn(ld(pos, 3 * p + 1), len2pix), half);
+    const double x = add_rn(mul_rn(ld(pos, 3 * pl + 0), len2pix), half);
+    const double y = add_rn(mul_rn(ld(pos, 3 * pl + 1), len2pix), half);
 
     const double lim = (double)npix + 1.0;
     double xa = floor(sub_rn(x, h)), xb = floor(add_rn(x, h));
@@ -221,9 +226,8 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
                                                   const int32_t *__restrict__ pairs,
                                                   const int32_t *__restrict__ large,
                                                   const Rec2 *__restrict__ rec,
-                                                  const T *__restrict__ binq, int nq,
-                                                  int plane0, int ntj,
-                                                  double *__restrict__ partial)
+                                                  const Map2D *__restrict__ maps, int nq,
+                                                  int plane0, double *__restrict__ partial)
 {
     __shared__ float2 s_row[kGBatch][kGT];
     __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch];
@@ -232,7 +236,10 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
     __shared__ double s_mult[NP][kGBatch];
 
     const int4 it = items[blockIdx.x];
-    const int tile = it.x;
+    const int ntj = maps[it.w].ntj;
+    const int tile = it.x - maps[it.w].tbase;
+    const T *binq = (const T *)maps[it.w].binq;
+    const int64_t pbase = maps[it.w].pbase;
     const int ti0 = (tile / ntj) * kGT, tj0 = (tile % ntj) * kGT;
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     const int rg = ln >> 4;
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             for (int q = 0; q < NP; ++q) {
                 const int P = plane0 + q;
                 const double mult =
-                    P == nq ? 1.0 : ld(binq, (int64_t)p * nq + (P < nq ? P : 0));
+                    P == nq ? 1.0 : ld(binq, (int64_t)(p - pbase) * nq + (P < nq ? P : 0));
                 s_mult[q][threadIdx.x] = mult * r.s;
             }
         }
@@ -324,16 +331,24 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             dst[(q * kGT + 4 * k + rg) * kGT + 16 * wv + (ln & 15)] = acc[k][q];
 }
 
-// fixed-order sum of a tile's item partials -> image planes (plain stores)
+// fixed-order sum of a tile's item partials -> image planes (plain stores).
+// Every pixel of every tile of the pass is written (zero where no large
+// particle reaches it), which initialises the images for the small scatter.
 template <int NP>
 __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ partial,
                                                     const int64_t *__restrict__ tile_items,
-                                                    int ntj, int64_t npix, int plane0,
-                                                    double *__restrict__ image)
+                                                    const Map2D *__restrict__ maps, int nmaps,
+                                                    int plane0)
 {
     const int t = blockIdx.x;
-    const int ti0 = (t / ntj) * kGT, tj0 = (t % ntj) * kGT;
-    const int64_t b = tile_items[2 * t], k = tile_items[2 * t + 1];
+    const int mi = find_entry(maps, nmaps, t, [](const Map2D &e) { return e.tbase; });
+    const int ntj = maps[mi].ntj;
+    const int64_t npix = maps[mi].npix;
+    double *image = maps[mi].image;
+    const int tl = t - maps[mi].tbase;
+    const int ti0 = (tl / ntj) * kGT, tj0 = (tl % ntj) * kGT;
+    const int64_t b = tile_items ? tile_items[2 * t] : 0;
+    const int64_t k = tile_items ? tile_items[2 * t + 1] : 0;
     const int64_t nd = npix * npix;
     for (int px = threadIdx.x; px < kGT * kGT; px += 256) {
         const int i = ti0 + px / kGT, j = tj0 + px % kGT;
@@ -354,12 +369,17 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
 // wave-reduced normalisation, FP64 atomics into the image.
 // ---------------------------------------------------------------------------
 template <typename T, int K>
-__device__ __forceinline__ void k2d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list, int64_t ns, const Rec2 *__restrict__ rec,
-        const T *__restrict__ binq, int nq, int64_t npix, double *__restrict__ image)
+__device__ __forceinline__ void k2d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list,
+                                                     const Rec2 *__restrict__ rec,
+                                                     const Map2D *__restrict__ maps, int nq)
 {
     const int ln = threadIdx.x % kWave;
     const int p = list[gw];
     const Rec2 r = rec[p];
+    const int64_t npix = maps[r.map].npix;
+    double *image = maps[r.map].image;
+    const T *binq = (const T *)maps[r.map].binq;
+    const int64_t pl = p - maps[r.map].pbase;
     const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
     const int fp = iw * jw;
     const double hinv = r.fb ? 0.0 : 1.0 / r.h;
@@ -392,32 +412,29 @@ __device__ __forceinline__ void k2d_small_atomic_one(int64_t gw, const int32_t *
             const int64_t idx = (int64_t)(r.i0 + di) * npix + (r.j0 + dj);
             unsafeAtomicAdd(&wimg[idx], pw);
             for (int q = 0; q < nq; ++q)
-                unsafeAtomicAdd(&image[(int64_t)q * nd + idx],
-                                ld(binq, (int64_t)p * nq + q) * pw);
+                unsafeAtomicAdd(&image[(int64_t)q * nd + idx], ld(binq, pl * nq + q) * pw);
         }
     }
 }
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list, int64_t ns, const Rec2 *__restrict__ rec,
-        const T *__restrict__ binq, int nq, int64_t npix, double *__restrict__ image)
+__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list,
+                                                        int64_t ns, const Rec2 *__restrict__ rec,
+                                                        const Map2D *__restrict__ maps, int nq)
 {
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ns;
          gw += nw)
-        k2d_small_atomic_one<T, K>(gw, list, ns, rec, binq, nq, npix, image);
+        k2d_small_atomic_one<T, K>(gw, list, rec, maps, nq);
 }
-
 
 // ---------------------------------------------------------------------------
 // reduce_image_2D (reduce_image.jl:8-31): out[q][y][x] = img[q][x*N+y] / w.
 // 64x64 tiles transposed through LDS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k2d_reduce(const double *__restrict__ image,
-                                                  int64_t npix, int nq, int reduce,
-                                                  double *__restrict__ out)
+__device__ __forceinline__ void reduce_tile(const double *__restrict__ image, int64_t npix,
+                                            int nq, int reduce, double *__restrict__ out,
+                                            int ti, int tj, double (*tile)[65])
 {
-    __shared__ double tile[64][65];
-    const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;  // i = x (slow), j = y (fast)
     const int64_t nd = npix * npix;
     const double *w = image + (int64_t)nq * nd;
     const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
@@ -452,56 +469,69 @@ __global__ __launch_bounds__(256) void k2d_reduce(const double *__restrict__ ima
     }
 }
 
+__global__ __launch_bounds__(256) void k2d_reduce(const double *__restrict__ image,
+                                                  int64_t npix, int nq, int reduce,
+                                                  double *__restrict__ out)
+{
+    __shared__ double tile[64][65];
+    reduce_tile(image, npix, nq, reduce, out, blockIdx.y * 64, blockIdx.x * 64, tile);
+}
+
+// one block per gather tile of the pass (gather and reduce tiles are both
+// 64 x 64); maps without an output are skipped
+__global__ __launch_bounds__(256) void k2d_reduce_maps(const Map2D *__restrict__ maps,
+                                                       int nmaps, int nq)
+{
+    __shared__ double tile[64][65];
+    const int t = blockIdx.x;
+    const int mi = find_entry(maps, nmaps, t, [](const Map2D &e) { return e.tbase; });
+    const Map2D &md = maps[mi];
+    if (!md.out)
+        return;
+    const int tl = t - md.tbase;
+    reduce_tile(md.image, md.npix, nq, (md.flags & S2G_FLAG_REDUCE) ? 1 : 0, md.out,
+                (tl / md.ntj) * kGT, (tl % md.ntj) * kGT, tile);
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-template <typename T>
-static void launch_setup_t(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s)
+#define S2G_K2D_SWITCH(kernel, CALL)                                                    \
+    switch (kernel) {                                                                  \
+    case S2G_KERNEL_CUBIC: { constexpr int KK = S2G_KERNEL_CUBIC; CALL; } break;       \
+    case S2G_KERNEL_QUINTIC: { constexpr int KK = S2G_KERNEL_QUINTIC; CALL; } break;   \
+    case S2G_KERNEL_WENDLAND_C2: { constexpr int KK = S2G_KERNEL_WENDLAND_C2; CALL; } break; \
+    case S2G_KERNEL_WENDLAND_C4: { constexpr int KK = S2G_KERNEL_WENDLAND_C4; CALL; } break; \
+    case S2G_KERNEL_WENDLAND_C6: { constexpr int KK = S2G_KERNEL_WENDLAND_C6; CALL; } break; \
+    default: { constexpr int KK = S2G_KERNEL_WENDLAND_C8; CALL; } break;               \
+    }
+
+void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int nq, int kernel,
+                     Rec2 *rec, int32_t *cls, hipStream_t s)
 {
-    if (a.n == 0)
+    if (ntot == 0)
         return;
-    hipLaunchKernelGGL(k2d_setup<T>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s,
-                       (const T *)a.pos, (const T *)a.hsml, (const T *)a.m,
-                       (const T *)a.rho, (const T *)a.binq, a.nq, (const T *)a.w, a.n,
-                       a.npix, a.len2pix, a.kernel, a.flags, rec, cls);
-}
-
-void launch_2d_setup(const Map2DArgs &a, Rec2 *rec, int32_t *cls, hipStream_t s)
-{
-    if (a.dtype == S2G_F32)
-        launch_setup_t<float>(a, rec, cls, s);
+    const dim3 g((unsigned)((ntot + 255) / 256)), b(256);
+    if (dtype == S2G_F32)
+        hipLaunchKernelGGL(k2d_setup<float>, g, b, 0, s, maps, nmaps, ntot, nq, kernel, rec, cls);
     else
-        launch_setup_t<double>(a, rec, cls, s);
-}
-
-template <int K>
-static void launch_norm_k(const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
-{
-    const int64_t blocks = wave_blocks(ne, kNormThreads);
-    hipLaunchKernelGGL(k2d_norm<K>, dim3((unsigned)blocks), dim3(kNormThreads), 0, s, list, ne,
-                       rec);
+        hipLaunchKernelGGL(k2d_setup<double>, g, b, 0, s, maps, nmaps, ntot, nq, kernel, rec,
+                           cls);
 }
 
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
 {
     if (ne == 0)
         return;
-    switch (kernel) {
-    case S2G_KERNEL_CUBIC: launch_norm_k<S2G_KERNEL_CUBIC>(list, ne, rec, s); break;
-    case S2G_KERNEL_QUINTIC: launch_norm_k<S2G_KERNEL_QUINTIC>(list, ne, rec, s); break;
-    case S2G_KERNEL_WENDLAND_C2: launch_norm_k<S2G_KERNEL_WENDLAND_C2>(list, ne, rec, s); break;
-    case S2G_KERNEL_WENDLAND_C4: launch_norm_k<S2G_KERNEL_WENDLAND_C4>(list, ne, rec, s); break;
-    case S2G_KERNEL_WENDLAND_C6: launch_norm_k<S2G_KERNEL_WENDLAND_C6>(list, ne, rec, s); break;
-    default: launch_norm_k<S2G_KERNEL_WENDLAND_C8>(list, ne, rec, s); break;
-    }
+    const dim3 g((unsigned)wave_blocks(ne, kNormThreads)), b(kNormThreads);
+    S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL(k2d_norm<KK>, g, b, 0, s, list, ne, rec));
 }
 
 template <typename T, int K, int NP>
 static void gather_knp(const GatherArgs &g, hipStream_t s)
 {
     hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
-                       g.items, g.pairs, g.large, g.rec, (const T *)g.binq, g.nq, g.plane0,
-                       g.ntj, g.partial);
+                       g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0, g.partial);
 }
 
 template <typename T, int K>
@@ -515,67 +545,46 @@ static void gather_k(const GatherArgs &g, hipStream_t s)
     }
 }
 
-template <typename T>
-static void gather_t(const GatherArgs &g, hipStream_t s)
-{
-    switch (g.kernel) {
-    case S2G_KERNEL_CUBIC: gather_k<T, S2G_KERNEL_CUBIC>(g, s); break;
-    case S2G_KERNEL_QUINTIC: gather_k<T, S2G_KERNEL_QUINTIC>(g, s); break;
-    case S2G_KERNEL_WENDLAND_C2: gather_k<T, S2G_KERNEL_WENDLAND_C2>(g, s); break;
-    case S2G_KERNEL_WENDLAND_C4: gather_k<T, S2G_KERNEL_WENDLAND_C4>(g, s); break;
-    case S2G_KERNEL_WENDLAND_C6: gather_k<T, S2G_KERNEL_WENDLAND_C6>(g, s); break;
-    default: gather_k<T, S2G_KERNEL_WENDLAND_C8>(g, s); break;
-    }
-}
-
 void launch_2d_gather(const GatherArgs &g, hipStream_t s)
 {
-    if (g.dtype == S2G_F32)
-        gather_t<float>(g, s);
-    else
-        gather_t<double>(g, s);
+    if (g.dtype == S2G_F32) {
+        S2G_K2D_SWITCH(g.kernel, (gather_k<float, KK>(g, s)));
+    } else {
+        S2G_K2D_SWITCH(g.kernel, (gather_k<double, KK>(g, s)));
+    }
 }
 
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
 {
     const dim3 grid((unsigned)g.ntiles), block(256);
     switch (g.np) {
-    case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
-    case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
-    case 3: hipLaunchKernelGGL(k2d_finalize<3>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
-    default: hipLaunchKernelGGL(k2d_finalize<4>, grid, block, 0, s, g.partial, g.tile_items, g.ntj, g.npix, g.plane0, g.image); break;
+    case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
+    case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
+    case 3: hipLaunchKernelGGL(k2d_finalize<3>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
+    default: hipLaunchKernelGGL(k2d_finalize<4>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
     }
 }
 
-template <typename T, int K>
-static void small_tk(const SmallArgs &a, hipStream_t s)
+void launch_2d_small(int dtype, int kernel, int nq, const int32_t *list, int64_t ns,
+                     const Rec2 *rec, const Map2D *maps, hipStream_t s)
 {
-    hipLaunchKernelGGL((k2d_small_atomic<T, K>), dim3((unsigned)wave_blocks(a.ns, 256)),
-                       dim3(256), 0, s, a.list, a.ns, a.rec, (const T *)a.binq, a.nq, a.npix,
-                       a.image);
-}
-
-template <typename T>
-static void small_t(const SmallArgs &a, hipStream_t s)
-{
-    switch (a.kernel) {
-    case S2G_KERNEL_CUBIC: small_tk<T, S2G_KERNEL_CUBIC>(a, s); break;
-    case S2G_KERNEL_QUINTIC: small_tk<T, S2G_KERNEL_QUINTIC>(a, s); break;
-    case S2G_KERNEL_WENDLAND_C2: small_tk<T, S2G_KERNEL_WENDLAND_C2>(a, s); break;
-    case S2G_KERNEL_WENDLAND_C4: small_tk<T, S2G_KERNEL_WENDLAND_C4>(a, s); break;
-    case S2G_KERNEL_WENDLAND_C6: small_tk<T, S2G_KERNEL_WENDLAND_C6>(a, s); break;
-    default: small_tk<T, S2G_KERNEL_WENDLAND_C8>(a, s); break;
-    }
-}
-
-void launch_2d_small(const SmallArgs &a, hipStream_t s)
-{
-    if (a.ns == 0)
+    if (ns == 0)
         return;
-    if (a.dtype == S2G_F32)
-        small_t<float>(a, s);
-    else
-        small_t<double>(a, s);
+    const dim3 g((unsigned)wave_blocks(ns, 256)), b(256);
+    if (dtype == S2G_F32) {
+        S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL((k2d_small_atomic<float, KK>), g, b, 0, s,
+                                                  list, ns, rec, maps, nq));
+    } else {
+        S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL((k2d_small_atomic<double, KK>), g, b, 0, s,
+                                                  list, ns, rec, maps, nq));
+    }
+}
+
+void launch_2d_reduce_maps(const Map2D *maps, int nmaps, int ntiles, int nq, hipStream_t s)
+{
+    if (ntiles == 0)
+        return;
+    hipLaunchKernelGGL(k2d_reduce_maps, dim3((unsigned)ntiles), dim3(256), 0, s, maps, nmaps, nq);
 }
 
 void launch_2d_reduce(const double *image, int64_t npix, int nq, int reduce, double *out,

@@ -9,7 +9,7 @@ namespace s2g {
 constexpr int kPartBlock = 256;
 // large-list entries per counting-sort chunk (one LDS histogram per chunk)
 constexpr int kPairChunk = 4096;
-// per-workgroup LDS bound of the tile histograms (ints)
+// per-workgroup LDS bound of the tile histograms (ints): tiles of one map
 constexpr int kMaxTiles = 16384;
 
 int64_t partition_blocks(int64_t n);
@@ -17,15 +17,53 @@ int64_t partition_blocks(int64_t n);
 // index of class c in the concatenation and counts[3*nb] the grand total
 void launch_partition(const int32_t *cls, int64_t n, int64_t *counts, int64_t *scan_tmp,
                       int32_t *l0, int32_t *l1, int32_t *l2, hipStream_t s);
-// fill == false: count + scan into cnt (ntiles*nchunks+1 int64, last = total)
-// fill == true : fill pairs from the scanned cnt
+// After launch_partition over the particles of several maps: out[0..3] =
+// class offsets counts[0], counts[nb], counts[2nb], counts[3nb]; out[4 + m] =
+// first entry of map m in the large list (particles of map m are
+// [pbase[m], pbase[m+1]), and the partition is stable), out[4 + nmaps] = size
+// of the large list.
+void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *large,
+                         const int64_t *pbase, int nmaps, int64_t *out, hipStream_t s);
+
+// Tile grid of one map in the (tile, particle) counting sort.  Local tile id
+// t = (a * nt + b) * ntc + c; global tile id tbase + t.  The map's count block
+// is cnt[cbase + t * nch + k] for its chunks k < nch.
+struct TileMap {
+    int64_t cbase;
+    int32_t tbase, ntiles;
+    int32_t nt, ntc;
+    int32_t nch, pad;
+};
+// chunk k of map `map`: large-list entries [l0, l1)
+struct PairChunk {
+    int32_t map, k, l0, l1;
+};
+
+// fill == false: count + scan into cnt (ncnt int64, the last is the total)
+// fill == true : fill pairs (large-list indices) from the scanned cnt
 template <typename R>
-void launch_pairs(const int32_t *large, int64_t nl, const R *rec, int nt, int ntc, int ntiles,
-                  int nchunks, int64_t *cnt, int64_t *scan_tmp, int32_t *pairs_or_null,
-                  bool fill, hipStream_t s);
-// fill == false: nitems (ntiles+1, last = total) ; fill == true: items + tile_items
-void launch_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs, int64_t G,
+void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
+                  const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
+                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s);
+// fill == false: nitems (ntiles+1, last = total) ; fill == true: items + tile_items.
+// items[i] = (global tile, first pair, count, map)
+void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles, int64_t G,
                   int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items,
                   bool fill, hipStream_t s);
+
+// index of the last table entry whose key is <= v (keys ascending, key[0] <= v)
+template <typename E, typename K, typename F>
+__device__ __forceinline__ int find_entry(const E *tab, int n, K v, F key)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(tab[mid]) <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 }  // namespace s2g

@@ -93,11 +93,51 @@ void launch_partition(const int32_t *cls, int64_t n, int64_t *counts, int64_t *s
                        l1, l2);
 }
 
+// class offsets and the per-map ranges of the large list
+__global__ void k_class_bounds(const int64_t *__restrict__ counts, int64_t nb,
+                               const int32_t *__restrict__ large,
+                               const int64_t *__restrict__ pbase, int nmaps,
+                               int64_t *__restrict__ out)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t l_lo = counts[nb], l_hi = counts[2 * nb];
+    const int64_t nl = l_hi - l_lo;
+    if (m < 4)
+        out[m] = counts[(int64_t)m * nb];
+    if (m > nmaps)
+        return;
+    if (m == nmaps) {
+        out[4 + m] = nl;
+        return;
+    }
+    // first large entry whose particle index is >= pbase[m]
+    const int64_t key = pbase[m];
+    int64_t lo = 0, hi = nl;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)large[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[4 + m] = lo;
+}
+
+void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *large,
+                         const int64_t *pbase, int nmaps, int64_t *out, hipStream_t s)
+{
+    const int nt = nmaps + 1 > 4 ? nmaps + 1 : 4;
+    hipLaunchKernelGGL(k_class_bounds, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                       counts, nb, large, pbase, nmaps, out);
+}
+
 // ---------------------------------------------------------------------------
 // (tile, particle) pair lists: a counting sort with deterministic order.
-// Chunk c holds large-list entries [c*CH, (c+1)*CH).  cnt is tile-major
-// [tile][chunk] so one exclusive scan yields every list offset.  Tiles are
-// numbered t = (a * nt + b) * ntc + c over the tile grid (ntc = 1 in 2D).
+// Chunks are map-local runs of at most kPairChunk large-list entries; map m's
+// counts are tile-major [tile][chunk] at cnt[cbase], so one exclusive scan
+// over all maps yields every list offset, ordered by (map, tile, particle).
+// Local tiles are t = (a * nt + b) * ntc + c over the map's tile grid
+// (ntc = 1 in 2D).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void tile_rect(const Rec2 &r, int &a0, int &a1, int &b0, int &b1,
                                           int &c0, int &c1)
@@ -122,48 +162,50 @@ __device__ __forceinline__ void tile_rect(const Rec3 &r, int &a0, int &a1, int &
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ large,
-                                                    int64_t nl, const R *__restrict__ rec,
-                                                    int nt, int ntc, int ntiles, int nchunks,
-                                                    int64_t *__restrict__ cnt)
+                                                    const R *__restrict__ rec,
+                                                    const PairChunk *__restrict__ chunks,
+                                                    const TileMap *__restrict__ tmaps,
+                                                    int64_t ncnt, int64_t *__restrict__ cnt)
 {
     extern __shared__ int hist[];
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
+    const PairChunk ch = chunks[blockIdx.x];
+    const TileMap tm = tmaps[ch.map];
+    for (int t = threadIdx.x; t < tm.ntiles; t += blockDim.x)
         hist[t] = 0;
     __syncthreads();
-    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
-    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
-    for (int64_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+    for (int64_t l = ch.l0 + threadIdx.x; l < ch.l1; l += blockDim.x) {
         const R r = rec[large[l]];
         int a0, a1, b0, b1, c0, c1;
         tile_rect(r, a0, a1, b0, b1, c0, c1);
         for (int a = a0; a <= a1; ++a)
             for (int b = b0; b <= b1; ++b)
                 for (int c = c0; c <= c1; ++c)
-                    atomicAdd(&hist[(a * nt + b) * ntc + c], 1);
+                    atomicAdd(&hist[(a * tm.nt + b) * tm.ntc + c], 1);
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-        cnt[(int64_t)t * nchunks + blockIdx.x] = hist[t];
+    for (int t = threadIdx.x; t < tm.ntiles; t += blockDim.x)
+        cnt[tm.cbase + (int64_t)t * tm.nch + ch.k] = hist[t];
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        cnt[(int64_t)ntiles * nchunks] = 0;  // becomes the pair total after the scan
+        cnt[ncnt - 1] = 0;  // becomes the pair total after the scan
 }
 
 // one wave per chunk: sequential over the chunk's particles, lanes over the
 // particle's tiles (distinct per lane), so every list is in particle order
 template <typename R>
 __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__ large,
-                                                     int64_t nl, const R *__restrict__ rec,
-                                                     int nt, int ntc, int ntiles, int nchunks,
+                                                     const R *__restrict__ rec,
+                                                     const PairChunk *__restrict__ chunks,
+                                                     const TileMap *__restrict__ tmaps,
                                                      const int64_t *__restrict__ off,
                                                      int32_t *__restrict__ pairs)
 {
     extern __shared__ int cursor[];
-    for (int t = threadIdx.x; t < ntiles; t += kWave)
-        cursor[t] = (int)off[(int64_t)t * nchunks + blockIdx.x];
+    const PairChunk ch = chunks[blockIdx.x];
+    const TileMap tm = tmaps[ch.map];
+    for (int t = threadIdx.x; t < tm.ntiles; t += kWave)
+        cursor[t] = (int)off[tm.cbase + (int64_t)t * tm.nch + ch.k];
     __syncthreads();
-    const int64_t l0 = (int64_t)blockIdx.x * kPairChunk;
-    const int64_t l1 = l0 + kPairChunk < nl ? l0 + kPairChunk : nl;
-    for (int64_t l = l0; l < l1; ++l) {
+    for (int64_t l = ch.l0; l < ch.l1; ++l) {
         const R r = rec[large[l]];
         int a0, a1, b0, b1, c0, c1;
         tile_rect(r, a0, a1, b0, b1, c0, c1);
@@ -173,81 +215,97 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
             const int a = a0 + k / (wb * wc);
             const int rem = k % (wb * wc);
             const int b = b0 + rem / wc, c = c0 + rem % wc;
-            const int slot = atomicAdd(&cursor[(a * nt + b) * ntc + c], 1);
+            const int slot = atomicAdd(&cursor[(a * tm.nt + b) * tm.ntc + c], 1);
             pairs[slot] = (int32_t)l;
         }
     }
 }
 
 template <typename R>
-void launch_pairs(const int32_t *large, int64_t nl, const R *rec, int nt, int ntc, int ntiles,
-                  int nchunks, int64_t *cnt, int64_t *scan_tmp, int32_t *pairs_or_null,
-                  bool fill, hipStream_t s)
+void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
+                  const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
+                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s)
 {
-    const size_t lds = sizeof(int) * (size_t)ntiles;
+    const size_t lds = sizeof(int) * (size_t)(max_tiles > 0 ? max_tiles : 1);
     if (!fill) {
         hipLaunchKernelGGL(k_pair_count<R>, dim3((unsigned)nchunks), dim3(256), lds, s, large,
-                           nl, rec, nt, ntc, ntiles, nchunks, cnt);
-        exclusive_scan_i64(cnt, (int64_t)ntiles * nchunks + 1, scan_tmp, s);
+                           rec, chunks, tmaps, ncnt, cnt);
+        exclusive_scan_i64(cnt, ncnt, scan_tmp, s);
     } else {
         hipLaunchKernelGGL(k_pair_fill<R>, dim3((unsigned)nchunks), dim3(kWave), lds, s, large,
-                           nl, rec, nt, ntc, ntiles, nchunks, cnt, pairs_or_null);
+                           rec, chunks, tmaps, cnt, pairs_or_null);
     }
 }
 
-template void launch_pairs<Rec2>(const int32_t *, int64_t, const Rec2 *, int, int, int, int,
-                                 int64_t *, int64_t *, int32_t *, bool, hipStream_t);
-template void launch_pairs<Rec3>(const int32_t *, int64_t, const Rec3 *, int, int, int, int,
-                                 int64_t *, int64_t *, int32_t *, bool, hipStream_t);
+template void launch_pairs<Rec2>(const int32_t *, const Rec2 *, const PairChunk *, int,
+                                 const TileMap *, int, int64_t *, int64_t, int64_t *, int32_t *,
+                                 bool, hipStream_t);
+template void launch_pairs<Rec3>(const int32_t *, const Rec3 *, const PairChunk *, int,
+                                 const TileMap *, int, int64_t *, int64_t, int64_t *, int32_t *,
+                                 bool, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Work items: tile t's list [start, start+count) is split into ceil(count/G)
-// items (tile, first pair, count).
+// items (global tile, first pair, count, map).
 // ---------------------------------------------------------------------------
-__global__ void k_item_count(const int64_t *__restrict__ off, int ntiles, int nchunks,
-                             int64_t npairs, int64_t G, int64_t *__restrict__ nitems)
+__device__ __forceinline__ void tile_list(const int64_t *__restrict__ off,
+                                          const TileMap *__restrict__ tmaps, int nmaps, int t,
+                                          int &m, int64_t &s, int64_t &e)
+{
+    m = find_entry(tmaps, nmaps, t, [](const TileMap &x) { return x.tbase; });
+    const TileMap tm = tmaps[m];
+    const int64_t tl = t - tm.tbase;
+    // a map without chunks has an empty block: both ends read the same entry
+    s = off[tm.cbase + tl * tm.nch];
+    e = off[tm.cbase + (tl + 1) * tm.nch];
+}
+
+__global__ void k_item_count(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
+                             int nmaps, int ntiles, int64_t G, int64_t *__restrict__ nitems)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == ntiles)
         nitems[t] = 0;  // becomes the item total after the scan
     if (t >= ntiles)
         return;
-    const int64_t s = off[(int64_t)t * nchunks];
-    const int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    int m;
+    int64_t s, e;
+    tile_list(off, tmaps, nmaps, t, m, s, e);
     nitems[t] = (e - s + G - 1) / G;
 }
 
-__global__ void k_item_fill(const int64_t *__restrict__ off, int ntiles, int nchunks,
-                            int64_t npairs, int64_t G, const int64_t *__restrict__ ibase,
+__global__ void k_item_fill(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
+                            int nmaps, int ntiles, int64_t G, const int64_t *__restrict__ ibase,
                             int4 *__restrict__ items, int64_t *__restrict__ tile_items)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles)
         return;
-    const int64_t s = off[(int64_t)t * nchunks];
-    const int64_t e = (t + 1 < ntiles) ? off[(int64_t)(t + 1) * nchunks] : npairs;
+    int m;
+    int64_t s, e;
+    tile_list(off, tmaps, nmaps, t, m, s, e);
     const int64_t b = ibase[t];
     int64_t k = 0;
     for (int64_t q = s; q < e; q += G, ++k) {
         const int64_t len = e - q < G ? e - q : G;
-        items[b + k] = make_int4(t, (int)q, (int)len, 0);
+        items[b + k] = make_int4(t, (int)q, (int)len, m);
     }
     tile_items[2 * t] = b;
     tile_items[2 * t + 1] = k;
 }
 
-void launch_items(const int64_t *off, int ntiles, int nchunks, int64_t npairs, int64_t G,
+void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles, int64_t G,
                   int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items,
                   bool fill, hipStream_t s)
 {
     const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
     if (!fill) {
-        hipLaunchKernelGGL(k_item_count, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks, npairs,
+        hipLaunchKernelGGL(k_item_count, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles,
                            G, nitems);
         exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
     } else {
-        hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, ntiles, nchunks, npairs,
-                           G, nitems, items_or_null, tile_items);
+        hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles, G,
+                           nitems, items_or_null, tile_items);
     }
 }
 

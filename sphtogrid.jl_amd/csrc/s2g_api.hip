@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "k_cic3d.h"
 #include "k_healpix.h"
@@ -21,8 +22,11 @@ struct s2g_ctx {
     std::string err;
     // staged host inputs
     DevBuf in[7];
-    // 2D pipeline
+    // 2D / 3D pipelines
     DevBuf rec, cls, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
+    // per-pass tables: map table + particle bases, class bounds, tile maps + chunks
+    DevBuf tab, bounds, tiles;
+    PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
     DevBuf hp_rec, hp_nunits, hp_units, hp_part;
     // outputs
@@ -114,22 +118,56 @@ struct TilePlan {
     int64_t *tile_items = nullptr;
 };
 
-template <typename R>
-TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, int64_t nl, const R *rec, int nt,
-                          int ntc, int ntiles, hipStream_t s, double *ms_sort)
+// tm: tile grid of every map of the pass (cbase / nch filled here from the
+// per-map large-list bounds lbase[0..nmaps])
+TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, bool rec3,
+                          std::vector<TileMap> &tm, const std::vector<int64_t> &lbase,
+                          int ntiles, hipStream_t s, double *ms_sort)
 {
     hipEvent_t *E = c->stage;
     TilePlan tp;
-    if (ntiles > kMaxTiles)
-        throw Error(S2G_EUNSUPPORTED, "grid too large for the tile lists (" +
-                                          std::to_string(ntiles) + " tiles > " +
-                                          std::to_string(kMaxTiles) + ")");
-    const int nchunks = (int)((nl + kPairChunk - 1) / kPairChunk);
-    const int64_t ncnt = (int64_t)ntiles * nchunks + 1;
+    const int nmaps = (int)tm.size();
+    std::vector<PairChunk> chunks;
+    int64_t cb = 0;
+    int max_tiles = 1;
+    for (int m = 0; m < nmaps; ++m) {
+        if (tm[m].ntiles > kMaxTiles)
+            throw Error(S2G_EUNSUPPORTED, "grid too large for the tile lists (" +
+                                              std::to_string(tm[m].ntiles) + " tiles > " +
+                                              std::to_string(kMaxTiles) + ")");
+        max_tiles = tm[m].ntiles > max_tiles ? tm[m].ntiles : max_tiles;
+        const int64_t l0 = lbase[m], l1 = lbase[m + 1];
+        const int nch = (int)((l1 - l0 + kPairChunk - 1) / kPairChunk);
+        tm[m].cbase = cb;
+        tm[m].nch = nch;
+        cb += (int64_t)tm[m].ntiles * nch;
+        for (int k = 0; k < nch; ++k) {
+            const int64_t a = l0 + (int64_t)k * kPairChunk;
+            const int64_t e = a + kPairChunk < l1 ? a + kPairChunk : l1;
+            chunks.push_back(PairChunk{m, k, (int32_t)a, (int32_t)e});
+        }
+    }
+    const int nchunks = (int)chunks.size();
+    const int64_t ncnt = cb + 1;
+    // one upload of the tile maps and the chunk table
+    const size_t btm = sizeof(TileMap) * nmaps, bch = sizeof(PairChunk) * nchunks;
+    char *h = (char *)c->h_tiles.get(btm + bch);
+    std::memcpy(h, tm.data(), btm);
+    std::memcpy(h + btm, chunks.data(), bch);
+    char *d = (char *)c->tiles.get(btm + bch);
+    S2G_HIP(hipMemcpyAsync(d, h, btm + bch, hipMemcpyHostToDevice, s));
+    const TileMap *d_tm = (const TileMap *)d;
+    const PairChunk *d_ch = (const PairChunk *)(d + btm);
+
     int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
     int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
     S2G_HIP(hipEventRecord(E[4], s));
-    launch_pairs<R>(large, nl, rec, nt, ntc, ntiles, nchunks, cnt, tmp, nullptr, false, s);
+    if (rec3)
+        launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
+                           tmp, nullptr, false, s);
+    else
+        launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
+                           tmp, nullptr, false, s);
     S2G_HIP(hipEventRecord(E[5], s));
     read_i64(c, s, {cnt + ncnt - 1}, &tp.npairs);
     if (tp.npairs >= (int64_t)INT32_MAX)
@@ -142,14 +180,19 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, int64_t nl, const R 
         G = 64;
     int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
     S2G_HIP(hipEventRecord(E[6], s));
-    launch_pairs<R>(large, nl, rec, nt, ntc, ntiles, nchunks, cnt, tmp, tp.pairs, true, s);
-    launch_items(cnt, ntiles, nchunks, tp.npairs, G, nit, tmp, nullptr, nullptr, false, s);
+    if (rec3)
+        launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
+                           tmp, tp.pairs, true, s);
+    else
+        launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
+                           tmp, tp.pairs, true, s);
+    launch_items(cnt, d_tm, nmaps, ntiles, G, nit, tmp, nullptr, nullptr, false, s);
     S2G_HIP(hipEventRecord(E[7], s));
     read_i64(c, s, {nit + ntiles}, &tp.nitems);
     tp.items = (int4 *)c->items.get(sizeof(int4) * (tp.nitems ? tp.nitems : 1));
     tp.tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
     S2G_HIP(hipEventRecord(E[8], s));
-    launch_items(cnt, ntiles, nchunks, tp.npairs, G, nit, tmp, tp.items, tp.tile_items, true, s);
+    launch_items(cnt, d_tm, nmaps, ntiles, G, nit, tmp, tp.items, tp.tile_items, true, s);
     S2G_HIP(hipEventRecord(E[9], s));
     S2G_HIP(hipEventSynchronize(E[9]));
     *ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]) + elapsed(E[8], E[9]);
@@ -159,9 +202,13 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, int64_t nl, const R 
 struct ClassLists {
     int64_t ns = 0, nl = 0, ne = 0;
     int32_t *l0 = nullptr, *l1 = nullptr, *l2 = nullptr;
+    std::vector<int64_t> lbase;  // per-map first large-list entry (+ the total)
 };
 
-ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, hipStream_t s)
+// stable class partition of the pass's particles and one read-back of the
+// class sizes and of the per-map large-list bounds
+ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, const int64_t *d_pbase,
+                             int nmaps, hipStream_t s)
 {
     ClassLists cl;
     const int64_t nb = partition_blocks(n);
@@ -171,12 +218,16 @@ ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, hipStrea
     cl.l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
     int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
     launch_partition(cls, n, counts, tmp, cl.l0, cl.l1, cl.l2, s);
+    int64_t *bounds = (int64_t *)c->bounds.get(sizeof(int64_t) * (nmaps + 5));
+    launch_class_bounds(counts, nb, cl.l1, d_pbase, nmaps, bounds, s);
     S2G_HIP(hipEventRecord(c->stage[1], s));
-    int64_t o[4];
-    read_i64(c, s, {counts, counts + nb, counts + 2 * nb, counts + 3 * nb}, o);
-    cl.ns = o[1] - o[0];
-    cl.nl = o[2] - o[1];
-    cl.ne = o[3] - o[2];
+    int64_t *h = (int64_t *)c->h_bounds.get(sizeof(int64_t) * (nmaps + 5));
+    S2G_HIP(hipMemcpyAsync(h, bounds, sizeof(int64_t) * (nmaps + 5), hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipStreamSynchronize(s));
+    cl.ns = h[1] - h[0];
+    cl.nl = h[2] - h[1];
+    cl.ne = h[3] - h[2];
+    cl.lbase.assign(h + 4, h + 5 + nmaps);
     return cl;
 }
 
@@ -191,109 +242,144 @@ void fill_class_stats(s2g_stats *st, int64_t n, const ClassLists &cl)
 }
 
 // ---------------------------------------------------------------------------
-// 2D deposit on device pointers
+// 2D deposit pass over one or more maps on device pointers.  All maps share
+// dtype, kernel and nq.  Each map's flat image (nq + 1 planes) is written in
+// full; maps with an output also get reduce_image_2D.
 // ---------------------------------------------------------------------------
-void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream_t s,
-                      s2g_stats *st)
+struct MapIn {
+    Map2DArgs a;
+    double *image;
+    double *out;
+};
+
+void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_stats *st)
 {
-    const int64_t n = a.n, npix = a.npix;
-    const int64_t nd = npix * npix;
-    const int nplanes = a.nq + 1;
+    const int dtype = mi[0].a.dtype, kernel = mi[0].a.kernel, nq = mi[0].a.nq;
+    const int nplanes = nq + 1;
+    // per-map table: particle and tile offsets of the pass
+    const size_t btab = sizeof(Map2D) * nmaps, bpb = sizeof(int64_t) * (nmaps + 1);
+    char *h = (char *)c->h_tab.get(btab + bpb);
+    Map2D *tab = (Map2D *)h;
+    int64_t *pb = (int64_t *)(h + btab);
+    std::vector<TileMap> tm(nmaps);
+    int64_t ntot = 0, ttot = 0;
+    bool any_out = false;
+    for (int m = 0; m < nmaps; ++m) {
+        const Map2DArgs &a = mi[m].a;
+        Map2D &e = tab[m];
+        e.pos = a.pos;
+        e.hsml = a.hsml;
+        e.m = a.m;
+        e.rho = a.rho;
+        e.binq = a.binq;
+        e.w = a.w;
+        e.image = mi[m].image;
+        e.out = mi[m].out;
+        e.len2pix = a.len2pix;
+        e.npix = a.npix;
+        e.n = a.n;
+        e.pbase = ntot;
+        e.flags = a.flags;
+        e.pad = 0;
+        const int ntj = (int)((a.npix + kGT - 1) / kGT);
+        e.ntj = ntj;
+        e.tbase = (int32_t)ttot;
+        pb[m] = ntot;
+        tm[m] = TileMap{0, (int32_t)ttot, ntj * ntj, ntj, 1, 0, 0};
+        ntot += a.n;
+        ttot += (int64_t)ntj * ntj;
+        any_out = any_out || mi[m].out;
+        if (ntot > (int64_t)INT32_MAX || ttot > (int64_t)INT32_MAX)
+            throw Error(S2G_EUNSUPPORTED, "deposit pass too large (2^31 particles or tiles)");
+    }
+    pb[nmaps] = ntot;
+    char *d = (char *)c->tab.get(btab + bpb);
+    const Map2D *d_maps = (const Map2D *)d;
+    const int64_t *d_pbase = (const int64_t *)(d + btab);
+    const int ntiles = (int)ttot;
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        st->n_particles = n;
+        st->n_particles = ntot;
     }
     hipEvent_t *E = c->stage;
     S2G_HIP(hipEventRecord(E[0], s));
-    if (n == 0) {
-        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
-        S2G_HIP(hipEventRecord(E[1], s));
-        S2G_HIP(hipEventSynchronize(E[1]));
-        return;
-    }
-    Rec2 *rec = (Rec2 *)c->rec.get(sizeof(Rec2) * n);
-    int32_t *cls = (int32_t *)c->cls.get(sizeof(int32_t) * n);
-    launch_2d_setup(a, rec, cls, s);
-    const ClassLists cl = partition_classes(c, cls, n, s);
-    fill_class_stats(st, n, cl);
+    S2G_HIP(hipMemcpyAsync(d, h, btab + bpb, hipMemcpyHostToDevice, s));
 
+    ClassLists cl;
+    Rec2 *rec = nullptr;
+    if (ntot > 0) {
+        rec = (Rec2 *)c->rec.get(sizeof(Rec2) * ntot);
+        int32_t *cls = (int32_t *)c->cls.get(sizeof(int32_t) * ntot);
+        launch_2d_setup(dtype, d_maps, nmaps, ntot, nq, kernel, rec, cls, s);
+        cl = partition_classes(c, cls, ntot, d_pbase, nmaps, s);
+        fill_class_stats(st, ntot, cl);
+    } else {
+        S2G_HIP(hipEventRecord(E[1], s));
+    }
     S2G_HIP(hipEventRecord(E[2], s));
-    launch_2d_norm(a.kernel, cl.l2, cl.ne, rec, s);
+    launch_2d_norm(kernel, cl.l2, cl.ne, rec, s);
     S2G_HIP(hipEventRecord(E[3], s));
 
-    const int ntj = (int)((npix + kGT - 1) / kGT);
-    const int ntiles = ntj * ntj;
     double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;
     int64_t launches = 0;
+    TilePlan tp;
     if (cl.nl > 0) {
-        const TilePlan tp = build_tile_lists<Rec2>(c, cl.l1, cl.nl, rec, ntj, 1, ntiles, s,
-                                                   &ms_sort);
+        tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, ntiles, s, &ms_sort);
         if (st)
             st->n_pairs = tp.npairs;
-        int ev = 10;
-        for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
-            const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
-            GatherArgs g;
-            g.dtype = a.dtype;
-            g.kernel = a.kernel;
-            g.np = np;
-            g.plane0 = plane0;
-            g.nq = a.nq;
-            g.ntj = ntj;
-            g.ntiles = ntiles;
-            g.nitems = tp.nitems;
-            g.npix = npix;
-            g.items = tp.items;
-            g.pairs = tp.pairs;
-            g.large = cl.l1;
-            g.rec = rec;
-            g.binq = a.binq;
-            g.partial = (double *)c->partial.get(sizeof(double) * (tp.nitems ? tp.nitems : 1) *
-                                                 np * kGT * kGT);
-            g.tile_items = tp.tile_items;
-            g.image = d_image;
-            const bool timed = ev + 2 < s2g_ctx::kNumStageEvents - 3;
-            if (timed)
-                S2G_HIP(hipEventRecord(E[ev], s));
-            if (tp.nitems > 0) {
-                launch_2d_gather(g, s);
-                ++launches;
-            }
-            if (timed)
-                S2G_HIP(hipEventRecord(E[ev + 1], s));
-            launch_2d_finalize(g, s);
-            if (timed) {
-                S2G_HIP(hipEventRecord(E[ev + 2], s));
-                ev += 3;
-            }
+    }
+    int ev = 10;
+    for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
+        const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
+        GatherArgs g;
+        g.dtype = dtype;
+        g.kernel = kernel;
+        g.np = np;
+        g.plane0 = plane0;
+        g.nq = nq;
+        g.nmaps = nmaps;
+        g.ntiles = ntiles;
+        g.nitems = tp.nitems;
+        g.items = tp.items;
+        g.pairs = tp.pairs;
+        g.large = cl.l1;
+        g.rec = rec;
+        g.maps = d_maps;
+        g.partial = tp.nitems > 0 ? (double *)c->partial.get(sizeof(double) * tp.nitems * np *
+                                                             kGT * kGT)
+                                  : nullptr;
+        g.tile_items = tp.tile_items;
+        const bool timed = ev + 2 < s2g_ctx::kNumStageEvents - 3;
+        if (timed)
+            S2G_HIP(hipEventRecord(E[ev], s));
+        if (tp.nitems > 0) {
+            launch_2d_gather(g, s);
+            ++launches;
         }
-        S2G_HIP(hipEventSynchronize(E[ev - 1]));
+        if (timed)
+            S2G_HIP(hipEventRecord(E[ev + 1], s));
+        launch_2d_finalize(g, s);
+        if (timed) {
+            S2G_HIP(hipEventRecord(E[ev + 2], s));
+            ev += 3;
+        }
+    }
+
+    const int eS = s2g_ctx::kNumStageEvents - 3;
+    S2G_HIP(hipEventRecord(E[eS], s));
+    launch_2d_small(dtype, kernel, nq, cl.l0, cl.ns, rec, d_maps, s);
+    S2G_HIP(hipEventRecord(E[eS + 1], s));
+    if (any_out)
+        launch_2d_reduce_maps(d_maps, nmaps, ntiles, nq, s);
+    S2G_HIP(hipGetLastError());
+    S2G_HIP(hipEventRecord(E[eS + 2], s));
+    S2G_HIP(hipEventSynchronize(E[eS + 2]));
+    if (st) {
         for (int k = 10; k < ev; k += 3) {
             ms_gather += elapsed(E[k], E[k + 1]);
             ms_fin += elapsed(E[k + 1], E[k + 2]);
         }
-    } else {
-        S2G_HIP(hipMemsetAsync(d_image, 0, sizeof(double) * nd * nplanes, s));
-    }
-
-    SmallArgs sa;
-    sa.dtype = a.dtype;
-    sa.kernel = a.kernel;
-    sa.nq = a.nq;
-    sa.ns = cl.ns;
-    sa.npix = npix;
-    sa.list = cl.l0;
-    sa.rec = rec;
-    sa.binq = a.binq;
-    sa.image = d_image;
-    const int eS = s2g_ctx::kNumStageEvents - 3;
-    S2G_HIP(hipEventRecord(E[eS], s));
-    launch_2d_small(sa, s);
-    S2G_HIP(hipGetLastError());
-    S2G_HIP(hipEventRecord(E[eS + 1], s));
-    S2G_HIP(hipEventSynchronize(E[eS + 1]));
-    if (st) {
-        st->ms_device = elapsed(E[0], E[eS + 1]);
+        st->ms_device = elapsed(E[0], E[eS + 2]);
         st->ms_setup = elapsed(E[0], E[1]);
         st->ms_norm = elapsed(E[2], E[3]);
         st->ms_sort = ms_sort;
@@ -302,6 +388,14 @@ void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream
         st->ms_small = elapsed(E[eS], E[eS + 1]);
         st->n_gather_launches = launches;
     }
+}
+
+// one map, flat image only
+void deposit2d_device(s2g_ctx *c, const Map2DArgs &a, double *d_image, hipStream_t s,
+                      s2g_stats *st)
+{
+    MapIn m{a, d_image, nullptr};
+    deposit2d_pass(c, &m, 1, s, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,8 +420,13 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     }
     Rec3 *rec = (Rec3 *)c->rec.get(sizeof(Rec3) * n);
     int32_t *cls = (int32_t *)c->cls.get(sizeof(int32_t) * n);
+    int64_t *hpb = (int64_t *)c->h_tab.get(2 * sizeof(int64_t));
+    hpb[0] = 0;
+    hpb[1] = n;
+    int64_t *d_pbase = (int64_t *)c->tab.get(2 * sizeof(int64_t));
+    S2G_HIP(hipMemcpyAsync(d_pbase, hpb, 2 * sizeof(int64_t), hipMemcpyHostToDevice, s));
     launch_3d_setup(a, rec, cls, s);
-    const ClassLists cl = partition_classes(c, cls, n, s);
+    const ClassLists cl = partition_classes(c, cls, n, d_pbase, 1, s);
     fill_class_stats(st, n, cl);
 
     S2G_HIP(hipEventRecord(E[2], s));
@@ -339,8 +438,9 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;
     int64_t launches = 0;
     if (cl.nl > 0) {
-        const TilePlan tp = build_tile_lists<Rec3>(c, cl.l1, cl.nl, rec, nb, nb, ntiles, s,
-                                                   &ms_sort);
+        std::vector<TileMap> tm{TileMap{0, 0, ntiles, nb, nb, 0, 0}};
+        const TilePlan tp = build_tile_lists(c, cl.l1, rec, true, tm, cl.lbase, ntiles, s,
+                                             &ms_sort);
         if (st)
             st->n_pairs = tp.npairs;
         Gather3Args g;
@@ -559,6 +659,12 @@ void s2g_destroy(s2g_ctx *c)
     c->items.release();
     c->tile_items.release();
     c->partial.release();
+    c->tab.release();
+    c->bounds.release();
+    c->tiles.release();
+    c->h_tab.release();
+    c->h_bounds.release();
+    c->h_tiles.release();
     c->hp_rec.release();
     c->hp_nunits.release();
     c->hp_units.release();
@@ -906,7 +1012,86 @@ int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin
 int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps, void *stream,
                       s2g_stats *stats)
 {
-    return fail(c, S2G_EUNSUPPORTED, "batched deposit not available in this build");
+    return guarded(c, [&]() {
+        if (nmaps < 0 || (nmaps > 0 && !maps))
+            throw Error(S2G_EINVAL, "maps is NULL or nmaps < 0");
+        if (dtype != S2G_F32 && dtype != S2G_F64)
+            throw Error(S2G_EINVAL, "dtype must be S2G_F32 or S2G_F64");
+        for (int k = 0; k < nmaps; ++k) {
+            const s2g_map_desc &d = maps[k];
+            const std::string at = "map " + std::to_string(k) + ": ";
+            if (d.dims != 2)
+                throw Error(S2G_EUNSUPPORTED, at + "batched deposits are 2D (dims == 2)");
+            try {
+                check_common(dtype, d.kernel, 2, d.n, &d.params);
+                check_dims_nq(2, d.nq);
+            } catch (const Error &e) {
+                throw Error(e.code, at + e.msg);
+            }
+            if (!d.d_out)
+                throw Error(S2G_EINVAL, at + "output pointer is NULL");
+            if (d.n > 0 && (!d.d_pos || !d.d_hsml || !d.d_m || !d.d_rho || !d.d_bin_q ||
+                            !d.d_weights))
+                throw Error(S2G_EINVAL, at + "input array pointer is NULL");
+        }
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        s2g_stats total;
+        std::memset(&total, 0, sizeof(total));
+        // one pass per (kernel, nq) group, maps in their given order
+        std::vector<char> done(nmaps, 0);
+        for (int k = 0; k < nmaps; ++k) {
+            if (done[k])
+                continue;
+            std::vector<int> grp;
+            for (int j = k; j < nmaps; ++j)
+                if (!done[j] && maps[j].kernel == maps[k].kernel && maps[j].nq == maps[k].nq) {
+                    grp.push_back(j);
+                    done[j] = 1;
+                }
+            // reduced maps deposit into context-owned flat images first
+            int64_t scratch = 0;
+            for (int j : grp)
+                if (maps[j].flags & S2G_FLAG_REDUCE)
+                    scratch += maps[j].params.npixels * maps[j].params.npixels * (maps[j].nq + 1);
+            double *img = scratch ? (double *)c->image.get(sizeof(double) * scratch) : nullptr;
+            std::vector<MapIn> in;
+            int64_t off = 0;
+            for (int j : grp) {
+                const s2g_map_desc &d = maps[j];
+                MapIn m;
+                m.a = make_args(dtype, d.d_pos, d.d_hsml, d.d_m, d.d_rho, d.d_bin_q, d.nq,
+                                d.d_weights, d.n, &d.params, d.kernel, d.flags);
+                if (d.flags & S2G_FLAG_REDUCE) {
+                    m.image = img + off;
+                    m.out = d.d_out;
+                    off += d.params.npixels * d.params.npixels * (d.nq + 1);
+                } else {
+                    m.image = d.d_out;
+                    m.out = nullptr;
+                }
+                in.push_back(m);
+            }
+            s2g_stats st;
+            deposit2d_pass(c, in.data(), (int)in.size(), s, &st);
+            total.ms_device += st.ms_device;
+            total.ms_setup += st.ms_setup;
+            total.ms_norm += st.ms_norm;
+            total.ms_sort += st.ms_sort;
+            total.ms_gather += st.ms_gather;
+            total.ms_finalize += st.ms_finalize;
+            total.ms_small += st.ms_small;
+            total.n_particles += st.n_particles;
+            total.n_skipped += st.n_skipped;
+            total.n_small += st.n_small;
+            total.n_large += st.n_large;
+            total.n_large_exact += st.n_large_exact;
+            total.n_pairs += st.n_pairs;
+            total.n_gather_launches += st.n_gather_launches;
+        }
+        if (stats)
+            *stats = total;
+        return S2G_OK;
+    });
 }
 
 }  // extern "C"

@@ -40,8 +40,9 @@ struct Rec2 {
     double h;        // hsml in pixels, cic_2D.jl:83
     double s;        // area*w*dz (small / exact) or the final pixel scale (large)
     int32_t i0, i1, j0, j1;  // footprint, cic_shared.jl:46-52
-    int32_t cls;
-    int32_t fb;      // fallback (no pixel centre inside the kernel), cic_2D.jl:51-69
+    int16_t cls;
+    int16_t fb;      // fallback (no pixel centre inside the kernel), cic_2D.jl:51-69
+    int32_t map;     // map of the deposit pass the particle belongs to
 };
 
 // Per-particle record of the 3D deposit (cic_3D.jl:87-97, 150-155).
@@ -245,6 +246,37 @@ struct DevBuf {
     {
         if (p)
             (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// grow-only pinned host buffer (staging of small tables and read-backs).
+// Only grown while no copy from or into it is in flight.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *get(size_t need)
+    {
+        if (need > bytes) {
+            if (p)
+                (void)hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+            size_t b = need < 4096 ? 4096 : 2 * need;
+            if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) {
+                p = nullptr;
+                throw Error(S2G_ENOMEM, "pinned host allocation of " + std::to_string(b) +
+                                            " bytes failed");
+            }
+            bytes = b;
+        }
+        return p;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipHostFree(p);
         p = nullptr;
         bytes = 0;
     }

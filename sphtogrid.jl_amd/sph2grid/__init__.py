@@ -8,6 +8,7 @@ types.  All deposits run as HIP kernels through libsph2grid's C ABI
 (include/sph2grid.h); there is no CPU fallback.
 """
 from ._lib import Context, LibraryError, default_context, version  # noqa: F401
+from .batch import deposit_batch_device, map_batch  # noqa: F401
 from .healpix import BoundsError, filter_sort_particles, healpix_map  # noqa: F401
 from .kernels import (AbstractSPHKernel, Cubic, Quintic, WendlandC2,  # noqa: F401
                       WendlandC4, WendlandC6, WendlandC8)
@@ -26,5 +27,5 @@ __all__ = [
     "part_weight_XrayBand", "AbstractSPHKernel", "Cubic", "Quintic", "WendlandC2",
     "WendlandC4", "WendlandC6", "WendlandC8", "Context", "LibraryError",
     "default_context", "version", "healpix_map", "filter_sort_particles", "BoundsError",
-    "tsc_mapping",
+    "tsc_mapping", "map_batch", "deposit_batch_device",
 ]

@@ -75,22 +75,19 @@ def _check_float(name, a):
                         "strip units before mapping")
 
 
-def sphMapping(Pos, HSML, M, Rho, Bin_Quant, Weights=None, RM=None, *,
-               param: MappingParameters, kernel, show_progress: bool = True,
-               parallel: bool = False, reduce_image: bool = True,
-               return_both_maps: bool = False, dimensions: int = 2,
-               calc_mean: bool = False, stokes: bool = False, sort_z: bool = False,
-               exact_norm: bool = False, context=None, stats: dict | None = None):
-    """cic_interpolation.jl:35-273 on the GPU.
+class _Prepared:
+    """A sphMapping call after the host steps: centred, filtered, converted."""
+    __slots__ = ("arrays", "nq", "npix", "n", "flags", "kid", "kdim", "params", "dtype",
+                 "dtype_id", "dimensions", "return_both_maps")
 
-    ``Pos`` is mutated (centred) exactly as the reference mutates it.  ``RM``
-    is accepted and, as in the reference, has no effect on the maps (the
-    Faraday rotation is never enabled on this path, cic_interpolation.jl:152-155,
-    and is dropped on the parallel path, :191-196).  ``stokes`` implies
-    ``sort_z`` (:74-81).  ``exact_norm`` disables the closed-form per-particle
-    normalisation of large unclipped footprints (it changes results by less
-    than 1e-9 relative).
-    """
+
+def _prepare(Pos, HSML, M, Rho, Bin_Quant, Weights=None, *, param: MappingParameters, kernel,
+             show_progress: bool = True, reduce_image: bool = True, dimensions: int = 2,
+             calc_mean: bool = False, stokes: bool = False, sort_z: bool = False,
+             exact_norm: bool = False, return_both_maps: bool = False) -> _Prepared:
+    """The host steps of sphMapping before the deposit (cic_interpolation.jl:
+    60-150): argument checks, center_particles (mutates Pos), the image
+    filter, Float32/Float64 promotion and the interleaved layouts of the C ABI."""
     Pos = np.asarray(Pos)
     if Pos.ndim != 2 or Pos.shape[0] != 3:
         raise ValueError("Pos must be a 3 x N array")
@@ -106,11 +103,9 @@ def sphMapping(Pos, HSML, M, Rho, Bin_Quant, Weights=None, RM=None, *,
     kid, kdim = kernel_args(kernel)
     N_in = arrays["M"].shape[0]
 
-    t0 = time.perf_counter()
     Pos, par = center_particles(Pos, param)
     if stokes:
         sort_z = True
-        parallel = False
     p_in_image = filter_particles_in_image(Pos, par, sort_z)
     x = Pos[:, p_in_image]
     hsml = arrays["HSML"][p_in_image]
@@ -131,22 +126,52 @@ def sphMapping(Pos, HSML, M, Rho, Bin_Quant, Weights=None, RM=None, *,
         calc_mean = False
     dt = _common_dtype([x, hsml, m, rho, bin_q, weights])
     cvt = lambda a: np.ascontiguousarray(a, dtype=dt)
+    pm = _Prepared()
     # interleaved xyz per particle (Julia's 3 x N column-major memory)
-    x_il = np.ascontiguousarray(x.T, dtype=dt)
-    bq_il = cvt(bin_q.T if nq > 1 else bin_q)
-    hsml_c, m_c, rho_c, w_c = cvt(hsml), cvt(m), cvt(rho), cvt(weights)
+    pm.arrays = (np.ascontiguousarray(x.T, dtype=dt), cvt(hsml), cvt(m), cvt(rho),
+                 cvt(bin_q.T if nq > 1 else bin_q), cvt(weights))
+    pm.nq = nq
+    pm.npix = int(param.Npixels[0])
+    pm.n = N_map
+    pm.flags = ((_lib.S2G_FLAG_CALC_MEAN if calc_mean else 0) |
+                (_lib.S2G_FLAG_EXACT_NORM if exact_norm else 0) |
+                (_lib.S2G_FLAG_REDUCE if reduce_image else 0))
+    pm.kid, pm.kdim = kid, kdim
+    pm.params = _lib.Params(npixels=int(par.Npixels[0]), len2pix=float(par.len2pix))
+    pm.dtype = dt
+    pm.dtype_id = _lib.S2G_F32 if dt == np.float32 else _lib.S2G_F64
+    pm.dimensions = dimensions
+    pm.return_both_maps = return_both_maps
+    return pm
 
+
+def sphMapping(Pos, HSML, M, Rho, Bin_Quant, Weights=None, RM=None, *,
+               param: MappingParameters, kernel, show_progress: bool = True,
+               parallel: bool = False, reduce_image: bool = True,
+               return_both_maps: bool = False, dimensions: int = 2,
+               calc_mean: bool = False, stokes: bool = False, sort_z: bool = False,
+               exact_norm: bool = False, context=None, stats: dict | None = None):
+    """cic_interpolation.jl:35-273 on the GPU.
+
+    ``Pos`` is mutated (centred) exactly as the reference mutates it.  ``RM``
+    is accepted and, as in the reference, has no effect on the maps (the
+    Faraday rotation is never enabled on this path, cic_interpolation.jl:152-155,
+    and is dropped on the parallel path, :191-196).  ``stokes`` implies
+    ``sort_z`` (:74-81).  ``exact_norm`` disables the closed-form per-particle
+    normalisation of large unclipped footprints (it changes results by less
+    than 1e-9 relative).
+    """
+    t0 = time.perf_counter()
+    pm = _prepare(Pos, HSML, M, Rho, Bin_Quant, Weights, param=param, kernel=kernel,
+                  show_progress=show_progress, reduce_image=reduce_image,
+                  dimensions=dimensions, calc_mean=calc_mean, stokes=stokes, sort_z=sort_z,
+                  exact_norm=exact_norm)
+    x_il, hsml_c, m_c, rho_c, bq_il, w_c = pm.arrays
+    nq, npix, N_map, flags = pm.nq, pm.npix, pm.n, pm.flags
+    kid, kdim = pm.kid, pm.kdim
+    p = pm.params
+    dtype_id = pm.dtype_id
     ctx = context or _lib.default_context()
-    npix = int(param.Npixels[0])
-    p = _lib.Params(npixels=int(par.Npixels[0]), len2pix=float(par.len2pix))
-    flags = 0
-    if calc_mean:
-        flags |= _lib.S2G_FLAG_CALC_MEAN
-    if exact_norm:
-        flags |= _lib.S2G_FLAG_EXACT_NORM
-    if reduce_image:
-        flags |= _lib.S2G_FLAG_REDUCE
-    dtype_id = _lib.S2G_F32 if dt == np.float32 else _lib.S2G_F64
     st = _lib.Stats()
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
     args = (ctx.handle, dimensions, dtype_id, ptr(x_il), ptr(hsml_c), ptr(m_c), ptr(rho_c),

@@ -1,0 +1,152 @@
+"""GPU tests of the batched 2D deposit (s2g_deposit_batch): a batch must give
+each map what a separate deposit of that map gives (same kernels, different
+work split: <= 1e-12 relative), and the oracle's result for the map
+(rel-L1 <= 1e-5, as the single-map tests).  Covers mixed kernels and nq
+(separate passes), sides that are not multiples of the 64-pixel tile, empty
+maps, flat and reduced outputs, Float32 and Float64, and the argument checks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def _particles(n, seed, hmin_px, hmax_px, npix, nq, dtype):
+    r = np.random.default_rng(seed)
+    pos = r.uniform(0.0, 1.0, (3, n))
+    h = np.exp(r.uniform(np.log(hmin_px / npix), np.log(hmax_px / npix), n))
+    m, rho, w = (r.uniform(0.5, 2.0, n) for _ in range(3))
+    q = r.uniform(0.1, 5.0, (nq, n)) if nq > 1 else r.uniform(0.1, 5.0, n)
+    c = lambda a: np.asarray(a, dtype=dtype)
+    return dict(Pos=c(pos), HSML=c(h), M=c(m), Rho=c(rho), Bin_Quant=c(q), Weights=c(w))
+
+
+def _calls(s2g, dtype=np.float64):
+    spec = [  # (n, npix, nq, kernel, reduce, calc_mean, hmin_px, hmax_px)
+        (900, 96, 1, s2g.WendlandC6(2), True, False, 0.1, 40.0),
+        (0, 64, 1, s2g.WendlandC6(2), True, False, 0.1, 10.0),
+        (1500, 130, 2, s2g.Cubic(2), False, True, 0.05, 70.0),
+        (400, 17, 1, s2g.WendlandC6(2), False, True, 0.5, 30.0),
+        (2000, 200, 3, s2g.WendlandC6(2), True, True, 0.2, 90.0),
+        (700, 64, 2, s2g.Cubic(2), True, False, 0.1, 20.0),
+        (1200, 150, 1, s2g.WendlandC6(2), True, True, 0.05, 120.0),
+    ]
+    calls = []
+    for k, (n, npix, nq, ker, red, cm, hmin, hmax) in enumerate(spec):
+        d = _particles(n, 700 + k, hmin, hmax, npix, nq, dtype)
+        par = s2g.mappingParameters(x_lim=[0.0, 1.0], y_lim=[0.0, 1.0], z_lim=[0.0, 1.0],
+                                    Npixels=npix)
+        calls.append(dict(d, param=par, kernel=ker, reduce_image=red,
+                          return_both_maps=not red, calc_mean=cm))
+    return calls
+
+
+def _copy_call(c):
+    return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.items()}
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_matches_single_maps(s2g, dtype):
+    calls = _calls(s2g, dtype)
+    st = {}
+    got = s2g.map_batch([_copy_call(c) for c in calls], stats=st)
+    print("batch stats", st, flush=True)
+    assert st["n_particles"] == sum(c["M"].shape[0] for c in calls)
+    for k, c in enumerate(calls):
+        kw = _copy_call(c)
+        args = [kw.pop(x) for x in ("Pos", "HSML", "M", "Rho", "Bin_Quant", "Weights")]
+        ref = s2g.sphMapping(*args, show_progress=False, **kw)
+        assert got[k].shape == ref.shape, k
+        np.testing.assert_allclose(got[k], ref, rtol=1e-12, atol=1e-300 + 1e-13 * np.abs(ref).max(),
+                                   err_msg=f"map {k}")
+
+
+def test_batch_vs_oracle_flat(s2g, orc):
+    calls = _calls(s2g, np.float64)
+    got = s2g.map_batch([dict(_copy_call(c), reduce_image=False, return_both_maps=True)
+                         for c in calls])
+    for k, c in enumerate(calls):
+        n = c["M"].shape[0]
+        if n == 0:
+            assert not np.any(got[k]), k
+            continue
+        npix = int(c["param"].Npixels[0])
+        x = (c["Pos"] - 0.5).T.copy()  # centred on (0.5, 0.5, 0.5); every particle is inside
+        q = c["Bin_Quant"].T.copy() if c["Bin_Quant"].ndim > 1 else c["Bin_Quant"]
+        kname = type(c["kernel"]).__name__.lower()
+        ref = orc.cic_mapping_2D(x, c["HSML"], c["M"], c["Rho"], q, c["Weights"], npix,
+                                 float(npix), kname, 2, c["calc_mean"])
+        for p in range(ref.shape[1]):
+            e1, em = rel_l1(got[k][:, p], ref[:, p]), max_rel(got[k][:, p], ref[:, p])
+            print(f"map {k} plane {p}: rel_l1 {e1:.2e} max_rel {em:.2e}", flush=True)
+            assert e1 <= 1e-5 and em <= 1e-4, (k, p)
+
+
+def test_batch_workload_scaled(s2g):
+    """The batched workload (sedov-like maps, Nq = 3, Weights = Rho, sides
+    128-2048) at 1/200 of the particles, 12 maps, against per-map sphMapping."""
+    from sph2grid import workloads
+    maps = workloads.batched(nmaps=12, scale=1.0 / 200)
+    calls = []
+    for d in maps:
+        par = s2g.mappingParameters(x_lim=d["lims"][0], y_lim=d["lims"][1], z_lim=d["lims"][2],
+                                    Npixels=d["Npixels"])
+        calls.append(dict(Pos=d["Pos"], HSML=d["HSML"], M=d["M"], Rho=d["Rho"],
+                          Bin_Quant=d["Bin_Quant"], Weights=d["Weights"], param=par,
+                          kernel=s2g.WendlandC6(2), reduce_image=True, calc_mean=True))
+    st = {}
+    got = s2g.map_batch([_copy_call(c) for c in calls], stats=st)
+    print("batched workload stats", st, flush=True)
+    for k, c in enumerate(calls):
+        kw = _copy_call(c)
+        args = [kw.pop(x) for x in ("Pos", "HSML", "M", "Rho", "Bin_Quant", "Weights")]
+        ref = s2g.sphMapping(*args, show_progress=False, **kw)
+        err = np.abs(got[k] - ref).max() / max(np.abs(ref).max(), 1e-300)
+        assert err <= 1e-12, (k, err)
+
+
+def test_batch_repeat_and_empty(s2g):
+    """Repeated batches reuse the context's buffers; an empty batch is a no-op."""
+    calls = _calls(s2g, np.float32)
+    a = s2g.map_batch([_copy_call(c) for c in calls])
+    b = s2g.map_batch([_copy_call(c) for c in calls[2:4]])
+    c2 = s2g.map_batch([_copy_call(c) for c in calls])
+    for k in range(len(calls)):
+        np.testing.assert_allclose(a[k], c2[k], rtol=1e-12, atol=1e-13 * np.abs(a[k]).max())
+    assert len(b) == 2
+    assert s2g.map_batch([]) == []
+    s2g.deposit_batch_device([])
+
+
+def test_batch_argument_errors(s2g):
+    import torch
+    from sph2grid import _lib
+    dev = torch.device("cuda:0")
+    n, npix = 10, 32
+    t = lambda k: torch.zeros(k, dtype=torch.float32, device=dev)
+    good = dict(pos=t(3 * n), hsml=t(n), m=t(n), rho=t(n), bin_q=t(n), weights=t(n),
+                out=torch.zeros(2 * npix * npix, dtype=torch.float64, device=dev), n=n, nq=1,
+                npix=npix, len2pix=float(npix), kernel=s2g.Cubic(2))
+    s2g.deposit_batch_device([good])
+    ctx = s2g.default_context()
+    with pytest.raises(_lib.LibraryError) as e:
+        s2g.deposit_batch_device([good, dict(good, out=0)])
+    assert e.value.code == _lib.S2G_EINVAL and "map 1" in str(e.value)
+    with pytest.raises(_lib.LibraryError) as e:
+        s2g.deposit_batch_device([dict(good, kernel=17)])
+    assert e.value.code == _lib.S2G_EUNSUPPORTED
+    with pytest.raises(_lib.LibraryError) as e:
+        s2g.deposit_batch_device([dict(good, npix=0)])
+    assert e.value.code == _lib.S2G_EINVAL
+    d = (_lib.MapDesc * 1)()
+    d[0] = _lib.MapDesc(dims=3, nq=1, n=0, params=_lib.Params(npixels=8, len2pix=8.0),
+                        kernel=0, flags=0, d_out=good["out"].data_ptr())
+    rc = ctx.lib.s2g_deposit_batch(ctx.handle, _lib.S2G_F32, d, 1, None, None)
+    assert rc == _lib.S2G_EUNSUPPORTED
+    rc = ctx.lib.s2g_deposit_batch(ctx.handle, 7, d, 1, None, None)
+    assert rc == _lib.S2G_EINVAL
+    rc = ctx.lib.s2g_deposit_batch(ctx.handle, _lib.S2G_F32, None, 2, None, None)
+    assert rc == _lib.S2G_EINVAL
