@@ -19,9 +19,17 @@ struct Map2D {
     int64_t pbase;    // first particle of the map in the pass
     int32_t tbase;    // first gather tile of the map in the pass
     int32_t ntj;      // gather tiles per side
+    int32_t sbase;    // first small-footprint tile of the map in the pass
+    int32_t nts;      // small-footprint tiles per side
     uint32_t flags;
     int32_t pad;
 };
+
+// small-footprint path: particles are bucketed by the kST x kST tile holding
+// their footprint's first pixel; a work item accumulates its particles in a
+// kSW x kSW LDS window (the tile plus the largest footprint) per plane
+constexpr int kST = 32;
+constexpr int kSW = kST + kSmallSide2D;
 
 // host-side arguments of one map (Map2D without the pass offsets)
 struct Map2DArgs {
@@ -51,8 +59,16 @@ void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int 
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s);
 void launch_2d_gather(const GatherArgs &g, hipStream_t s);
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s);
-void launch_2d_small(int dtype, int kernel, int nq, const int32_t *list, int64_t ns,
-                     const Rec2 *rec, const Map2D *maps, hipStream_t s);
+// keys (global small tile) and values (particle) of the small list
+void launch_2d_small_keys(const int32_t *list, int64_t ns, const Rec2 *rec, const Map2D *maps,
+                          uint32_t *keys, int32_t *vals, hipStream_t s);
+// off[t] = first sorted entry of tile t, t in [0, ntiles]
+void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *off,
+                         hipStream_t s);
+// one block per work item (grid = max_items; blocks past *nitems exit)
+void launch_2d_small_tile(int dtype, int kernel, int np, const int4 *items,
+                          const int64_t *nitems, int64_t max_items, const int32_t *plist,
+                          const Rec2 *rec, const Map2D *maps, int nq, int plane0, hipStream_t s);
 // reduce_image_2D of every map of the pass that has an output
 void launch_2d_reduce_maps(const Map2D *maps, int nmaps, int ntiles, int nq, hipStream_t s);
 // reduce_image_2D of one flat image

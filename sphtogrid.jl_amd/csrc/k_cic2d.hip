@@ -14,7 +14,8 @@
 //   k_item_*         split every tile list into work items of <= G pairs
 //   k2d_gather       per-tile gather, FP64 register accumulation per pixel
 //   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
-//   k2d_small_*      scatter of small footprints (atomics on top of the image)
+//   k2d_small_*      small footprints: sorted by tile, LDS accumulation per item,
+//                    one atomic add per touched pixel into the image
 //   k2d_reduce_maps  reduce_image_2D: divide by the weight plane + transpose
 #include "k_cic2d.h"
 
@@ -234,6 +235,7 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
     __shared__ float s_hinv[kGBatch];
     __shared__ int s_i0[kGBatch], s_i1[kGBatch], s_j0[kGBatch], s_j1[kGBatch];
     __shared__ double s_mult[NP][kGBatch];
+    __shared__ int s_k[kGBatch][4];  // row-group range per (particle, strip), -1: none
 
     const int4 it = items[blockIdx.x];
     const int ntj = maps[it.w].ntj;
@@ -285,20 +287,47 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             const float ax = fmaxf((float)overlap1(s_x[b], s_h[b], fi), 0.0f);
             s_row[b][rr] = make_float2(xd * xd, ax);
         }
+        // row groups each wave's strip needs for each particle: the footprint
+        // box, narrowed to the rows that can reach the kernel disc from the
+        // strip's 16 columns, |x - i - 1/2| < sqrt(h^2 - dy^2) + 1 with dy the
+        // smallest distance of a column centre from y.  The 1-pixel margin
+        // keeps every skipped pixel at u >= 1 up to FP32 rounding (W = 0, or
+        // below 1e-20 relative).
+        if (threadIdx.x < cnt * 4) {
+            const int b = threadIdx.x >> 2, w = threadIdx.x & 3;
+            const int c0 = tj0 + 16 * w;
+            int packed = -1;
+            if (!(c0 + 15 < s_j0[b] || c0 > s_j1[b])) {
+                const int i0 = s_i0[b] - ti0, i1 = s_i1[b] - ti0;
+                int klo = i0 > 0 ? (i0 >> 2) : 0;
+                int khi = (i1 >> 2) < kGRows - 1 ? (i1 >> 2) : kGRows - 1;
+                if (s_hinv[b] != 0.0f) {
+                    const float xc = (float)s_x[b], yc = (float)s_y[b], hf = (float)s_h[b];
+                    const float dy = fmaxf(0.0f, fmaxf((float)c0 + 0.5f - yc,
+                                                       yc - ((float)c0 + 15.5f)));
+                    if (dy < hf + 1.0f) {
+                        const float hx = sqrtf(fmaxf(hf * hf - dy * dy, 0.0f)) + 1.0f;
+                        const int ra = (int)floorf(xc - 0.5f - hx) - ti0;
+                        const int rb = (int)ceilf(xc - 0.5f + hx) - ti0;
+                        klo = (ra >> 2) > klo ? (ra >> 2) : klo;
+                        khi = (rb >> 2) < khi ? (rb >> 2) : khi;
+                    } else {
+                        khi = -1;
+                    }
+                }
+                if (klo <= khi)
+                    packed = klo | (khi << 8);
+            }
+            s_k[b][w] = packed;
+        }
         __syncthreads();
         for (int b = 0; b < cnt; ++b) {
-            const int j0 = __builtin_amdgcn_readfirstlane(s_j0[b]);
-            const int j1 = __builtin_amdgcn_readfirstlane(s_j1[b]);
-            if (sj0 + 15 < j0 || sj0 > j1)
+            const int kk = __builtin_amdgcn_readfirstlane(s_k[b][wv]);
+            if (kk < 0)
                 continue;
-            const int i0 = __builtin_amdgcn_readfirstlane(s_i0[b]) - ti0;
-            const int i1 = __builtin_amdgcn_readfirstlane(s_i1[b]) - ti0;
-            const int klo = i0 > 0 ? (i0 >> 2) : 0;
-            const int khi = (i1 >> 2) < kGRows - 1 ? (i1 >> 2) : kGRows - 1;
-            if (klo > khi)
-                continue;
-            const double y = s_y[b], h = s_h[b];
+            const int klo = kk & 0xff, khi = kk >> 8;
             const float hinv = s_hinv[b];
+            const double y = s_y[b], h = s_h[b];
             const float yd = (float)coff1(y, fj);
             const float yd2 = yd * yd;
             const float ay = fmaxf((float)overlap1(y, h, fj), 0.0f);
@@ -365,66 +394,176 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Small footprints (box side <= 34): one wave per particle, FP64 weights,
-// wave-reduced normalisation, FP64 atomics into the image.
+// Small footprints (box side <= 34).  Particles are sorted by the 32 x 32
+// tile that holds their footprint's first pixel; a block takes a work item of
+// one tile, each wave one particle at a time: the FP64 normalisation sum over
+// the footprint (wave-reduced), then the FP64 pixel weights added into the
+// block's LDS window of the tile + 34 pixels (ds_add_f64).  The window is
+// added to the image once per item (FP64 atomics, only non-zero cells).
 // ---------------------------------------------------------------------------
-template <typename T, int K>
-__device__ __forceinline__ void k2d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list,
-                                                     const Rec2 *__restrict__ rec,
-                                                     const Map2D *__restrict__ maps, int nq)
+constexpr int kSmallThreads = 512;
+// particles whose records a block loads into LDS at a time
+constexpr int kSmallBatch = 64;
+// pixels per lane of the largest small footprint (34 x 34 over 64 lanes)
+constexpr int kPerLane = (kSmallSide2D * kSmallSide2D + kWave - 1) / kWave;
+
+__global__ void k2d_small_keys(const int32_t *__restrict__ list, int64_t ns,
+                               const Rec2 *__restrict__ rec, const Map2D *__restrict__ maps,
+                               uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
-    const int ln = threadIdx.x % kWave;
-    const int p = list[gw];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ns)
+        return;
+    const int p = list[e];
     const Rec2 r = rec[p];
-    const int64_t npix = maps[r.map].npix;
-    double *image = maps[r.map].image;
-    const T *binq = (const T *)maps[r.map].binq;
-    const int64_t pl = p - maps[r.map].pbase;
+    const Map2D &md = maps[r.map];
+    keys[e] = (uint32_t)(md.sbase + (r.i0 / kST) * md.nts + r.j0 / kST);
+    vals[e] = p;
+}
+
+__global__ void k_tile_offsets(const uint32_t *__restrict__ keys, int64_t ns, int ntiles,
+                               int64_t *__restrict__ off)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles)
+        return;
+    int64_t lo = 0, hi = ns;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint32_t)t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    off[t] = lo;
+}
+
+// One small particle on one wave: FP64 row / column terms in the wave's LDS
+// tables, the pixel weights kept in registers (at most KMAX per lane), the
+// wave-reduced normalisation sum, then the weights times the particle's plane
+// multipliers added into the block's LDS window (ds_add_f64).
+template <int K, int NP, int KMAX>
+__device__ __forceinline__ void small_particle(const Rec2 &r, int e, int ln, int oi, int oj,
+                                               double2 *rt_tab, double2 *ct_tab,
+                                               double (*s_mult)[kSmallBatch],
+                                               double (*acc)[kSW * kSW])
+{
     const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
     const int fp = iw * jw;
     const double hinv = r.fb ? 0.0 : 1.0 / r.h;
+    if (ln < iw) {
+        const double fi = (double)(r.i0 + ln);
+        const double xd = coff1(r.x, fi);
+        rt_tab[ln] = make_double2(xd * xd, overlap1(r.x, r.h, fi));
+    }
+    if (ln < jw) {
+        const double fj = (double)(r.j0 + ln);
+        const double yd = coff1(r.y, fj);
+        ct_tab[ln] = make_double2(yd * yd, overlap1(r.y, r.h, fj));
+    }
     const float invjw = 1.0f / (float)jw;
+    double wk[KMAX];
+    int cell[KMAX];
     double sum = 0.0;
-    for (int f = ln; f < fp; f += kWave) {
-        const int di = (int)(((float)f + 0.5f) * invjw);
-        const int dj = f - di * jw;
-        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj);
-        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj);
-        const double A = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj);
-        const double u = sqrt(xd * xd + yd * yd) * hinv;
-        sum += shape_of<K, double>(u) * A;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        wk[k] = 0.0;
+        cell[k] = 0;
+        const int f = ln + k * kWave;
+        if (f < fp) {
+            const int di = (int)(((float)f + 0.5f) * invjw);
+            const int dj = f - di * jw;
+            const double2 rt = rt_tab[di], ct = ct_tab[dj];
+            const double u = sqrt(rt.x + ct.x) * hinv;
+            wk[k] = shape_of<K, double>(u) * (rt.y * ct.y);
+            cell[k] = (r.i0 + di - oi) * kSW + (r.j0 + dj - oj);
+            sum += wk[k];
+        }
     }
     sum = wave_sum(sum);
     if (!(sum != 0.0))
         return;
-    const double sc = r.s / sum;
-    const int64_t nd = npix * npix;
-    double *wimg = image + (int64_t)nq * nd;
-    for (int f = ln; f < fp; f += kWave) {
-        const int di = (int)(((float)f + 0.5f) * invjw);
-        const int dj = f - di * jw;
-        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj);
-        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj);
-        const double A = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj);
-        const double u = sqrt(xd * xd + yd * yd) * hinv;
-        const double pw = shape_of<K, double>(u) * A * sc;
-        if (pw != 0.0) {
-            const int64_t idx = (int64_t)(r.i0 + di) * npix + (r.j0 + dj);
-            unsafeAtomicAdd(&wimg[idx], pw);
-            for (int q = 0; q < nq; ++q)
-                unsafeAtomicAdd(&image[(int64_t)q * nd + idx], ld(binq, pl * nq + q) * pw);
+    const double inv = 1.0 / sum;
+    double mult[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+        mult[q] = s_mult[q][e] * inv;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (wk[k] != 0.0) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                unsafeAtomicAdd(&acc[q][cell[k]], wk[k] * mult[q]);
         }
     }
 }
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k2d_small_atomic(const int32_t *__restrict__ list,
-                                                        int64_t ns, const Rec2 *__restrict__ rec,
-                                                        const Map2D *__restrict__ maps, int nq)
+
+template <typename T, int K, int NP>
+__global__ __launch_bounds__(kSmallThreads) void k2d_small_tile(
+    const int4 *__restrict__ items, const int64_t *__restrict__ nitems,
+    const int32_t *__restrict__ plist, const Rec2 *__restrict__ rec,
+    const Map2D *__restrict__ maps, int nq, int plane0)
 {
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ns;
-         gw += nw)
-        k2d_small_atomic_one<T, K>(gw, list, rec, maps, nq);
+    __shared__ double acc[NP][kSW * kSW];
+    // per-wave row / column terms of the current particle: (offset^2, overlap)
+    __shared__ double2 s_rt[kSmallThreads / kWave][kSmallSide2D];
+    __shared__ double2 s_ct[kSmallThreads / kWave][kSmallSide2D];
+    // records and plane multipliers of a batch of particles, loaded together
+    __shared__ Rec2 s_rec[kSmallBatch];
+    __shared__ double s_mult[NP][kSmallBatch];
+    if ((int64_t)blockIdx.x >= *nitems)
+        return;
+    const int4 it = items[blockIdx.x];
+    const int nts = maps[it.w].nts;
+    const int tl = it.x - maps[it.w].sbase;
+    const int oi = (tl / nts) * kST, oj = (tl % nts) * kST;
+    const int64_t npix = maps[it.w].npix, pbase = maps[it.w].pbase;
+    const T *binq = (const T *)maps[it.w].binq;
+    double *image = maps[it.w].image;
+    for (int c = threadIdx.x; c < NP * kSW * kSW; c += kSmallThreads)
+        (&acc[0][0])[c] = 0.0;
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    for (int base = 0; base < it.z; base += kSmallBatch) {
+        const int cnt = it.z - base < kSmallBatch ? it.z - base : kSmallBatch;
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const int p = plist[it.y + base + threadIdx.x];
+            const Rec2 r = rec[p];
+            s_rec[threadIdx.x] = r;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const int P = plane0 + q;
+                s_mult[q][threadIdx.x] =
+                    P == nq ? r.s : r.s * ld(binq, (int64_t)(p - pbase) * nq + (P < nq ? P : 0));
+            }
+        }
+        __syncthreads();
+        for (int e = wv; e < cnt; e += kSmallThreads / kWave) {
+            const Rec2 r = s_rec[e];
+            const int fp = (r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1);
+            // most footprints need at most 4 pixels per lane: a short
+            // unrolled variant for them, the full one for the rest
+            if (__builtin_amdgcn_readfirstlane(fp) <= 4 * kWave)
+                small_particle<K, NP, 4>(r, e, ln, oi, oj, s_rt[wv], s_ct[wv], s_mult, acc);
+            else
+                small_particle<K, NP, kPerLane>(r, e, ln, oi, oj, s_rt[wv], s_ct[wv], s_mult,
+                                                acc);
+        }
+    }
+    __syncthreads();
+    const int64_t nd = npix * npix;
+    for (int c = threadIdx.x; c < kSW * kSW; c += kSmallThreads) {
+        const int i = oi + c / kSW, j = oj + c % kSW;
+        if (i >= npix || j >= npix)
+            continue;
+        const int64_t idx = (int64_t)i * npix + j;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const double v = acc[q][c];
+            if (v != 0.0)
+                unsafeAtomicAdd(&image[(int64_t)(plane0 + q) * nd + idx], v);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -565,18 +704,49 @@ void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
     }
 }
 
-void launch_2d_small(int dtype, int kernel, int nq, const int32_t *list, int64_t ns,
-                     const Rec2 *rec, const Map2D *maps, hipStream_t s)
+void launch_2d_small_keys(const int32_t *list, int64_t ns, const Rec2 *rec, const Map2D *maps,
+                          uint32_t *keys, int32_t *vals, hipStream_t s)
 {
     if (ns == 0)
         return;
-    const dim3 g((unsigned)wave_blocks(ns, 256)), b(256);
+    hipLaunchKernelGGL(k2d_small_keys, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, list,
+                       ns, rec, maps, keys, vals);
+}
+
+void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *off,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tile_offsets, dim3((unsigned)((ntiles + 1 + 255) / 256)), dim3(256), 0,
+                       s, keys, ns, ntiles, off);
+}
+
+template <typename T, int K>
+static void small_tile_k(int np, dim3 g, const int4 *items, const int64_t *nitems,
+                         const int32_t *plist, const Rec2 *rec, const Map2D *maps, int nq,
+                         int plane0, hipStream_t s)
+{
+    const dim3 b(kSmallThreads);
+    switch (np) {
+    case 1: hipLaunchKernelGGL((k2d_small_tile<T, K, 1>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
+    case 2: hipLaunchKernelGGL((k2d_small_tile<T, K, 2>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
+    case 3: hipLaunchKernelGGL((k2d_small_tile<T, K, 3>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
+    default: hipLaunchKernelGGL((k2d_small_tile<T, K, 4>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
+    }
+}
+
+void launch_2d_small_tile(int dtype, int kernel, int np, const int4 *items,
+                          const int64_t *nitems, int64_t max_items, const int32_t *plist,
+                          const Rec2 *rec, const Map2D *maps, int nq, int plane0, hipStream_t s)
+{
+    if (max_items == 0)
+        return;
+    const dim3 g((unsigned)max_items);
     if (dtype == S2G_F32) {
-        S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL((k2d_small_atomic<float, KK>), g, b, 0, s,
-                                                  list, ns, rec, maps, nq));
+        S2G_K2D_SWITCH(kernel, (small_tile_k<float, KK>(np, g, items, nitems, plist, rec, maps,
+                                                         nq, plane0, s)));
     } else {
-        S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL((k2d_small_atomic<double, KK>), g, b, 0, s,
-                                                  list, ns, rec, maps, nq));
+        S2G_K2D_SWITCH(kernel, (small_tile_k<double, KK>(np, g, items, nitems, plist, rec, maps,
+                                                          nq, plane0, s)));
     }
 }
 

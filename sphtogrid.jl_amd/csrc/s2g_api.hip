@@ -7,6 +7,7 @@
 
 #include "k_cic3d.h"
 #include "k_healpix.h"
+#include "k_sort.h"
 #include "k_tsc.h"
 #include "s2g_internal.h"
 
@@ -26,6 +27,9 @@ struct s2g_ctx {
     DevBuf rec, cls, counts, scan_tmp, lists[3], cnt, pairs, nitems, items, tile_items, partial;
     // per-pass tables: map table + particle bases, class bounds, tile maps + chunks
     DevBuf tab, bounds, tiles;
+    // small-footprint path: sort keys / values (double-buffered), sort scratch,
+    // tile offsets, work items
+    DevBuf skeys, svals, sort_tmp, soff, snitems, sitems, stile;
     PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
     DevBuf hp_rec, hp_nunits, hp_units, hp_part;
@@ -38,13 +42,6 @@ struct s2g_ctx {
 namespace {
 
 const char *kVersion = "sph2grid 0.1.0 gfx950";
-
-int fail(s2g_ctx *c, int code, const std::string &m)
-{
-    if (c)
-        c->err = m;
-    return code;
-}
 
 template <typename F>
 int guarded(s2g_ctx *c, F &&f)
@@ -256,13 +253,16 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
 {
     const int dtype = mi[0].a.dtype, kernel = mi[0].a.kernel, nq = mi[0].a.nq;
     const int nplanes = nq + 1;
-    // per-map table: particle and tile offsets of the pass
+    // per-map table: particle and tile offsets of the pass, and the grids of
+    // the small-footprint tiles (one bucket per tile)
     const size_t btab = sizeof(Map2D) * nmaps, bpb = sizeof(int64_t) * (nmaps + 1);
-    char *h = (char *)c->h_tab.get(btab + bpb);
+    const size_t bst = sizeof(TileMap) * nmaps;
+    char *h = (char *)c->h_tab.get(btab + bpb + bst);
     Map2D *tab = (Map2D *)h;
     int64_t *pb = (int64_t *)(h + btab);
+    TileMap *stm = (TileMap *)(h + btab + bpb);
     std::vector<TileMap> tm(nmaps);
-    int64_t ntot = 0, ttot = 0;
+    int64_t ntot = 0, ttot = 0, stot = 0;
     bool any_out = false;
     for (int m = 0; m < nmaps; ++m) {
         const Map2DArgs &a = mi[m].a;
@@ -284,26 +284,33 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         const int ntj = (int)((a.npix + kGT - 1) / kGT);
         e.ntj = ntj;
         e.tbase = (int32_t)ttot;
+        const int nts = (int)((a.npix + kST - 1) / kST);
+        e.nts = nts;
+        e.sbase = (int32_t)stot;
         pb[m] = ntot;
         tm[m] = TileMap{0, (int32_t)ttot, ntj * ntj, ntj, 1, 0, 0};
+        // bucket t of the small path spans sorted entries [off[t], off[t+1])
+        stm[m] = TileMap{stot, (int32_t)stot, nts * nts, nts, 1, 1, 0};
         ntot += a.n;
         ttot += (int64_t)ntj * ntj;
+        stot += (int64_t)nts * nts;
         any_out = any_out || mi[m].out;
-        if (ntot > (int64_t)INT32_MAX || ttot > (int64_t)INT32_MAX)
+        if (ntot > (int64_t)INT32_MAX || stot > (int64_t)INT32_MAX)
             throw Error(S2G_EUNSUPPORTED, "deposit pass too large (2^31 particles or tiles)");
     }
     pb[nmaps] = ntot;
-    char *d = (char *)c->tab.get(btab + bpb);
+    char *d = (char *)c->tab.get(btab + bpb + bst);
     const Map2D *d_maps = (const Map2D *)d;
     const int64_t *d_pbase = (const int64_t *)(d + btab);
-    const int ntiles = (int)ttot;
+    const TileMap *d_stm = (const TileMap *)(d + btab + bpb);
+    const int ntiles = (int)ttot, nstiles = (int)stot;
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->n_particles = ntot;
     }
     hipEvent_t *E = c->stage;
     S2G_HIP(hipEventRecord(E[0], s));
-    S2G_HIP(hipMemcpyAsync(d, h, btab + bpb, hipMemcpyHostToDevice, s));
+    S2G_HIP(hipMemcpyAsync(d, h, btab + bpb + bst, hipMemcpyHostToDevice, s));
 
     ClassLists cl;
     Rec2 *rec = nullptr;
@@ -367,7 +374,40 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
 
     const int eS = s2g_ctx::kNumStageEvents - 3;
     S2G_HIP(hipEventRecord(E[eS], s));
-    launch_2d_small(dtype, kernel, nq, cl.l0, cl.ns, rec, d_maps, s);
+    if (cl.ns > 0) {
+        // bucket the small list by tile: stable radix sort of (tile, particle)
+        const int64_t ns = cl.ns;
+        uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * ns);
+        int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * ns);
+        launch_2d_small_keys(cl.l0, ns, rec, d_maps, keys, vals, s);
+        int end_bit = 1;
+        while (end_bit < 32 && ((int64_t)1 << end_bit) < (int64_t)nstiles)
+            ++end_bit;
+        const size_t tb = sort_pairs_tmp_bytes(ns, end_bit);
+        void *stmp = c->sort_tmp.get(tb ? tb : 1);
+        sort_pairs(stmp, tb, keys, keys + ns, vals, vals + ns, ns, end_bit, s);
+        int64_t *soff = (int64_t *)c->soff.get(sizeof(int64_t) * (nstiles + 1));
+        launch_tile_offsets(keys + ns, ns, nstiles, soff, s);
+        // work items of <= Gs particles; the item count stays on the device and
+        // the grid is its upper bound nstiles + ns / Gs
+        int64_t Gs = (ns + (int64_t)c->ncu * 8 - 1) / ((int64_t)c->ncu * 8);
+        if (Gs < 256)
+            Gs = 256;
+        const int64_t max_items = nstiles + (ns + Gs - 1) / Gs;
+        if (max_items > ((int64_t)1 << 31) / 512)
+            throw Error(S2G_EUNSUPPORTED, "too many small-footprint work items");
+        int64_t *snit = (int64_t *)c->snitems.get(sizeof(int64_t) * (nstiles + 1));
+        int4 *sitems = (int4 *)c->sitems.get(sizeof(int4) * max_items);
+        int64_t *stile = (int64_t *)c->stile.get(sizeof(int64_t) * 2 * nstiles);
+        int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nstiles + 1));
+        launch_items(soff, d_stm, nmaps, nstiles, Gs, snit, tmp, nullptr, nullptr, false, s);
+        launch_items(soff, d_stm, nmaps, nstiles, Gs, snit, tmp, sitems, stile, true, s);
+        for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
+            const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
+            launch_2d_small_tile(dtype, kernel, np, sitems, snit + nstiles, max_items, vals + ns,
+                                 rec, d_maps, nq, plane0, s);
+        }
+    }
     S2G_HIP(hipEventRecord(E[eS + 1], s));
     if (any_out)
         launch_2d_reduce_maps(d_maps, nmaps, ntiles, nq, s);
@@ -662,6 +702,9 @@ void s2g_destroy(s2g_ctx *c)
     c->tab.release();
     c->bounds.release();
     c->tiles.release();
+    for (DevBuf *b : {&c->skeys, &c->svals, &c->sort_tmp, &c->soff, &c->snitems, &c->sitems,
+                      &c->stile})
+        b->release();
     c->h_tab.release();
     c->h_bounds.release();
     c->h_tiles.release();

@@ -196,11 +196,31 @@ inline int64_t wave_blocks(int64_t items, int threads_per_block)
     return b < 1 ? 1 : (b > cap ? cap : b);
 }
 
+// v + the value of another lane of the same 16-lane row (DPP lane pattern)
+template <int kCtrl>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, true);
+    return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a fully active wave, returned on every lane.  DPP
+// steps inside each 16-lane row (quad swaps, half-row and row mirrors), then
+// the four row sums read from lanes 0, 16, 32, 48.  The order is fixed, so
+// the result is reproducible.
 __device__ __forceinline__ double wave_sum(double v)
 {
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, kWave);
-    return v;
+    v = dpp_add<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x141>(v);  // row_half_mirror
+    v = dpp_add<0x140>(v);  // row_mirror
+    double r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r[k] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * k),
+                                __builtin_amdgcn_readlane(__double2loint(v), 16 * k));
+    return (r[0] + r[1]) + (r[2] + r[3]);
 }
 
 // ---------------------------------------------------------------------------
