@@ -188,21 +188,27 @@ int s2g_tsc3d_device(s2g_ctx *ctx, int dtype, const void *d_pos,
                      const void *d_bin_q, int64_t n, const s2g_params *params,
                      uint32_t flags, double *d_grid_out, void *stream);
 
-/* One map of a batch (device pointers). */
+/* One map of a batch (device pointers; inputs centred and filtered as for
+ * s2g_deposit). */
 typedef struct s2g_map_desc {
-    int32_t dims;      /* 2 (3 reserved) */
+    int32_t dims;      /* 2 (other values: S2G_EUNSUPPORTED) */
     int32_t nq;
     int64_t n;
     s2g_params params;
     int32_t kernel;
-    uint32_t flags;    /* S2G_FLAG_CALC_MEAN / S2G_FLAG_REDUCE */
+    uint32_t flags;    /* S2G_FLAG_CALC_MEAN / S2G_FLAG_EXACT_NORM / S2G_FLAG_REDUCE */
     const void *d_pos, *d_hsml, *d_m, *d_rho, *d_bin_q, *d_weights;
-    double *d_out;     /* flat image, or reduced map with S2G_FLAG_REDUCE */
+    double *d_out;     /* flat image (nq + 1 planes), or with S2G_FLAG_REDUCE the
+                          reduced map of reduce_image_2D(reduce_image = true) */
 } s2g_map_desc;
 
-/* Many independent maps (all with the same dtype) in one device pass.  The
- * launches are recorded once into a hipGraph keyed by the batch shape and
- * replayed on later calls with the same shape. */
+/* Many independent 2D maps, all with the same dtype.  Replaces a loop of
+ * sphMapping calls (cic_interpolation.jl:35-273, one call per map, as in
+ * distributed_mapping/cic.jl:24-110).  Maps with the same kernel and nq share
+ * one device pass: their particles and tiles are concatenated, and the host
+ * synchronises a fixed number of times per pass, not per map.  Synchronous
+ * on return; `stream` may be NULL (the context's stream).  Errors name the
+ * offending map. */
 int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
                       int nmaps, void *stream, s2g_stats *stats);
 
