@@ -38,8 +38,12 @@
  *   - Return value: S2G_OK (0) or a negative S2G_E* code; the message is
  *     available from s2g_last_error().  No C++ exception crosses the ABI.
  *   - A context is not thread safe; use one per host thread.
- *   - *_device variants take device pointers and a hipStream_t (as void*),
- *     enqueue asynchronously and return without synchronising.
+ *   - *_device variants and s2g_deposit_batch take device pointers and a
+ *     hipStream_t (as void*; NULL = the context's stream).  The deposits
+ *     (s2g_deposit_device, s2g_healpix_deposit_device, s2g_deposit_batch)
+ *     size their work lists from device-side counts, so they synchronise the
+ *     stream and return with the result complete; s2g_reduce_device and
+ *     s2g_tsc3d_device only enqueue.
  */
 #ifndef SPH2GRID_H
 #define SPH2GRID_H
