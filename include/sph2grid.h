@@ -22,6 +22,12 @@
  *                           commented out in the reference)
  *   s2g_deposit_batch    <- many sphMapping calls (no single reference entry;
  *                           the closest is distributed_mapping/cic.jl:24-110)
+ *   s2g_accumulate_device,
+ *   s2g_healpix_reduce_device
+ *                        <- the per-subfile summation and the final reduce of
+ *                           distributed_cic_map / distributed_allsky_map
+ *                           (distributed_mapping/cic.jl:63-73,
+ *                           distributed_mapping/healpix.jl:50-73)
  *
  * Conventions
  *   - All arrays are plain pointers with explicit sizes; no ownership is
@@ -215,6 +221,19 @@ typedef struct s2g_map_desc {
  * offending map. */
 int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
                       int nmaps, void *stream, s2g_stats *stats);
+
+/* Subfile reductions of distributed_cic_map / distributed_allsky_map: add
+ * one subfile's map into a resident sum, skipping NaN and Inf values
+ * (distributed_mapping/cic.jl:63-73, distributed_mapping/healpix.jl:50-58).
+ * d_dst[i] += d_src[i] for finite d_src[i].  Enqueues only. */
+int s2g_accumulate_device(s2g_ctx *ctx, const double *d_src, double *d_dst, int64_t count,
+                          void *stream);
+
+/* reduce_image of distributed_allsky_map (distributed_mapping/healpix.jl:67-73):
+ * d_map[i] /= d_weights[i] where the weight is finite and non-zero.
+ * Enqueues only. */
+int s2g_healpix_reduce_device(s2g_ctx *ctx, double *d_map, const double *d_weights,
+                              int64_t npix, void *stream);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "k_accum.h"
 #include "k_cic3d.h"
 #include "k_healpix.h"
 #include "k_sort.h"
@@ -1133,6 +1134,36 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
         }
         if (stats)
             *stats = total;
+        return S2G_OK;
+    });
+}
+
+int s2g_accumulate_device(s2g_ctx *c, const double *d_src, double *d_dst, int64_t count,
+                          void *stream)
+{
+    return guarded(c, [&]() {
+        if (count < 0)
+            throw Error(S2G_EINVAL, "count < 0");
+        if (count > 0 && (!d_src || !d_dst))
+            throw Error(S2G_EINVAL, "array pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        launch_accumulate(d_src, d_dst, count, s);
+        S2G_HIP(hipGetLastError());
+        return S2G_OK;
+    });
+}
+
+int s2g_healpix_reduce_device(s2g_ctx *c, double *d_map, const double *d_weights, int64_t npix,
+                              void *stream)
+{
+    return guarded(c, [&]() {
+        if (npix < 0)
+            throw Error(S2G_EINVAL, "npix < 0");
+        if (npix > 0 && (!d_map || !d_weights))
+            throw Error(S2G_EINVAL, "map pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        launch_hp_reduce(d_map, d_weights, npix, s);
+        S2G_HIP(hipGetLastError());
         return S2G_OK;
     });
 }

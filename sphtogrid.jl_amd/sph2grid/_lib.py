@@ -32,7 +32,9 @@ S2G_FLAG_REDUCE = 0x8
 EXPORTS = (
     "s2g_create", "s2g_destroy", "s2g_last_error", "s2g_version",
     "s2g_deposit", "s2g_deposit_device", "s2g_reduce", "s2g_reduce_device",
-    "s2g_map", "s2g_healpix_deposit", "s2g_healpix_deposit_device", "s2g_tsc3d", "s2g_tsc3d_device", "s2g_deposit_batch",
+    "s2g_map", "s2g_healpix_deposit", "s2g_healpix_deposit_device", "s2g_tsc3d",
+    "s2g_tsc3d_device", "s2g_deposit_batch", "s2g_accumulate_device",
+    "s2g_healpix_reduce_device",
 )
 
 
@@ -131,6 +133,10 @@ def load(path: str | None = None):
         L.s2g_deposit_batch.restype = C.c_int
         L.s2g_deposit_batch.argtypes = [vp, C.c_int, C.POINTER(MapDesc), C.c_int, vp,
                                         C.POINTER(Stats)]
+        L.s2g_accumulate_device.restype = C.c_int
+        L.s2g_accumulate_device.argtypes = [vp, vp, vp, i64, vp]
+        L.s2g_healpix_reduce_device.restype = C.c_int
+        L.s2g_healpix_reduce_device.argtypes = [vp, vp, vp, i64, vp]
         if path is None:
             _lib = L
         return L

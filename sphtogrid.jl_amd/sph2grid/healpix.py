@@ -58,11 +58,12 @@ def filter_sort_particles(Pos, Hsml, M, Rho, Bin_q, Weights, center, radius_limi
             np.asarray(Bin_q)[idx], np.asarray(Weights)[idx])
 
 
-def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
-                radius_limits=(0.0, np.inf), Nside: int = 1024, kernel,
-                show_progress: bool = True, output_from_all_workers: bool = False,
-                calc_mean: bool = True, context=None, stats: dict | None = None):
-    """main.jl:92-227 on the GPU.  ``Pos`` (3 x N, Float64) is mutated."""
+def _prepare_healpix(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
+                     radius_limits=(0.0, np.inf), kernel, calc_mean: bool = True):
+    """The host steps of healpix_map before the particle loop (main.jl:100-143):
+    type check, the early return of :114-121 (None), filter_sort_particles
+    (mutates Pos) and the interleaved layout of the C ABI.  Returns
+    (arrays, n, kernel id, kernel dim, flags) or None."""
     Pos = np.asarray(Pos)
     arrays = [Pos] + [np.asarray(a) for a in (Hsml, M, Rho, Bin_q, Weights)]
     if any(a.dtype != np.float64 for a in arrays):
@@ -72,22 +73,34 @@ def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
     if Pos.ndim != 2 or Pos.shape[0] != 3:
         raise ValueError("Pos must be a 3 x N array")
     kid, kdim = kernel_args(kernel)
-    npix = 12 * int(Nside) ** 2
-    allsky = np.zeros(npix)
-    weight = np.zeros(npix)
     # main.jl:114-121
     if not calc_mean and np.sum(Bin_q) == 0:
-        return allsky, weight
-    t0 = time.perf_counter()
+        return None
     pos, hsml, m, rho, bq, w = filter_sort_particles(Pos, Hsml, M, Rho, Bin_q, Weights,
                                                      np.asarray(center, dtype=np.float64),
                                                      radius_limits, calc_mean)
     n = hsml.shape[0]
-    x_il = np.ascontiguousarray(pos.T)
     cv = lambda a: np.ascontiguousarray(a, dtype=np.float64)
-    hsml, m, rho, bq, w = (cv(a) for a in (hsml, m, rho, bq, w))
-    ctx = context or _lib.default_context()
+    arrs = (np.ascontiguousarray(pos.T),) + tuple(cv(a) for a in (hsml, m, rho, bq, w))
     flags = _lib.S2G_FLAG_CALC_MEAN if calc_mean else 0
+    return arrs, n, kid, kdim, flags
+
+
+def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
+                radius_limits=(0.0, np.inf), Nside: int = 1024, kernel,
+                show_progress: bool = True, output_from_all_workers: bool = False,
+                calc_mean: bool = True, context=None, stats: dict | None = None):
+    """main.jl:92-227 on the GPU.  ``Pos`` (3 x N, Float64) is mutated."""
+    npix = 12 * int(Nside) ** 2
+    allsky = np.zeros(npix)
+    weight = np.zeros(npix)
+    t0 = time.perf_counter()
+    prep = _prepare_healpix(Pos, Hsml, M, Rho, Bin_q, Weights, center=center,
+                            radius_limits=radius_limits, kernel=kernel, calc_mean=calc_mean)
+    if prep is None:
+        return allsky, weight
+    (x_il, hsml, m, rho, bq, w), n, kid, kdim, flags = prep
+    ctx = context or _lib.default_context()
     st = _lib.Stats()
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
     rc = ctx.lib.s2g_healpix_deposit(ctx.handle, _lib.S2G_F64, ptr(x_il), ptr(hsml), ptr(m),
