@@ -192,11 +192,10 @@ def distributed_cic_map(cic_filename: str, Nsubfiles: int, mapping_function,
     out = torch.empty(nd * nout, dtype=torch.float64, device=dev)
     ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, Ndim, npix, Nimages, flags, vp(total),
                                         vp(out), cstream), "s2g_reduce_device")
-    torch.cuda.synchronize(dev)
-    st["wall_s"] = time.perf_counter() - t0
-    st["wait_s"] = wait
     shape = (npix, npix, nout) if Ndim == 2 else (npix, npix, npix)
     image = out.cpu().numpy().reshape(shape, order="F")
+    st["wall_s"] = time.perf_counter() - t0
+    st["wait_s"] = wait
     print("saving")
     if Ndim == 3 and vtk:
         write_vtk_image(cic_filename, image, "map", param, snap=snap, units=units)
@@ -282,10 +281,9 @@ def distributed_allsky_map(allsky_filename: str, Nside: int, Nsubfiles: int, map
         ctx.check(ctx.lib.s2g_healpix_reduce_device(ctx.handle, vp(total),
                                                     C.c_void_p(total.data_ptr() + 8 * npx),
                                                     npx, cstream), "s2g_healpix_reduce_device")
-    torch.cuda.synchronize(dev)
+    both = total.cpu().numpy()
     st["wall_s"] = time.perf_counter() - t0
     st["wait_s"] = wait
-    both = total.cpu().numpy()
     allsky, weights = both[:npx].copy(), both[npx:].copy()
     print("saving")
     if os.path.isfile(allsky_filename):
