@@ -22,6 +22,7 @@
  *                           commented out in the reference)
  *   s2g_deposit_batch    <- many sphMapping calls (no single reference entry;
  *                           the closest is distributed_mapping/cic.jl:24-110)
+ *   s2g_mass_density     <- mass_density (effects/mass_density.jl:30-99)
  *   s2g_accumulate_device,
  *   s2g_healpix_reduce_device
  *                        <- the per-subfile summation and the final reduce of
@@ -234,6 +235,23 @@ int s2g_accumulate_device(s2g_ctx *ctx, const double *d_src, double *d_dst, int6
  * Enqueues only. */
 int s2g_healpix_reduce_device(s2g_ctx *ctx, double *d_map, const double *d_weights,
                               int64_t npix, void *stream);
+
+/* mass_density (effects/mass_density.jl:30-99): SPH density and smoothing
+ * length of an arbitrary particle set.  hsml_i = distance to the farthest of
+ * the nneighbors + 1 nearest particles (the particle itself included);
+ * rho_i = sum_j m_j W(r_ij / hsml_i, 1 / hsml_i) over them, with the kernel's
+ * own normalisation in kernel_dim (2 or 3) dimensions, then the Wendland
+ * kernels' bias correction of Dehnen & Aly (2012, eqs. 18-19).  pos is 3 x n
+ * interleaved; boxsize = NULL or all zero: Euclidean distances, all positive:
+ * periodic box (minimum image per dimension).  nneighbors + 1 <= n. */
+int s2g_mass_density(s2g_ctx *ctx, int dtype, const void *pos, const void *mass, int64_t n,
+                     int kernel, int kernel_dim, int64_t nneighbors, const double *boxsize,
+                     double *rho_out, double *hsml_out, s2g_stats *stats);
+/* Same on device pointers (synchronises the stream before returning). */
+int s2g_mass_density_device(s2g_ctx *ctx, int dtype, const void *d_pos, const void *d_mass,
+                            int64_t n, int kernel, int kernel_dim, int64_t nneighbors,
+                            const double *boxsize, double *d_rho_out, double *d_hsml_out,
+                            void *stream, s2g_stats *stats);
 
 #ifdef __cplusplus
 }

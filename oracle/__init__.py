@@ -81,6 +81,11 @@ def lib():
             L.orc_kernel_norm.argtypes = [C.c_int, C.c_int]
             L.orc_kernel_value.restype = C.c_double
             L.orc_kernel_value.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+            L.orc_bias_correction.restype = C.c_double
+            L.orc_bias_correction.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                              C.c_double, _i64]
+            L.orc_mass_density.restype = C.c_int
+            L.orc_mass_density.argtypes = [_d, _d, _i64, C.c_int, C.c_int, _i64, _d, _d, _d]
             L.orc_mapping_parameters.restype = C.c_int
             L.orc_mapping_parameters.argtypes = [
                 C.POINTER(Params), _d, _d, _d, _d, C.c_double, C.c_double,
@@ -412,3 +417,25 @@ def tsc_3d(pos, bin_q, npix, len2pix, periodic=False):
     lib().orc_tsc_3d(_p(pos), _p(bq), pos.shape[0], npix, float(len2pix), int(periodic),
                      _p(g))
     return g.reshape((npix ** 3, 2), order="F")
+
+
+# ---------------------------------------------------------------- mass_density
+def bias_correction(kernel, dim, density, m, h_inv, nneighbors):
+    """Dehnen & Aly (2012) eqs. 18-19 (SPHKernels.jl bias_correction)."""
+    return lib().orc_bias_correction(_kid(kernel), int(dim), float(density), float(m),
+                                     float(h_inv), int(nneighbors))
+
+
+def mass_density(pos, mass, kernel="cubic", dim=3, nneighbors=32, boxsize=(0.0, 0.0, 0.0)):
+    """effects/mass_density.jl:30-99 by brute force.  ``pos`` is 3 x N (the
+    reference's layout).  Returns (rho, hsml)."""
+    x = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T)
+    m = _f64(mass)
+    n = m.shape[0]
+    box = np.ascontiguousarray(np.asarray(boxsize, dtype=np.float64))
+    rho, hsml = np.zeros(n), np.zeros(n)
+    rc = lib().orc_mass_density(_p(x), _p(m), n, _kid(kernel), int(dim), int(nneighbors),
+                                _p(box), _p(rho), _p(hsml))
+    if rc:
+        raise ValueError("oracle mass_density: fewer particles than neighbours, or no memory")
+    return rho, hsml

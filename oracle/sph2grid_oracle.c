@@ -1215,3 +1215,109 @@ int orc_tsc_3d(const double *pos, const double *bin_q, int64_t n, int64_t npix,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* mass_density (effects/mass_density.jl:30-99)                             */
+/* ------------------------------------------------------------------------ */
+
+/* Dehnen & Aly (2012, MNRAS 425, 1068) eqs. 18-19: the self-contribution
+ * bias of the Wendland kernels, eps = eps100 * (N / 100)^(-alpha), subtracted
+ * as eps * m * W(0) when that leaves a positive density (SPHKernels.jl
+ * bias_correction; called at mass_density.jl:86).  Other kernels: none. */
+double orc_bias_correction(int kernel, int dim, double density, double m, double h_inv,
+                           int64_t nneighbors)
+{
+    double e100, alpha;
+    switch (kernel) {
+    case ORC_KERNEL_WENDLAND_C2: e100 = 0.0294; alpha = 0.977; break;
+    case ORC_KERNEL_WENDLAND_C4: e100 = 0.01342; alpha = 1.579; break;
+    case ORC_KERNEL_WENDLAND_C6: e100 = 0.0116; alpha = 2.236; break;
+    default: return density;
+    }
+    double hd = h_inv;
+    for (int d = 1; d < dim; ++d)
+        hd *= h_inv;
+    const double corr = e100 * pow((double)nneighbors * 0.01, -alpha) * m *
+                        orc_kernel_norm(kernel, dim) * hd;
+    return density - corr > 0.0 ? density - corr : density;
+}
+
+static double md_dist(const double *a, const double *b, const double *box, int periodic)
+{
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        double x = a[d] - b[d];
+        if (periodic) {
+            /* Distances.jl PeriodicEuclidean: minimum image per dimension */
+            x = fabs(x);
+            x = fmod(x, box[d]);
+            if (x > 0.5 * box[d])
+                x = box[d] - x;
+        }
+        s += x * x;
+    }
+    return sqrt(s);
+}
+
+/* k-th smallest (0-based) of v[0..n) by quickselect (v is reordered) */
+static double md_select(double *v, int64_t n, int64_t k)
+{
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const double piv = v[lo + (hi - lo) / 2];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (v[i] < piv) ++i;
+            while (v[j] > piv) --j;
+            if (i <= j) {
+                const double t = v[i];
+                v[i] = v[j];
+                v[j] = t;
+                ++i;
+                --j;
+            }
+        }
+        if (k <= j)
+            hi = j;
+        else if (k >= i)
+            lo = i;
+        else
+            return v[k];
+    }
+    return v[k];
+}
+
+int orc_mass_density(const double *pos, const double *mass, int64_t n, int kernel, int dim,
+                     int64_t nneighbors, const double *boxsize, double *rho, double *hsml)
+{
+    const int64_t k1 = nneighbors + 1;
+    if (n < k1 || nneighbors < 1)
+        return -1;
+    const int periodic = boxsize && boxsize[0] > 0.0 && boxsize[1] > 0.0 && boxsize[2] > 0.0;
+    double *d = (double *)malloc(sizeof(double) * (size_t)n);
+    double *work = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!d || !work) {
+        free(d);
+        free(work);
+        return -2;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t j = 0; j < n; ++j)
+            d[j] = md_dist(pos + 3 * i, pos + 3 * j, boxsize, periodic);
+        memcpy(work, d, sizeof(double) * (size_t)n);
+        /* mass_density.jl:64-68: hsml = the largest of the k1 nearest distances */
+        const double h = md_select(work, n, k1 - 1);
+        const double h_inv = 1.0 / h;
+        /* mass_density.jl:73-81: sum over the neighbours; those at distance h
+           have u = 1 and W = 0, so ties there do not matter */
+        double r = 0.0;
+        for (int64_t j = 0; j < n; ++j)
+            if (d[j] < h)
+                r += mass[j] * orc_kernel_value(kernel, dim, d[j] * h_inv, h_inv);
+        hsml[i] = h;
+        rho[i] = orc_bias_correction(kernel, dim, r, mass[i], h_inv, nneighbors);
+    }
+    free(d);
+    free(work);
+    return 0;
+}

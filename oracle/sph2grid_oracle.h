@@ -152,6 +152,18 @@ int orc_healpix_deposit_parallel(int nworkers, const double *pos,
 int orc_tsc_3d(const double *pos, const double *bin_q, int64_t n,
                int64_t npix, double len2pix, int periodic, double *grid);
 
+/* ---- mass_density (effects/mass_density.jl:30-99) ----------------------
+ * Brute-force k nearest neighbours (nneighbors + 1 of them, the particle
+ * itself included), hsml = the largest of their distances, the SPH density
+ * sum m_j W(r_ij / h, 1 / h) and the kernel bias correction of Dehnen & Aly
+ * (2012, eqs. 18-19; SPHKernels.jl bias_correction).  pos is 3 x n
+ * interleaved; boxsize[d] > 0 for all d selects the periodic (minimum image)
+ * metric, all zero the Euclidean one.  O(n^2). */
+double orc_bias_correction(int kernel, int dim, double density, double m, double h_inv,
+                           int64_t nneighbors);
+int orc_mass_density(const double *pos, const double *mass, int64_t n, int kernel, int dim,
+                     int64_t nneighbors, const double *boxsize, double *rho, double *hsml);
+
 #ifdef __cplusplus
 }
 #endif

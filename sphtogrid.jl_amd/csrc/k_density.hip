@@ -1,0 +1,505 @@
+// k_density.hip -- mass_density (effects/mass_density.jl:30-99) on gfx950:
+// SPH density and smoothing length of an arbitrary particle set from its
+// k = Nneighbors + 1 nearest neighbours (the particle itself included).
+//
+// Neighbour search without a tree build: particles are sorted by a 63-bit
+// Morton key (21 bits per axis over the domain), so the cells of every level
+// of the implicit octree are contiguous ranges of the sorted order.  A table
+// of cell offsets at one level (8 at most, 8^8 cells) turns a cell into its range
+// with two loads; finer cells are found by binary search inside their table
+// cell.
+//
+//   k_md_prep      FP64 (x, y, z, m), positions wrapped into a periodic box,
+//                  per-block bounding boxes
+//   k_md_keys      Morton key per particle -> radix sort
+//   k_md_sorted    particles in key order
+//   k_md_table     cell offsets at the table level (binary search per cell)
+//   k_md_knn       one wave per particle:
+//                  1. a first bound T0 >= the true k-th distance: the k-th
+//                     smallest distance to the particles next to it in key
+//                     order (they are k real particles);
+//                  2. the level whose cells are between T0/2 and T0 wide and
+//                     the cube of <= 5^3 cells around the particle that covers
+//                     the ball of radius T0;
+//                  3. every particle in the cube at distance <= T goes to a
+//                     per-wave LDS buffer (T shrinks as the buffer is cut back
+//                     to the k nearest); the exact k-th smallest distance is
+//                     an 8-bit radix select on the FP64 bit patterns;
+//                  4. the kernel sum over the buffer and the bias correction
+//                     of Dehnen & Aly (2012, eqs. 18-19).
+// The cube holds the whole ball of radius T0 and T0 is an upper bound, so the
+// k nearest neighbours are all found: the result is exact, and the work per
+// particle follows the local density (clustered sets cost what uniform ones
+// cost).  Distances are FP64 without contraction, minimum image per
+// dimension in a periodic box.
+#include "k_density.h"
+
+namespace s2g {
+
+constexpr int kMdWaves = 4;     // waves per block of the search
+constexpr int kMdCap = 768;     // candidate buffer per wave
+constexpr int kMdBits = 21;     // Morton bits per axis
+constexpr int kMdMaxCells = 128;  // cube cells per particle (5^3 = 125)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_md_prep(const T *__restrict__ pos,
+                                                 const T *__restrict__ mass, int64_t n,
+                                                 MdGrid g, double4 *__restrict__ P,
+                                                 double *__restrict__ bbox)
+{
+    __shared__ double s_lo[3][256], s_hi[3][256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    if (i < n) {
+        double x[3];
+        for (int d = 0; d < 3; ++d) {
+            double v = (double)pos[3 * i + d];
+            if (g.periodic) {
+                v = v - g.box[d] * floor(v / g.box[d]);
+                if (v >= g.box[d])
+                    v -= g.box[d];
+            }
+            x[d] = v;
+            lo[d] = hi[d] = v;
+            if (!isfinite(v)) {  // reported by the host as an invalid input
+                lo[d] = -INFINITY;
+                hi[d] = INFINITY;
+            }
+        }
+        P[i] = make_double4(x[0], x[1], x[2], (double)mass[i]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        s_lo[d][threadIdx.x] = lo[d];
+        s_hi[d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int d = 0; d < 3; ++d) {
+                s_lo[d][threadIdx.x] = fmin(s_lo[d][threadIdx.x], s_lo[d][threadIdx.x + s]);
+                s_hi[d][threadIdx.x] = fmax(s_hi[d][threadIdx.x], s_hi[d][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        bbox[6 * blockIdx.x + threadIdx.x] = s_lo[threadIdx.x][0];
+        bbox[6 * blockIdx.x + 3 + threadIdx.x] = s_hi[threadIdx.x][0];
+    }
+}
+
+// spread the low 21 bits of a over every third bit
+__host__ __device__ __forceinline__ uint64_t md_split3(uint32_t a)
+{
+    uint64_t x = a & 0x1fffffu;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t md_morton(uint32_t x, uint32_t y, uint32_t z)
+{
+    return md_split3(x) | md_split3(y) << 1 | md_split3(z) << 2;
+}
+
+__device__ __forceinline__ uint32_t md_quant(double v, const MdGrid &g, int d)
+{
+    const double t = (v - g.lo[d]) * g.inv_ext[d] * (double)(1u << kMdBits);
+    const double c = floor(t);
+    return c < 0.0 ? 0u : (c >= (double)((1u << kMdBits) - 1) ? (1u << kMdBits) - 1 : (uint32_t)c);
+}
+
+__global__ void k_md_keys(const double4 *__restrict__ P, int64_t n, MdGrid g,
+                          uint64_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double4 p = P[i];
+    keys[i] = md_morton(md_quant(p.x, g, 0), md_quant(p.y, g, 1), md_quant(p.z, g, 2));
+    vals[i] = (int32_t)i;
+}
+
+__global__ void k_md_sorted(const double4 *__restrict__ P, const int32_t *__restrict__ idx,
+                            int64_t n, double4 *__restrict__ S)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        S[i] = P[idx[i]];
+}
+
+__device__ __forceinline__ int64_t md_lower_bound(const uint64_t *__restrict__ K, int64_t lo,
+                                                  int64_t hi, uint64_t key)
+{
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (K[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// offsets of the 8^lt cells of level lt (+ the end)
+__global__ void k_md_table(const uint64_t *__restrict__ K, int64_t n, int lt,
+                           int32_t *__restrict__ tab)
+{
+    const int64_t ncell = (int64_t)1 << (3 * lt);
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > ncell)
+        return;
+    tab[e] = e == ncell ? (int32_t)n
+                        : (int32_t)md_lower_bound(K, 0, n, (uint64_t)e << (3 * (kMdBits - lt)));
+}
+
+// minimum-image component (mass_density.jl:45-52, Distances.jl PeriodicEuclidean)
+__device__ __forceinline__ double md_comp(double a, double b, double box, int periodic)
+{
+    double x = fabs(a - b);
+    if (periodic && x > 0.5 * box)
+        x = box - x;
+    return x;
+}
+
+__device__ __forceinline__ double md_dist(const double4 &a, const double4 &b, const MdGrid &g)
+{
+    const double dx = md_comp(a.x, b.x, g.box[0], g.periodic);
+    const double dy = md_comp(a.y, b.y, g.box[1], g.periodic);
+    const double dz = md_comp(a.z, b.z, g.box[2], g.periodic);
+    return sqrt(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)));
+}
+
+// exclusive prefix sum over the lanes of a wave
+__device__ __forceinline__ int wave_excl_scan(int v, int ln)
+{
+    int x = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(x, o, kWave);
+        if (ln >= o)
+            x += y;
+    }
+    return x - v;
+}
+
+// k-th smallest (1-based) of the non-negative doubles buf[0..cnt), exactly:
+// most-significant-byte-first radix select on the bit patterns
+__device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln)
+{
+    uint64_t prefix = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int b = ln; b < 256; b += kWave)
+            hist[b] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int e = ln; e < cnt; e += kWave) {
+            const uint64_t key = (uint64_t)__double_as_longlong(buf[e]);
+            const bool match = shift == 56 || (key >> (shift + 8)) == (prefix >> (shift + 8));
+            if (match)
+                atomicAdd(&hist[(key >> shift) & 255u], 1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // lane l owns bins 4l .. 4l+3
+        int c4[4], loc = 0;
+        for (int j = 0; j < 4; ++j) {
+            c4[j] = hist[4 * ln + j];
+            loc += c4[j];
+        }
+        const int before = wave_excl_scan(loc, ln);
+        const bool mine = before < k && k <= before + loc;
+        const uint64_t owner = __ballot(mine);
+        const int src = __ffsll((unsigned long long)owner) - 1;
+        int bin = 0, below = 0;
+        if (mine) {
+            int acc = before;
+            for (int j = 0; j < 4; ++j) {
+                if (k <= acc + c4[j]) {
+                    bin = 4 * ln + j;
+                    below = acc;
+                    break;
+                }
+                acc += c4[j];
+            }
+        }
+        bin = __shfl(bin, src, kWave);
+        below = __shfl(below, src, kWave);
+        k -= below;
+        prefix |= (uint64_t)bin << shift;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    return __longlong_as_double((long long)prefix);
+}
+
+// keep the entries below T and, of those equal to T, as many as make k in
+// total (they are at u = 1, W = 0); returns the new count
+__device__ int md_compact(double *buf, int *jb, int cnt, double T, int k, int ln)
+{
+    int nless = 0;
+    for (int e0 = 0; e0 < cnt; e0 += kWave) {
+        const int e = e0 + ln;
+        nless += __popcll(__ballot(e < cnt && buf[e] < T));
+    }
+    int quota = k - nless;  // entries equal to T to keep
+    int out = 0;
+    const uint64_t below = ln == 0 ? 0ull : (~0ull >> (kWave - ln));
+    for (int e0 = 0; e0 < cnt; e0 += kWave) {
+        const int e = e0 + ln;
+        const double d = e < cnt ? buf[e] : 0.0;
+        const int j = e < cnt ? jb[e] : 0;
+        const bool less = e < cnt && d < T;
+        const bool eq = e < cnt && d == T;
+        const uint64_t beq = __ballot(eq);
+        const bool keep_eq = eq && (int)__popcll(beq & below) < quota;
+        const bool keep = less || keep_eq;
+        const uint64_t bk = __ballot(keep);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (keep) {
+            const int at = out + (int)__popcll(bk & below);
+            buf[at] = d;
+            jb[at] = j;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        out += (int)__popcll(bk);
+        const int taken = (int)__popcll(beq);
+        quota -= taken < quota ? taken : (quota > 0 ? quota : 0);
+    }
+    return out;
+}
+
+// Append the candidates (lane's j, distance d, valid flag) with d <= T to the
+// buffer; when it is about to overflow, cut it back to the k nearest.
+__device__ __forceinline__ void md_offer(bool valid, double d, int j, double &T, int &cnt,
+                                         double *buf, int *jb, int *hist, int k1, int ln,
+                                         uint64_t below)
+{
+    const bool take = valid && d <= T;
+    const uint64_t bt = __ballot(take);
+    if (take) {
+        const int at = cnt + (int)__popcll(bt & below);
+        buf[at] = d;
+        jb[at] = j;
+    }
+    cnt += (int)__popcll(bt);
+    if (cnt > kMdCap - kWave) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        T = md_select(buf, cnt, k1, hist, ln);
+        cnt = md_compact(buf, jb, cnt, T, k1, ln);
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kMdWaves * kWave) void k_md_knn(
+    const double4 *__restrict__ S, const uint64_t *__restrict__ SK,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ tab, int lt, int64_t n,
+    MdGrid g, int k1, int dim, int nneigh, double norm, double *__restrict__ rho_out,
+    double *__restrict__ hsml_out)
+{
+    __shared__ double s_d[kMdWaves][kMdCap];
+    __shared__ int s_j[kMdWaves][kMdCap];
+    __shared__ int s_h[kMdWaves][256];
+    __shared__ int s_cs[kMdWaves][kMdMaxCells];  // cube cell: first particle
+    __shared__ int s_cp[kMdWaves][kMdMaxCells + 1];  // prefix of the cells' counts
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    double *buf = s_d[wv];
+    int *jb = s_j[wv];
+    int *hist = s_h[wv];
+    int *cstart = s_cs[wv];
+    int *cpre = s_cp[wv];
+    const uint64_t below = ln == 0 ? 0ull : (~0ull >> (kWave - ln));
+    const int64_t nw = (int64_t)gridDim.x * kMdWaves;
+    for (int64_t q = (int64_t)blockIdx.x * kMdWaves + wv; q < n; q += nw) {
+        const double4 me = S[q];
+        // 1. first bound: the k-th nearest of the particles next in key order
+        int cnt = 0;
+        double T = 1e300;
+        {
+            const int64_t a = q - k1 > 0 ? q - k1 : 0;
+            const int64_t b = q + k1 < n - 1 ? q + k1 : n - 1;
+            for (int64_t t0 = a; t0 <= b; t0 += kWave) {
+                const int64_t t = t0 + ln;
+                const bool valid = t <= b;
+                const double d = valid ? md_dist(me, S[t], g) : 0.0;
+                md_offer(valid, d, (int)t, T, cnt, buf, jb, hist, k1, ln, below);
+            }
+            T = md_select(buf, cnt, k1, hist, ln);
+        }
+        const double T0 = T;
+        // 2. level with cells in [T0/2, T0) along the narrowest axis, cube
+        //    of half-width ceil(T0 / cell) per axis
+        int lev = kMdBits;
+        if (T0 > 0.0) {
+            const int l = (int)floor(log2(g.min_ext / T0)) + 1;
+            lev = l < 0 ? 0 : (l > kMdBits ? kMdBits : l);
+        }
+        const uint32_t ncl = 1u << lev;
+        int ca[3], cl[3];  // first cell and cell count per axis
+        const double mv[3] = {me.x, me.y, me.z};
+        for (int d = 0; d < 3; ++d) {
+            const uint32_t c = md_quant(mv[d], g, d) >> (kMdBits - lev);
+            const double cw = g.ext[d] / (double)ncl;
+            int w = T0 > 0.0 ? (int)ceil(T0 / cw) : 0;
+            w = w > 2 ? 2 : w;  // the narrowest axis needs 2; wider ones at most 2
+            if (g.periodic) {
+                if (2 * w + 1 >= (int)ncl) {
+                    ca[d] = 0;
+                    cl[d] = (int)ncl;
+                } else {
+                    ca[d] = (int)c - w;
+                    cl[d] = 2 * w + 1;
+                }
+            } else {
+                const int lo = (int)c - w < 0 ? 0 : (int)c - w;
+                const int hi = (int)c + w > (int)ncl - 1 ? (int)ncl - 1 : (int)c + w;
+                ca[d] = lo;
+                cl[d] = hi - lo + 1;
+            }
+        }
+        const int ncube = cl[0] * cl[1] * cl[2];
+        // 3. the cube's particles; cells are ranges of the sorted order
+        int C = 0;  // candidates so far (uniform)
+        for (int t0 = 0; t0 < ncube; t0 += kWave) {
+            const int t = t0 + ln;
+            int s = 0, e = 0;
+            if (t < ncube) {
+                uint32_t cc[3];
+                int r = t;
+                for (int d = 0; d < 3; ++d) {
+                    int o = ca[d] + r % cl[d];
+                    r /= cl[d];
+                    o = o < 0 ? o + (int)ncl : (o >= (int)ncl ? o - (int)ncl : o);
+                    cc[d] = (uint32_t)o;
+                }
+                const uint64_t pre = md_morton(cc[0], cc[1], cc[2]);  // 3*lev bits
+                if (lev <= lt) {
+                    s = tab[pre << (3 * (lt - lev))];
+                    e = tab[(pre + 1) << (3 * (lt - lev))];
+                } else {
+                    const uint64_t par = pre >> (3 * (lev - lt));
+                    const int64_t ps = tab[par], pe = tab[par + 1];
+                    const int sh = 3 * (kMdBits - lev);
+                    s = (int)md_lower_bound(SK, ps, pe, pre << sh);
+                    e = (int)md_lower_bound(SK, s, pe, (pre + 1) << sh);
+                }
+            }
+            const int len = e - s;
+            const int ex = wave_excl_scan(len, ln);
+            if (t < ncube) {
+                cstart[t] = s;
+                cpre[t] = C + ex;
+            }
+            C += __shfl(ex + len, kWave - 1, kWave);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        cnt = 0;  // fresh buffer, threshold T0 (inclusive)
+        T = T0;
+        for (int u0 = 0; u0 < C; u0 += kWave) {
+            const int u = u0 + ln;
+            bool valid = u < C;
+            double d = 0.0;
+            int j = 0;
+            if (valid) {
+                // cell of candidate u: last c with cpre[c] <= u
+                int lo = 0, hi = ncube - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (cpre[mid] <= u)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                j = cstart[lo] + (u - cpre[lo]);
+                d = md_dist(me, S[j], g);
+            }
+            md_offer(valid, d, j, T, cnt, buf, jb, hist, k1, ln, below);
+        }
+        T = md_select(buf, cnt, k1, hist, ln);
+        cnt = md_compact(buf, jb, cnt, T, k1, ln);
+        // 4. mass_density.jl:70-86: kernel sum over the neighbours, bias correction
+        const double h = T, h_inv = 1.0 / h;
+        double rho = 0.0;
+        for (int e = ln; e < cnt; e += kWave) {
+            const double d = buf[e];
+            if (d < h)
+                rho += S[jb[e]].w * shape_of<K, double>(d * h_inv);
+        }
+        rho = wave_sum(rho);
+        double hd = h_inv;
+        for (int d = 1; d < dim; ++d)
+            hd *= h_inv;
+        rho *= norm * hd;
+        rho = md_bias_correction(K, rho, me.w, norm * hd, nneigh);
+        if (ln == 0) {
+            const int oi = order[q];
+            rho_out[oi] = rho;
+            hsml_out[oi] = h;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int64_t md_prep_blocks(int64_t n) { return (n + 255) / 256; }
+
+int md_max_neighbours() { return kMdCap - 2 * kWave - 1; }
+
+int md_bits() { return kMdBits; }
+
+void launch_md_prep(int dtype, const void *pos, const void *mass, int64_t n, const MdGrid &g,
+                    double4 *P, double *bbox, hipStream_t s)
+{
+    const dim3 grid((unsigned)md_prep_blocks(n)), blk(256);
+    if (dtype == S2G_F32)
+        hipLaunchKernelGGL(k_md_prep<float>, grid, blk, 0, s, (const float *)pos,
+                           (const float *)mass, n, g, P, bbox);
+    else
+        hipLaunchKernelGGL(k_md_prep<double>, grid, blk, 0, s, (const double *)pos,
+                           (const double *)mass, n, g, P, bbox);
+}
+
+void launch_md_keys(const double4 *P, int64_t n, const MdGrid &g, uint64_t *keys, int32_t *vals,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_md_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, n, g,
+                       keys, vals);
+}
+
+void launch_md_sorted(const double4 *P, const int32_t *idx, int64_t n, double4 *S,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_md_sorted, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, idx,
+                       n, S);
+}
+
+void launch_md_table(const uint64_t *K, int64_t n, int lt, int32_t *tab, hipStream_t s)
+{
+    const int64_t ne = ((int64_t)1 << (3 * lt)) + 1;
+    hipLaunchKernelGGL(k_md_table, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, K, n, lt,
+                       tab);
+}
+
+void launch_md_knn(int kernel, const double4 *S, const uint64_t *SK, const int32_t *order,
+                   const int32_t *tab, int lt, int64_t n, const MdGrid &g, int k1, int dim,
+                   int nneigh, double norm, double *rho, double *hsml, hipStream_t s)
+{
+    int64_t blocks = (n + kMdWaves - 1) / kMdWaves;
+    if (blocks > (1 << 18))
+        blocks = 1 << 18;
+    const dim3 grid((unsigned)blocks), blk(kMdWaves * kWave);
+#define S2G_MD_CASE(KK)                                                                      \
+    hipLaunchKernelGGL(k_md_knn<KK>, grid, blk, 0, s, S, SK, order, tab, lt, n, g, k1, dim, \
+                       nneigh, norm, rho, hsml)
+    switch (kernel) {
+    case S2G_KERNEL_CUBIC: S2G_MD_CASE(S2G_KERNEL_CUBIC); break;
+    case S2G_KERNEL_QUINTIC: S2G_MD_CASE(S2G_KERNEL_QUINTIC); break;
+    case S2G_KERNEL_WENDLAND_C2: S2G_MD_CASE(S2G_KERNEL_WENDLAND_C2); break;
+    case S2G_KERNEL_WENDLAND_C4: S2G_MD_CASE(S2G_KERNEL_WENDLAND_C4); break;
+    case S2G_KERNEL_WENDLAND_C6: S2G_MD_CASE(S2G_KERNEL_WENDLAND_C6); break;
+    default: S2G_MD_CASE(S2G_KERNEL_WENDLAND_C8); break;
+    }
+#undef S2G_MD_CASE
+}
+
+}  // namespace s2g

@@ -7,6 +7,7 @@
 
 #include "k_accum.h"
 #include "k_cic3d.h"
+#include "k_density.h"
 #include "k_healpix.h"
 #include "k_sort.h"
 #include "k_tsc.h"
@@ -31,6 +32,8 @@ struct s2g_ctx {
     // small-footprint path: sort keys / values (double-buffered), sort scratch,
     // tile offsets, work items
     DevBuf skeys, svals, sort_tmp, soff, snitems, sitems, stile;
+    // mass_density: particles, bounding-box partials, particles in cell order
+    DevBuf md_P, md_bbox, md_S, md_tab;
     PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
     DevBuf hp_rec, hp_nunits, hp_units, hp_part;
@@ -632,6 +635,115 @@ void deposit_device_any(s2g_ctx *c, int dims, const Map2DArgs &a, double *d_imag
         deposit3d_device(c, a, d_image, s, st);
 }
 
+// ---------------------------------------------------------------------------
+// mass_density on device pointers
+// ---------------------------------------------------------------------------
+void mass_density_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_mass, int64_t n,
+                         int kernel, int dim, int64_t nneigh, const double *boxsize,
+                         double *d_rho, double *d_hsml, hipStream_t s, s2g_stats *st)
+{
+    if (dtype != S2G_F32 && dtype != S2G_F64)
+        throw Error(S2G_EINVAL, "dtype must be S2G_F32 or S2G_F64");
+    if (kernel < S2G_KERNEL_CUBIC || kernel > S2G_KERNEL_WENDLAND_C8)
+        throw Error(S2G_EUNSUPPORTED, "unsupported kernel id " + std::to_string(kernel));
+    if (dim != 2 && dim != 3)
+        throw Error(S2G_EINVAL, "kernel_dim must be 2 or 3");
+    if (nneigh < 1 || nneigh > md_max_neighbours())
+        throw Error(S2G_EUNSUPPORTED, "Nneighbors must be in [1, " +
+                                          std::to_string(md_max_neighbours()) + "]");
+    if (n < nneigh + 1 || n > (int64_t)INT32_MAX)
+        throw Error(S2G_EINVAL, "need at least Nneighbors + 1 particles (and fewer than 2^31)");
+    if (!d_pos || !d_mass || !d_rho || !d_hsml)
+        throw Error(S2G_EINVAL, "array pointer is NULL");
+    MdGrid g;
+    int npos = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double b = boxsize ? boxsize[d] : 0.0;
+        if (!(b >= 0.0) || !std::isfinite(b))
+            throw Error(S2G_EINVAL, "boxsize must be finite and >= 0");
+        g.box[d] = b;
+        npos += b > 0.0;
+    }
+    if (npos != 0 && npos != 3)
+        throw Error(S2G_EINVAL, "boxsize must be zero (non-periodic) or positive in all "
+                                "three dimensions");
+    g.periodic = npos == 3;
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+    }
+    hipEvent_t *E = c->stage;
+    S2G_HIP(hipEventRecord(E[0], s));
+    const int64_t nb = md_prep_blocks(n);
+    double4 *P = (double4 *)c->md_P.get(sizeof(double4) * n);
+    double *bbox = (double *)c->md_bbox.get(sizeof(double) * 6 * nb);
+    launch_md_prep(dtype, d_pos, d_mass, n, g, P, bbox, s);
+    // key domain: the periodic box, or the bounding box of the particles
+    double *h = (double *)c->h_bounds.get(sizeof(double) * 6 * nb);
+    S2G_HIP(hipMemcpyAsync(h, bbox, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipStreamSynchronize(s));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t b = 0; b < nb; ++b)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::fmin(lo[d], h[6 * b + d]);
+            hi[d] = std::fmax(hi[d], h[6 * b + 3 + d]);
+        }
+    for (int d = 0; d < 3; ++d)
+        if (!(lo[d] > -1e300) || !(hi[d] < 1e300))
+            throw Error(S2G_EINVAL, "positions must be finite");
+    if (g.periodic) {
+        for (int d = 0; d < 3; ++d) {
+            g.lo[d] = 0.0;
+            g.ext[d] = g.box[d];
+        }
+    } else {
+        double emax = 0.0;
+        for (int d = 0; d < 3; ++d)
+            emax = std::fmax(emax, hi[d] - lo[d]);
+        if (!(emax > 0.0))
+            emax = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            g.lo[d] = lo[d];
+            const double e = hi[d] - lo[d];
+            // a little wider than the particles, so the largest maps inside
+            g.ext[d] = (e > 0.0 ? e : emax) * (1.0 + 1e-9);
+        }
+    }
+    g.min_ext = 1e300;
+    for (int d = 0; d < 3; ++d) {
+        g.inv_ext[d] = 1.0 / g.ext[d];
+        g.min_ext = std::fmin(g.min_ext, g.ext[d]);
+    }
+    uint64_t *keys = (uint64_t *)c->skeys.get(sizeof(uint64_t) * 2 * n);
+    int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * n);
+    launch_md_keys(P, n, g, keys, vals, s);
+    const int end_bit = 3 * md_bits();
+    const size_t tb = sort_pairs64_tmp_bytes(n, end_bit);
+    void *stmp = c->sort_tmp.get(tb ? tb : 1);
+    sort_pairs64(stmp, tb, keys, keys + n, vals, vals + n, n, end_bit, s);
+    // offset table at the level with about one particle per cell (<= 8)
+    int lt = 1;
+    while (lt < 8 && ((int64_t)1 << (3 * lt)) < n)
+        ++lt;
+    int32_t *tab = (int32_t *)c->md_tab.get(sizeof(int32_t) * (((size_t)1 << (3 * lt)) + 1));
+    launch_md_table(keys + n, n, lt, tab, s);
+    double4 *S = (double4 *)c->md_S.get(sizeof(double4) * n);
+    launch_md_sorted(P, vals + n, n, S, s);
+    S2G_HIP(hipEventRecord(E[1], s));
+    const double norm = dim == 3 ? 1.0 / shape_ball_integral(kernel)
+                                 : 1.0 / shape_disc_integral(kernel);
+    launch_md_knn(kernel, S, keys + n, vals + n, tab, lt, n, g, (int)nneigh + 1, dim,
+                  (int)nneigh, norm, d_rho, d_hsml, s);
+    S2G_HIP(hipGetLastError());
+    S2G_HIP(hipEventRecord(E[2], s));
+    S2G_HIP(hipEventSynchronize(E[2]));
+    if (st) {
+        st->ms_device = elapsed(E[0], E[2]);
+        st->ms_sort = elapsed(E[0], E[1]);
+        st->ms_gather = elapsed(E[1], E[2]);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -704,7 +816,7 @@ void s2g_destroy(s2g_ctx *c)
     c->bounds.release();
     c->tiles.release();
     for (DevBuf *b : {&c->skeys, &c->svals, &c->sort_tmp, &c->soff, &c->snitems, &c->sitems,
-                      &c->stile})
+                      &c->stile, &c->md_P, &c->md_bbox, &c->md_S, &c->md_tab})
         b->release();
     c->h_tab.release();
     c->h_bounds.release();
@@ -1164,6 +1276,53 @@ int s2g_healpix_reduce_device(s2g_ctx *c, double *d_map, const double *d_weights
         hipStream_t s = stream ? (hipStream_t)stream : c->stream;
         launch_hp_reduce(d_map, d_weights, npix, s);
         S2G_HIP(hipGetLastError());
+        return S2G_OK;
+    });
+}
+
+int s2g_mass_density_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_mass,
+                            int64_t n, int kernel, int kernel_dim, int64_t nneighbors,
+                            const double *boxsize, double *d_rho_out, double *d_hsml_out,
+                            void *stream, s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        mass_density_device(c, dtype, d_pos, d_mass, n, kernel, kernel_dim, nneighbors, boxsize,
+                            d_rho_out, d_hsml_out, s, stats);
+        return S2G_OK;
+    });
+}
+
+int s2g_mass_density(s2g_ctx *c, int dtype, const void *pos, const void *mass, int64_t n,
+                     int kernel, int kernel_dim, int64_t nneighbors, const double *boxsize,
+                     double *rho_out, double *hsml_out, s2g_stats *stats)
+{
+    return guarded(c, [&]() {
+        if (!pos || !mass || !rho_out || !hsml_out)
+            throw Error(S2G_EINVAL, "array pointer is NULL");
+        if (n < 0 || n > (int64_t)INT32_MAX)
+            throw Error(S2G_EINVAL, "particle count out of range [0, 2^31)");
+        const size_t es = elem_size(dtype);
+        hipStream_t s = c->stream;
+        void *dp = c->in[0].get(es * 3 * (n ? n : 1));
+        void *dm = c->in[1].get(es * (n ? n : 1));
+        double *dr = (double *)c->out.get(sizeof(double) * 2 * (n ? n : 1));
+        double h2d = 0.0;
+        S2G_HIP(hipEventRecord(c->ev0, s));
+        S2G_HIP(hipMemcpyAsync(dp, pos, es * 3 * n, hipMemcpyHostToDevice, s));
+        S2G_HIP(hipMemcpyAsync(dm, mass, es * n, hipMemcpyHostToDevice, s));
+        S2G_HIP(hipEventRecord(c->ev1, s));
+        S2G_HIP(hipEventSynchronize(c->ev1));
+        h2d = elapsed(c->ev0, c->ev1);
+        mass_density_device(c, dtype, dp, dm, n, kernel, kernel_dim, nneighbors, boxsize, dr,
+                            dr + n, s, stats);
+        double t1 = 0.0, t2 = 0.0;
+        d2h(c, rho_out, dr, sizeof(double) * n, s, &t1);
+        d2h(c, hsml_out, dr + n, sizeof(double) * n, s, &t2);
+        if (stats) {
+            stats->ms_h2d = h2d;
+            stats->ms_d2h = t1 + t2;
+        }
         return S2G_OK;
     });
 }

@@ -9,6 +9,7 @@ types.  All deposits run as HIP kernels through libsph2grid's C ABI
 """
 from ._lib import Context, LibraryError, default_context, version  # noqa: F401
 from .batch import deposit_batch_device, map_batch  # noqa: F401
+from .density import mass_density  # noqa: F401
 from .distributed import (AllskySubfile, Subfile, distributed_allsky_map,  # noqa: F401
                           distributed_cic_map)
 from .io import (get_map_grid_3D, read_allsky_fits_image, read_fits_image,  # noqa: F401
@@ -35,5 +36,5 @@ __all__ = [
     "tsc_mapping", "map_batch", "deposit_batch_device", "distributed_cic_map",
     "distributed_allsky_map", "Subfile", "AllskySubfile", "write_fits_image",
     "read_fits_image", "read_allsky_fits_image", "save_healpix_fits", "read_healpix_fits",
-    "write_vtk_image", "read_vtk_image", "get_map_grid_3D",
+    "write_vtk_image", "read_vtk_image", "get_map_grid_3D", "mass_density",
 ]

@@ -34,7 +34,7 @@ EXPORTS = (
     "s2g_deposit", "s2g_deposit_device", "s2g_reduce", "s2g_reduce_device",
     "s2g_map", "s2g_healpix_deposit", "s2g_healpix_deposit_device", "s2g_tsc3d",
     "s2g_tsc3d_device", "s2g_deposit_batch", "s2g_accumulate_device",
-    "s2g_healpix_reduce_device",
+    "s2g_healpix_reduce_device", "s2g_mass_density", "s2g_mass_density_device",
 )
 
 
@@ -137,6 +137,12 @@ def load(path: str | None = None):
         L.s2g_accumulate_device.argtypes = [vp, vp, vp, i64, vp]
         L.s2g_healpix_reduce_device.restype = C.c_int
         L.s2g_healpix_reduce_device.argtypes = [vp, vp, vp, i64, vp]
+        L.s2g_mass_density.restype = C.c_int
+        L.s2g_mass_density.argtypes = [vp, C.c_int, vp, vp, i64, C.c_int, C.c_int, i64, vp, vp,
+                                       vp, C.POINTER(Stats)]
+        L.s2g_mass_density_device.restype = C.c_int
+        L.s2g_mass_density_device.argtypes = [vp, C.c_int, vp, vp, i64, C.c_int, C.c_int, i64,
+                                              vp, vp, vp, vp, C.POINTER(Stats)]
         if path is None:
             _lib = L
         return L
