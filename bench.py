@@ -308,6 +308,7 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
                            "weight_sum_over_n": wsum / na}
     log(f"extra tsc: {out['tsc_2e22_512']}")
     out["batched_64maps"] = batched_extra(args, torch, s2g, _lib)
+    out["mass_density_2e20_k64"] = density_extra(args, torch, s2g, _lib)
     return out
 
 
@@ -460,6 +461,48 @@ def streaming_extra(args, d, torch, s2g, _lib):
            "speedup_resident_vs_host_sum": t_host / t_loop,
            "max_rel_diff_vs_host_sum": diff}
     log(f"extra streaming: {rec}")
+    return rec
+
+
+def density_extra(args, torch, s2g, _lib):
+    """mass_density (effects/mass_density.jl:30-99) of 2^20 clustered
+    particles (half in 64 Gaussian clumps, half uniform) in a periodic box,
+    WendlandC4, 64 neighbours: device time, and 512 particles against the
+    brute-force oracle (all 2^20 candidates each)."""
+    import oracle
+    r = np.random.default_rng(0x5348544F47520006)
+    n, L = 1 << 20, 10.0
+    pos = r.random((3, n)) * L
+    c = n // 2
+    centres = r.random((3, 64)) * L
+    pos[:, :c] = np.mod(centres[:, r.integers(0, 64, c)] + 0.02 * L * r.standard_normal((3, c)),
+                        L)
+    m = r.uniform(0.5, 2.0, n)
+    ctx = s2g.default_context()
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    dpos, dm = T(pos.T.copy()), T(m)
+    rho = torch.empty(n, dtype=torch.float64, device=dev)
+    hs = torch.empty(n, dtype=torch.float64, device=dev)
+    box = (C.c_double * 3)(L, L, L)
+    st = _lib.Stats()
+    vp = lambda t: C.c_void_p(t.data_ptr())
+
+    def step():
+        ctx.check(ctx.lib.s2g_mass_density_device(ctx.handle, _lib.S2G_F64, vp(dpos), vp(dm), n,
+                                                  3, 3, 64, box, vp(rho), vp(hs), None,
+                                                  C.byref(st)), "mass_density")
+    ms = _time_steps(step, torch, steps=3, warmup=1)
+    rec = {"particles": n, "nneighbors": 64, "kernel": "wendlandc4", "ms_per_call": ms,
+           "particles_per_s": n / (ms / 1e3), "ms_sort": st.ms_sort, "ms_search": st.ms_gather}
+    if not args.no_oracle:
+        idx = r.choice(n, 512, replace=False)
+        orho, oh = oracle.mass_density_subset(pos, m, idx, "wendlandc4", 3, 64, (L, L, L))
+        g_rho, g_h = rho.cpu().numpy()[idx], hs.cpu().numpy()[idx]
+        rec["sample_512_vs_oracle"] = {
+            "hsml_max_rel": float(np.abs(g_h - oh).max() / oh.max()),
+            "rho_max_rel": float((np.abs(g_rho - orho) / np.abs(orho)).max())}
+    log(f"extra mass_density: {rec}")
     return rec
 
 

@@ -46,7 +46,9 @@
  *     available from s2g_last_error().  No C++ exception crosses the ABI.
  *   - A context is not thread safe; use one per host thread.
  *   - *_device variants and s2g_deposit_batch take device pointers and a
- *     hipStream_t (as void*; NULL = the context's stream).  The deposits
+ *     hipStream_t (as void*; NULL = the context's stream, a blocking stream,
+ *     so work the caller queued on the null stream is ordered before it and
+ *     work queued there afterwards waits for it).  The deposits
  *     (s2g_deposit_device, s2g_healpix_deposit_device, s2g_deposit_batch)
  *     size their work lists from device-side counts, so they synchronise the
  *     stream and return with the result complete; s2g_reduce_device and
@@ -123,8 +125,8 @@ typedef struct s2g_stats {
     int64_t n_gather_launches;
 } s2g_stats;
 
-/* Context: owns the device buffers (grown monotonically) and one stream on
- * `device`. */
+/* Context: owns the device buffers (grown monotonically) and one blocking
+ * stream (default priority) on `device`. */
 int s2g_create(s2g_ctx **out, int device);
 void s2g_destroy(s2g_ctx *ctx);
 const char *s2g_last_error(const s2g_ctx *ctx);

@@ -86,6 +86,9 @@ def lib():
                                               C.c_double, _i64]
             L.orc_mass_density.restype = C.c_int
             L.orc_mass_density.argtypes = [_d, _d, _i64, C.c_int, C.c_int, _i64, _d, _d, _d]
+            L.orc_mass_density_subset.restype = C.c_int
+            L.orc_mass_density_subset.argtypes = [_d, _d, _i64, C.c_void_p, _i64, C.c_int,
+                                                  C.c_int, _i64, _d, _d, _d]
             L.orc_mapping_parameters.restype = C.c_int
             L.orc_mapping_parameters.argtypes = [
                 C.POINTER(Params), _d, _d, _d, _d, C.c_double, C.c_double,
@@ -436,6 +439,22 @@ def mass_density(pos, mass, kernel="cubic", dim=3, nneighbors=32, boxsize=(0.0, 
     rho, hsml = np.zeros(n), np.zeros(n)
     rc = lib().orc_mass_density(_p(x), _p(m), n, _kid(kernel), int(dim), int(nneighbors),
                                 _p(box), _p(rho), _p(hsml))
+    if rc:
+        raise ValueError("oracle mass_density: fewer particles than neighbours, or no memory")
+    return rho, hsml
+
+
+def mass_density_subset(pos, mass, idx, kernel="cubic", dim=3, nneighbors=32,
+                        boxsize=(0.0, 0.0, 0.0)):
+    """mass_density of the particles ``idx`` only, against the whole set."""
+    x = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).T)
+    m = _f64(mass)
+    ii = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+    box = np.ascontiguousarray(np.asarray(boxsize, dtype=np.float64))
+    rho, hsml = np.zeros(ii.size), np.zeros(ii.size)
+    rc = lib().orc_mass_density_subset(_p(x), _p(m), m.shape[0], ii.ctypes.data_as(C.c_void_p),
+                                       ii.size, _kid(kernel), int(dim), int(nneighbors),
+                                       _p(box), _p(rho), _p(hsml))
     if rc:
         raise ValueError("oracle mass_density: fewer particles than neighbours, or no memory")
     return rho, hsml

@@ -1290,6 +1290,15 @@ static double md_select(double *v, int64_t n, int64_t k)
 int orc_mass_density(const double *pos, const double *mass, int64_t n, int kernel, int dim,
                      int64_t nneighbors, const double *boxsize, double *rho, double *hsml)
 {
+    return orc_mass_density_subset(pos, mass, n, NULL, n, kernel, dim, nneighbors, boxsize, rho,
+                                   hsml);
+}
+
+int orc_mass_density_subset(const double *pos, const double *mass, int64_t n,
+                            const int64_t *idx, int64_t nidx, int kernel, int dim,
+                            int64_t nneighbors, const double *boxsize, double *rho,
+                            double *hsml)
+{
     const int64_t k1 = nneighbors + 1;
     if (n < k1 || nneighbors < 1)
         return -1;
@@ -1301,7 +1310,8 @@ int orc_mass_density(const double *pos, const double *mass, int64_t n, int kerne
         free(work);
         return -2;
     }
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t a = 0; a < nidx; ++a) {
+        const int64_t i = idx ? idx[a] : a;
         for (int64_t j = 0; j < n; ++j)
             d[j] = md_dist(pos + 3 * i, pos + 3 * j, boxsize, periodic);
         memcpy(work, d, sizeof(double) * (size_t)n);
@@ -1314,8 +1324,8 @@ int orc_mass_density(const double *pos, const double *mass, int64_t n, int kerne
         for (int64_t j = 0; j < n; ++j)
             if (d[j] < h)
                 r += mass[j] * orc_kernel_value(kernel, dim, d[j] * h_inv, h_inv);
-        hsml[i] = h;
-        rho[i] = orc_bias_correction(kernel, dim, r, mass[i], h_inv, nneighbors);
+        hsml[a] = h;
+        rho[a] = orc_bias_correction(kernel, dim, r, mass[i], h_inv, nneighbors);
     }
     free(d);
     free(work);

@@ -163,6 +163,12 @@ double orc_bias_correction(int kernel, int dim, double density, double m, double
                            int64_t nneighbors);
 int orc_mass_density(const double *pos, const double *mass, int64_t n, int kernel, int dim,
                      int64_t nneighbors, const double *boxsize, double *rho, double *hsml);
+/* The same for the particles idx[0..nidx) only (against all n); results in
+ * rho[a], hsml[a] for a < nidx. */
+int orc_mass_density_subset(const double *pos, const double *mass, int64_t n,
+                            const int64_t *idx, int64_t nidx, int kernel, int dim,
+                            int64_t nneighbors, const double *boxsize, double *rho,
+                            double *hsml);
 
 #ifdef __cplusplus
 }

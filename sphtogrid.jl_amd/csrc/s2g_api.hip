@@ -771,7 +771,7 @@ int s2g_create(s2g_ctx **out, int device)
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess)
             c->ncu = prop.multiProcessorCount;
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess ||
             hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
             hipHostMalloc((void **)&c->h_small, 64 * sizeof(int64_t), hipHostMallocDefault) !=
                 hipSuccess) {

@@ -126,7 +126,10 @@ def distributed_cic_map(cic_filename: str, Nsubfiles: int, mapping_function,
     print("allocating image")
     total = torch.zeros(nd * ncol, dtype=torch.float64, device=dev)
     scratch = torch.empty(nd * ncol, dtype=torch.float64, device=dev)
-    compute = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize(dev)
+    # explicit streams (default priority): uploads on `copy`; deposits, sums
+    # and the reduce on `compute`, which waits for each upload's event
+    compute = torch.cuda.Stream(dev)
     copy = torch.cuda.Stream(dev)
     cstream = C.c_void_p(compute.cuda_stream)
     vp = lambda t: C.c_void_p(t.data_ptr())
@@ -192,6 +195,7 @@ def distributed_cic_map(cic_filename: str, Nsubfiles: int, mapping_function,
     out = torch.empty(nd * nout, dtype=torch.float64, device=dev)
     ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, Ndim, npix, Nimages, flags, vp(total),
                                         vp(out), cstream), "s2g_reduce_device")
+    compute.synchronize()
     shape = (npix, npix, nout) if Ndim == 2 else (npix, npix, npix)
     image = out.cpu().numpy().reshape(shape, order="F")
     st["wall_s"] = time.perf_counter() - t0
@@ -219,7 +223,8 @@ def distributed_allsky_map(allsky_filename: str, Nside: int, Nsubfiles: int, map
     print("allocating images")
     total = torch.zeros(2 * npx, dtype=torch.float64, device=dev)
     scratch = torch.empty(2 * npx, dtype=torch.float64, device=dev)
-    compute = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize(dev)
+    compute = torch.cuda.Stream(dev)
     copy = torch.cuda.Stream(dev)
     cstream = C.c_void_p(compute.cuda_stream)
     vp = lambda t: C.c_void_p(t.data_ptr())
@@ -281,6 +286,7 @@ def distributed_allsky_map(allsky_filename: str, Nside: int, Nsubfiles: int, map
         ctx.check(ctx.lib.s2g_healpix_reduce_device(ctx.handle, vp(total),
                                                     C.c_void_p(total.data_ptr() + 8 * npx),
                                                     npx, cstream), "s2g_healpix_reduce_device")
+    compute.synchronize()
     both = total.cpu().numpy()
     st["wall_s"] = time.perf_counter() - t0
     st["wait_s"] = wait
