@@ -232,7 +232,7 @@ int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
 int s2g_accumulate_device(s2g_ctx *ctx, const double *d_src, double *d_dst, int64_t count,
                           void *stream);
 
-/* reduce_image of distributed_allsky_map (distributed_mapping/healpix.jl:67-73):
+/* reduce_image of distributed_allsky_map (distributed_mapping/healpix.jl:62-70):
  * d_map[i] /= d_weights[i] where the weight is finite and non-zero.
  * Enqueues only. */
 int s2g_healpix_reduce_device(s2g_ctx *ctx, double *d_map, const double *d_weights,
