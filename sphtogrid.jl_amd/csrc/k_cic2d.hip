@@ -217,10 +217,56 @@ __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restri
 // all 64 tile rows are tabulated once in LDS; per particle each lane computes
 // its column terms, and waves skip particles that miss their strip or row
 // groups (wave-uniform scalar tests).  Per pixel: one sqrt, the kernel
-// polynomial, one FP32 product chain and NP FP64 FMAs (the plane multiplier
-// already carries the particle's scale).
+// polynomial with the row overlap folded into its (1 - u) factor (shape_row),
+// and NP FP64 FMAs (the plane multiplier already carries the particle's scale
+// and the lane's column overlap).
 // ---------------------------------------------------------------------------
 constexpr int kGBatch = 32;
+
+// Power n of the (1 - u) factor of each kernel shape.  The gather folds a
+// row's x overlap ax into that factor as a = ax^(1/n), so the pixel weight
+// ax * shape(u) costs no multiply of its own; a^n equals ax to a few FP32
+// ulps.  ax < 1 only in the first and last rows of a footprint
+// (cic_shared.jl:60-76), ax = 0 outside it.
+template <int K>
+constexpr int kTPow = K == S2G_KERNEL_CUBIC          ? 3
+                      : K == S2G_KERNEL_QUINTIC      ? 5
+                      : K == S2G_KERNEL_WENDLAND_C2  ? 4
+                      : K == S2G_KERNEL_WENDLAND_C4  ? 6
+                      : K == S2G_KERNEL_WENDLAND_C6  ? 8
+                                                     : 10;
+
+// ax * shape(u) with a = ax^(1/n): the SPHKernels closed forms with every
+// (c - u)_+ factor written as a (c - u) clamped to [0, 1] (the clamp is the
+// VALU output modifier)
+template <int K>
+__device__ __forceinline__ float shape_row(float u, float a)
+{
+    const float t = __builtin_amdgcn_fmed3f(fmaf(-a, u, a), 0.0f, 1.0f);
+    if constexpr (K == S2G_KERNEL_CUBIC) {
+        // 2 (1-u)^3 - 8 (1/2-u)^3_+ (= 1 + 6 (u-1) u^2 below u = 1/2)
+        const float v = a * __builtin_amdgcn_fmed3f(0.5f - u, 0.0f, 1.0f);
+        return fmaf(-8.0f, v * v * v, 2.0f * (t * t * t));
+    } else if constexpr (K == S2G_KERNEL_QUINTIC) {
+        const float p = a * __builtin_amdgcn_fmed3f(2.0f / 3.0f - u, 0.0f, 1.0f);
+        const float q = a * __builtin_amdgcn_fmed3f(1.0f / 3.0f - u, 0.0f, 1.0f);
+        const float t2 = t * t, p2 = p * p, q2 = q * q;
+        return fmaf(15.0f, q2 * q2 * q, fmaf(-6.0f, p2 * p2 * p, t2 * t2 * t));
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C2) {
+        const float t2 = t * t;
+        return t2 * t2 * fmaf(4.0f, u, 1.0f);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C4) {
+        const float t2 = t * t;
+        return t2 * t2 * t2 * fmaf(u, fmaf(u, 35.0f / 3.0f, 6.0f), 1.0f);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C6) {
+        const float t2 = t * t, t4 = t2 * t2;
+        return t4 * t4 * fmaf(u, fmaf(u, fmaf(u, 32.0f, 25.0f), 8.0f), 1.0f);
+    } else {  // WENDLAND_C8
+        const float t2 = t * t, t4 = t2 * t2;
+        return t4 * t4 * t2 *
+               fmaf(u, fmaf(u, fmaf(u, fmaf(u, 429.0f / 5.0f, 90.0f), 42.0f), 10.0f), 1.0f);
+    }
+}
 
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items,
@@ -279,13 +325,17 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             }
         }
         __syncthreads();
-        // row table of the batch: (x offset^2, x overlap) for the 64 tile rows
+        // row table of the batch for the 64 tile rows: ((x offset / h)^2,
+        // x overlap^(1/n))
         for (int e = threadIdx.x; e < cnt * kGT; e += 256) {
             const int b = e / kGT, rr = e % kGT;
             const double fi = (double)(ti0 + rr);
             const float xd = (float)coff1(s_x[b], fi);
-            const float ax = fmaxf((float)overlap1(s_x[b], s_h[b], fi), 0.0f);
-            s_row[b][rr] = make_float2(xd * xd, ax);
+            const float ax = (float)overlap1(s_x[b], s_h[b], fi);
+            const float a = ax >= 1.0f  ? 1.0f
+                            : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>))
+                                        : 0.0f;
+            s_row[b][rr] = make_float2(xd * xd * (s_hinv[b] * s_hinv[b]), a);
         }
         // row groups each wave's strip needs for each particle: the footprint
         // box, narrowed to the rows that can reach the kernel disc from the
@@ -329,7 +379,7 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             const float hinv = s_hinv[b];
             const double y = s_y[b], h = s_h[b];
             const float yd = (float)coff1(y, fj);
-            const float yd2 = yd * yd;
+            const float yd2 = yd * yd * (hinv * hinv);
             const float ay = fmaxf((float)overlap1(y, h, fj), 0.0f);
             // the lane's column overlap is common to its 16 pixels: fold it
             // into the FP64 plane multipliers
@@ -343,8 +393,8 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
                 if (k < klo || k > khi)
                     continue;
                 const float2 rvk = s_row[b][4 * k + rg];
-                const float u = __builtin_amdgcn_sqrtf(rvk.x + yd2) * hinv;
-                const double f = (double)(shape_of<K, float>(u) * rvk.y);
+                const float u = __builtin_amdgcn_sqrtf(rvk.x + yd2);
+                const double f = (double)shape_row<K>(u, rvk.y);
 #pragma unroll
                 for (int q = 0; q < NP; ++q)
                     acc[k][q] = fma(f, mult[q], acc[k][q]);
