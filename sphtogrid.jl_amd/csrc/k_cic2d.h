@@ -56,7 +56,8 @@ struct GatherArgs {
 // setup over all particles of the pass: records and the class array
 void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int nq, int kernel,
                      Rec2 *rec, int32_t *cls, hipStream_t s);
-void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s);
+void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *maps, Rec2 *rec,
+                    hipStream_t s);
 void launch_2d_gather(const GatherArgs &g, hipStream_t s);
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s);
 // keys (global small tile) and values (particle) of the small list

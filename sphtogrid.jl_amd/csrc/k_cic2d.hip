@@ -139,90 +139,11 @@ __global__ __launch_bounds__(256) void k2d_setup(const Map2D *__restrict__ maps,
 }
 
 // ---------------------------------------------------------------------------
-// Pixel weight factor shared by the normalisation and the gather so that the
-// sum and the deposited values are computed identically:
-//   f = shape(u) * ay * ax   (FP32, in [0, 1])
+// Pixel weights shared by the normalisation sum and the tile gather: both
+// form s = (x offset^2 + y offset^2) / h^2 with the same FP32 operations and
+// evaluate ax * shape(sqrt(s)) with shape_row, so a particle's deposited
+// weights and its normalisation sum come from the same per-pixel values.
 // ---------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ float pix_factor(float xd2, float yd2, float hinv, float ax,
-                                            float ay)
-{
-    float u = __builtin_amdgcn_sqrtf(xd2 + yd2) * hinv;
-    float W = shape_of<K, float>(u);
-    return (W * ay) * ax;
-}
-
-// ---------------------------------------------------------------------------
-// Explicit normalisation for large footprints that are clipped by the image
-// border (or below the closed-form threshold): one block per particle,
-// deterministic FP64 sum (fixed per-thread order + fixed tree).
-// ---------------------------------------------------------------------------
-constexpr int kNormThreads = 256;
-
-// One wave per particle.  Rows are taken 64 at a time: lane l computes the
-// row terms of row i0+64c+l once, and the row loop broadcasts them with
-// v_readlane; lanes run over the columns.  Per-lane FP64 sums in a fixed
-// order and a fixed butterfly make the result bit-reproducible.
-template <int K>
-__device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restrict__ list, int64_t ne,
-                                             Rec2 *__restrict__ rec)
-{
-    const int ln = threadIdx.x % kWave;
-    const int p = list[gw];
-    const Rec2 r = rec[p];
-    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
-    double acc = 0.0;
-    for (int ic = r.i0; ic <= r.i1; ic += kWave) {
-        const double fi = (double)(ic + ln);
-        const float xd = (float)coff1(r.x, fi);
-        const float xd2 = xd * xd;
-        const float ax = fmaxf((float)overlap1(r.x, r.h, fi), 0.0f);
-        const int nrow = r.i1 - ic + 1 < kWave ? r.i1 - ic + 1 : kWave;
-        for (int jc = r.j0; jc <= r.j1; jc += kWave) {
-            const int j = jc + ln;
-            const double fj = (double)j;
-            const float yd = (float)coff1(r.y, fj);
-            const float yd2 = yd * yd;
-            const float ay = (j <= r.j1) ? fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) : 0.0f;
-            for (int rr = 0; rr < nrow; ++rr) {
-                const float xd2r =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), rr));
-                const float axr =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), rr));
-                acc += (double)pix_factor<K>(xd2r, yd2, hinv, axr, ay);
-            }
-        }
-    }
-    acc = wave_sum(acc);
-    if (ln == 0)
-        rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
-}
-
-template <int K>
-__global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
-                                                         int64_t ne, Rec2 *__restrict__ rec)
-{
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
-         gw += nw)
-        k2d_norm_one<K>(gw, list, ne, rec);
-}
-
-
-// ---------------------------------------------------------------------------
-// Tile gather.  Block = 4 waves = one 64x64 tile.  Wave w owns the 16-column
-// strip [16w, 16w+16) of the tile; lane l works on column 16w + (l & 15) and
-// rows 4k + (l >> 4), k = 0..15, so each lane accumulates 16 pixels in FP64
-// registers.  Per batch of particles the row terms (x offset^2, x overlap) of
-// all 64 tile rows are tabulated once in LDS; per particle each lane computes
-// its column terms, and waves skip particles that miss their strip or row
-// groups (wave-uniform scalar tests).  Per pixel: one sqrt, the kernel
-// polynomial with the row overlap folded into its (1 - u) factor (shape_row),
-// and NP FP64 FMAs (the plane multiplier already carries the particle's scale
-// and the lane's column overlap).
-// ---------------------------------------------------------------------------
-constexpr int kGBatch = 32;
-
 // Power n of the (1 - u) factor of each kernel shape.  The gather folds a
 // row's x overlap ax into that factor as a = ax^(1/n), so the pixel weight
 // ax * shape(u) costs no multiply of its own; a^n equals ax to a few FP32
@@ -267,6 +188,148 @@ __device__ __forceinline__ float shape_row(float u, float a)
                fmaf(u, fmaf(u, fmaf(u, fmaf(u, 429.0f / 5.0f, 90.0f), 42.0f), 10.0f), 1.0f);
     }
 }
+
+// a = ax^(1/n) of a row (or column) overlap ax, as shape_row takes it
+template <int K>
+__device__ __forceinline__ float overlap_root(float ax)
+{
+    return ax >= 1.0f ? 1.0f : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>)) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit normalisation for large footprints that are clipped by the image
+// border, below the closed-form threshold or with S2G_FLAG_EXACT_NORM: one
+// wave per particle, deterministic FP64 sum (fixed per-lane order + fixed
+// butterfly).
+// ---------------------------------------------------------------------------
+constexpr int kNormThreads = 256;
+
+// This lane's share of sum over rows [ia, ib] x columns [ja, jb] (any
+// integers: pixels outside the image follow the same formulas) of
+// shape_row(u, ax^(1/n)) * ay in FP64.  Lanes take the columns; rows are
+// taken 64 at a time, lane l computing the terms of row ic + l once and the
+// row loop broadcasting them with v_readlane.  With `disc`, each 64-column
+// chunk runs only over the rows that can reach the kernel disc from it
+// (|x - i - 1/2| < sqrt(h^2 - dy^2) + 1, dy the chunk's nearest column-centre
+// distance): the rows left out have u >= 1, so the sum is unchanged.
+template <int K>
+__device__ __forceinline__ double norm_rect(const Rec2 &r, float h2, bool disc, int64_t ia,
+                                            int64_t ib, int64_t ja, int64_t jb, int ln)
+{
+    double acc = 0.0;
+    for (int64_t jc = ja; jc <= jb; jc += kWave) {
+        int64_t ra = ia, rb = ib;
+        if (disc) {
+            const float xc = (float)r.x, yc = (float)r.y, hf = (float)r.h;
+            const int64_t jh = jc + kWave - 1 < jb ? jc + kWave - 1 : jb;
+            const float dy = fmaxf(0.0f, fmaxf((float)jc + 0.5f - yc, yc - ((float)jh + 0.5f)));
+            if (!(dy < hf + 1.0f))
+                continue;
+            const float hx = sqrtf(fmaxf(hf * hf - dy * dy, 0.0f)) + 1.0f;
+            const int64_t da = (int64_t)floorf(xc - 0.5f - hx), db = (int64_t)ceilf(xc - 0.5f + hx);
+            ra = da > ra ? da : ra;
+            rb = db < rb ? db : rb;
+        }
+        const int64_t j = jc + ln;
+        const double fj = (double)j;
+        const float yd = (float)coff1(r.y, fj);
+        const float yd2 = yd * yd * h2;
+        const float ay = j <= jb ? fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) : 0.0f;
+        double col = 0.0;
+        for (int64_t ic = ra; ic <= rb; ic += kWave) {
+            const double fi = (double)(ic + ln);
+            const float xd = (float)coff1(r.x, fi);
+            const float rx = xd * xd * h2;
+            const float a = overlap_root<K>((float)overlap1(r.x, r.h, fi));
+            const int nrow = rb - ic + 1 < kWave ? (int)(rb - ic + 1) : kWave;
+            for (int rr = 0; rr < nrow; ++rr) {
+                const float rxr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rx), rr));
+                const float ar = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), rr));
+                col += (double)shape_row<K>(__builtin_amdgcn_sqrtf(rxr + yd2), ar);
+            }
+        }
+        acc += col * (double)ay;
+    }
+    return acc;
+}
+
+// Sum over the footprint in the image.  When the closed form h^2 I_k holds for
+// the unclipped footprint (h >= the kernel's threshold, no fallback, no
+// S2G_FLAG_EXACT_NORM), the particle's centre is in the image and the part cut
+// off by the border is the smaller one, the sum is h^2 I_k minus the sum over
+// that part (the border rows, then the border columns of the kept rows);
+// otherwise the footprint in the image is summed directly.
+template <int K>
+__device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restrict__ list,
+                                             const Map2D *__restrict__ maps,
+                                             Rec2 *__restrict__ rec)
+{
+    const int ln = threadIdx.x % kWave;
+    const int p = list[gw];
+    const Rec2 r = rec[p];
+    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
+    const float h2 = hinv * hinv;
+    const int64_t i0 = r.i0, i1 = r.i1, j0 = r.j0, j1 = r.j1;
+    bool cut = false;
+    double A0 = 0.0, A1 = 0.0, B0 = 0.0, B1 = 0.0;
+    if (!r.fb && r.h >= closed_form_min_h(K) && !(maps[r.map].flags & S2G_FLAG_EXACT_NORM)) {
+        const double np = (double)maps[r.map].npix;
+        A0 = floor(sub_rn(r.x, r.h));
+        A1 = floor(add_rn(r.x, r.h));
+        B0 = floor(sub_rn(r.y, r.h));
+        B1 = floor(add_rn(r.y, r.h));
+        const double kept = (double)(i1 - i0 + 1) * (double)(j1 - j0 + 1);
+        const double full = (A1 - A0 + 1.0) * (B1 - B0 + 1.0);
+        cut = r.x >= 0.0 && r.x < np && r.y >= 0.0 && r.y < np && full - kept < kept;
+    }
+    double acc;
+    if (cut) {
+        const int64_t a0 = (int64_t)A0, a1 = (int64_t)A1, b0 = (int64_t)B0, b1 = (int64_t)B1;
+        double out = 0.0;
+        if (a0 < i0)
+            out += norm_rect<K>(r, h2, true, a0, i0 - 1, b0, b1, ln);
+        if (a1 > i1)
+            out += norm_rect<K>(r, h2, true, i1 + 1, a1, b0, b1, ln);
+        if (b0 < j0)
+            out += norm_rect<K>(r, h2, true, i0, i1, b0, j0 - 1, ln);
+        if (b1 > j1)
+            out += norm_rect<K>(r, h2, true, i0, i1, j1 + 1, b1, ln);
+        out = wave_sum(out);
+        acc = mul_rn(r.h, r.h) * shape_disc_integral(K) - out;
+    } else {
+        acc = wave_sum(norm_rect<K>(r, h2, !r.fb, i0, i1, j0, j1, ln));
+    }
+    if (ln == 0)
+        rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
+}
+
+template <int K>
+__global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
+                                                         int64_t ne,
+                                                         const Map2D *__restrict__ maps,
+                                                         Rec2 *__restrict__ rec)
+{
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
+         gw += nw)
+        k2d_norm_one<K>(gw, list, maps, rec);
+}
+
+
+// ---------------------------------------------------------------------------
+// Tile gather.  Block = 4 waves = one 64x64 tile.  Wave w owns the 16-column
+// strip [16w, 16w+16) of the tile; lane l works on column 16w + (l & 15) and
+// rows 4k + (l >> 4), k = 0..15, so each lane accumulates 16 pixels in FP64
+// registers.  Per batch of particles the row terms (x offset^2, x overlap) of
+// all 64 tile rows are tabulated once in LDS; per particle each lane computes
+// its column terms, and waves skip particles that miss their strip or row
+// groups (wave-uniform scalar tests).  Per pixel: one sqrt, the kernel
+// polynomial with the row overlap folded into its (1 - u) factor (shape_row),
+// and NP FP64 FMAs (the plane multiplier already carries the particle's scale
+// and the lane's column overlap).
+// ---------------------------------------------------------------------------
+constexpr int kGBatch = 32;
+constexpr int kGStep = 4;  // row groups per skip test (measured best of 1, 2, 4, 8, 16)
 
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items,
@@ -331,10 +394,7 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             const int b = e / kGT, rr = e % kGT;
             const double fi = (double)(ti0 + rr);
             const float xd = (float)coff1(s_x[b], fi);
-            const float ax = (float)overlap1(s_x[b], s_h[b], fi);
-            const float a = ax >= 1.0f  ? 1.0f
-                            : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>))
-                                        : 0.0f;
+            const float a = overlap_root<K>((float)overlap1(s_x[b], s_h[b], fi));
             s_row[b][rr] = make_float2(xd * xd * (s_hinv[b] * s_hinv[b]), a);
         }
         // row groups each wave's strip needs for each particle: the footprint
@@ -388,16 +448,27 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
 #pragma unroll
             for (int q = 0; q < NP; ++q)
                 mult[q] = s_mult[q][b] * ayd;
+            // kGStep row groups per wave-uniform test, so their LDS reads and
+            // dependency chains overlap.  A group outside [klo, khi] adds
+            // exact zeros (a = 0 outside the footprint, u >= 1 outside the
+            // row range), so the image is the same as with one group per test.
 #pragma unroll
-            for (int k = 0; k < kGRows; ++k) {
-                if (k < klo || k > khi)
+            for (int k0 = 0; k0 < kGRows; k0 += kGStep) {
+                if (k0 + kGStep - 1 < klo || k0 > khi)
                     continue;
-                const float2 rvk = s_row[b][4 * k + rg];
-                const float u = __builtin_amdgcn_sqrtf(rvk.x + yd2);
-                const double f = (double)shape_row<K>(u, rvk.y);
+                float wk[kGStep];
 #pragma unroll
-                for (int q = 0; q < NP; ++q)
-                    acc[k][q] = fma(f, mult[q], acc[k][q]);
+                for (int d = 0; d < kGStep; ++d) {
+                    const float2 rvk = s_row[b][4 * (k0 + d) + rg];
+                    wk[d] = shape_row<K>(__builtin_amdgcn_sqrtf(rvk.x + yd2), rvk.y);
+                }
+#pragma unroll
+                for (int d = 0; d < kGStep; ++d) {
+                    const double f = (double)wk[d];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q)
+                        acc[k0 + d][q] = fma(f, mult[q], acc[k0 + d][q]);
+                }
             }
         }
         __syncthreads();
@@ -708,12 +779,13 @@ void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int 
                            cls);
 }
 
-void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, Rec2 *rec, hipStream_t s)
+void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *maps, Rec2 *rec,
+                    hipStream_t s)
 {
     if (ne == 0)
         return;
     const dim3 g((unsigned)wave_blocks(ne, kNormThreads)), b(kNormThreads);
-    S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL(k2d_norm<KK>, g, b, 0, s, list, ne, rec));
+    S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL(k2d_norm<KK>, g, b, 0, s, list, ne, maps, rec));
 }
 
 template <typename T, int K, int NP>

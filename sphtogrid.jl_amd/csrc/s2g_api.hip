@@ -328,7 +328,7 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         S2G_HIP(hipEventRecord(E[1], s));
     }
     S2G_HIP(hipEventRecord(E[2], s));
-    launch_2d_norm(kernel, cl.l2, cl.ne, rec, s);
+    launch_2d_norm(kernel, cl.l2, cl.ne, d_maps, rec, s);
     S2G_HIP(hipEventRecord(E[3], s));
 
     double ms_sort = 0.0, ms_gather = 0.0, ms_fin = 0.0;

@@ -71,6 +71,34 @@ def test_flat_image_kernels_dtypes(s2g, orc, kernel, dtype):
     _check_flat(gpu, ref, f"{kernel}/{dtype.__name__}")
 
 
+@pytest.mark.parametrize("kernel", KNAMES)
+def test_clipped_footprints_border_sum(s2g, orc, kernel):
+    """Large footprints cut by the image border: the normalisation is the
+    closed form minus the sum over the cut-off part where that part is the
+    smaller one, else the direct sum.  Both must equal the exact sum
+    (S2G_FLAG_EXACT_NORM) and the oracle."""
+    npix, n = 160, 400
+    r = np.random.default_rng(300 + KNAMES.index(kernel))
+    x = r.uniform(-0.5, 0.5, (n, 3))
+    side = r.integers(0, 2, n)
+    x[np.arange(n), side] = np.where(r.random(n) < 0.5, 1, -1) * r.uniform(0.38, 0.499, n)
+    h = r.uniform(70.0, 160.0, n) / npix  # >= every kernel's closed-form threshold
+    m, rho, w = (r.uniform(0.5, 2.0, n) for _ in range(3))
+    q = r.uniform(0.1, 5.0, n)
+    st_a, st_b = {}, {}
+    a = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix),
+                    kernel=_kernel(s2g, kernel), calc_mean=True, stats=st_a)
+    b = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix),
+                    kernel=_kernel(s2g, kernel), calc_mean=True, exact_norm=True, stats=st_b)
+    assert st_a["n_large_exact"] > n // 2, st_a
+    for k in range(2):
+        e = rel_l1(a[:, k], b[:, k])
+        print(kernel, "plane", k, "border sum vs exact sum rel_l1", e, flush=True)
+        assert e <= 1e-8, (kernel, k, e)
+    ref = orc.cic_mapping_2D(x, h, m, rho, q, w, npix, float(npix), kernel, 2, True)
+    _check_flat(a, ref, f"border/{kernel}")
+
+
 @pytest.mark.parametrize("nq", [1, 2, 3, 4, 6])
 def test_multi_quantity_planes(s2g, orc, nq):
     npix, n = 80, 1200
