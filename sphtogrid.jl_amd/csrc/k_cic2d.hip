@@ -340,6 +340,7 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
                                                   int plane0, double *__restrict__ partial)
 {
     __shared__ float2 s_row[kGBatch][kGT];
+    __shared__ float2 s_col[kGBatch][kGT];
     __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch];
     __shared__ float s_hinv[kGBatch];
     __shared__ int s_i0[kGBatch], s_i1[kGBatch], s_j0[kGBatch], s_j1[kGBatch];
@@ -354,9 +355,6 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
     const int ti0 = (tile / ntj) * kGT, tj0 = (tile % ntj) * kGT;
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     const int rg = ln >> 4;
-    const int sj0 = tj0 + 16 * wv;  // first column of this wave's strip
-    const int j = sj0 + (ln & 15);
-    const double fj = (double)j;
 
     double acc[kGRows][NP];
 #pragma unroll
@@ -388,14 +386,19 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             }
         }
         __syncthreads();
-        // row table of the batch for the 64 tile rows: ((x offset / h)^2,
-        // x overlap^(1/n))
+        // row and column tables of the batch for the 64 tile rows and
+        // columns: ((x offset / h)^2, x overlap^(1/n)) and
+        // ((y offset / h)^2, y overlap)
         for (int e = threadIdx.x; e < cnt * kGT; e += 256) {
             const int b = e / kGT, rr = e % kGT;
-            const double fi = (double)(ti0 + rr);
+            const float h2 = s_hinv[b] * s_hinv[b];
+            const double fi = (double)(ti0 + rr), fj = (double)(tj0 + rr);
             const float xd = (float)coff1(s_x[b], fi);
             const float a = overlap_root<K>((float)overlap1(s_x[b], s_h[b], fi));
-            s_row[b][rr] = make_float2(xd * xd * (s_hinv[b] * s_hinv[b]), a);
+            s_row[b][rr] = make_float2(xd * xd * h2, a);
+            const float yd = (float)coff1(s_y[b], fj);
+            const float ay = fmaxf((float)overlap1(s_y[b], s_h[b], fj), 0.0f);
+            s_col[b][rr] = make_float2(yd * yd * h2, ay);
         }
         // row groups each wave's strip needs for each particle: the footprint
         // box, narrowed to the rows that can reach the kernel disc from the
@@ -436,14 +439,11 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             if (kk < 0)
                 continue;
             const int klo = kk & 0xff, khi = kk >> 8;
-            const float hinv = s_hinv[b];
-            const double y = s_y[b], h = s_h[b];
-            const float yd = (float)coff1(y, fj);
-            const float yd2 = yd * yd * (hinv * hinv);
-            const float ay = fmaxf((float)overlap1(y, h, fj), 0.0f);
+            const float2 cv = s_col[b][16 * wv + (ln & 15)];
+            const float yd2 = cv.x;
             // the lane's column overlap is common to its 16 pixels: fold it
             // into the FP64 plane multipliers
-            const double ayd = (double)ay;
+            const double ayd = (double)cv.y;
             double mult[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q)
