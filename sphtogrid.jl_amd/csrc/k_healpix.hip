@@ -302,15 +302,37 @@ __device__ __forceinline__ void hp_weights(float d, float theta, float inv_theta
     A = fmaxf(0.0f, fminf(ang, a)) * inv_ang;
 }
 
+// One ring of a work unit, as the pixel loop reads it (per-wave LDS table).
+struct HpRing {
+    int off;     // first position of the ring's chord in the unit's pixel list
+    int lo;      // ring-local index of the chord's first pixel (< 0: wraps)
+    int start;   // RING index of the ring's first pixel
+    int nr;      // pixels in the ring
+    float s1, s2;
+    float base;  // phi of the chord's first pixel - phi0, in [-pi, pi)
+    float step;  // 2 pi / nr
+};
+
+__device__ __forceinline__ int wave_incl_scan(int v, int ln)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o, kWave);
+        v += ln >= o ? t : 0;
+    }
+    return v;
+}
+
 // Iterate the pixels of unit `ui` of particle r: rings [ra + 64*ui, ...].
 // Phase 1: lane l computes the pixel interval and the terms of ring
-// ra + 64*ui + l.  Phase 2: the pixels are spread over sub-groups of
-// G = nextpow2(longest interval) lanes, one ring per sub-group, 64/G rings at
-// a time, so short rings (small discs) still fill the wave.
+// ra + 64*ui + l and writes them to the wave's LDS table `tab`, with its
+// chord's offset in the unit's pixel list (wave prefix sum).  Phase 2: lane
+// l takes list positions l, l + 64, ..., walking forward through the table,
+// so every lane holds a pixel until the list ends.
 // F(pix, W, A) is called per pixel on every lane that holds a pixel.
 template <int K, typename F>
 __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, int ui, int ln,
-                                               float ang, F &&f)
+                                               float ang, HpRing *tab, F &&f)
 {
     const float theta = (float)r.theta, inv_theta = (float)(1.0 / r.theta);
     const float inv_ang = 1.0f / ang;
@@ -322,9 +344,12 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
     hp_pix2ring(nside, r.cpix, cring, ck);
 
     // phase 1: one ring per lane
-    int lo = 0, len = 0, start = 0, nr = 1;
-    float s1 = 0.0f, s2 = 0.0f;
-    double dphi = 0.0, phase = 0.0;
+    HpRing e;
+    e.lo = 0;
+    e.start = 0;
+    e.nr = 1;
+    e.s1 = e.s2 = e.base = e.step = 0.0f;
+    int len = 0;
     bool cov = false;
     if (ln < nrings) {
         const int64_t iz = r0 + ln;
@@ -342,66 +367,51 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
         }
         if (any) {
             const RingTerms t = ring_terms(nside, iz, r);
-            lo = (int)a;
+            e.lo = (int)a;
             len = (int)(b - a + 1);
-            start = (int)t.start;
-            nr = (int)t.nr;
-            s1 = t.s1;
-            s2 = t.s2;
-            dphi = t.dphi_pix;
-            phase = t.phase;
+            e.start = (int)t.start;
+            e.nr = (int)t.nr;
+            e.s1 = t.s1;
+            e.s2 = t.s2;
+            double d0 = (double)a * t.dphi_pix + t.phase;  // phi_a - phi0
+            d0 = d0 >= kPi ? d0 - 2.0 * kPi : d0;
+            d0 = d0 < -kPi ? d0 + 2.0 * kPi : d0;
+            e.base = (float)d0;
+            e.step = (float)t.dphi_pix;
             if (iz == cring)
                 cov = (a >= 0 && ck >= a && ck <= b) || (a < 0 && (ck <= b || ck >= a + t.nr));
         }
     }
     const bool covered_cp = __ballot(cov) != 0ull;
-    int maxlen = len;
-    for (int o = 32; o > 0; o >>= 1) {
-        const int v = __shfl_xor(maxlen, o, kWave);
-        maxlen = v > maxlen ? v : maxlen;
-    }
-    // phase 2: sub-groups of G lanes per ring
-    int G = 1;
-    while (G < maxlen && G < kWave)
-        G <<= 1;
-    const int P = kWave / G;
-    const int sub = ln % G, slot = ln / G;
-    for (int g = 0; g < nrings; g += P) {
-        // every lane takes part in every shuffle: a cross-lane read placed
-        // under a lane condition reads 0 from source lanes that the
-        // condition switched off
-        const int j = g + slot;
-        const int src = j < nrings ? j : 0;
-        const int jlo = __shfl(lo, src, kWave);
-        const int jlen_src = __shfl(len, src, kWave);
-        const int jlen = j < nrings ? jlen_src : 0;
-        const int jstart = __shfl(start, src, kWave);
-        const int jnr = __shfl(nr, src, kWave);
-        RingTerms t;
-        t.s1 = __shfl(s1, src, kWave);
-        t.s2 = __shfl(s2, src, kWave);
-        t.dphi_pix = __shfl(dphi, src, kWave);
-        t.phase = __shfl(phase, src, kWave);
-        t.start = jstart;
-        t.nr = jnr;
-        int glen = jlen;
-        for (int o = 32; o > 0; o >>= 1) {
-            const int v = __shfl_xor(glen, o, kWave);
-            glen = v > glen ? v : glen;
+    const int incl = wave_incl_scan(len, ln);
+    const int total = __shfl(incl, kWave - 1, kWave);
+    e.off = incl - len;
+    __builtin_amdgcn_wave_barrier();
+    tab[ln] = e;
+    __builtin_amdgcn_wave_barrier();
+
+    // phase 2: the unit's pixel list, 64 positions at a time
+    int j = -1, next = 0;
+    HpRing cur = e;
+    for (int q = ln; q < total; q += kWave) {
+        while (q >= next) {
+            ++j;
+            cur = tab[j];
+            next = j + 1 < kWave ? tab[j + 1].off : total;
         }
-        for (int c0 = 0; c0 < glen; c0 += G) {
-            const int c = c0 + sub;
-            if (c < jlen) {
-                int k = jlo + c;
-                if (k < 0)
-                    k += jnr;
-                const float d = hp_dist(t, k);
-                float W, A;
-                hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
-                f((int64_t)jstart + k, W, A);
-            }
-        }
+        const int c = q - cur.off;
+        int k = cur.lo + c;
+        k = k < 0 ? k + cur.nr : k;
+        float dphi = fmaf((float)c, cur.step, cur.base);
+        dphi = dphi >= (float)kPi ? dphi - (float)(2.0 * kPi) : dphi;
+        const float sx = sin_half_pi(0.5f * dphi);
+        const float hav = fminf(fmaf(cur.s2, sx * sx, cur.s1), 1.0f);
+        const float d = 2.0f * asinf(sqrtf(hav));
+        float W, A;
+        hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+        f((int64_t)cur.start + k, W, A);
     }
+    __builtin_amdgcn_wave_barrier();
     // constributing_pixels.jl:16-19: the pixel containing the particle is
     // added if the disc query did not return it (handled by the unit that
     // owns its ring, or by unit 0 if its ring is outside [ra, rb])
@@ -416,34 +426,39 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
     }
 }
 
+// Per unit: sum of W*A in FP64 and of A in FP32 (A is needed only in the
+// fallback, i.e. when no pixel centre lies inside the kernel, which happens
+// only for discs of a few pixels).
 template <int K>
-__device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
-        int64_t nside, double4 *__restrict__ part)
+__device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict__ units,
+                                              const HpRec *__restrict__ rec, int64_t nside,
+                                              HpRing *tab, double4 *__restrict__ part)
 {
     const int ln = threadIdx.x % kWave;
     const int2 u = units[gw];
     const HpRec r = rec[u.x];
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
-    double swa = 0.0, sa = 0.0, nin = 0.0;
-    hp_unit_pixels<K>(nside, r, u.y, ln, ang, [&](int64_t, float W, float A) {
+    double swa = 0.0;
+    float sa = 0.0f;
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, [&](int64_t, float W, float A) {
         swa += (double)(W * A);
-        sa += (double)A;
-        nin += W > 0.0f ? 1.0 : 0.0;
+        sa += A;
     });
     swa = wave_sum(swa);
-    sa = wave_sum(sa);
-    nin = wave_sum(nin);
+    const double sad = wave_sum((double)sa);
     if (ln == 0)
-        part[gw] = make_double4(swa, sa, nin, 0.0);
+        part[gw] = make_double4(swa, sad, 0.0, 0.0);
 }
 template <int K>
-__global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
-        int64_t nside, double4 *__restrict__ part)
+__global__ __launch_bounds__(256) void k_hp_norm(const int2 *__restrict__ units, int64_t nu,
+                                                 const HpRec *__restrict__ rec, int64_t nside,
+                                                 double4 *__restrict__ part)
 {
+    __shared__ HpRing s_tab[256 / kWave][kWave];
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
          gw += nw)
-        k_hp_norm_one<K>(gw, units, nu, rec, nside, part);
+        k_hp_norm_one<K>(gw, units, rec, nside, s_tab[threadIdx.x / kWave], part);
 }
 
 
@@ -475,9 +490,11 @@ __global__ void k_hp_finish(const int64_t *__restrict__ ubase, int64_t n,
 }
 
 template <typename T, int K>
-__device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
-        const T *__restrict__ binq, int64_t nside, double *__restrict__ map,
-        double *__restrict__ wmap)
+__device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restrict__ units,
+                                                 const HpRec *__restrict__ rec,
+                                                 const T *__restrict__ binq, int64_t nside,
+                                                 HpRing *tab, double *__restrict__ map,
+                                                 double *__restrict__ wmap)
 {
     const int ln = threadIdx.x % kWave;
     const int2 u = units[gw];
@@ -487,7 +504,7 @@ __device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restr
     const double bq = ld(binq, u.x);
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     const bool fb = r.fb != 0;
-    hp_unit_pixels<K>(nside, r, u.y, ln, ang, [&](int64_t pix, float W, float A) {
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, [&](int64_t pix, float W, float A) {
         const double pw = (double)((fb ? 1.0f : W) * A) * r.s;
         if (pw != 0.0) {
             unsafeAtomicAdd(&wmap[pix], pw);
@@ -496,14 +513,18 @@ __device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restr
     });
 }
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu, const HpRec *__restrict__ rec,
-        const T *__restrict__ binq, int64_t nside, double *__restrict__ map,
-        double *__restrict__ wmap)
+__global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu,
+                                                    const HpRec *__restrict__ rec,
+                                                    const T *__restrict__ binq, int64_t nside,
+                                                    double *__restrict__ map,
+                                                    double *__restrict__ wmap)
 {
+    __shared__ HpRing s_tab[256 / kWave][kWave];
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
          gw += nw)
-        k_hp_deposit_one<T, K>(gw, units, nu, rec, binq, nside, map, wmap);
+        k_hp_deposit_one<T, K>(gw, units, rec, binq, nside, s_tab[threadIdx.x / kWave], map,
+                               wmap);
 }
 
 
