@@ -559,47 +559,38 @@ __global__ void k_tile_offsets(const uint32_t *__restrict__ keys, int64_t ns, in
     off[t] = lo;
 }
 
-// One small particle on one wave: FP64 row / column terms in the wave's LDS
-// tables, the pixel weights kept in registers (at most KMAX per lane), the
-// wave-reduced normalisation sum, then the weights times the particle's plane
-// multipliers added into the block's LDS window (ds_add_f64).
-template <int K, int NP, int KMAX>
-__device__ __forceinline__ void small_particle(const Rec2 &r, int e, int ln, int oi, int oj,
-                                               double2 *rt_tab, double2 *ct_tab,
-                                               double (*s_mult)[kSmallBatch],
-                                               double (*acc)[kSW * kSW])
+// One small particle on one wave: row / column terms in the wave's LDS
+// tables (((x offset / h)^2, x overlap^(1/n)) and ((y offset / h)^2,
+// y overlap), as the tile gather forms them), the pixel weights
+// ay * shape_row(u, a) in FP32 kept in registers (at most KMAX per lane), the
+// normalisation sum in FP64 reduced across the wave, then the weights times
+// the particle's FP64 plane multipliers added into the block's LDS window
+// (ds_add_f64).
+// FP64 weights, computed per pixel without tables (two passes: the sum,
+// then the deposit), for the footprints whose FP32 weights cannot resolve the
+// reference's: h < 2 px, where every pixel centre inside the kernel may lie
+// near its edge (1 - u of the order of the FP32 rounding of u), and the
+// fallback to area weights.  Such footprints have at most 5 x 5 pixels.
+template <int K, int NP>
+__device__ __forceinline__ void small_particle64(const Rec2 &r, int e, int ln, int oi, int oj,
+                                                 double (*s_mult)[kSmallBatch],
+                                                 double (*acc)[kSW * kSW])
 {
-    const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
-    const int fp = iw * jw;
+    const int jw = r.j1 - r.j0 + 1;
+    const int fp = (r.i1 - r.i0 + 1) * jw;
     const double hinv = r.fb ? 0.0 : 1.0 / r.h;
-    if (ln < iw) {
-        const double fi = (double)(r.i0 + ln);
-        const double xd = coff1(r.x, fi);
-        rt_tab[ln] = make_double2(xd * xd, overlap1(r.x, r.h, fi));
-    }
-    if (ln < jw) {
-        const double fj = (double)(r.j0 + ln);
-        const double yd = coff1(r.y, fj);
-        ct_tab[ln] = make_double2(yd * yd, overlap1(r.y, r.h, fj));
-    }
-    const float invjw = 1.0f / (float)jw;
-    double wk[KMAX];
-    int cell[KMAX];
+    auto weight = [&](int f, int &cell) {
+        const int di = f / jw, dj = f - di * jw;
+        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj);
+        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj);
+        const double u = sqrt(xd * xd + yd * yd) * hinv;
+        cell = (r.i0 + di - oi) * kSW + (r.j0 + dj - oj);
+        return shape_of<K, double>(u) * (overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj));
+    };
     double sum = 0.0;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        wk[k] = 0.0;
-        cell[k] = 0;
-        const int f = ln + k * kWave;
-        if (f < fp) {
-            const int di = (int)(((float)f + 0.5f) * invjw);
-            const int dj = f - di * jw;
-            const double2 rt = rt_tab[di], ct = ct_tab[dj];
-            const double u = sqrt(rt.x + ct.x) * hinv;
-            wk[k] = shape_of<K, double>(u) * (rt.y * ct.y);
-            cell[k] = (r.i0 + di - oi) * kSW + (r.j0 + dj - oj);
-            sum += wk[k];
-        }
+    for (int f = ln; f < fp; f += kWave) {
+        int c;
+        sum += weight(f, c);
     }
     sum = wave_sum(sum);
     if (!(sum != 0.0))
@@ -609,14 +600,79 @@ __device__ __forceinline__ void small_particle(const Rec2 &r, int e, int ln, int
 #pragma unroll
     for (int q = 0; q < NP; ++q)
         mult[q] = s_mult[q][e] * inv;
-#pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        if (wk[k] != 0.0) {
+    for (int f = ln; f < fp; f += kWave) {
+        int c;
+        const double w = weight(f, c);
+        if (w != 0.0) {
 #pragma unroll
             for (int q = 0; q < NP; ++q)
-                unsafeAtomicAdd(&acc[q][cell[k]], wk[k] * mult[q]);
+                unsafeAtomicAdd(&acc[q][c], w * mult[q]);
         }
     }
+}
+
+// FP32 weights for h >= 2 px: the pixel holding the particle's centre has
+// u <= 0.36, so the largest weights are well conditioned and the FP32
+// rounding of u only moves weights far below them.  Returns false (nothing
+// deposited) if the weight sum is below 1e-30, which leaves the particle to
+// the FP64 path.
+template <int K, int NP, int KMAX>
+__device__ __forceinline__ bool small_particle(const Rec2 &r, int e, int ln, int oi, int oj,
+                                               float2 *rt_tab, float2 *ct_tab,
+                                               double (*s_mult)[kSmallBatch],
+                                               double (*acc)[kSW * kSW])
+{
+    const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
+    const int fp = iw * jw;
+    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
+    const float h2 = hinv * hinv;
+    if (ln < iw) {
+        const double fi = (double)(r.i0 + ln);
+        const float xd = (float)coff1(r.x, fi);
+        rt_tab[ln] = make_float2(xd * xd * h2, overlap_root<K>((float)overlap1(r.x, r.h, fi)));
+    }
+    if (ln < jw) {
+        const double fj = (double)(r.j0 + ln);
+        const float yd = (float)coff1(r.y, fj);
+        ct_tab[ln] = make_float2(yd * yd * h2, fmaxf((float)overlap1(r.y, r.h, fj), 0.0f));
+    }
+    const float invjw = 1.0f / (float)jw;
+    const int cbase = (r.i0 - oi) * kSW + (r.j0 - oj);
+    float wk[KMAX];
+    int cell[KMAX];
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        wk[k] = 0.0f;
+        cell[k] = 0;
+        const int f = ln + k * kWave;
+        if (f < fp) {
+            const int di = (int)(((float)f + 0.5f) * invjw);
+            const int dj = f - di * jw;
+            const float2 rt = rt_tab[di], ct = ct_tab[dj];
+            wk[k] = shape_row<K>(__builtin_amdgcn_sqrtf(rt.x + ct.x), rt.y) * ct.y;
+            cell[k] = cbase + di * kSW + dj;
+            sum += (double)wk[k];
+        }
+    }
+    sum = wave_sum(sum);
+    if (!(sum >= 1e-30))
+        return false;
+    const double inv = 1.0 / sum;
+    double mult[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+        mult[q] = s_mult[q][e] * inv;
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+        if (wk[k] != 0.0f) {
+            const double w = (double)wk[k];
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                unsafeAtomicAdd(&acc[q][cell[k]], w * mult[q]);
+        }
+    }
+    return true;
 }
 
 template <typename T, int K, int NP>
@@ -627,8 +683,8 @@ __global__ __launch_bounds__(kSmallThreads) void k2d_small_tile(
 {
     __shared__ double acc[NP][kSW * kSW];
     // per-wave row / column terms of the current particle: (offset^2, overlap)
-    __shared__ double2 s_rt[kSmallThreads / kWave][kSmallSide2D];
-    __shared__ double2 s_ct[kSmallThreads / kWave][kSmallSide2D];
+    __shared__ float2 s_rt[kSmallThreads / kWave][kSmallSide2D];
+    __shared__ float2 s_ct[kSmallThreads / kWave][kSmallSide2D];
     // records and plane multipliers of a batch of particles, loaded together
     __shared__ Rec2 s_rec[kSmallBatch];
     __shared__ double s_mult[NP][kSmallBatch];
@@ -662,13 +718,19 @@ __global__ __launch_bounds__(kSmallThreads) void k2d_small_tile(
         for (int e = wv; e < cnt; e += kSmallThreads / kWave) {
             const Rec2 r = s_rec[e];
             const int fp = (r.i1 - r.i0 + 1) * (r.j1 - r.j0 + 1);
-            // most footprints need at most 4 pixels per lane: a short
-            // unrolled variant for them, the full one for the rest
-            if (__builtin_amdgcn_readfirstlane(fp) <= 4 * kWave)
-                small_particle<K, NP, 4>(r, e, ln, oi, oj, s_rt[wv], s_ct[wv], s_mult, acc);
-            else
-                small_particle<K, NP, kPerLane>(r, e, ln, oi, oj, s_rt[wv], s_ct[wv], s_mult,
-                                                acc);
+            bool done = false;
+            if (!r.fb && r.h >= 2.0) {
+                // most footprints need at most 4 pixels per lane: a short
+                // unrolled variant for them, the full one for the rest
+                if (__builtin_amdgcn_readfirstlane(fp) <= 4 * kWave)
+                    done = small_particle<K, NP, 4>(r, e, ln, oi, oj, s_rt[wv], s_ct[wv],
+                                                    s_mult, acc);
+                else
+                    done = small_particle<K, NP, kPerLane>(r, e, ln, oi, oj, s_rt[wv],
+                                                           s_ct[wv], s_mult, acc);
+            }
+            if (!done)
+                small_particle64<K, NP>(r, e, ln, oi, oj, s_mult, acc);
         }
     }
     __syncthreads();
