@@ -141,62 +141,10 @@ __global__ __launch_bounds__(256) void k2d_setup(const Map2D *__restrict__ maps,
 // ---------------------------------------------------------------------------
 // Pixel weights shared by the normalisation sum and the tile gather: both
 // form s = (x offset^2 + y offset^2) / h^2 with the same FP32 operations and
-// evaluate ax * shape(sqrt(s)) with shape_row, so a particle's deposited
-// weights and its normalisation sum come from the same per-pixel values.
-// ---------------------------------------------------------------------------
-// Power n of the (1 - u) factor of each kernel shape.  The gather folds a
-// row's x overlap ax into that factor as a = ax^(1/n), so the pixel weight
-// ax * shape(u) costs no multiply of its own; a^n equals ax to a few FP32
-// ulps.  ax < 1 only in the first and last rows of a footprint
-// (cic_shared.jl:60-76), ax = 0 outside it.
-template <int K>
-constexpr int kTPow = K == S2G_KERNEL_CUBIC          ? 3
-                      : K == S2G_KERNEL_QUINTIC      ? 5
-                      : K == S2G_KERNEL_WENDLAND_C2  ? 4
-                      : K == S2G_KERNEL_WENDLAND_C4  ? 6
-                      : K == S2G_KERNEL_WENDLAND_C6  ? 8
-                                                     : 10;
-
-// ax * shape(u) with a = ax^(1/n): the SPHKernels closed forms with every
-// (c - u)_+ factor written as a (c - u) clamped to [0, 1] (the clamp is the
-// VALU output modifier)
-template <int K>
-__device__ __forceinline__ float shape_row(float u, float a)
-{
-    const float t = __builtin_amdgcn_fmed3f(fmaf(-a, u, a), 0.0f, 1.0f);
-    if constexpr (K == S2G_KERNEL_CUBIC) {
-        // 2 (1-u)^3 - 8 (1/2-u)^3_+ (= 1 + 6 (u-1) u^2 below u = 1/2)
-        const float v = a * __builtin_amdgcn_fmed3f(0.5f - u, 0.0f, 1.0f);
-        return fmaf(-8.0f, v * v * v, 2.0f * (t * t * t));
-    } else if constexpr (K == S2G_KERNEL_QUINTIC) {
-        const float p = a * __builtin_amdgcn_fmed3f(2.0f / 3.0f - u, 0.0f, 1.0f);
-        const float q = a * __builtin_amdgcn_fmed3f(1.0f / 3.0f - u, 0.0f, 1.0f);
-        const float t2 = t * t, p2 = p * p, q2 = q * q;
-        return fmaf(15.0f, q2 * q2 * q, fmaf(-6.0f, p2 * p2 * p, t2 * t2 * t));
-    } else if constexpr (K == S2G_KERNEL_WENDLAND_C2) {
-        const float t2 = t * t;
-        return t2 * t2 * fmaf(4.0f, u, 1.0f);
-    } else if constexpr (K == S2G_KERNEL_WENDLAND_C4) {
-        const float t2 = t * t;
-        return t2 * t2 * t2 * fmaf(u, fmaf(u, 35.0f / 3.0f, 6.0f), 1.0f);
-    } else if constexpr (K == S2G_KERNEL_WENDLAND_C6) {
-        const float t2 = t * t, t4 = t2 * t2;
-        return t4 * t4 * fmaf(u, fmaf(u, fmaf(u, 32.0f, 25.0f), 8.0f), 1.0f);
-    } else {  // WENDLAND_C8
-        const float t2 = t * t, t4 = t2 * t2;
-        return t4 * t4 * t2 *
-               fmaf(u, fmaf(u, fmaf(u, fmaf(u, 429.0f / 5.0f, 90.0f), 42.0f), 10.0f), 1.0f);
-    }
-}
-
-// a = ax^(1/n) of a row (or column) overlap ax, as shape_row takes it
-template <int K>
-__device__ __forceinline__ float overlap_root(float ax)
-{
-    return ax >= 1.0f ? 1.0f : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>)) : 0.0f;
-}
-
-// ---------------------------------------------------------------------------
+// evaluate ax * shape(sqrt(s)) with shape_row (s2g_internal.h), so a
+// particle's deposited weights and its normalisation sum come from the same
+// per-pixel values.
+//
 // Explicit normalisation for large footprints that are clipped by the image
 // border, below the closed-form threshold or with S2G_FLAG_EXACT_NORM: one
 // wave per particle, deterministic FP64 sum (fixed per-lane order + fixed
@@ -213,7 +161,7 @@ constexpr int kNormThreads = 256;
 // (|x - i - 1/2| < sqrt(h^2 - dy^2) + 1, dy the chunk's nearest column-centre
 // distance): the rows left out have u >= 1, so the sum is unchanged.
 template <int K>
-__device__ __forceinline__ double norm_rect(const Rec2 &r, float h2, bool disc, int64_t ia,
+__device__ __forceinline__ double norm_rect(const Rec2 &r, double hinv, bool disc, int64_t ia,
                                             int64_t ib, int64_t ja, int64_t jb, int ln)
 {
     double acc = 0.0;
@@ -232,14 +180,14 @@ __device__ __forceinline__ double norm_rect(const Rec2 &r, float h2, bool disc, 
         }
         const int64_t j = jc + ln;
         const double fj = (double)j;
-        const float yd = (float)coff1(r.y, fj);
-        const float yd2 = yd * yd * h2;
+        const float ys = (float)(coff1(r.y, fj) * hinv);
+        const float yd2 = ys * ys;
         const float ay = j <= jb ? fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) : 0.0f;
         double col = 0.0;
         for (int64_t ic = ra; ic <= rb; ic += kWave) {
             const double fi = (double)(ic + ln);
-            const float xd = (float)coff1(r.x, fi);
-            const float rx = xd * xd * h2;
+            const float xs = (float)(coff1(r.x, fi) * hinv);
+            const float rx = xs * xs;
             const float a = overlap_root<K>((float)overlap1(r.x, r.h, fi));
             const int nrow = rb - ic + 1 < kWave ? (int)(rb - ic + 1) : kWave;
             for (int rr = 0; rr < nrow; ++rr) {
@@ -267,8 +215,7 @@ __device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restri
     const int ln = threadIdx.x % kWave;
     const int p = list[gw];
     const Rec2 r = rec[p];
-    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
-    const float h2 = hinv * hinv;
+    const double hinv = r.fb ? 0.0 : 1.0 / r.h;
     const int64_t i0 = r.i0, i1 = r.i1, j0 = r.j0, j1 = r.j1;
     bool cut = false;
     double A0 = 0.0, A1 = 0.0, B0 = 0.0, B1 = 0.0;
@@ -287,17 +234,17 @@ __device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restri
         const int64_t a0 = (int64_t)A0, a1 = (int64_t)A1, b0 = (int64_t)B0, b1 = (int64_t)B1;
         double out = 0.0;
         if (a0 < i0)
-            out += norm_rect<K>(r, h2, true, a0, i0 - 1, b0, b1, ln);
+            out += norm_rect<K>(r, hinv, true, a0, i0 - 1, b0, b1, ln);
         if (a1 > i1)
-            out += norm_rect<K>(r, h2, true, i1 + 1, a1, b0, b1, ln);
+            out += norm_rect<K>(r, hinv, true, i1 + 1, a1, b0, b1, ln);
         if (b0 < j0)
-            out += norm_rect<K>(r, h2, true, i0, i1, b0, j0 - 1, ln);
+            out += norm_rect<K>(r, hinv, true, i0, i1, b0, j0 - 1, ln);
         if (b1 > j1)
-            out += norm_rect<K>(r, h2, true, i0, i1, j1 + 1, b1, ln);
+            out += norm_rect<K>(r, hinv, true, i0, i1, j1 + 1, b1, ln);
         out = wave_sum(out);
         acc = mul_rn(r.h, r.h) * shape_disc_integral(K) - out;
     } else {
-        acc = wave_sum(norm_rect<K>(r, h2, !r.fb, i0, i1, j0, j1, ln));
+        acc = wave_sum(norm_rect<K>(r, hinv, !r.fb, i0, i1, j0, j1, ln));
     }
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
@@ -332,17 +279,18 @@ constexpr int kGBatch = 32;
 constexpr int kGStep = 4;  // row groups per skip test (measured best of 1, 2, 4, 8, 16)
 
 template <typename T, int K, int NP>
-__global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items,
-                                                  const int32_t *__restrict__ pairs,
-                                                  const int32_t *__restrict__ large,
-                                                  const Rec2 *__restrict__ rec,
-                                                  const Map2D *__restrict__ maps, int nq,
-                                                  int plane0, double *__restrict__ partial)
+__device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
+                                            const int32_t *__restrict__ pairs,
+                                            const int32_t *__restrict__ large,
+                                            const Rec2 *__restrict__ rec,
+                                            const Map2D *__restrict__ maps, int nq, int plane0,
+                                            double *__restrict__ partial)
 {
     __shared__ float2 s_row[kGBatch][kGT];
     __shared__ float2 s_col[kGBatch][kGT];
     __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch];
     __shared__ float s_hinv[kGBatch];
+    __shared__ double s_hinvd[kGBatch];
     __shared__ int s_i0[kGBatch], s_i1[kGBatch], s_j0[kGBatch], s_j1[kGBatch];
     __shared__ double s_mult[NP][kGBatch];
     __shared__ int s_k[kGBatch][4];  // row-group range per (particle, strip), -1: none
@@ -372,7 +320,9 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
             s_x[threadIdx.x] = r.x;
             s_y[threadIdx.x] = r.y;
             s_h[threadIdx.x] = r.h;
-            s_hinv[threadIdx.x] = r.fb ? 0.0f : (float)(1.0 / r.h);
+            const double hi = r.fb ? 0.0 : 1.0 / r.h;
+            s_hinv[threadIdx.x] = (float)hi;
+            s_hinvd[threadIdx.x] = hi;
             s_i0[threadIdx.x] = r.i0;
             s_i1[threadIdx.x] = r.i1;
             s_j0[threadIdx.x] = r.j0;
@@ -388,17 +338,17 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
         __syncthreads();
         // row and column tables of the batch for the 64 tile rows and
         // columns: ((x offset / h)^2, x overlap^(1/n)) and
-        // ((y offset / h)^2, y overlap)
+        // ((y offset / h)^2, y overlap); offset / h is formed in FP64 and
+        // rounded once, so no rounding of 1/h scales a whole footprint
         for (int e = threadIdx.x; e < cnt * kGT; e += 256) {
             const int b = e / kGT, rr = e % kGT;
-            const float h2 = s_hinv[b] * s_hinv[b];
             const double fi = (double)(ti0 + rr), fj = (double)(tj0 + rr);
-            const float xd = (float)coff1(s_x[b], fi);
+            const float xs = (float)(coff1(s_x[b], fi) * s_hinvd[b]);
             const float a = overlap_root<K>((float)overlap1(s_x[b], s_h[b], fi));
-            s_row[b][rr] = make_float2(xd * xd * h2, a);
-            const float yd = (float)coff1(s_y[b], fj);
+            s_row[b][rr] = make_float2(xs * xs, a);
+            const float ys = (float)(coff1(s_y[b], fj) * s_hinvd[b]);
             const float ay = fmaxf((float)overlap1(s_y[b], s_h[b], fj), 0.0f);
-            s_col[b][rr] = make_float2(yd * yd * h2, ay);
+            s_col[b][rr] = make_float2(ys * ys, ay);
         }
         // row groups each wave's strip needs for each particle: the footprint
         // box, narrowed to the rows that can reach the kernel disc from the
@@ -479,6 +429,29 @@ __global__ __launch_bounds__(256) void k2d_gather(const int4 *__restrict__ items
 #pragma unroll
         for (int k = 0; k < kGRows; ++k)
             dst[(q * kGT + 4 * k + rg) * kGT + 16 * wv + (ln & 15)] = acc[k][q];
+}
+
+// up to two planes: 128 VGPRs, four waves per SIMD (the accumulators need
+// 64); three or four planes need more registers and run with fewer waves
+template <typename T, int K, int NP>
+__global__ __launch_bounds__(256, 4) void k2d_gather(const int4 *__restrict__ items,
+                                                     const int32_t *__restrict__ pairs,
+                                                     const int32_t *__restrict__ large,
+                                                     const Rec2 *__restrict__ rec,
+                                                     const Map2D *__restrict__ maps, int nq,
+                                                     int plane0, double *__restrict__ partial)
+{
+    gather_tile<T, K, NP>(items, pairs, large, rec, maps, nq, plane0, partial);
+}
+template <typename T, int K, int NP>
+__global__ __launch_bounds__(256) void k2d_gather_wide(const int4 *__restrict__ items,
+                                                       const int32_t *__restrict__ pairs,
+                                                       const int32_t *__restrict__ large,
+                                                       const Rec2 *__restrict__ rec,
+                                                       const Map2D *__restrict__ maps, int nq,
+                                                       int plane0, double *__restrict__ partial)
+{
+    gather_tile<T, K, NP>(items, pairs, large, rec, maps, nq, plane0, partial);
 }
 
 // fixed-order sum of a tile's item partials -> image planes (plain stores).
@@ -624,17 +597,16 @@ __device__ __forceinline__ bool small_particle(const Rec2 &r, int e, int ln, int
 {
     const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1;
     const int fp = iw * jw;
-    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
-    const float h2 = hinv * hinv;
+    const double hinv = r.fb ? 0.0 : 1.0 / r.h;
     if (ln < iw) {
         const double fi = (double)(r.i0 + ln);
-        const float xd = (float)coff1(r.x, fi);
-        rt_tab[ln] = make_float2(xd * xd * h2, overlap_root<K>((float)overlap1(r.x, r.h, fi)));
+        const float xs = (float)(coff1(r.x, fi) * hinv);
+        rt_tab[ln] = make_float2(xs * xs, overlap_root<K>((float)overlap1(r.x, r.h, fi)));
     }
     if (ln < jw) {
         const double fj = (double)(r.j0 + ln);
-        const float yd = (float)coff1(r.y, fj);
-        ct_tab[ln] = make_float2(yd * yd * h2, fmaxf((float)overlap1(r.y, r.h, fj), 0.0f));
+        const float ys = (float)(coff1(r.y, fj) * hinv);
+        ct_tab[ln] = make_float2(ys * ys, fmaxf((float)overlap1(r.y, r.h, fj), 0.0f));
     }
     const float invjw = 1.0f / (float)jw;
     const int cbase = (r.i0 - oi) * kSW + (r.j0 - oj);
@@ -853,8 +825,13 @@ void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *ma
 template <typename T, int K, int NP>
 static void gather_knp(const GatherArgs &g, hipStream_t s)
 {
-    hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
-                       g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0, g.partial);
+    if constexpr (NP <= 2)
+        hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
+                           g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0, g.partial);
+    else
+        hipLaunchKernelGGL((k2d_gather_wide<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0,
+                           s, g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0,
+                           g.partial);
 }
 
 template <typename T, int K>

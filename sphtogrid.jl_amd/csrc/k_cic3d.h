@@ -29,7 +29,10 @@ struct Small3Args {
 };
 
 void launch_3d_setup(const Map3DArgs &a, Rec3 *rec, int32_t *cls, hipStream_t s);
-void launch_3d_norm(int kernel, const int32_t *list, int64_t ne, Rec3 *rec, hipStream_t s);
+// explicit normalisation sums (flags: S2G_FLAG_EXACT_NORM turns off the
+// border-cut closed form)
+void launch_3d_norm(int kernel, const int32_t *list, int64_t ne, int64_t npix, uint32_t flags,
+                    Rec3 *rec, hipStream_t s);
 void launch_3d_gather(const Gather3Args &g, hipStream_t s);
 void launch_3d_finalize(const Gather3Args &g, hipStream_t s);
 void launch_3d_small(const Small3Args &a, hipStream_t s);

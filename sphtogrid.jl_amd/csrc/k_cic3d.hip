@@ -119,69 +119,120 @@ __global__ __launch_bounds__(256) void k3d_setup(const T *__restrict__ pos,
     cls[p] = r.cls;
 }
 
-// voxel weight factor shared by the normalisation and the gather:
-//   f = shape(sqrt(r2xy + zd2) * hinv) * (ax*ay) * az
-template <int K>
-__device__ __forceinline__ float vox_factor(float r2xy, float zd2, float hinv, float axy,
-                                            float az)
-{
-    const float u = __builtin_amdgcn_sqrtf(r2xy + zd2) * hinv;
-    return (shape_of<K, float>(u) * axy) * az;
-}
+// ---------------------------------------------------------------------------
+// Explicit normalisation: one wave per particle.  The voxel weights are
+// formed as the brick gather forms them (s = (x offset^2 + y offset^2) / h^2
+// + (z offset / h)^2 in the same FP32 operations, shape_row with the overlap
+// product), so the sum and the deposited weights agree.
+// ---------------------------------------------------------------------------
 
-// ---------------------------------------------------------------------------
-// Explicit normalisation: one wave per particle.  The (j, k) plane of the
-// footprint is flattened over the lanes (64 at a time, per-lane y/z terms
-// computed once per chunk); x rows are taken 64 at a time with the row terms
-// broadcast by v_readlane.  Fixed per-lane order + butterfly: reproducible.
-// ---------------------------------------------------------------------------
+// This lane's share of the sum over the box [ia, ib] x [ja, jb] x [ka, kb]
+// (any integers; voxels outside the grid follow the same formulas).  The
+// (j, k) plane is flattened over the lanes, 64 at a time; x rows are taken 64
+// at a time with the row terms broadcast by v_readlane.  Fixed per-lane
+// order: reproducible.
 template <int K>
-__device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restrict__ list, int64_t ne, Rec3 *__restrict__ rec)
+__device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t ia, int64_t ib,
+                                            int64_t ja, int64_t jb, int64_t ka, int64_t kb,
+                                            int ln)
 {
-    const int ln = threadIdx.x % kWave;
-    const int p = list[gw];
-    const Rec3 r = rec[p];
-    const float hinv = r.fb ? 0.0f : (float)(1.0 / r.h);
-    const int jw = r.j1 - r.j0 + 1, kw = r.k1 - r.k0 + 1;
-    const int njk = jw * kw;
+    const int64_t kw = kb - ka + 1;
+    const int64_t njk = (jb - ja + 1) * kw;
     double acc = 0.0;
-    for (int fc = 0; fc < njk; fc += kWave) {
-        const int f = fc + ln;
-        const int dj = f / kw, dk = f - dj * kw;
-        const double fj = (double)(r.j0 + dj), fk = (double)(r.k0 + dk);
-        const float yd = (float)coff1(r.y, fj), zd = (float)coff1(r.z, fk);
+    for (int64_t fc = 0; fc < njk; fc += kWave) {
+        const int64_t f = fc + ln;
+        const int64_t dj = f / kw, dk = f - dj * kw;
+        const double fj = (double)(ja + dj), fk = (double)(ka + dk);
+        const float ys = (float)(coff1(r.y, fj) * hinv), zs = (float)(coff1(r.z, fk) * hinv);
         const float ay = fmaxf((float)overlap1(r.y, r.h, fj), 0.0f);
         const float az = fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
-        const bool on = f < njk;
-        const float yzd2 = yd * yd + zd * zd;
-        const float ayz = on ? ay * az : 0.0f;
-        for (int ic = r.i0; ic <= r.i1; ic += kWave) {
+        const float yd2 = ys * ys, zd2 = zs * zs;
+        const float ayz = f < njk ? ay * az : 0.0f;
+        for (int64_t ic = ia; ic <= ib; ic += kWave) {
             const double fi = (double)(ic + ln);
-            const float xd = (float)coff1(r.x, fi);
-            const float xd2 = xd * xd;
+            const float xs = (float)(coff1(r.x, fi) * hinv);
+            const float xd2 = xs * xs;
             const float ax = fmaxf((float)overlap1(r.x, r.h, fi), 0.0f);
-            const int nrow = r.i1 - ic + 1 < kWave ? r.i1 - ic + 1 : kWave;
+            const int nrow = ib - ic + 1 < kWave ? (int)(ib - ic + 1) : kWave;
             for (int rr = 0; rr < nrow; ++rr) {
                 const float xd2r =
                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), rr));
                 const float axr =
                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), rr));
-                const float u = __builtin_amdgcn_sqrtf(xd2r + yzd2) * hinv;
-                acc += (double)((shape_of<K, float>(u) * axr) * ayz);
+                const float u = __builtin_amdgcn_sqrtf((xd2r + yd2) + zd2);
+                acc += (double)(shape_row<K>(u, 1.0f) * (axr * ayz));
             }
         }
     }
-    acc = wave_sum(acc);
+    return acc;
+}
+
+// Sum over the footprint in the grid.  When the closed form h^3 I_k holds for
+// the unclipped footprint (h >= the kernel's threshold, no fallback, no
+// S2G_FLAG_EXACT_NORM), the particle's centre is in the grid and the part cut
+// off by the border is the smaller one, the sum is h^3 I_k minus the sum over
+// that part (x slabs, then y slabs of the kept x range, then z slabs of the
+// kept x and y ranges); otherwise the footprint in the grid is summed.
+template <int K>
+__device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restrict__ list,
+                                             int64_t npix, uint32_t flags,
+                                             Rec3 *__restrict__ rec)
+{
+    const int ln = threadIdx.x % kWave;
+    const int p = list[gw];
+    const Rec3 r = rec[p];
+    const double hinv = r.fb ? 0.0 : 1.0 / r.h;
+    const int64_t i0 = r.i0, i1 = r.i1, j0 = r.j0, j1 = r.j1, k0 = r.k0, k1 = r.k1;
+    bool cut = false;
+    double A0 = 0.0, A1 = 0.0, B0 = 0.0, B1 = 0.0, C0 = 0.0, C1 = 0.0;
+    if (!r.fb && r.h >= closed_form_min_h3(K) && !(flags & S2G_FLAG_EXACT_NORM)) {
+        const double np = (double)npix;
+        A0 = floor(sub_rn(r.x, r.h));
+        A1 = floor(add_rn(r.x, r.h));
+        B0 = floor(sub_rn(r.y, r.h));
+        B1 = floor(add_rn(r.y, r.h));
+        C0 = floor(sub_rn(r.z, r.h));
+        C1 = floor(add_rn(r.z, r.h));
+        const double kept =
+            (double)(i1 - i0 + 1) * (double)(j1 - j0 + 1) * (double)(k1 - k0 + 1);
+        const double full = (A1 - A0 + 1.0) * (B1 - B0 + 1.0) * (C1 - C0 + 1.0);
+        cut = r.x >= 0.0 && r.x < np && r.y >= 0.0 && r.y < np && r.z >= 0.0 && r.z < np &&
+              full - kept < kept;
+    }
+    double acc;
+    if (cut) {
+        const int64_t a0 = (int64_t)A0, a1 = (int64_t)A1, b0 = (int64_t)B0, b1 = (int64_t)B1;
+        const int64_t c0 = (int64_t)C0, c1 = (int64_t)C1;
+        double out = 0.0;
+        if (a0 < i0)
+            out += norm_box3<K>(r, hinv, a0, i0 - 1, b0, b1, c0, c1, ln);
+        if (a1 > i1)
+            out += norm_box3<K>(r, hinv, i1 + 1, a1, b0, b1, c0, c1, ln);
+        if (b0 < j0)
+            out += norm_box3<K>(r, hinv, i0, i1, b0, j0 - 1, c0, c1, ln);
+        if (b1 > j1)
+            out += norm_box3<K>(r, hinv, i0, i1, j1 + 1, b1, c0, c1, ln);
+        if (c0 < k0)
+            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, c0, k0 - 1, ln);
+        if (c1 > k1)
+            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, k1 + 1, c1, ln);
+        out = wave_sum(out);
+        acc = mul_rn(mul_rn(r.h, r.h), r.h) * shape_ball_integral(K) - out;
+    } else {
+        acc = wave_sum(norm_box3<K>(r, hinv, i0, i1, j0, j1, k0, k1, ln));
+    }
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
 }
 template <int K>
-__global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list, int64_t ne, Rec3 *__restrict__ rec)
+__global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list, int64_t ne,
+                                                int64_t npix, uint32_t flags,
+                                                Rec3 *__restrict__ rec)
 {
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
          gw += nw)
-        k3d_norm_one<K>(gw, list, ne, rec);
+        k3d_norm_one<K>(gw, list, npix, flags, rec);
 }
 
 
@@ -206,8 +257,9 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
                                                   double *__restrict__ partial)
 {
     __shared__ float2 s_row[kGBatch3][kBT * kBT];
+    __shared__ float2 s_zc[kGBatch3][kBT];
     __shared__ double s_x[kGBatch3], s_y[kGBatch3], s_z[kGBatch3], s_h[kGBatch3];
-    __shared__ float s_hinv[kGBatch3];
+    __shared__ double s_hinvd[kGBatch3];
     __shared__ int s_i0[kGBatch3], s_i1[kGBatch3], s_j0[kGBatch3], s_j1[kGBatch3];
     __shared__ double s_mult[NP][kGBatch3];
 
@@ -217,7 +269,6 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     const int rg = ln >> 4;
     const int kk = ln & 15;
-    const double fk = (double)(bk0 + kk);
 
     double acc[16][NP];
 #pragma unroll
@@ -236,7 +287,7 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
             s_y[threadIdx.x] = r.y;
             s_z[threadIdx.x] = r.z;
             s_h[threadIdx.x] = r.h;
-            s_hinv[threadIdx.x] = r.fb ? 0.0f : (float)(1.0 / r.h);
+            s_hinvd[threadIdx.x] = r.fb ? 0.0 : 1.0 / r.h;
             s_i0[threadIdx.x] = r.i0;
             s_i1[threadIdx.x] = r.i1;
             s_j0[threadIdx.x] = r.j0;
@@ -250,13 +301,23 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
             }
         }
         __syncthreads();
+        // row table ((x offset / h)^2 + (y offset / h)^2, (ax ay)^(1/n)) of
+        // the 256 (i, j) rows and z table ((z offset / h)^2, az) of the 16 z;
+        // offset / h in FP64, rounded once
         for (int e = threadIdx.x; e < cnt * kBT * kBT; e += 256) {
             const int b = e / (kBT * kBT), rr = e % (kBT * kBT);
             const double fi = (double)(bi0 + (rr >> 4)), fj = (double)(bj0 + (rr & 15));
-            const float xd = (float)coff1(s_x[b], fi), yd = (float)coff1(s_y[b], fj);
+            const float xs = (float)(coff1(s_x[b], fi) * s_hinvd[b]);
+            const float ys = (float)(coff1(s_y[b], fj) * s_hinvd[b]);
             const float ax = fmaxf((float)overlap1(s_x[b], s_h[b], fi), 0.0f);
             const float ay = fmaxf((float)overlap1(s_y[b], s_h[b], fj), 0.0f);
-            s_row[b][rr] = make_float2(xd * xd + yd * yd, ax * ay);
+            s_row[b][rr] = make_float2(xs * xs + ys * ys, overlap_root<K>(ax * ay));
+        }
+        if (threadIdx.x < cnt * kBT) {
+            const int b = threadIdx.x / kBT, zz = threadIdx.x % kBT;
+            const double fk = (double)(bk0 + zz);
+            const float zs = (float)(coff1(s_z[b], fk) * s_hinvd[b]);
+            s_zc[b][zz] = make_float2(zs * zs, fmaxf((float)overlap1(s_z[b], s_h[b], fk), 0.0f));
         }
         __syncthreads();
         for (int b = 0; b < cnt; ++b) {
@@ -264,30 +325,37 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
             const int i1 = __builtin_amdgcn_readfirstlane(s_i1[b]) - bi0;
             if (4 * wv + 3 < i0 || 4 * wv > i1)
                 continue;
-            const int j0 = __builtin_amdgcn_readfirstlane(s_j0[b]) - bj0;
-            const int j1 = __builtin_amdgcn_readfirstlane(s_j1[b]) - bj0;
-            const double z = s_z[b], h = s_h[b];
-            const float hinv = s_hinv[b];
-            const float zd = (float)coff1(z, fk);
-            const float zd2 = zd * zd;
-            const double azd = (double)fmaxf((float)overlap1(z, h, fk), 0.0f);
+            const float2 zc = s_zc[b][kk];
+            const float zd2 = zc.x;
+            // the lane's z overlap is common to its 16 voxels: fold it into
+            // the FP64 plane multipliers
+            const double azd = (double)zc.y;
             double mult[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q)
                 mult[q] = s_mult[q][b] * azd;
+            // the four row groups of one x plane (ii = 4w + g) per test, so
+            // their LDS reads and chains overlap; rows outside the footprint
+            // have a = 0 and add exact zeros
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                // row 64w + 4m + rg: ii = 4w + m/4, jj = 4(m%4) + rg
-                const int ii = 4 * wv + (m >> 2);
-                const int jg = 4 * (m & 3);
-                if (ii < i0 || ii > i1 || jg + 3 < j0 || jg > j1)
+            for (int g = 0; g < 4; ++g) {
+                const int ii = 4 * wv + g;
+                if (ii < i0 || ii > i1)
                     continue;
-                const float2 rv = s_row[b][64 * wv + 4 * m + rg];
-                const float u = __builtin_amdgcn_sqrtf(rv.x + zd2) * hinv;
-                const double f = (double)(shape_of<K, float>(u) * rv.y);
+                float wk[4];
 #pragma unroll
-                for (int q = 0; q < NP; ++q)
-                    acc[m][q] = fma(f, mult[q], acc[m][q]);
+                for (int d = 0; d < 4; ++d) {
+                    // row 64w + 4m + rg with m = 4g + d: jj = 4d + rg
+                    const float2 rv = s_row[b][64 * wv + 16 * g + 4 * d + rg];
+                    wk[d] = shape_row<K>(__builtin_amdgcn_sqrtf(rv.x + zd2), rv.y);
+                }
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const double f = (double)wk[d];
+#pragma unroll
+                    for (int q = 0; q < NP; ++q)
+                        acc[4 * g + d][q] = fma(f, mult[q], acc[4 * g + d][q]);
+                }
             }
         }
         __syncthreads();
@@ -430,12 +498,14 @@ void launch_3d_setup(const Map3DArgs &a, Rec3 *rec, int32_t *cls, hipStream_t s)
                            a.len2pix, a.kernel, a.flags, rec, cls);
 }
 
-void launch_3d_norm(int kernel, const int32_t *list, int64_t ne, Rec3 *rec, hipStream_t s)
+void launch_3d_norm(int kernel, const int32_t *list, int64_t ne, int64_t npix, uint32_t flags,
+                    Rec3 *rec, hipStream_t s)
 {
     if (ne == 0)
         return;
     const dim3 g((unsigned)wave_blocks(ne, 256)), blk(256);
-    S2G_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k3d_norm<KK>, g, blk, 0, s, list, ne, rec));
+    S2G_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k3d_norm<KK>, g, blk, 0, s, list, ne, npix,
+                                                 flags, rec));
 }
 
 template <typename T, int K>

@@ -474,7 +474,7 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     fill_class_stats(st, n, cl);
 
     S2G_HIP(hipEventRecord(E[2], s));
-    launch_3d_norm(a.kernel, cl.l2, cl.ne, rec, s);
+    launch_3d_norm(a.kernel, cl.l2, cl.ne, a.npix, a.flags, rec, s);
     S2G_HIP(hipEventRecord(E[3], s));
 
     const int nb = (int)((npix + kBT - 1) / kBT);

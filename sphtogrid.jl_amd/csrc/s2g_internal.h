@@ -100,6 +100,58 @@ __device__ __forceinline__ R shape_of(R u)
     }
 }
 
+// Power n of the (1 - u) factor of each kernel shape.  The gather folds a
+// row's x overlap ax into that factor as a = ax^(1/n), so the pixel weight
+// ax * shape(u) costs no multiply of its own; a^n equals ax to a few FP32
+// ulps.  ax < 1 only in the first and last rows of a footprint
+// (cic_shared.jl:60-76), ax = 0 outside it.
+template <int K>
+constexpr int kTPow = K == S2G_KERNEL_CUBIC          ? 3
+                      : K == S2G_KERNEL_QUINTIC      ? 5
+                      : K == S2G_KERNEL_WENDLAND_C2  ? 4
+                      : K == S2G_KERNEL_WENDLAND_C4  ? 6
+                      : K == S2G_KERNEL_WENDLAND_C6  ? 8
+                                                     : 10;
+
+// ax * shape(u) with a = ax^(1/n): the SPHKernels closed forms with every
+// (c - u)_+ factor written as a (c - u) clamped to [0, 1] (the clamp is the
+// VALU output modifier)
+template <int K>
+__device__ __forceinline__ float shape_row(float u, float a)
+{
+    const float t = __builtin_amdgcn_fmed3f(fmaf(-a, u, a), 0.0f, 1.0f);
+    if constexpr (K == S2G_KERNEL_CUBIC) {
+        // 2 (1-u)^3 - 8 (1/2-u)^3_+ (= 1 + 6 (u-1) u^2 below u = 1/2)
+        const float v = a * __builtin_amdgcn_fmed3f(0.5f - u, 0.0f, 1.0f);
+        return fmaf(-8.0f, v * v * v, 2.0f * (t * t * t));
+    } else if constexpr (K == S2G_KERNEL_QUINTIC) {
+        const float p = a * __builtin_amdgcn_fmed3f(2.0f / 3.0f - u, 0.0f, 1.0f);
+        const float q = a * __builtin_amdgcn_fmed3f(1.0f / 3.0f - u, 0.0f, 1.0f);
+        const float t2 = t * t, p2 = p * p, q2 = q * q;
+        return fmaf(15.0f, q2 * q2 * q, fmaf(-6.0f, p2 * p2 * p, t2 * t2 * t));
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C2) {
+        const float t2 = t * t;
+        return t2 * t2 * fmaf(4.0f, u, 1.0f);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C4) {
+        const float t2 = t * t;
+        return t2 * t2 * t2 * fmaf(u, fmaf(u, 35.0f / 3.0f, 6.0f), 1.0f);
+    } else if constexpr (K == S2G_KERNEL_WENDLAND_C6) {
+        const float t2 = t * t, t4 = t2 * t2;
+        return t4 * t4 * fmaf(u, fmaf(u, fmaf(u, 32.0f, 25.0f), 8.0f), 1.0f);
+    } else {  // WENDLAND_C8
+        const float t2 = t * t, t4 = t2 * t2;
+        return t4 * t4 * t2 *
+               fmaf(u, fmaf(u, fmaf(u, fmaf(u, 429.0f / 5.0f, 90.0f), 42.0f), 10.0f), 1.0f);
+    }
+}
+
+// a = ax^(1/n) of a row (or column) overlap ax, as shape_row takes it
+template <int K>
+__device__ __forceinline__ float overlap_root(float ax)
+{
+    return ax >= 1.0f ? 1.0f : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>)) : 0.0f;
+}
+
 // integral of shape(|v|) over the unit disc (= 1 / norm_2D), used for the
 // closed-form normalisation of large unclipped footprints
 __host__ __device__ inline double shape_disc_integral(int kernel)

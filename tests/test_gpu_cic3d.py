@@ -91,6 +91,34 @@ def test_exact_norm_matches_closed_form(s2g):
     assert rel_l1(a[:, 1], b[:, 1]) < 1e-7
 
 
+@pytest.mark.parametrize("kernel", KNAMES)
+def test_clipped_footprints_border_sum(s2g, orc, kernel):
+    """Large footprints cut by the grid border: closed form minus the cut-off
+    part where that part is the smaller one, else the direct sum; both equal
+    the exact sum (S2G_FLAG_EXACT_NORM) and the oracle."""
+    npix, n = 128, 60
+    r = np.random.default_rng(500 + KNAMES.index(kernel))
+    x = r.uniform(-0.25, 0.25, (n, 3))
+    side = r.integers(0, 3, n)
+    x[np.arange(n), side] = np.where(r.random(n) < 0.5, 1, -1) * r.uniform(0.36, 0.49, n)
+    h = r.uniform(41.0, 50.0, n) / npix  # >= every kernel's 3D closed-form threshold
+    m, rho, w, q = (r.uniform(0.5, 2.0, n) for _ in range(4))
+    sa, sb = {}, {}
+    a = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix), dimensions=3,
+                    kernel=_kernel(s2g, kernel), calc_mean=False, stats=sa)
+    b = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix), dimensions=3,
+                    kernel=_kernel(s2g, kernel), calc_mean=False, exact_norm=True, stats=sb)
+    assert sa["n_large_exact"] == n, sa
+    for k in range(2):
+        e = rel_l1(a[:, k], b[:, k])
+        print(kernel, "plane", k, "border sum vs exact sum rel_l1", e, flush=True)
+        # the closed form's own error (< 2e-9) and the FP32 weights' bias,
+        # relative to the kept part: the bound of test_exact_norm_matches_closed_form
+        assert e <= 1e-7, (kernel, k, e)
+    ref = orc.cic_mapping_3D(x, h, m, rho, q, w, npix, float(npix), kernel, 3, False)
+    _check(a, ref, f"border3d/{kernel}")
+
+
 def test_large_only_is_deterministic(s2g):
     npix, n = 48, 150
     x, h, m, rho, q, w = _particles(n, npix, 8, hmin=4.0, hmax=12.0)
