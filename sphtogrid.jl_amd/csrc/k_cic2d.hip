@@ -8,11 +8,13 @@
 //                    zero-quantity skip, fallback decision, class
 //   k2d_partition    stable (particle-order) lists of small / large / exact
 //   k2d_norm         explicit FP64 normalisation sum of large clipped footprints
+//                    (closed form minus the border-cut part when smaller)
 //   k_pair_count     counting sort of (tile, particle) pairs: per-chunk counts
 //   (scan)           tile-major exclusive scan -> deterministic list offsets
 //   k_pair_fill      one wave per chunk fills the per-tile lists in particle order
 //   k_item_*         split every tile list into work items of <= G pairs
 //   k2d_gather       per-tile gather, FP64 register accumulation per pixel
+//                    (k2d_gather_wide for three or four planes)
 //   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
 //   k2d_small_*      small footprints: sorted by tile, LDS accumulation per item,
 //                    one atomic add per touched pixel into the image

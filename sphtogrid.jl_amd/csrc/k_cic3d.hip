@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
     __shared__ float2 s_zc[kGBatch3][kBT];
     __shared__ double s_x[kGBatch3], s_y[kGBatch3], s_z[kGBatch3], s_h[kGBatch3];
     __shared__ double s_hinvd[kGBatch3];
-    __shared__ int s_i0[kGBatch3], s_i1[kGBatch3], s_j0[kGBatch3], s_j1[kGBatch3];
+    __shared__ int s_i0[kGBatch3], s_i1[kGBatch3];
     __shared__ double s_mult[NP][kGBatch3];
 
     const int4 it = items[blockIdx.x];
@@ -290,8 +290,6 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
             s_hinvd[threadIdx.x] = r.fb ? 0.0 : 1.0 / r.h;
             s_i0[threadIdx.x] = r.i0;
             s_i1[threadIdx.x] = r.i1;
-            s_j0[threadIdx.x] = r.j0;
-            s_j1[threadIdx.x] = r.j1;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 const int P = plane0 + q;
