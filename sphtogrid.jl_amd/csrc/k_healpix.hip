@@ -529,6 +529,211 @@ __global__ __launch_bounds__(256) void k_hp_deposit(const int2 *__restrict__ uni
 
 
 // ---------------------------------------------------------------------------
+// Ring-major deposit (Nside <= kHpRingLds / 4): one block per ring holds the
+// ring's two map rows in LDS, takes the particles whose disc (or centre
+// pixel) touches the ring from a per-ring list, adds W*A*scale with LDS
+// atomics and writes the ring with plain stores.  No global atomics; every
+// pixel of the map is written, so no memset either.  The pixel weights are
+// formed exactly as in the norm pass (same chord, ring terms and FP32
+// arithmetic), so the deposited weights sum to the normalisation.
+// ---------------------------------------------------------------------------
+constexpr int kHpRingLds = 4096;  // pixels of the longest ring held in LDS
+
+// the rings a particle deposits to: [ra, rb] and the ring of its centre pixel
+__device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int64_t &r0,
+                                             int64_t &r1, int64_t &cring, int64_t &ck)
+{
+    if (!r.ok || r.s == 0.0)
+        return false;
+    hp_pix2ring(nside, r.cpix, cring, ck);
+    if (r.rb >= r.ra) {
+        r0 = r.ra < cring ? r.ra : cring;
+        r1 = r.rb > cring ? r.rb : cring;
+    } else {
+        r0 = r1 = cring;
+    }
+    return true;
+}
+
+// ring counts as a difference array: diff[r0] += 1, diff[r1 + 1] -= 1
+__global__ void k_hp_ring_count(const HpRec *__restrict__ rec, int64_t n, int64_t nside,
+                                unsigned long long *__restrict__ diff)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    int64_t r0, r1, cring, ck;
+    if (!hp_dep_rings(nside, rec[p], r0, r1, cring, ck))
+        return;
+    atomicAdd(&diff[r0], 1ull);
+    atomicAdd(&diff[r1 + 1], ~0ull);  // -1 modulo 2^64
+}
+
+// count[r] = inclusive prefix of diff (= exclusive prefix at r + 1); count[last] = 0
+__global__ void k_hp_ring_shift(const int64_t *__restrict__ ex, int64_t nslots,
+                                int64_t *__restrict__ count)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nslots - 1)
+        count[r] = ex[r + 1];
+    else if (r == nslots - 1)
+        count[r] = 0;
+}
+
+// per-ring particle lists (order within a ring is not fixed: the LDS
+// additions of the deposit are unordered anyway)
+__global__ void k_hp_ring_fill(const HpRec *__restrict__ rec, int64_t n, int64_t nside,
+                               const int64_t *__restrict__ off, int *__restrict__ fill,
+                               int32_t *__restrict__ list)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    int64_t r0, r1, cring, ck;
+    if (!hp_dep_rings(nside, rec[p], r0, r1, cring, ck))
+        return;
+    for (int64_t rr = r0; rr <= r1; ++rr)
+        list[off[rr] + atomicAdd(&fill[rr], 1)] = (int32_t)p;
+}
+
+// one (particle, ring) chord of the ring-major deposit (per-wave LDS table)
+struct HpPair {
+    int off;    // first position of the chord in the wave's pixel list
+    int lo;     // ring-local index of the chord's first pixel (< 0: wraps)
+    int ck;     // ring-local index of the centre pixel, or -1 if not added
+    int fb;     // fallback: area weights only
+    float s1, s2, base, step;
+    float theta, inv_theta;
+    double scale, bq;
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_hp_ring_deposit(
+    const int32_t *__restrict__ list, const int64_t *__restrict__ off,
+    const HpRec *__restrict__ rec, const T *__restrict__ binq, int64_t nside,
+    double *__restrict__ map, double *__restrict__ wmap)
+{
+    __shared__ double s_m[kHpRingLds], s_w[kHpRingLds];
+    __shared__ HpPair s_tab[256 / kWave][kWave];
+    const int64_t iz = (int64_t)blockIdx.x + 1;  // ring, 1-based
+    int64_t start, nr;
+    double shift;
+    hp_ring_info(nside, iz, start, nr, shift);
+    for (int k = threadIdx.x; k < nr; k += 256) {
+        s_m[k] = 0.0;
+        s_w[k] = 0.0;
+    }
+    __syncthreads();
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    HpPair *tab = s_tab[wv];
+    const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
+    const float inv_ang = 1.0f / ang;
+    const int64_t b = off[iz], e = off[iz + 1];
+    for (int64_t c0 = b + (int64_t)wv * kWave; c0 < e; c0 += (256 / kWave) * kWave) {
+        // phase 1: lane l takes pair c0 + l: its chord on this ring
+        HpPair q;
+        q.lo = 0;
+        q.ck = -1;
+        q.fb = 0;
+        q.s1 = q.s2 = q.base = q.step = q.theta = q.inv_theta = 0.0f;
+        q.scale = q.bq = 0.0;
+        int len = 0;
+        if (c0 + ln < e) {
+            const int p = list[c0 + ln];
+            const HpRec r = rec[p];
+            int64_t r0, r1, cring, ck;
+            hp_dep_rings(nside, r, r0, r1, cring, ck);
+            int64_t a = 0, bb = -1;
+            bool any = false;
+            if (iz >= r.ra && iz <= r.rb) {
+                if (iz < r.irmin || iz > r.irmax) {  // a ring fully inside a polar cap
+                    a = 0;
+                    bb = nr - 1;
+                    any = true;
+                } else {
+                    any = hp_ring_interval(nside, iz, r.z0, r.xa, cos(r.theta), r.phi0, a, bb);
+                }
+            }
+            const RingTerms t = ring_terms(nside, iz, r);
+            if (any) {
+                q.lo = (int)a;
+                len = (int)(bb - a + 1);
+                double d0 = (double)a * t.dphi_pix + t.phase;  // phi_a - phi0
+                d0 = d0 >= kPi ? d0 - 2.0 * kPi : d0;
+                d0 = d0 < -kPi ? d0 + 2.0 * kPi : d0;
+                q.base = (float)d0;
+                q.step = (float)t.dphi_pix;
+            }
+            q.s1 = t.s1;
+            q.s2 = t.s2;
+            // constributing_pixels.jl:16-19: the centre pixel, if the disc
+            // query did not return it
+            if (iz == cring) {
+                const bool cov = any && ((a >= 0 && ck >= a && ck <= bb) ||
+                                         (a < 0 && (ck <= bb || ck >= a + nr)));
+                if (!cov) {
+                    q.ck = (int)ck;
+                    len += 1;
+                }
+            }
+            q.fb = r.fb;
+            q.theta = (float)r.theta;
+            q.inv_theta = (float)(1.0 / r.theta);
+            q.scale = r.s;
+            q.bq = ld(binq, p);
+        }
+        const int incl = wave_incl_scan(len, ln);
+        const int total = __shfl(incl, kWave - 1, kWave);
+        q.off = incl - len;
+        __builtin_amdgcn_wave_barrier();
+        tab[ln] = q;
+        __builtin_amdgcn_wave_barrier();
+        // phase 2: the chords' pixels, 64 list positions at a time
+        int j = -1, next = 0;
+        HpPair cur = q;
+        int clen = 0;
+        for (int pos = ln; pos < total; pos += kWave) {
+            while (pos >= next) {
+                ++j;
+                cur = tab[j];
+                next = j + 1 < kWave ? tab[j + 1].off : total;
+                clen = next - cur.off - (cur.ck >= 0 ? 1 : 0);
+            }
+            const int c = pos - cur.off;
+            int k;
+            float d;
+            if (c < clen) {
+                k = cur.lo + c;
+                k = k < 0 ? k + (int)nr : k;
+                float dphi = fmaf((float)c, cur.step, cur.base);
+                dphi = dphi >= (float)kPi ? dphi - (float)(2.0 * kPi) : dphi;
+                const float sx = sin_half_pi(0.5f * dphi);
+                const float hav = fminf(fmaf(cur.s2, sx * sx, cur.s1), 1.0f);
+                d = 2.0f * asinf(sqrtf(hav));
+            } else {
+                // the centre pixel, as the norm pass computes it
+                k = cur.ck;
+                const HpRec &r = rec[list[c0 + j]];
+                d = hp_dist(ring_terms(nside, iz, r), k);
+            }
+            float W, A;
+            hp_weights<K>(d, cur.theta, cur.inv_theta, ang, inv_ang, W, A);
+            const double pw = (double)((cur.fb ? 1.0f : W) * A) * cur.scale;
+            if (pw != 0.0) {
+                unsafeAtomicAdd(&s_w[k], pw);
+                unsafeAtomicAdd(&s_m[k], cur.bq * pw);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nr; k += 256) {
+        map[start + k] = s_m[k];
+        wmap[start + k] = s_w[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 #define S2G_HP_KERNEL_SWITCH(kernel, CALL)                                      \
@@ -594,6 +799,45 @@ void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpR
                              hipLaunchKernelGGL((k_hp_deposit<double, KK>), g, blk, 0, s, units,
                                                 nu, rec, (const double *)a.binq, a.nside, map,
                                                 wmap));
+    }
+}
+
+bool hp_ring_deposit_fits(int64_t nside) { return 4 * nside <= kHpRingLds; }
+
+void launch_hp_ring_count(const HpRec *rec, int64_t n, int64_t nside, int64_t *diff,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec,
+                       n, nside, (unsigned long long *)diff);
+}
+
+void launch_hp_ring_shift(const int64_t *ex, int64_t nslots, int64_t *count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_shift, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                       ex, nslots, count);
+}
+
+void launch_hp_ring_fill(const HpRec *rec, int64_t n, int64_t nside, const int64_t *off,
+                         int *fill, int32_t *list, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec,
+                       n, nside, off, fill, list);
+}
+
+void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
+                            const HpRec *rec, double *map, double *wmap, hipStream_t s)
+{
+    const dim3 g((unsigned)(4 * a.nside - 1)), blk(256);
+    if (a.dtype == S2G_F32) {
+        S2G_HP_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k_hp_ring_deposit<float, KK>), g, blk,
+                                                          0, s, list, off, rec,
+                                                          (const float *)a.binq, a.nside, map,
+                                                          wmap));
+    } else {
+        S2G_HP_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k_hp_ring_deposit<double, KK>), g,
+                                                          blk, 0, s, list, off, rec,
+                                                          (const double *)a.binq, a.nside, map,
+                                                          wmap));
     }
 }
 

@@ -36,7 +36,7 @@ struct s2g_ctx {
     DevBuf md_P, md_bbox, md_S, md_tab;
     PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
-    DevBuf hp_rec, hp_nunits, hp_units, hp_part;
+    DevBuf hp_rec, hp_nunits, hp_units, hp_part, hp_ring, hp_list;
     // outputs
     DevBuf image, out;
     // pinned host scratch for small device->host reads
@@ -825,6 +825,8 @@ void s2g_destroy(s2g_ctx *c)
     c->hp_nunits.release();
     c->hp_units.release();
     c->hp_part.release();
+    c->hp_ring.release();
+    c->hp_list.release();
     c->image.release();
     c->out.release();
     if (c->h_small)
@@ -980,8 +982,12 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
     const int64_t npx = 12 * nside * nside;
     hipEvent_t *E = c->stage;
     S2G_HIP(hipEventRecord(E[0], s));
-    S2G_HIP(hipMemsetAsync(d_map, 0, sizeof(double) * npx, s));
-    S2G_HIP(hipMemsetAsync(d_wmap, 0, sizeof(double) * npx, s));
+    // the ring-major deposit writes every pixel; the atomic one adds
+    const bool rings = n > 0 && hp_ring_deposit_fits(nside);
+    if (!rings) {
+        S2G_HIP(hipMemsetAsync(d_map, 0, sizeof(double) * npx, s));
+        S2G_HIP(hipMemsetAsync(d_wmap, 0, sizeof(double) * npx, s));
+    }
     if (n > 0) {
         HpRec *rec = (HpRec *)c->hp_rec.get(sizeof(HpRec) * n);
         int64_t *nun = (int64_t *)c->hp_nunits.get(sizeof(int64_t) * (n + 1));
@@ -1000,7 +1006,30 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
         launch_hp_norm(a.kernel, units, nu, rec, nside, part, s);
         launch_hp_finish(nun, n, part, rec, s);
         S2G_HIP(hipEventRecord(E[3], s));
-        launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
+        if (rings) {
+            // per-ring particle lists: counts (difference array, scans), fill
+            const int64_t nslots = 4 * nside + 1;
+            int64_t *rb = (int64_t *)c->hp_ring.get(sizeof(int64_t) * 2 * nslots +
+                                                    sizeof(int) * nslots);
+            int64_t *diff = rb, *off = rb + nslots;
+            int *fill = (int *)(rb + 2 * nslots);
+            int64_t *stmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nslots));
+            S2G_HIP(hipMemsetAsync(diff, 0, sizeof(int64_t) * nslots, s));
+            S2G_HIP(hipMemsetAsync(fill, 0, sizeof(int) * nslots, s));
+            launch_hp_ring_count(rec, n, nside, diff, s);
+            exclusive_scan_i64(diff, nslots, stmp, s);
+            launch_hp_ring_shift(diff, nslots, off, s);
+            exclusive_scan_i64(off, nslots, stmp, s);
+            int64_t npairs;
+            read_i64(c, s, {off + nslots - 1}, &npairs);
+            if (npairs >= (int64_t)INT32_MAX)
+                throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
+            int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
+            launch_hp_ring_fill(rec, n, nside, off, fill, list, s);
+            launch_hp_ring_deposit(a, list, off, rec, d_map, d_wmap, s);
+        } else {
+            launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
+        }
         S2G_HIP(hipGetLastError());
         if (st)
             st->n_pairs = nu;

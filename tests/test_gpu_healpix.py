@@ -52,6 +52,13 @@ def test_random_shell(s2g, orc, kname):
     _compare(s2g, orc, pos, h, m, rho, q, w, 32, kname, tag="shell")
 
 
+def test_nside_above_ring_lds(s2g, orc):
+    """Nside 2048: rings of 8192 pixels exceed the ring-major deposit's LDS
+    row, so the deposit takes the per-unit atomic path."""
+    pos, h, m, rho, q, w = _shell(200, 77, hmin=2e-3, hmax=2e-2, rmin=0.3)
+    _compare(s2g, orc, pos, h, m, rho, q, w, 2048, "wendlandc4", tag="atomic")
+
+
 @pytest.mark.parametrize("nside", [1, 4, 64])
 def test_nsides(s2g, orc, nside):
     pos, h, m, rho, q, w = _shell(300, 20 + nside)
