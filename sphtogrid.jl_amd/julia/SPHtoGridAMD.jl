@@ -11,6 +11,8 @@ Binding points in SPHtoGrid.jl (src/ paths):
   cic_interpolation/cic_interpolation.jl:152-155, 191-196  -> cic_mapping_2D
   cic_interpolation/cic_interpolation.jl:219-221, 248-252  -> cic_mapping_3D
   healpix_interpolation/main.jl:146-213                    -> the particle loop
+  effects/mass_density.jl:30-99                            -> mass_density
+  tsc_interpolation/tsc_interpolation.jl:131-133           -> TSCInterpolation
 
 The shared library path is taken from ENV["SPH2GRID_LIB"] or the default
 build location next to this file.
@@ -169,6 +171,69 @@ function healpix_deposit!(allsky_map::Vector{Float64}, weight_map::Vector{Float6
     end
     check(c, rc, "s2g_healpix_deposit")
     return allsky_map, weight_map
+end
+
+"""
+    mass_density(pos, mass; kernel=Cubic(Float64, 3), Nneighbors=32,
+                 boxsize=zeros(3), verbose=false)
+
+Drop-in for SPHtoGrid.mass_density (effects/mass_density.jl:30-99): SPH
+density and smoothing length (distance to the (Nneighbors+1)-th nearest
+particle, itself included) with the Dehnen & Aly bias correction; a boxsize
+of zeros means non-periodic.  Returns `(rho, hsml)`.
+"""
+function mass_density(pos::Matrix{<:Real}, mass::Vector{<:Real};
+                      kernel=Cubic(Float64, 3), Nneighbors::Integer=32,
+                      boxsize::Vector{<:Real}=zeros(3), verbose::Bool=false)
+    T = _common(pos, mass)
+    x = convert(Matrix{T}, pos)
+    m = convert(Vector{T}, mass)
+    n = length(m)
+    box = convert(Vector{Float64}, boxsize)
+    rho = Vector{Float64}(undef, n)
+    hsml = Vector{Float64}(undef, n)
+    c = context()
+    GC.@preserve x m box rho hsml begin
+        rc = ccall((:s2g_mass_density, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Int64, Cint, Cint, Int64,
+                    Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
+                   c.handle, _dtype(T), x, m, n, kernel_id(kernel),
+                   Cint(kernel.dim in (2, 3) ? kernel.dim : 3), Int64(Nneighbors), box,
+                   rho, hsml, C_NULL)
+    end
+    check(c, rc, "s2g_mass_density")
+    return rho, hsml
+end
+
+"""
+    TSCInterpolation(bin_q, pos_tsc, Npixels; average=true, periodic=false,
+                     len2pix=1.0)
+
+The grid deposit tsc_interpolation.jl:131-133 calls (that file is commented
+out in SPHtoGrid.jl): textbook TSC of `bin_q` at `pos_tsc` (3 x N, centred)
+onto an Npixels^3 grid in grid units x*len2pix + Npixels/2.  Returns the
+grid [z, y, x] as Σ q W / Σ W (0 where no weight) with `average`, else Σ q W.
+"""
+function TSCInterpolation(bin_q, pos_tsc, Npixels::Integer; average::Bool=true,
+                          periodic::Bool=false, len2pix::Real=1.0)
+    T = _common(bin_q, pos_tsc)
+    x = convert(Matrix{T}, pos_tsc)
+    q = convert(Vector{T}, bin_q)
+    nd = Npixels^3
+    grid = Vector{Float64}(undef, 2 * nd)
+    p = Ref(S2GParams(Npixels, Float64(len2pix)))
+    c = context()
+    GC.@preserve x q grid begin
+        rc = ccall((:s2g_tsc3d, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Int64, Ref{S2GParams}, UInt32,
+                    Ptr{Float64}, Ptr{Cvoid}),
+                   c.handle, _dtype(T), x, q, length(q), p,
+                   periodic ? S2G_FLAG_PERIODIC : UInt32(0), grid, C_NULL)
+    end
+    check(c, rc, "s2g_tsc3d")
+    gq, gw = view(grid, 1:nd), view(grid, nd+1:2nd)
+    out = average ? [w > 0 ? a / w : 0.0 for (a, w) in zip(gq, gw)] : collect(gq)
+    return reshape(out, Npixels, Npixels, Npixels)
 end
 
 end # module
