@@ -39,7 +39,7 @@ namespace s2g {
 constexpr int kMdWaves = 4;     // waves per block of the search
 constexpr int kMdCap = 768;     // candidate buffer per wave
 constexpr int kMdBits = 21;     // Morton bits per axis
-constexpr int kMdMaxCells = 128;  // cube cells per particle (5^3 = 125)
+constexpr int kMdMaxCells = 216;  // cube cells per particle (at most 6^3)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_md_prep(const T *__restrict__ pos,
@@ -325,51 +325,71 @@ __global__ __launch_bounds__(kMdWaves * kWave) void k_md_knn(
             T = md_select(buf, cnt, k1, hist, ln);
         }
         const double T0 = T;
-        // 2. level with cells in [T0/2, T0) along the narrowest axis, cube
-        //    of half-width ceil(T0 / cell) per axis
+        // 2. level with cells in [T0/2, T0) along the narrowest axis; per
+        //    axis the cells that meet [x - T0, x + T0], found with the
+        //    particles' own quantisation widened by one quantum: the span is
+        //    2 T0 + 2 quanta < 5 cells (cells are >= T0 / 2 wide and a
+        //    quantum is at most half a cell below level 21, where the cell
+        //    exceeds T0), so at most six cells per axis
         int lev = kMdBits;
         if (T0 > 0.0) {
             const int l = (int)floor(log2(g.min_ext / T0)) + 1;
             lev = l < 0 ? 0 : (l > kMdBits ? kMdBits : l);
         }
         const uint32_t ncl = 1u << lev;
-        int ca[3], cl[3];  // first cell and cell count per axis
+        const int sh = kMdBits - lev;
+        int ca[3], cl[3];  // first cell (unwrapped) and cell count per axis
         const double mv[3] = {me.x, me.y, me.z};
         for (int d = 0; d < 3; ++d) {
-            const uint32_t c = md_quant(mv[d], g, d) >> (kMdBits - lev);
-            const double cw = g.ext[d] / (double)ncl;
-            int w = T0 > 0.0 ? (int)ceil(T0 / cw) : 0;
-            w = w > 2 ? 2 : w;  // the narrowest axis needs 2; wider ones at most 2
+            const double qs = (double)(1u << kMdBits);
+            const int64_t qlo = (int64_t)floor((mv[d] - T0 - g.lo[d]) * g.inv_ext[d] * qs) - 1;
+            const int64_t qhi = (int64_t)floor((mv[d] + T0 - g.lo[d]) * g.inv_ext[d] * qs) + 1;
+            int64_t clo = qlo >> sh, chi = qhi >> sh;  // floor division
             if (g.periodic) {
-                if (2 * w + 1 >= (int)ncl) {
-                    ca[d] = 0;
-                    cl[d] = (int)ncl;
-                } else {
-                    ca[d] = (int)c - w;
-                    cl[d] = 2 * w + 1;
+                if (chi - clo + 1 >= (int64_t)ncl) {
+                    clo = 0;
+                    chi = (int64_t)ncl - 1;
                 }
             } else {
-                const int lo = (int)c - w < 0 ? 0 : (int)c - w;
-                const int hi = (int)c + w > (int)ncl - 1 ? (int)ncl - 1 : (int)c + w;
-                ca[d] = lo;
-                cl[d] = hi - lo + 1;
+                clo = clo < 0 ? 0 : clo;
+                chi = chi > (int64_t)ncl - 1 ? (int64_t)ncl - 1 : chi;
             }
+            ca[d] = (int)clo;
+            cl[d] = (int)(chi - clo + 1);
         }
+        // cells whose nearest point is farther than T0 (with a margin of two
+        // quanta per axis) hold no candidate
+        const double T0m = T0 * (1.0 + 1e-12);
         const int ncube = cl[0] * cl[1] * cl[2];
         // 3. the cube's particles; cells are ranges of the sorted order
         int C = 0;  // candidates so far (uniform)
         for (int t0 = 0; t0 < ncube; t0 += kWave) {
             const int t = t0 + ln;
             int s = 0, e = 0;
+            bool near = false;
+            uint32_t cc[3];
             if (t < ncube) {
-                uint32_t cc[3];
                 int r = t;
+                double dd = 0.0;
                 for (int d = 0; d < 3; ++d) {
                     int o = ca[d] + r % cl[d];
                     r /= cl[d];
+                    if (cl[d] < (int)ncl || !g.periodic) {
+                        // unwrapped cell o spans [o, o + 1) * cw around x
+                        const double cw = g.ext[d] / (double)ncl;
+                        const double eps = 2.0 * g.ext[d] / (double)(1u << kMdBits);
+                        const double xr = mv[d] - g.lo[d];
+                        const double a = (double)o * cw - xr - eps;
+                        const double b = xr - (double)(o + 1) * cw - eps;
+                        const double dx = a > 0.0 ? a : (b > 0.0 ? b : 0.0);
+                        dd += dx * dx;
+                    }
                     o = o < 0 ? o + (int)ncl : (o >= (int)ncl ? o - (int)ncl : o);
                     cc[d] = (uint32_t)o;
                 }
+                near = dd <= T0m * T0m;
+            }
+            if (near) {
                 const uint64_t pre = md_morton(cc[0], cc[1], cc[2]);  // 3*lev bits
                 if (lev <= lt) {
                     s = tab[pre << (3 * (lt - lev))];
@@ -377,9 +397,9 @@ __global__ __launch_bounds__(kMdWaves * kWave) void k_md_knn(
                 } else {
                     const uint64_t par = pre >> (3 * (lev - lt));
                     const int64_t ps = tab[par], pe = tab[par + 1];
-                    const int sh = 3 * (kMdBits - lev);
-                    s = (int)md_lower_bound(SK, ps, pe, pre << sh);
-                    e = (int)md_lower_bound(SK, s, pe, (pre + 1) << sh);
+                    const int ksh = 3 * (kMdBits - lev);
+                    s = (int)md_lower_bound(SK, ps, pe, pre << ksh);
+                    e = (int)md_lower_bound(SK, s, pe, (pre + 1) << ksh);
                 }
             }
             const int len = e - s;
