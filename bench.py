@@ -650,7 +650,12 @@ def main():
                 "scatter/gather); peak = MI355X FP32 vector rate",
                 "alg_flops_per_launch": fp_large * fk,
                 "alg_pixel_updates_per_launch": fp_large,
-                "ms_per_launch": stage["ms_gather"]}
+                "ms_per_launch": stage["ms_gather"],
+                # HBM bytes the gather moves per launch (rocprofv3 FETCH_SIZE +
+                # WRITE_SIZE, calibrated) over its measured duration
+                "hbm_measured_GBps": (traffic / (stage["ms_gather"] / 1e3) / 1e9
+                                      if traffic and stage["ms_gather"] else None),
+                "hbm_peak_GBps": 8000.0}
 
     cpu = None
     if not args.no_cpu:
