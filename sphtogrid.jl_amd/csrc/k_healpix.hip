@@ -541,6 +541,8 @@ __global__ __launch_bounds__(256, 4) void k_hp_deposit(const int2 *__restrict__ 
 // arithmetic), so the deposited weights sum to the normalisation.
 // ---------------------------------------------------------------------------
 constexpr int kHpRingLds = 4096;  // pixels of the longest ring held in LDS
+constexpr int kHpRingThreads = 512;  // 8 waves per ring block
+constexpr int kHpRingPairs = 32;     // (particle, ring) chords per wave and round
 
 // the rings a particle deposits to: [ra, rb] and the ring of its centre pixel
 __device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int64_t &r0,
@@ -610,19 +612,21 @@ struct HpPair {
     double scale, bq;
 };
 
+// Two blocks of 8 waves per CU (78 KB of LDS each), held to 128 VGPRs so
+// that four waves per SIMD run.
 template <typename T, int K>
-__global__ __launch_bounds__(256) void k_hp_ring_deposit(
+__global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
     const int32_t *__restrict__ list, const int64_t *__restrict__ off,
     const HpRec *__restrict__ rec, const T *__restrict__ binq, int64_t nside,
     double *__restrict__ map, double *__restrict__ wmap)
 {
     __shared__ double s_m[kHpRingLds], s_w[kHpRingLds];
-    __shared__ HpPair s_tab[256 / kWave][kWave];
+    __shared__ HpPair s_tab[kHpRingThreads / kWave][kHpRingPairs];
     const int64_t iz = (int64_t)blockIdx.x + 1;  // ring, 1-based
     int64_t start, nr;
     double shift;
     hp_ring_info(nside, iz, start, nr, shift);
-    for (int k = threadIdx.x; k < nr; k += 256) {
+    for (int k = threadIdx.x; k < nr; k += kHpRingThreads) {
         s_m[k] = 0.0;
         s_w[k] = 0.0;
     }
@@ -632,8 +636,9 @@ __global__ __launch_bounds__(256) void k_hp_ring_deposit(
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     const float inv_ang = 1.0f / ang;
     const int64_t b = off[iz], e = off[iz + 1];
-    for (int64_t c0 = b + (int64_t)wv * kWave; c0 < e; c0 += (256 / kWave) * kWave) {
-        // phase 1: lane l takes pair c0 + l: its chord on this ring
+    for (int64_t c0 = b + (int64_t)wv * kHpRingPairs; c0 < e;
+         c0 += (kHpRingThreads / kWave) * kHpRingPairs) {
+        // phase 1: lane l < kHpRingPairs takes pair c0 + l: its chord on this ring
         HpPair q;
         q.lo = 0;
         q.ck = -1;
@@ -641,7 +646,7 @@ __global__ __launch_bounds__(256) void k_hp_ring_deposit(
         q.s1 = q.s2 = q.base = q.step = q.theta = q.inv_theta = 0.0f;
         q.scale = q.bq = 0.0;
         int len = 0;
-        if (c0 + ln < e) {
+        if (ln < kHpRingPairs && c0 + ln < e) {
             const int p = list[c0 + ln];
             const HpRec r = rec[p];
             int64_t r0, r1, cring, ck;
@@ -689,7 +694,8 @@ __global__ __launch_bounds__(256) void k_hp_ring_deposit(
         const int total = __shfl(incl, kWave - 1, kWave);
         q.off = incl - len;
         __builtin_amdgcn_wave_barrier();
-        tab[ln] = q;
+        if (ln < kHpRingPairs)
+            tab[ln] = q;
         __builtin_amdgcn_wave_barrier();
         // phase 2: the chords' pixels, 64 list positions at a time
         int j = -1, next = 0;
@@ -699,7 +705,7 @@ __global__ __launch_bounds__(256) void k_hp_ring_deposit(
             while (pos >= next) {
                 ++j;
                 cur = tab[j];
-                next = j + 1 < kWave ? tab[j + 1].off : total;
+                next = j + 1 < kHpRingPairs ? tab[j + 1].off : total;
                 clen = next - cur.off - (cur.ck >= 0 ? 1 : 0);
             }
             const int c = pos - cur.off;
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(256) void k_hp_ring_deposit(
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < nr; k += 256) {
+    for (int k = threadIdx.x; k < nr; k += kHpRingThreads) {
         map[start + k] = s_m[k];
         wmap[start + k] = s_w[k];
     }
@@ -830,7 +836,7 @@ void launch_hp_ring_fill(const HpRec *rec, int64_t n, int64_t nside, const int64
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
                             const HpRec *rec, double *map, double *wmap, hipStream_t s)
 {
-    const dim3 g((unsigned)(4 * a.nside - 1)), blk(256);
+    const dim3 g((unsigned)(4 * a.nside - 1)), blk(kHpRingThreads);
     if (a.dtype == S2G_F32) {
         S2G_HP_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k_hp_ring_deposit<float, KK>), g, blk,
                                                           0, s, list, off, rec,
