@@ -245,7 +245,8 @@ int s2g_healpix_reduce_device(s2g_ctx *ctx, double *d_map, const double *d_weigh
  * own normalisation in kernel_dim (2 or 3) dimensions, then the Wendland
  * kernels' bias correction of Dehnen & Aly (2012, eqs. 18-19).  pos is 3 x n
  * interleaved; boxsize = NULL or all zero: Euclidean distances, all positive:
- * periodic box (minimum image per dimension).  nneighbors + 1 <= n. */
+ * periodic box (minimum image per dimension).  nneighbors + 1 <= n and
+ * nneighbors <= 471 (the search buffer per particle). */
 int s2g_mass_density(s2g_ctx *ctx, int dtype, const void *pos, const void *mass, int64_t n,
                      int kernel, int kernel_dim, int64_t nneighbors, const double *boxsize,
                      double *rho_out, double *hsml_out, s2g_stats *stats);

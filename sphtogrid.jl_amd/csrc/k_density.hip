@@ -37,7 +37,7 @@
 namespace s2g {
 
 constexpr int kMdWaves = 4;     // waves per block of the search
-constexpr int kMdCap = 768;     // candidate buffer per wave
+constexpr int kMdCap = 600;     // candidate buffer per wave
 constexpr int kMdBits = 21;     // Morton bits per axis
 constexpr int kMdMaxCells = 216;  // cube cells per particle (at most 6^3)
 
@@ -289,7 +289,7 @@ __device__ __forceinline__ void md_offer(bool valid, double d, int j, double &T,
 }
 
 template <int K>
-__global__ __launch_bounds__(kMdWaves * kWave) void k_md_knn(
+__global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
     const double4 *__restrict__ S, const uint64_t *__restrict__ SK,
     const int32_t *__restrict__ order, const int32_t *__restrict__ tab, int lt, int64_t n,
     MdGrid g, int k1, int dim, int nneigh, double norm, double *__restrict__ rho_out,
