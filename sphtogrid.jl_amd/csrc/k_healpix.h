@@ -41,9 +41,11 @@ void launch_hp_ring_count(const HpRec *rec, int64_t n, int64_t nside, int64_t *d
                           hipStream_t s);
 // count[r] = ex[r + 1] for r < nslots - 1, count[nslots - 1] = 0
 void launch_hp_ring_shift(const int64_t *ex, int64_t nslots, int64_t *count, hipStream_t s);
-// off: exclusive scan of the ring counts; fill: 4 Nside zeroed int
-void launch_hp_ring_fill(const HpRec *rec, int64_t n, int64_t nside, const int64_t *off,
-                         int *fill, int32_t *list, hipStream_t s);
+// fill counters: one every hp_ring_fill_stride() ints
+int hp_ring_fill_stride();
+// off: exclusive scan of the ring counts; fill: 4 Nside + 1 zeroed counters
+void launch_hp_ring_fill(const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
+                         const int64_t *off, int *fill, int32_t *list, hipStream_t s);
 // writes every pixel of both maps
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
                             const HpRec *rec, double *map, double *wmap, hipStream_t s);

@@ -544,34 +544,40 @@ constexpr int kHpRingLds = 4096;  // pixels of the longest ring held in LDS
 constexpr int kHpRingThreads = 512;  // 8 waves per ring block
 constexpr int kHpRingPairs = 32;     // (particle, ring) chords per wave and round
 
-// the rings a particle deposits to: [ra, rb] and the ring of its centre pixel
-__device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int64_t &r0,
-                                             int64_t &r1, int64_t &cring, int64_t &ck)
+// The rings a particle deposits to: its disc's rings [ra, rb] and the ring of
+// its centre pixel (constributing_pixels.jl:16-19), which lies outside that
+// range only when the disc query returns no pixel of it.  False for
+// particles that deposit nothing.
+__device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int64_t &cring,
+                                             int64_t &ck, bool &cring_extra)
 {
     if (!r.ok || r.s == 0.0)
         return false;
     hp_pix2ring(nside, r.cpix, cring, ck);
-    if (r.rb >= r.ra) {
-        r0 = r.ra < cring ? r.ra : cring;
-        r1 = r.rb > cring ? r.rb : cring;
-    } else {
-        r0 = r1 = cring;
-    }
+    cring_extra = r.rb < r.ra || cring < r.ra || cring > r.rb;
     return true;
 }
 
-// ring counts as a difference array: diff[r0] += 1, diff[r1 + 1] -= 1
+// ring counts as a difference array over [ra, rb] and the extra centre ring
 __global__ void k_hp_ring_count(const HpRec *__restrict__ rec, int64_t n, int64_t nside,
                                 unsigned long long *__restrict__ diff)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n)
         return;
-    int64_t r0, r1, cring, ck;
-    if (!hp_dep_rings(nside, rec[p], r0, r1, cring, ck))
+    const HpRec r = rec[p];
+    int64_t cring, ck;
+    bool extra;
+    if (!hp_dep_rings(nside, r, cring, ck, extra))
         return;
-    atomicAdd(&diff[r0], 1ull);
-    atomicAdd(&diff[r1 + 1], ~0ull);  // -1 modulo 2^64
+    if (r.rb >= r.ra) {
+        atomicAdd(&diff[r.ra], 1ull);
+        atomicAdd(&diff[r.rb + 1], ~0ull);  // -1 modulo 2^64
+    }
+    if (extra) {
+        atomicAdd(&diff[cring], 1ull);
+        atomicAdd(&diff[cring + 1], ~0ull);
+    }
 }
 
 // count[r] = inclusive prefix of diff (= exclusive prefix at r + 1); count[last] = 0
@@ -585,20 +591,37 @@ __global__ void k_hp_ring_shift(const int64_t *__restrict__ ex, int64_t nslots,
         count[r] = 0;
 }
 
-// per-ring particle lists (order within a ring is not fixed: the LDS
-// additions of the deposit are unordered anyway)
-__global__ void k_hp_ring_fill(const HpRec *__restrict__ rec, int64_t n, int64_t nside,
-                               const int64_t *__restrict__ off, int *__restrict__ fill,
-                               int32_t *__restrict__ list)
+// Per-ring particle lists, one wave per work unit: lane l enters the particle
+// into ring ra + 64 ui + l; unit 0's lane 0 adds the extra centre ring.  The
+// ring fill counters are kHpFillStride ints apart (one cache line each), as
+// every ring's counter takes many increments.  The order within a ring is not
+// fixed: the deposit's LDS additions are unordered anyway.
+constexpr int kHpFillStride = 32;
+
+__global__ __launch_bounds__(256) void k_hp_ring_fill(const int2 *__restrict__ units,
+                                                      int64_t nu,
+                                                      const HpRec *__restrict__ rec,
+                                                      int64_t nside,
+                                                      const int64_t *__restrict__ off,
+                                                      int *__restrict__ fill,
+                                                      int32_t *__restrict__ list)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n)
-        return;
-    int64_t r0, r1, cring, ck;
-    if (!hp_dep_rings(nside, rec[p], r0, r1, cring, ck))
-        return;
-    for (int64_t rr = r0; rr <= r1; ++rr)
-        list[off[rr] + atomicAdd(&fill[rr], 1)] = (int32_t)p;
+    const int ln = threadIdx.x % kWave;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
+         gw += nw) {
+        const int2 u = units[gw];
+        const HpRec r = rec[u.x];
+        int64_t cring, ck;
+        bool extra;
+        if (!hp_dep_rings(nside, r, cring, ck, extra))
+            continue;
+        const int64_t rr = (int64_t)r.ra + (int64_t)u.y * kHpRingsPerUnit + ln;
+        if (rr <= r.rb)
+            list[off[rr] + atomicAdd(&fill[rr * kHpFillStride], 1)] = u.x;
+        if (extra && u.y == 0 && ln == 0)
+            list[off[cring] + atomicAdd(&fill[cring * kHpFillStride], 1)] = u.x;
+    }
 }
 
 // one (particle, ring) chord of the ring-major deposit (per-wave LDS table)
@@ -649,8 +672,9 @@ __global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
         if (ln < kHpRingPairs && c0 + ln < e) {
             const int p = list[c0 + ln];
             const HpRec r = rec[p];
-            int64_t r0, r1, cring, ck;
-            hp_dep_rings(nside, r, r0, r1, cring, ck);
+            int64_t cring, ck;
+            bool extra;
+            hp_dep_rings(nside, r, cring, ck, extra);
             int64_t a = 0, bb = -1;
             bool any = false;
             if (iz >= r.ra && iz <= r.rb) {
@@ -826,11 +850,15 @@ void launch_hp_ring_shift(const int64_t *ex, int64_t nslots, int64_t *count, hip
                        ex, nslots, count);
 }
 
-void launch_hp_ring_fill(const HpRec *rec, int64_t n, int64_t nside, const int64_t *off,
-                         int *fill, int32_t *list, hipStream_t s)
+int hp_ring_fill_stride() { return kHpFillStride; }
+
+void launch_hp_ring_fill(const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
+                         const int64_t *off, int *fill, int32_t *list, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec,
-                       n, nside, off, fill, list);
+    if (nu == 0)
+        return;
+    hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)wave_blocks(nu, 256)), dim3(256), 0, s,
+                       units, nu, rec, nside, off, fill, list);
 }
 
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,

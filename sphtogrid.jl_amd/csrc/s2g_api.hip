@@ -1009,13 +1009,14 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
         if (rings) {
             // per-ring particle lists: counts (difference array, scans), fill
             const int64_t nslots = 4 * nside + 1;
+            const int64_t nfill = nslots * hp_ring_fill_stride();
             int64_t *rb = (int64_t *)c->hp_ring.get(sizeof(int64_t) * 2 * nslots +
-                                                    sizeof(int) * nslots);
+                                                    sizeof(int) * nfill);
             int64_t *diff = rb, *off = rb + nslots;
             int *fill = (int *)(rb + 2 * nslots);
             int64_t *stmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nslots));
             S2G_HIP(hipMemsetAsync(diff, 0, sizeof(int64_t) * nslots, s));
-            S2G_HIP(hipMemsetAsync(fill, 0, sizeof(int) * nslots, s));
+            S2G_HIP(hipMemsetAsync(fill, 0, sizeof(int) * nfill, s));
             launch_hp_ring_count(rec, n, nside, diff, s);
             exclusive_scan_i64(diff, nslots, stmp, s);
             launch_hp_ring_shift(diff, nslots, off, s);
@@ -1025,7 +1026,7 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
             if (npairs >= (int64_t)INT32_MAX)
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
             int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
-            launch_hp_ring_fill(rec, n, nside, off, fill, list, s);
+            launch_hp_ring_fill(units, nu, rec, nside, off, fill, list, s);
             launch_hp_ring_deposit(a, list, off, rec, d_map, d_wmap, s);
         } else {
             launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
