@@ -36,16 +36,21 @@ void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpR
 
 // ring-major deposit (no global atomics) for 4 Nside <= its LDS ring size
 bool hp_ring_deposit_fits(int64_t nside);
-// diff: 4 Nside + 1 zeroed int64 (ring-count difference array)
-void launch_hp_ring_count(const HpRec *rec, int64_t n, int64_t nside, int64_t *diff,
-                          hipStream_t s);
-// count[r] = ex[r + 1] for r < nslots - 1, count[nslots - 1] = 0
-void launch_hp_ring_shift(const int64_t *ex, int64_t nslots, int64_t *count, hipStream_t s);
-// fill counters: one every hp_ring_fill_stride() ints
-int hp_ring_fill_stride();
-// off: exclusive scan of the ring counts; fill: 4 Nside + 1 zeroed counters
-void launch_hp_ring_fill(const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
-                         const int64_t *off, int *fill, int32_t *list, hipStream_t s);
+// per-ring particle lists: entry keys (units + one extra entry per particle,
+// nu + n of each), sorted with sort_pairs over hp_entry_key_bits(nside) bits
+void launch_hp_entry_keys(const int2 *units, int64_t nu, const HpRec *rec, int64_t n,
+                          int64_t nside, uint32_t *keys, int32_t *vals, hipStream_t s);
+int hp_entry_key_bits(int64_t nside);
+// off: 4 Nside + 1 first-ring offsets into the sorted entries
+void launch_hp_entry_offsets(const uint32_t *keys, int64_t ne, int64_t nside, int64_t *off,
+                             hipStream_t s);
+// count: 4 Nside + 1 int64 (entries touching each ring; ends 0)
+void launch_hp_ring_count(const uint32_t *keys, const int64_t *eoff, int64_t nside,
+                          int64_t *count, hipStream_t s);
+// loff: exclusive scan of count; list: the particles per ring
+void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_t *eoff,
+                         const int64_t *loff, const int2 *units, int64_t nu, int64_t nside,
+                         int32_t *list, hipStream_t s);
 // writes every pixel of both maps
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
                             const HpRec *rec, double *map, double *wmap, hipStream_t s);

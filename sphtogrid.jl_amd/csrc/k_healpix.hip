@@ -558,69 +558,133 @@ __device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int6
     return true;
 }
 
-// ring counts as a difference array over [ra, rb] and the extra centre ring
-__global__ void k_hp_ring_count(const HpRec *__restrict__ rec, int64_t n, int64_t nside,
-                                unsigned long long *__restrict__ diff)
+// Per-ring particle lists without atomics.  Entries are the work units
+// (rings [ra + 64 ui, min(ra + 64 ui + 63, rb)]) and, per particle, the ring
+// of its centre pixel when that lies outside [ra, rb]; an entry's key is
+// first ring * kHpSpan + last ring (sentinel for entries without a ring).
+// After a radix sort, the entries that can touch ring R are those whose first
+// ring lies in [R - 63, R], one contiguous key range; those whose last ring
+// is >= R touch it.  One block per ring counts them, a scan gives the list
+// offsets, and one block per ring writes them in sorted order.
+constexpr uint32_t kHpSpan = 8192;  // > 4 Nside for Nside <= kHpRingLds / 4
+
+__device__ __forceinline__ int hp_entry_particle(int64_t e, int64_t nu,
+                                                 const int2 *__restrict__ units)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n)
+    return e < nu ? units[e].x : (int)(e - nu);
+}
+
+__global__ void k_hp_entry_keys(const int2 *__restrict__ units, int64_t nu,
+                                const HpRec *__restrict__ rec, int64_t n, int64_t nside,
+                                uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nu + n)
         return;
+    const uint32_t none = (uint32_t)(4 * nside + 1) * kHpSpan;
+    uint32_t key = none;
+    const int p = hp_entry_particle(e, nu, units);
     const HpRec r = rec[p];
     int64_t cring, ck;
     bool extra;
-    if (!hp_dep_rings(nside, r, cring, ck, extra))
-        return;
-    if (r.rb >= r.ra) {
-        atomicAdd(&diff[r.ra], 1ull);
-        atomicAdd(&diff[r.rb + 1], ~0ull);  // -1 modulo 2^64
+    if (hp_dep_rings(nside, r, cring, ck, extra)) {
+        if (e < nu) {
+            const int64_t a = (int64_t)r.ra + (int64_t)units[e].y * kHpRingsPerUnit;
+            const int64_t bb = a + kHpRingsPerUnit - 1 < r.rb ? a + kHpRingsPerUnit - 1 : r.rb;
+            if (bb >= a)
+                key = (uint32_t)a * kHpSpan + (uint32_t)bb;
+        } else if (extra) {
+            key = (uint32_t)cring * kHpSpan + (uint32_t)cring;
+        }
     }
-    if (extra) {
-        atomicAdd(&diff[cring], 1ull);
-        atomicAdd(&diff[cring + 1], ~0ull);
-    }
+    keys[e] = key;
+    vals[e] = (int32_t)e;
 }
 
-// count[r] = inclusive prefix of diff (= exclusive prefix at r + 1); count[last] = 0
-__global__ void k_hp_ring_shift(const int64_t *__restrict__ ex, int64_t nslots,
-                                int64_t *__restrict__ count)
+// off[t] = first sorted entry with first ring >= t, t in [0, 4 Nside]
+__global__ void k_hp_entry_offsets(const uint32_t *__restrict__ keys, int64_t ne, int64_t nt,
+                                   int64_t *__restrict__ off)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nslots - 1)
-        count[r] = ex[r + 1];
-    else if (r == nslots - 1)
-        count[r] = 0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nt)
+        return;
+    int64_t lo = 0, hi = ne;
+    const uint32_t key = (uint32_t)t * kHpSpan;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    off[t] = lo;
 }
 
-// Per-ring particle lists, one wave per work unit: lane l enters the particle
-// into ring ra + 64 ui + l; unit 0's lane 0 adds the extra centre ring.  The
-// ring fill counters are kHpFillStride ints apart (one cache line each), as
-// every ring's counter takes many increments.  The order within a ring is not
-// fixed: the deposit's LDS additions are unordered anyway.
-constexpr int kHpFillStride = 32;
+// count[iz] = sorted entries touching ring iz (count[0] = count[4 Nside] = 0)
+__global__ __launch_bounds__(256) void k_hp_ring_count(const uint32_t *__restrict__ keys,
+                                                       const int64_t *__restrict__ eoff,
+                                                       int64_t nside,
+                                                       int64_t *__restrict__ count)
+{
+    __shared__ int s_part[256 / kWave];
+    const int64_t iz = blockIdx.x;  // 0 .. 4 Nside
+    if (iz == 0 || iz == 4 * nside) {
+        if (threadIdx.x == 0)
+            count[iz] = 0;
+        return;
+    }
+    const int64_t b = eoff[iz >= kHpRingsPerUnit - 1 ? iz - (kHpRingsPerUnit - 1) : 0];
+    const int64_t e = eoff[iz + 1];
+    int c = 0;
+    for (int64_t i = b + threadIdx.x; i < e; i += 256)
+        c += (int64_t)(keys[i] % kHpSpan) >= iz ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, kWave);
+    if (threadIdx.x % kWave == 0)
+        s_part[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 256 / kWave; ++w)
+            t += s_part[w];
+        count[iz] = t;
+    }
+}
 
-__global__ __launch_bounds__(256) void k_hp_ring_fill(const int2 *__restrict__ units,
-                                                      int64_t nu,
-                                                      const HpRec *__restrict__ rec,
-                                                      int64_t nside,
-                                                      const int64_t *__restrict__ off,
-                                                      int *__restrict__ fill,
+// list[loff[iz] ...] = the particles of the entries touching ring iz, in
+// sorted order (block-wide ordered compaction)
+__global__ __launch_bounds__(256) void k_hp_ring_fill(const uint32_t *__restrict__ keys,
+                                                      const int32_t *__restrict__ vals,
+                                                      const int64_t *__restrict__ eoff,
+                                                      const int64_t *__restrict__ loff,
+                                                      const int2 *__restrict__ units,
+                                                      int64_t nu, int64_t nside,
                                                       int32_t *__restrict__ list)
 {
-    const int ln = threadIdx.x % kWave;
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
-         gw += nw) {
-        const int2 u = units[gw];
-        const HpRec r = rec[u.x];
-        int64_t cring, ck;
-        bool extra;
-        if (!hp_dep_rings(nside, r, cring, ck, extra))
-            continue;
-        const int64_t rr = (int64_t)r.ra + (int64_t)u.y * kHpRingsPerUnit + ln;
-        if (rr <= r.rb)
-            list[off[rr] + atomicAdd(&fill[rr * kHpFillStride], 1)] = u.x;
-        if (extra && u.y == 0 && ln == 0)
-            list[off[cring] + atomicAdd(&fill[cring * kHpFillStride], 1)] = u.x;
+    __shared__ int s_wc[256 / kWave];
+    const int64_t iz = blockIdx.x + 1;  // rings 1 .. 4 Nside - 1
+    const int64_t b = eoff[iz >= kHpRingsPerUnit - 1 ? iz - (kHpRingsPerUnit - 1) : 0];
+    const int64_t e = eoff[iz + 1];
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const uint64_t below = ln == 0 ? 0ull : (~0ull >> (kWave - ln));
+    int64_t out = loff[iz];
+    for (int64_t c0 = b; c0 < e; c0 += 256) {
+        const int64_t i = c0 + threadIdx.x;
+        const bool take = i < e && (int64_t)(keys[i] % kHpSpan) >= iz;
+        const uint64_t bt = __ballot(take);
+        if (ln == 0)
+            s_wc[wv] = (int)__popcll(bt);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < 256 / kWave; ++w) {
+            before += w < wv ? s_wc[w] : 0;
+            total += s_wc[w];
+        }
+        if (take)
+            list[out + before + (int)__popcll(bt & below)] =
+                hp_entry_particle(vals[i], nu, units);
+        out += total;
+        __syncthreads();
     }
 }
 
@@ -837,28 +901,44 @@ void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpR
 
 bool hp_ring_deposit_fits(int64_t nside) { return 4 * nside <= kHpRingLds; }
 
-void launch_hp_ring_count(const HpRec *rec, int64_t n, int64_t nside, int64_t *diff,
-                          hipStream_t s)
+void launch_hp_entry_keys(const int2 *units, int64_t nu, const HpRec *rec, int64_t n,
+                          int64_t nside, uint32_t *keys, int32_t *vals, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hp_ring_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec,
-                       n, nside, (unsigned long long *)diff);
+    const int64_t ne = nu + n;
+    hipLaunchKernelGGL(k_hp_entry_keys, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                       units, nu, rec, n, nside, keys, vals);
 }
 
-void launch_hp_ring_shift(const int64_t *ex, int64_t nslots, int64_t *count, hipStream_t s)
+int hp_entry_key_bits(int64_t nside)
 {
-    hipLaunchKernelGGL(k_hp_ring_shift, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
-                       ex, nslots, count);
+    const uint64_t top = (uint64_t)(4 * nside + 1) * kHpSpan;
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) <= top)
+        ++b;
+    return b;
 }
 
-int hp_ring_fill_stride() { return kHpFillStride; }
-
-void launch_hp_ring_fill(const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
-                         const int64_t *off, int *fill, int32_t *list, hipStream_t s)
+void launch_hp_entry_offsets(const uint32_t *keys, int64_t ne, int64_t nside, int64_t *off,
+                             hipStream_t s)
 {
-    if (nu == 0)
-        return;
-    hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)wave_blocks(nu, 256)), dim3(256), 0, s,
-                       units, nu, rec, nside, off, fill, list);
+    const int64_t nt = 4 * nside;
+    hipLaunchKernelGGL(k_hp_entry_offsets, dim3((unsigned)((nt + 1 + 255) / 256)), dim3(256), 0,
+                       s, keys, ne, nt, off);
+}
+
+void launch_hp_ring_count(const uint32_t *keys, const int64_t *eoff, int64_t nside,
+                          int64_t *count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_count, dim3((unsigned)(4 * nside + 1)), dim3(256), 0, s, keys,
+                       eoff, nside, count);
+}
+
+void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_t *eoff,
+                         const int64_t *loff, const int2 *units, int64_t nu, int64_t nside,
+                         int32_t *list, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)(4 * nside - 1)), dim3(256), 0, s, keys,
+                       vals, eoff, loff, units, nu, nside, list);
 }
 
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,

@@ -1007,26 +1007,31 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
         launch_hp_finish(nun, n, part, rec, s);
         S2G_HIP(hipEventRecord(E[3], s));
         if (rings) {
-            // per-ring particle lists: counts (difference array, scans), fill
+            // per-ring particle lists: sorted entries (units + extra centre
+            // rings), per-ring counts, scan, ordered fill
             const int64_t nslots = 4 * nside + 1;
-            const int64_t nfill = nslots * hp_ring_fill_stride();
-            int64_t *rb = (int64_t *)c->hp_ring.get(sizeof(int64_t) * 2 * nslots +
-                                                    sizeof(int) * nfill);
-            int64_t *diff = rb, *off = rb + nslots;
-            int *fill = (int *)(rb + 2 * nslots);
+            const int64_t ne = nu + n;
+            if (ne >= (int64_t)INT32_MAX)
+                throw Error(S2G_EUNSUPPORTED, "too many all-sky work units");
+            uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * ne);
+            int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * ne);
+            launch_hp_entry_keys(units, nu, rec, n, nside, keys, vals, s);
+            const int kbits = hp_entry_key_bits(nside);
+            const size_t tb = sort_pairs_tmp_bytes(ne, kbits);
+            void *sortmp = c->sort_tmp.get(tb ? tb : 1);
+            sort_pairs(sortmp, tb, keys, keys + ne, vals, vals + ne, ne, kbits, s);
+            int64_t *rb = (int64_t *)c->hp_ring.get(sizeof(int64_t) * 2 * nslots);
+            int64_t *eoff = rb, *off = rb + nslots;
             int64_t *stmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nslots));
-            S2G_HIP(hipMemsetAsync(diff, 0, sizeof(int64_t) * nslots, s));
-            S2G_HIP(hipMemsetAsync(fill, 0, sizeof(int) * nfill, s));
-            launch_hp_ring_count(rec, n, nside, diff, s);
-            exclusive_scan_i64(diff, nslots, stmp, s);
-            launch_hp_ring_shift(diff, nslots, off, s);
+            launch_hp_entry_offsets(keys + ne, ne, nside, eoff, s);
+            launch_hp_ring_count(keys + ne, eoff, nside, off, s);
             exclusive_scan_i64(off, nslots, stmp, s);
             int64_t npairs;
             read_i64(c, s, {off + nslots - 1}, &npairs);
             if (npairs >= (int64_t)INT32_MAX)
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
             int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
-            launch_hp_ring_fill(units, nu, rec, nside, off, fill, list, s);
+            launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, units, nu, nside, list, s);
             launch_hp_ring_deposit(a, list, off, rec, d_map, d_wmap, s);
         } else {
             launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
