@@ -46,7 +46,7 @@ struct GatherArgs {
     int dtype, kernel, np, plane0, nq, nmaps;
     int64_t ntiles, nitems;
     const int4 *items;
-    const int32_t *pairs, *large;
+    const int32_t *pairs;  // particle index per (tile, particle) pair
     const Rec2 *rec;
     const Map2D *maps;
     double *partial;

@@ -283,7 +283,6 @@ constexpr int kGStep = 4;  // row groups per skip test (measured best of 1, 2, 4
 template <typename T, int K, int NP>
 __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
                                             const int32_t *__restrict__ pairs,
-                                            const int32_t *__restrict__ large,
                                             const Rec2 *__restrict__ rec,
                                             const Map2D *__restrict__ maps, int nq, int plane0,
                                             double *__restrict__ partial)
@@ -316,8 +315,7 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
     for (int base = 0; base < it.z; base += kGBatch) {
         const int cnt = it.z - base < kGBatch ? it.z - base : kGBatch;
         if (threadIdx.x < cnt) {
-            const int l = pairs[it.y + base + threadIdx.x];
-            const int p = large[l];
+            const int p = pairs[it.y + base + threadIdx.x];
             const Rec2 r = rec[p];
             s_x[threadIdx.x] = r.x;
             s_y[threadIdx.x] = r.y;
@@ -440,22 +438,20 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256, 4) void k2d_gather(const int4 *__restrict__ items,
                                                      const int32_t *__restrict__ pairs,
-                                                     const int32_t *__restrict__ large,
                                                      const Rec2 *__restrict__ rec,
                                                      const Map2D *__restrict__ maps, int nq,
                                                      int plane0, double *__restrict__ partial)
 {
-    gather_tile<T, K, NP>(items, pairs, large, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP>(items, pairs, rec, maps, nq, plane0, partial);
 }
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256, 3) void k2d_gather_wide(const int4 *__restrict__ items,
                                                        const int32_t *__restrict__ pairs,
-                                                       const int32_t *__restrict__ large,
                                                        const Rec2 *__restrict__ rec,
                                                        const Map2D *__restrict__ maps, int nq,
                                                        int plane0, double *__restrict__ partial)
 {
-    gather_tile<T, K, NP>(items, pairs, large, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP>(items, pairs, rec, maps, nq, plane0, partial);
 }
 
 // fixed-order sum of a tile's item partials -> image planes (plain stores).
@@ -831,10 +827,10 @@ static void gather_knp(const GatherArgs &g, hipStream_t s)
 {
     if constexpr (NP <= 2)
         hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
-                           g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0, g.partial);
+                           g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial);
     else
         hipLaunchKernelGGL((k2d_gather_wide<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0,
-                           s, g.items, g.pairs, g.large, g.rec, g.maps, g.nq, g.plane0,
+                           s, g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0,
                            g.partial);
 }
 

@@ -11,7 +11,7 @@ struct Gather3Args {
     int dtype, kernel, np, plane0, nq, nb;
     int64_t ntiles, nitems, npix;
     const int4 *items;
-    const int32_t *pairs, *large;
+    const int32_t *pairs;  // particle index per (tile, particle) pair
     const Rec3 *rec;
     const void *binq;
     double *partial;

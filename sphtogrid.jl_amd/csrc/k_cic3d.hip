@@ -237,38 +237,46 @@ __global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list
 
 
 // ---------------------------------------------------------------------------
-// Brick gather.  Block = 4 waves = one 16^3 brick (4096 voxels); voxel
-// (ii, jj, kk) of the brick, "row" r = ii*16 + jj.  Wave w owns rows
-// [64w, 64w+64) (ii in [4w, 4w+4)); lane l works on kk = l & 15 and rows
-// 64w + 4m + (l >> 4), m = 0..15.  Per batch of particles the row terms
-// (xd^2 + yd^2, ax*ay) of all 256 rows are tabulated in LDS; per particle
-// each lane computes its z terms, and the z overlap is folded into the FP64
-// plane multipliers.  Per voxel the work equals the 2D gather's per pixel.
+// Brick gather.  Block = 4 waves = one 16^3 brick (4096 voxels), cut into
+// 4^3 sub-bricks.  Wave w owns the z slab kk in [4w, 4w + 4): its 16
+// sub-bricks m = 4 bi + bj; lane l works on voxel (4 bi + (l >> 4),
+// 4 bj + ((l >> 2) & 3), 4w + (l & 3)) of sub-brick m.  Per batch of
+// particles the block tabulates per axis the 16 offset terms (offset / h)^2
+// and overlaps, and one bit per sub-brick that can reach the kernel sphere
+// (voxel-centre box within h + 1 voxel of the centre, inside the footprint
+// box); a wave evaluates only its sub-bricks with that bit set.  The x
+// overlap enters shape_row as ax^(1/n), the y and z overlaps fold into the
+// FP64 plane multipliers, so per voxel the work equals the 2D gather's per
+// pixel.
 // ---------------------------------------------------------------------------
-constexpr int kGBatch3 = 16;
+constexpr int kGBatch3 = 32;
+// sub-bricks per uniform test (1, 2 or 4: one bi per test)
+constexpr int kG3 = 2;
 
 template <typename T, int K, int NP>
-__global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items,
-                                                  const int32_t *__restrict__ pairs,
-                                                  const int32_t *__restrict__ large,
-                                                  const Rec3 *__restrict__ rec,
-                                                  const T *__restrict__ binq, int nq,
-                                                  int plane0, int nb,
-                                                  double *__restrict__ partial)
+__global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ items,
+                                                     const int32_t *__restrict__ pairs,
+                                                     const Rec3 *__restrict__ rec,
+                                                     const T *__restrict__ binq, int nq,
+                                                     int plane0, int nb,
+                                                     double *__restrict__ partial)
 {
-    __shared__ float2 s_row[kGBatch3][kBT * kBT];
-    __shared__ float2 s_zc[kGBatch3][kBT];
-    __shared__ double s_x[kGBatch3], s_y[kGBatch3], s_z[kGBatch3], s_h[kGBatch3];
+    __shared__ float2 s_tab[kGBatch3][3][kBT];
+    __shared__ uint32_t s_hit[kGBatch3][2];
+    __shared__ double s_c[kGBatch3][3], s_h[kGBatch3];
     __shared__ double s_hinvd[kGBatch3];
-    __shared__ int s_i0[kGBatch3], s_i1[kGBatch3];
+    __shared__ int s_lo[kGBatch3][3], s_hi[kGBatch3][3];
+    __shared__ int s_fb[kGBatch3];
     __shared__ double s_mult[NP][kGBatch3];
 
     const int4 it = items[blockIdx.x];
     const int t = it.x;
-    const int bi0 = (t / (nb * nb)) * kBT, bj0 = ((t / nb) % nb) * kBT, bk0 = (t % nb) * kBT;
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
-    const int rg = ln >> 4;
-    const int kk = ln & 15;
+    int b0[3];
+    b0[0] = (t / (nb * nb)) * kBT;
+    b0[1] = ((t / nb) % nb) * kBT;
+    b0[2] = (t % nb) * kBT;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), ln = threadIdx.x % kWave;
+    const int di = ln >> 4, dj = (ln >> 2) & 3, dk = ln & 3;
 
     double acc[16][NP];
 #pragma unroll
@@ -280,16 +288,20 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
     for (int base = 0; base < it.z; base += kGBatch3) {
         const int cnt = it.z - base < kGBatch3 ? it.z - base : kGBatch3;
         if (threadIdx.x < cnt) {
-            const int l = pairs[it.y + base + threadIdx.x];
-            const int p = large[l];
+            const int p = pairs[it.y + base + threadIdx.x];
             const Rec3 r = rec[p];
-            s_x[threadIdx.x] = r.x;
-            s_y[threadIdx.x] = r.y;
-            s_z[threadIdx.x] = r.z;
+            s_c[threadIdx.x][0] = r.x;
+            s_c[threadIdx.x][1] = r.y;
+            s_c[threadIdx.x][2] = r.z;
             s_h[threadIdx.x] = r.h;
             s_hinvd[threadIdx.x] = r.fb ? 0.0 : 1.0 / r.h;
-            s_i0[threadIdx.x] = r.i0;
-            s_i1[threadIdx.x] = r.i1;
+            s_lo[threadIdx.x][0] = r.i0 - b0[0];
+            s_lo[threadIdx.x][1] = r.j0 - b0[1];
+            s_lo[threadIdx.x][2] = r.k0 - b0[2];
+            s_hi[threadIdx.x][0] = r.i1 - b0[0];
+            s_hi[threadIdx.x][1] = r.j1 - b0[1];
+            s_hi[threadIdx.x][2] = r.k1 - b0[2];
+            s_fb[threadIdx.x] = r.fb;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 const int P = plane0 + q;
@@ -299,60 +311,78 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
             }
         }
         __syncthreads();
-        // row table ((x offset / h)^2 + (y offset / h)^2, (ax ay)^(1/n)) of
-        // the 256 (i, j) rows and z table ((z offset / h)^2, az) of the 16 z;
-        // offset / h in FP64, rounded once
-        for (int e = threadIdx.x; e < cnt * kBT * kBT; e += 256) {
-            const int b = e / (kBT * kBT), rr = e % (kBT * kBT);
-            const double fi = (double)(bi0 + (rr >> 4)), fj = (double)(bj0 + (rr & 15));
-            const float xs = (float)(coff1(s_x[b], fi) * s_hinvd[b]);
-            const float ys = (float)(coff1(s_y[b], fj) * s_hinvd[b]);
-            const float ax = fmaxf((float)overlap1(s_x[b], s_h[b], fi), 0.0f);
-            const float ay = fmaxf((float)overlap1(s_y[b], s_h[b], fj), 0.0f);
-            s_row[b][rr] = make_float2(xs * xs + ys * ys, overlap_root<K>(ax * ay));
+        // per axis the 16 terms ((offset / h)^2, overlap); offset / h in FP64,
+        // rounded once; the x overlap as its n-th root
+        for (int e = threadIdx.x; e < cnt * 3 * kBT; e += 256) {
+            const int b = e / (3 * kBT), ax = (e / kBT) % 3, v = e % kBT;
+            const double f = (double)(b0[ax] + v);
+            const double c = s_c[b][ax];
+            const float os = (float)(coff1(c, f) * s_hinvd[b]);
+            const float ov = fmaxf((float)overlap1(c, s_h[b], f), 0.0f);
+            s_tab[b][ax][v] = make_float2(os * os, ax == 0 ? overlap_root<K>(ov) : ov);
         }
-        if (threadIdx.x < cnt * kBT) {
-            const int b = threadIdx.x / kBT, zz = threadIdx.x % kBT;
-            const double fk = (double)(bk0 + zz);
-            const float zs = (float)(coff1(s_z[b], fk) * s_hinvd[b]);
-            s_zc[b][zz] = make_float2(zs * zs, fmaxf((float)overlap1(s_z[b], s_h[b], fk), 0.0f));
+        // sub-brick bits: lane = 16 bk + 4 bi + bj
+        for (int b = wv; b < cnt; b += 4) {
+            const int sb[3] = {(ln >> 2) & 3, ln & 3, ln >> 4};
+            bool in = true;
+            float d2 = 0.0f;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const int lo = 4 * sb[ax], hi = lo + 3;
+                in = in && lo <= s_hi[b][ax] && hi >= s_lo[b][ax];
+                const float c = (float)(s_c[b][ax] - (double)b0[ax]);
+                const float dd = fmaxf(fmaxf((float)lo + 0.5f - c, c - ((float)hi + 0.5f)), 0.0f);
+                d2 += dd * dd;
+            }
+            // one voxel of margin: FP32 rounding here cannot drop a voxel
+            // with u < 1
+            const float rr = (float)s_h[b] + 1.0f;
+            const uint64_t bits = __ballot(in && (s_fb[b] || d2 < rr * rr));
+            if (ln == 0) {
+                s_hit[b][0] = (uint32_t)bits;
+                s_hit[b][1] = (uint32_t)(bits >> 32);
+            }
         }
         __syncthreads();
         for (int b = 0; b < cnt; ++b) {
-            const int i0 = __builtin_amdgcn_readfirstlane(s_i0[b]) - bi0;
-            const int i1 = __builtin_amdgcn_readfirstlane(s_i1[b]) - bi0;
-            if (4 * wv + 3 < i0 || 4 * wv > i1)
+            const uint32_t word = __builtin_amdgcn_readfirstlane(s_hit[b][wv >> 1]);
+            const uint32_t wm = (word >> (16 * (wv & 1))) & 0xffffu;
+            if (wm == 0)
                 continue;
-            const float2 zc = s_zc[b][kk];
-            const float zd2 = zc.x;
-            // the lane's z overlap is common to its 16 voxels: fold it into
-            // the FP64 plane multipliers
-            const double azd = (double)zc.y;
-            double mult[NP];
+            const float2 zt = s_tab[b][2][4 * wv + dk];
+            const double az = (double)zt.y;
+            // the y and z overlaps are common to a lane's voxels of one bj:
+            // fold them into the FP64 plane multipliers
+            float yz[4];
+            double mult[4][NP];
 #pragma unroll
-            for (int q = 0; q < NP; ++q)
-                mult[q] = s_mult[q][b] * azd;
-            // the four row groups of one x plane (ii = 4w + g) per test, so
-            // their LDS reads and chains overlap; rows outside the footprint
-            // have a = 0 and add exact zeros
+            for (int bj = 0; bj < 4; ++bj) {
+                const float2 yt = s_tab[b][1][4 * bj + dj];
+                yz[bj] = yt.x + zt.x;
+                const double f = az * (double)yt.y;
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int ii = 4 * wv + g;
-                if (ii < i0 || ii > i1)
+                for (int q = 0; q < NP; ++q)
+                    mult[bj][q] = s_mult[q][b] * f;
+            }
+#pragma unroll
+            for (int m = 0; m < 16; m += kG3) {
+                if (!((wm >> m) & ((1u << kG3) - 1u)))
                     continue;
-                float wk[4];
+                // the kG3 sub-bricks of one test share bi
+                const float2 xt = s_tab[b][0][4 * (m >> 2) + di];
+                float wk[kG3];
 #pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    // row 64w + 4m + rg with m = 4g + d: jj = 4d + rg
-                    const float2 rv = s_row[b][64 * wv + 16 * g + 4 * d + rg];
-                    wk[d] = shape_row<K>(__builtin_amdgcn_sqrtf(rv.x + zd2), rv.y);
+                for (int g = 0; g < kG3; ++g) {
+                    const int bj = (m + g) & 3;
+                    wk[g] = shape_row<K>(__builtin_amdgcn_sqrtf(xt.x + yz[bj]), xt.y);
                 }
 #pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    const double f = (double)wk[d];
+                for (int g = 0; g < kG3; ++g) {
+                    const int bj = (m + g) & 3;
+                    const double f = (double)wk[g];
 #pragma unroll
                     for (int q = 0; q < NP; ++q)
-                        acc[4 * g + d][q] = fma(f, mult[q], acc[4 * g + d][q]);
+                        acc[m + g][q] = fma(f, mult[bj][q], acc[m + g][q]);
                 }
             }
         }
@@ -362,8 +392,10 @@ __global__ __launch_bounds__(256) void k3d_gather(const int4 *__restrict__ items
 #pragma unroll
     for (int q = 0; q < NP; ++q)
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
-            dst[(q * kBT * kBT + 64 * wv + 4 * m + rg) * kBT + kk] = acc[m][q];
+        for (int m = 0; m < 16; ++m) {
+            const int i = 4 * (m >> 2) + di, j = 4 * (m & 3) + dj, k = 4 * wv + dk;
+            dst[((q * kBT + i) * kBT + j) * kBT + k] = acc[m][q];
+        }
 }
 
 // fixed-order sum of a brick's item partials -> image planes (plain stores)
@@ -511,10 +543,10 @@ static void gather3_tk(const Gather3Args &g, hipStream_t s)
 {
     const dim3 grid((unsigned)g.nitems), blk(256);
     if (g.np == 1)
-        hipLaunchKernelGGL((k3d_gather<T, K, 1>), grid, blk, 0, s, g.items, g.pairs, g.large,
+        hipLaunchKernelGGL((k3d_gather<T, K, 1>), grid, blk, 0, s, g.items, g.pairs,
                            g.rec, (const T *)g.binq, g.nq, g.plane0, g.nb, g.partial);
     else
-        hipLaunchKernelGGL((k3d_gather<T, K, 2>), grid, blk, 0, s, g.items, g.pairs, g.large,
+        hipLaunchKernelGGL((k3d_gather<T, K, 2>), grid, blk, 0, s, g.items, g.pairs,
                            g.rec, (const T *)g.binq, g.nq, g.plane0, g.nb, g.partial);
 }
 

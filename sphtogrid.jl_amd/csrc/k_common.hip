@@ -206,7 +206,8 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
         cursor[t] = (int)off[tm.cbase + (int64_t)t * tm.nch + ch.k];
     __syncthreads();
     for (int64_t l = ch.l0; l < ch.l1; ++l) {
-        const R r = rec[large[l]];
+        const int32_t p = large[l];
+        const R r = rec[p];
         int a0, a1, b0, b1, c0, c1;
         tile_rect(r, a0, a1, b0, b1, c0, c1);
         const int wb = b1 - b0 + 1, wc = c1 - c0 + 1;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
             const int rem = k % (wb * wc);
             const int b = b0 + rem / wc, c = c0 + rem % wc;
             const int slot = atomicAdd(&cursor[(a * tm.nt + b) * tm.ntc + c], 1);
-            pairs[slot] = (int32_t)l;
+            pairs[slot] = p;
         }
     }
 }

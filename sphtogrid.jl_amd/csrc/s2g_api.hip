@@ -353,7 +353,6 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         g.nitems = tp.nitems;
         g.items = tp.items;
         g.pairs = tp.pairs;
-        g.large = cl.l1;
         g.rec = rec;
         g.maps = d_maps;
         g.partial = tp.nitems > 0 ? (double *)c->partial.get(sizeof(double) * tp.nitems * np *
@@ -499,7 +498,6 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
         g.npix = npix;
         g.items = tp.items;
         g.pairs = tp.pairs;
-        g.large = cl.l1;
         g.rec = rec;
         g.binq = a.binq;
         g.partial = (double *)c->partial.get(sizeof(double) * (tp.nitems ? tp.nitems : 1) * 2 *

@@ -93,6 +93,14 @@ def load(path: str | None = None):
         if not os.path.exists(p):
             raise LibraryError(f"libsph2grid is not built ({p} missing); "
                                "run __graft_entry__.build() or make -C csrc")
+        # One HIP runtime per process: torch's wheel ships its own
+        # libamdhip64 (soname libamdhip64.so.7, file name libamdhip64.so).
+        # Loaded first, it also satisfies this library's dependency; loaded
+        # after the system copy, it is a second runtime and finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(p)
         vp = C.c_void_p
         i64 = C.c_int64
