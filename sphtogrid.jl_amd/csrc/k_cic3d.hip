@@ -261,12 +261,12 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
                                                      int plane0, int nb,
                                                      double *__restrict__ partial)
 {
-    __shared__ float2 s_tab[kGBatch3][3][kBT];
+    __shared__ float2 s_tab[3][kGBatch3][kBT];
     __shared__ uint32_t s_hit[kGBatch3][2];
     __shared__ double s_c[kGBatch3][3], s_h[kGBatch3];
     __shared__ double s_hinvd[kGBatch3];
-    __shared__ int s_lo[kGBatch3][3], s_hi[kGBatch3][3];
-    __shared__ int s_fb[kGBatch3];
+    __shared__ float s_dd[kGBatch3][3][4];  // per axis: squared sub-brick distance
+    __shared__ float s_rr[kGBatch3];        // (h + 1)^2
     __shared__ double s_mult[NP][kGBatch3];
 
     const int4 it = items[blockIdx.x];
@@ -295,13 +295,27 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
             s_c[threadIdx.x][2] = r.z;
             s_h[threadIdx.x] = r.h;
             s_hinvd[threadIdx.x] = r.fb ? 0.0 : 1.0 / r.h;
-            s_lo[threadIdx.x][0] = r.i0 - b0[0];
-            s_lo[threadIdx.x][1] = r.j0 - b0[1];
-            s_lo[threadIdx.x][2] = r.k0 - b0[2];
-            s_hi[threadIdx.x][0] = r.i1 - b0[0];
-            s_hi[threadIdx.x][1] = r.j1 - b0[1];
-            s_hi[threadIdx.x][2] = r.k1 - b0[2];
-            s_fb[threadIdx.x] = r.fb;
+            // sub-brick test terms: per axis and sub-brick the squared
+            // distance from the centre to its voxel centres, infinite
+            // outside the footprint box (0 inside it for a fallback
+            // footprint, which keeps its whole box); one voxel of margin on
+            // h, so FP32 rounding cannot drop a voxel with u < 1
+            const int lo[3] = {r.i0 - b0[0], r.j0 - b0[1], r.k0 - b0[2]};
+            const int hi[3] = {r.i1 - b0[0], r.j1 - b0[1], r.k1 - b0[2]};
+            const double cc[3] = {r.x, r.y, r.z};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const float c = (float)(cc[ax] - (double)b0[ax]);
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) {
+                    const float dd =
+                        fmaxf(fmaxf(4.0f * sb + 0.5f - c, c - (4.0f * sb + 3.5f)), 0.0f);
+                    const bool inbox = 4 * sb <= hi[ax] && 4 * sb + 3 >= lo[ax];
+                    s_dd[threadIdx.x][ax][sb] = !inbox ? __builtin_inff() : r.fb ? 0.0f : dd * dd;
+                }
+            }
+            const float rr = (float)r.h + 1.0f;
+            s_rr[threadIdx.x] = rr * rr;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 const int P = plane0 + q;
@@ -312,32 +326,22 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
         }
         __syncthreads();
         // per axis the 16 terms ((offset / h)^2, overlap); offset / h in FP64,
-        // rounded once; the x overlap as its n-th root
-        for (int e = threadIdx.x; e < cnt * 3 * kBT; e += 256) {
-            const int b = e / (3 * kBT), ax = (e / kBT) % 3, v = e % kBT;
+        // rounded once; the x overlap as its n-th root.  One axis per wave.
+        for (int e = threadIdx.x; e < 3 * kGBatch3 * kBT; e += 256) {
+            const int ax = e / (kGBatch3 * kBT), b = (e / kBT) % kGBatch3, v = e % kBT;
+            if (b >= cnt)
+                continue;
             const double f = (double)(b0[ax] + v);
             const double c = s_c[b][ax];
             const float os = (float)(coff1(c, f) * s_hinvd[b]);
             const float ov = fmaxf((float)overlap1(c, s_h[b], f), 0.0f);
-            s_tab[b][ax][v] = make_float2(os * os, ax == 0 ? overlap_root<K>(ov) : ov);
+            s_tab[ax][b][v] = make_float2(os * os, ax == 0 ? overlap_root<K>(ov) : ov);
         }
         // sub-brick bits: lane = 16 bk + 4 bi + bj
+        const int sbx = (ln >> 2) & 3, sby = ln & 3, sbz = ln >> 4;
         for (int b = wv; b < cnt; b += 4) {
-            const int sb[3] = {(ln >> 2) & 3, ln & 3, ln >> 4};
-            bool in = true;
-            float d2 = 0.0f;
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const int lo = 4 * sb[ax], hi = lo + 3;
-                in = in && lo <= s_hi[b][ax] && hi >= s_lo[b][ax];
-                const float c = (float)(s_c[b][ax] - (double)b0[ax]);
-                const float dd = fmaxf(fmaxf((float)lo + 0.5f - c, c - ((float)hi + 0.5f)), 0.0f);
-                d2 += dd * dd;
-            }
-            // one voxel of margin: FP32 rounding here cannot drop a voxel
-            // with u < 1
-            const float rr = (float)s_h[b] + 1.0f;
-            const uint64_t bits = __ballot(in && (s_fb[b] || d2 < rr * rr));
+            const float d2 = s_dd[b][0][sbx] + s_dd[b][1][sby] + s_dd[b][2][sbz];
+            const uint64_t bits = __ballot(d2 < s_rr[b]);
             if (ln == 0) {
                 s_hit[b][0] = (uint32_t)bits;
                 s_hit[b][1] = (uint32_t)(bits >> 32);
@@ -349,15 +353,18 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
             const uint32_t wm = (word >> (16 * (wv & 1))) & 0xffffu;
             if (wm == 0)
                 continue;
-            const float2 zt = s_tab[b][2][4 * wv + dk];
+            const float2 zt = s_tab[2][b][4 * wv + dk];
             const double az = (double)zt.y;
             // the y and z overlaps are common to a lane's voxels of one bj:
-            // fold them into the FP64 plane multipliers
+            // fold them into the FP64 plane multipliers (for the bj pairs
+            // that a test of this particle will evaluate)
             float yz[4];
             double mult[4][NP];
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
-                const float2 yt = s_tab[b][1][4 * bj + dj];
+                if (!(wm & (((1u << kG3) - 1u) * 0x1111u << (bj / kG3 * kG3))))
+                    continue;
+                const float2 yt = s_tab[1][b][4 * bj + dj];
                 yz[bj] = yt.x + zt.x;
                 const double f = az * (double)yt.y;
 #pragma unroll
@@ -369,7 +376,7 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
                 if (!((wm >> m) & ((1u << kG3) - 1u)))
                     continue;
                 // the kG3 sub-bricks of one test share bi
-                const float2 xt = s_tab[b][0][4 * (m >> 2) + di];
+                const float2 xt = s_tab[0][b][4 * (m >> 2) + di];
                 float wk[kG3];
 #pragma unroll
                 for (int g = 0; g < kG3; ++g) {

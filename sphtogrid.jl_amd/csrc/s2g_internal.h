@@ -149,7 +149,10 @@ __device__ __forceinline__ float shape_row(float u, float a)
 template <int K>
 __device__ __forceinline__ float overlap_root(float ax)
 {
-    return ax >= 1.0f ? 1.0f : ax > 0.0f ? exp2f(log2f(ax) * (1.0f / kTPow<K>)) : 0.0f;
+    // v_log_f32 / v_exp_f32 directly: ax is 0 or a normal number here
+    return ax >= 1.0f   ? 1.0f
+           : ax > 0.0f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(ax) * (1.0f / kTPow<K>))
+                       : 0.0f;
 }
 
 // integral of shape(|v|) over the unit disc (= 1 / norm_2D), used for the
