@@ -352,10 +352,10 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
         }
         // row groups each wave's strip needs for each particle: the footprint
         // box, narrowed to the rows that can reach the kernel disc from the
-        // strip's 16 columns, |x - i - 1/2| < sqrt(h^2 - dy^2) + 1 with dy the
-        // smallest distance of a column centre from y.  The 1-pixel margin
-        // keeps every skipped pixel at u >= 1 up to FP32 rounding (W = 0, or
-        // below 1e-20 relative).
+        // strip's 16 columns, |x - i - 1/2| < sqrt(h^2 - dy^2) + 1/4 with dy
+        // the smallest distance of a column centre from y.  The quarter-pixel
+        // margin keeps every skipped pixel at u >= 1 up to FP32 rounding
+        // (W = 0, or below 1e-20 relative).
         if (threadIdx.x < cnt * 4) {
             const int b = threadIdx.x >> 2, w = threadIdx.x & 3;
             const int c0 = tj0 + 16 * w;
@@ -368,8 +368,8 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
                     const float xc = (float)s_x[b], yc = (float)s_y[b], hf = (float)s_h[b];
                     const float dy = fmaxf(0.0f, fmaxf((float)c0 + 0.5f - yc,
                                                        yc - ((float)c0 + 15.5f)));
-                    if (dy < hf + 1.0f) {
-                        const float hx = sqrtf(fmaxf(hf * hf - dy * dy, 0.0f)) + 1.0f;
+                    if (dy < hf + 0.25f) {
+                        const float hx = sqrtf(fmaxf(hf * hf - dy * dy, 0.0f)) + 0.25f;
                         const int ra = (int)floorf(xc - 0.5f - hx) - ti0;
                         const int rb = (int)ceilf(xc - 0.5f + hx) - ti0;
                         klo = (ra >> 2) > klo ? (ra >> 2) : klo;
@@ -401,7 +401,10 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
             // kGStep row groups per wave-uniform test, so their LDS reads and
             // dependency chains overlap.  A group outside [klo, khi] adds
             // exact zeros (a = 0 outside the footprint, u >= 1 outside the
-            // row range), so the image is the same as with one group per test.
+            // row range), so the image is the same as with one group per test
+            // (evaluating only the groups in the range, with the chunks at
+            // the range's ends taken one group at a time, was slower: 272 ->
+            // 314 ms on the cluster set).
 #pragma unroll
             for (int k0 = 0; k0 < kGRows; k0 += kGStep) {
                 if (k0 + kGStep - 1 < klo || k0 > khi)

@@ -121,46 +121,130 @@ __global__ __launch_bounds__(256) void k3d_setup(const T *__restrict__ pos,
 
 // ---------------------------------------------------------------------------
 // Explicit normalisation: one wave per particle.  The voxel weights are
-// formed as the brick gather forms them (s = (x offset^2 + y offset^2) / h^2
-// + (z offset / h)^2 in the same FP32 operations, shape_row with the overlap
-// product), so the sum and the deposited weights agree.
+// formed as the brick gather forms them (s = (x offset / h)^2 +
+// ((y offset / h)^2 + (z offset / h)^2) in the same FP32 operations,
+// shape_row with the x overlap's root, the y and z overlaps as an FP64
+// factor), so the sum and the deposited weights agree.
 // ---------------------------------------------------------------------------
 
+constexpr int kNX3 = 128;  // x terms tabulated per pass (per wave, in LDS)
+
+struct NRow3 {
+    int off;  // first position of the row in the wave's (j, k) list
+    int kl;   // first k of the row, relative to the box
+};
+
+__device__ __forceinline__ int norm_incl_scan(int v, int ln)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o, kWave);
+        v += ln >= o ? t : 0;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1)
+        v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
 // This lane's share of the sum over the box [ia, ib] x [ja, jb] x [ka, kb]
-// (any integers; voxels outside the grid follow the same formulas).  The
-// (j, k) plane is flattened over the lanes, 64 at a time; x rows are taken 64
-// at a time with the row terms broadcast by v_readlane.  Fixed per-lane
-// order: reproducible.
+// (any integers; voxels outside the grid follow the same formulas).  Only
+// voxels within h + 1 voxel of the centre are visited (the others have
+// u >= 1 in FP32 too, and weight 0); a fallback footprint keeps its whole
+// box.  Per 64 rows j the (j, k) pairs inside that ball's section are listed
+// (per-row k ranges, wave prefix sum) and taken 64 at a time, a pair per
+// lane; each 64 pairs run over the x range of their union, with the x terms
+// read from the wave's LDS table.  Fixed per-lane order: reproducible.
 template <int K>
 __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t ia, int64_t ib,
                                             int64_t ja, int64_t jb, int64_t ka, int64_t kb,
-                                            int ln)
+                                            int ln, float2 *xt, NRow3 *rows)
 {
-    const int64_t kw = kb - ka + 1;
-    const int64_t njk = (jb - ja + 1) * kw;
+    const float xc = (float)r.x, yc = (float)r.y, zc = (float)r.z;
+    const float hr = (float)r.h + 1.0f;
+    const bool all = r.fb != 0;
     double acc = 0.0;
-    for (int64_t fc = 0; fc < njk; fc += kWave) {
-        const int64_t f = fc + ln;
-        const int64_t dj = f / kw, dk = f - dj * kw;
-        const double fj = (double)(ja + dj), fk = (double)(ka + dk);
-        const float ys = (float)(coff1(r.y, fj) * hinv), zs = (float)(coff1(r.z, fk) * hinv);
-        const float ay = fmaxf((float)overlap1(r.y, r.h, fj), 0.0f);
-        const float az = fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
-        const float yd2 = ys * ys, zd2 = zs * zs;
-        const float ayz = f < njk ? ay * az : 0.0f;
-        for (int64_t ic = ia; ic <= ib; ic += kWave) {
-            const double fi = (double)(ic + ln);
+    for (int64_t xa = ia; xa <= ib; xa += kNX3) {
+        const int64_t xb = ib < xa + kNX3 - 1 ? ib : xa + kNX3 - 1;
+        __builtin_amdgcn_wave_barrier();
+        for (int e = ln; e <= (int)(xb - xa); e += kWave) {
+            const double fi = (double)(xa + e);
             const float xs = (float)(coff1(r.x, fi) * hinv);
-            const float xd2 = xs * xs;
-            const float ax = fmaxf((float)overlap1(r.x, r.h, fi), 0.0f);
-            const int nrow = ib - ic + 1 < kWave ? (int)(ib - ic + 1) : kWave;
-            for (int rr = 0; rr < nrow; ++rr) {
-                const float xd2r =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xd2), rr));
-                const float axr =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), rr));
-                const float u = __builtin_amdgcn_sqrtf((xd2r + yd2) + zd2);
-                acc += (double)(shape_row<K>(u, 1.0f) * (axr * ayz));
+            xt[e] = make_float2(xs * xs,
+                                overlap_root<K>(fmaxf((float)overlap1(r.x, r.h, fi), 0.0f)));
+        }
+        for (int64_t jc = ja; jc <= jb; jc += kWave) {
+            const int64_t j = jc + ln;
+            int len = 0, kl = 0;
+            if (j <= jb) {
+                int64_t lo = ka, hi = kb;
+                if (!all) {
+                    const float dy = fabsf((float)j + 0.5f - yc);
+                    if (dy < hr) {
+                        const float wz = sqrtf(hr * hr - dy * dy);
+                        const int64_t a = (int64_t)floorf(zc - 0.5f - wz);
+                        const int64_t c = (int64_t)ceilf(zc - 0.5f + wz);
+                        lo = a > lo ? a : lo;
+                        hi = c < hi ? c : hi;
+                    } else {
+                        hi = lo - 1;
+                    }
+                }
+                if (hi >= lo) {
+                    len = (int)(hi - lo + 1);
+                    kl = (int)(lo - ka);
+                }
+            }
+            const int incl = norm_incl_scan(len, ln);
+            const int total = __shfl(incl, kWave - 1, kWave);
+            if (total == 0)
+                continue;
+            __builtin_amdgcn_wave_barrier();
+            rows[ln] = NRow3{incl - len, kl};
+            __builtin_amdgcn_wave_barrier();
+            int jr = -1, next = 0;
+            NRow3 cur = {0, 0};
+            for (int q0 = 0; q0 < total; q0 += kWave) {
+                const int q = q0 + ln;
+                float yz2 = 0.0f, wx = -1.0f;
+                double ayz = 0.0;
+                if (q < total) {
+                    while (q >= next) {
+                        ++jr;
+                        cur = rows[jr];
+                        next = jr + 1 < kWave ? rows[jr + 1].off : total;
+                    }
+                    const int64_t jj = jc + jr, kk = ka + cur.kl + (q - cur.off);
+                    const double fj = (double)jj, fk = (double)kk;
+                    const float ys = (float)(coff1(r.y, fj) * hinv);
+                    const float zs = (float)(coff1(r.z, fk) * hinv);
+                    yz2 = ys * ys + zs * zs;
+                    ayz = (double)fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) *
+                          (double)fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
+                    const float dy = (float)jj + 0.5f - yc, dz = (float)kk + 0.5f - zc;
+                    wx = sqrtf(fmaxf(hr * hr - dy * dy - dz * dz, 0.0f));
+                }
+                // the x range of the 64 pairs: the union of their sections
+                int64_t ra = xa, rb = xb;
+                if (!all) {
+                    const float wm = wave_maxf(wx);
+                    const int64_t a = (int64_t)floorf(xc - 0.5f - wm);
+                    const int64_t c = (int64_t)ceilf(xc - 0.5f + wm);
+                    ra = a > ra ? a : ra;
+                    rb = c < rb ? c : rb;
+                }
+                double col = 0.0;
+#pragma unroll 4
+                for (int64_t i = ra; i <= rb; ++i) {
+                    const float2 t = xt[i - xa];
+                    col += (double)shape_row<K>(__builtin_amdgcn_sqrtf(t.x + yz2), t.y);
+                }
+                acc += col * ayz;
             }
         }
     }
@@ -176,7 +260,7 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
 template <int K>
 __device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restrict__ list,
                                              int64_t npix, uint32_t flags,
-                                             Rec3 *__restrict__ rec)
+                                             Rec3 *__restrict__ rec, float2 *xt, NRow3 *rows)
 {
     const int ln = threadIdx.x % kWave;
     const int p = list[gw];
@@ -205,21 +289,21 @@ __device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restri
         const int64_t c0 = (int64_t)C0, c1 = (int64_t)C1;
         double out = 0.0;
         if (a0 < i0)
-            out += norm_box3<K>(r, hinv, a0, i0 - 1, b0, b1, c0, c1, ln);
+            out += norm_box3<K>(r, hinv, a0, i0 - 1, b0, b1, c0, c1, ln, xt, rows);
         if (a1 > i1)
-            out += norm_box3<K>(r, hinv, i1 + 1, a1, b0, b1, c0, c1, ln);
+            out += norm_box3<K>(r, hinv, i1 + 1, a1, b0, b1, c0, c1, ln, xt, rows);
         if (b0 < j0)
-            out += norm_box3<K>(r, hinv, i0, i1, b0, j0 - 1, c0, c1, ln);
+            out += norm_box3<K>(r, hinv, i0, i1, b0, j0 - 1, c0, c1, ln, xt, rows);
         if (b1 > j1)
-            out += norm_box3<K>(r, hinv, i0, i1, j1 + 1, b1, c0, c1, ln);
+            out += norm_box3<K>(r, hinv, i0, i1, j1 + 1, b1, c0, c1, ln, xt, rows);
         if (c0 < k0)
-            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, c0, k0 - 1, ln);
+            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, c0, k0 - 1, ln, xt, rows);
         if (c1 > k1)
-            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, k1 + 1, c1, ln);
+            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, k1 + 1, c1, ln, xt, rows);
         out = wave_sum(out);
         acc = mul_rn(mul_rn(r.h, r.h), r.h) * shape_ball_integral(K) - out;
     } else {
-        acc = wave_sum(norm_box3<K>(r, hinv, i0, i1, j0, j1, k0, k1, ln));
+        acc = wave_sum(norm_box3<K>(r, hinv, i0, i1, j0, j1, k0, k1, ln, xt, rows));
     }
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
@@ -229,10 +313,13 @@ __global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list
                                                 int64_t npix, uint32_t flags,
                                                 Rec3 *__restrict__ rec)
 {
+    __shared__ float2 s_xt[256 / kWave][kNX3];
+    __shared__ NRow3 s_rows[256 / kWave][kWave];
+    const int wv = threadIdx.x / kWave;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
          gw += nw)
-        k3d_norm_one<K>(gw, list, npix, flags, rec);
+        k3d_norm_one<K>(gw, list, npix, flags, rec, s_xt[wv], s_rows[wv]);
 }
 
 
