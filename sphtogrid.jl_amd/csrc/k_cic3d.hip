@@ -144,12 +144,26 @@ __device__ __forceinline__ int norm_incl_scan(int v, int ln)
     return v;
 }
 
-__device__ __forceinline__ float wave_maxf(float v)
+// max over the 64 lanes of a fully active wave of values >= 0 (as integers:
+// the bit patterns of non-negative floats order as the floats do), returned
+// wave-uniform: DPP within each 16-lane row, then the four rows
+template <int kCtrl>
+__device__ __forceinline__ int dpp_max(int v)
 {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1)
-        v = fmaxf(v, __shfl_xor(v, o, kWave));
-    return v;
+    const int o = __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, true);
+    return v > o ? v : o;
+}
+__device__ __forceinline__ float wave_max_nonneg(float x)
+{
+    int v = __float_as_int(x);
+    v = dpp_max<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v = dpp_max<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v = dpp_max<0x141>(v);  // row_half_mirror
+    v = dpp_max<0x140>(v);  // row_mirror
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    const int m01 = r0 > r1 ? r0 : r1, m23 = r2 > r3 ? r2 : r3;
+    return __int_as_float(m01 > m23 ? m01 : m23);
 }
 
 // This lane's share of the sum over the box [ia, ib] x [ja, jb] x [ka, kb]
@@ -161,34 +175,34 @@ __device__ __forceinline__ float wave_maxf(float v)
 // lane; each 64 pairs run over the x range of their union, with the x terms
 // read from the wave's LDS table.  Fixed per-lane order: reproducible.
 template <int K>
-__device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t ia, int64_t ib,
-                                            int64_t ja, int64_t jb, int64_t ka, int64_t kb,
-                                            int ln, float2 *xt, NRow3 *rows)
+__device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int ia, int ib, int ja,
+                                            int jb, int ka, int kb, int ln, float2 *xt,
+                                            NRow3 *rows)
 {
     const float xc = (float)r.x, yc = (float)r.y, zc = (float)r.z;
     const float hr = (float)r.h + 1.0f;
     const bool all = r.fb != 0;
     double acc = 0.0;
-    for (int64_t xa = ia; xa <= ib; xa += kNX3) {
-        const int64_t xb = ib < xa + kNX3 - 1 ? ib : xa + kNX3 - 1;
+    for (int xa = ia; xa <= ib; xa += kNX3) {
+        const int xb = ib < xa + kNX3 - 1 ? ib : xa + kNX3 - 1;
         __builtin_amdgcn_wave_barrier();
-        for (int e = ln; e <= (int)(xb - xa); e += kWave) {
+        for (int e = ln; e <= xb - xa; e += kWave) {
             const double fi = (double)(xa + e);
             const float xs = (float)(coff1(r.x, fi) * hinv);
             xt[e] = make_float2(xs * xs,
                                 overlap_root<K>(fmaxf((float)overlap1(r.x, r.h, fi), 0.0f)));
         }
-        for (int64_t jc = ja; jc <= jb; jc += kWave) {
-            const int64_t j = jc + ln;
+        for (int jc = ja; jc <= jb; jc += kWave) {
+            const int j = jc + ln;
             int len = 0, kl = 0;
             if (j <= jb) {
-                int64_t lo = ka, hi = kb;
+                int lo = ka, hi = kb;
                 if (!all) {
                     const float dy = fabsf((float)j + 0.5f - yc);
                     if (dy < hr) {
                         const float wz = sqrtf(hr * hr - dy * dy);
-                        const int64_t a = (int64_t)floorf(zc - 0.5f - wz);
-                        const int64_t c = (int64_t)ceilf(zc - 0.5f + wz);
+                        const int a = (int)floorf(zc - 0.5f - wz);
+                        const int c = (int)ceilf(zc - 0.5f + wz);
                         lo = a > lo ? a : lo;
                         hi = c < hi ? c : hi;
                     } else {
@@ -196,12 +210,12 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
                     }
                 }
                 if (hi >= lo) {
-                    len = (int)(hi - lo + 1);
-                    kl = (int)(lo - ka);
+                    len = hi - lo + 1;
+                    kl = lo - ka;
                 }
             }
             const int incl = norm_incl_scan(len, ln);
-            const int total = __shfl(incl, kWave - 1, kWave);
+            const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
             if (total == 0)
                 continue;
             __builtin_amdgcn_wave_barrier();
@@ -211,7 +225,7 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
             NRow3 cur = {0, 0};
             for (int q0 = 0; q0 < total; q0 += kWave) {
                 const int q = q0 + ln;
-                float yz2 = 0.0f, wx = -1.0f;
+                float yz2 = 0.0f, wx = 0.0f;
                 double ayz = 0.0;
                 if (q < total) {
                     while (q >= next) {
@@ -219,7 +233,7 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
                         cur = rows[jr];
                         next = jr + 1 < kWave ? rows[jr + 1].off : total;
                     }
-                    const int64_t jj = jc + jr, kk = ka + cur.kl + (q - cur.off);
+                    const int jj = jc + jr, kk = ka + cur.kl + (q - cur.off);
                     const double fj = (double)jj, fk = (double)kk;
                     const float ys = (float)(coff1(r.y, fj) * hinv);
                     const float zs = (float)(coff1(r.z, fk) * hinv);
@@ -230,17 +244,17 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
                     wx = sqrtf(fmaxf(hr * hr - dy * dy - dz * dz, 0.0f));
                 }
                 // the x range of the 64 pairs: the union of their sections
-                int64_t ra = xa, rb = xb;
+                int ra = xa, rb = xb;
                 if (!all) {
-                    const float wm = wave_maxf(wx);
-                    const int64_t a = (int64_t)floorf(xc - 0.5f - wm);
-                    const int64_t c = (int64_t)ceilf(xc - 0.5f + wm);
+                    const float wm = wave_max_nonneg(wx);
+                    const int a = (int)floorf(xc - 0.5f - wm);
+                    const int c = (int)ceilf(xc - 0.5f + wm);
                     ra = a > ra ? a : ra;
                     rb = c < rb ? c : rb;
                 }
                 double col = 0.0;
 #pragma unroll 4
-                for (int64_t i = ra; i <= rb; ++i) {
+                for (int i = ra; i <= rb; ++i) {
                     const float2 t = xt[i - xa];
                     col += (double)shape_row<K>(__builtin_amdgcn_sqrtf(t.x + yz2), t.y);
                 }
@@ -256,7 +270,8 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int64_t 
 // S2G_FLAG_EXACT_NORM), the particle's centre is in the grid and the part cut
 // off by the border is the smaller one, the sum is h^3 I_k minus the sum over
 // that part (x slabs, then y slabs of the kept x range, then z slabs of the
-// kept x and y ranges); otherwise the footprint in the grid is summed.
+// kept x and y ranges); otherwise the footprint in the grid is summed.  The
+// boxes go through one loop (one copy of the box sum in the kernel's code).
 template <int K>
 __device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restrict__ list,
                                              int64_t npix, uint32_t flags,
@@ -266,7 +281,7 @@ __device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restri
     const int p = list[gw];
     const Rec3 r = rec[p];
     const double hinv = r.fb ? 0.0 : 1.0 / r.h;
-    const int64_t i0 = r.i0, i1 = r.i1, j0 = r.j0, j1 = r.j1, k0 = r.k0, k1 = r.k1;
+    const int i0 = r.i0, i1 = r.i1, j0 = r.j0, j1 = r.j1, k0 = r.k0, k1 = r.k1;
     bool cut = false;
     double A0 = 0.0, A1 = 0.0, B0 = 0.0, B1 = 0.0, C0 = 0.0, C1 = 0.0;
     if (!r.fb && r.h >= closed_form_min_h3(K) && !(flags & S2G_FLAG_EXACT_NORM)) {
@@ -283,28 +298,34 @@ __device__ __forceinline__ void k3d_norm_one(int64_t gw, const int32_t *__restri
         cut = r.x >= 0.0 && r.x < np && r.y >= 0.0 && r.y < np && r.z >= 0.0 && r.z < np &&
               full - kept < kept;
     }
-    double acc;
-    if (cut) {
-        const int64_t a0 = (int64_t)A0, a1 = (int64_t)A1, b0 = (int64_t)B0, b1 = (int64_t)B1;
-        const int64_t c0 = (int64_t)C0, c1 = (int64_t)C1;
-        double out = 0.0;
-        if (a0 < i0)
-            out += norm_box3<K>(r, hinv, a0, i0 - 1, b0, b1, c0, c1, ln, xt, rows);
-        if (a1 > i1)
-            out += norm_box3<K>(r, hinv, i1 + 1, a1, b0, b1, c0, c1, ln, xt, rows);
-        if (b0 < j0)
-            out += norm_box3<K>(r, hinv, i0, i1, b0, j0 - 1, c0, c1, ln, xt, rows);
-        if (b1 > j1)
-            out += norm_box3<K>(r, hinv, i0, i1, j1 + 1, b1, c0, c1, ln, xt, rows);
-        if (c0 < k0)
-            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, c0, k0 - 1, ln, xt, rows);
-        if (c1 > k1)
-            out += norm_box3<K>(r, hinv, i0, i1, j0, j1, k1 + 1, c1, ln, xt, rows);
-        out = wave_sum(out);
-        acc = mul_rn(mul_rn(r.h, r.h), r.h) * shape_ball_integral(K) - out;
-    } else {
-        acc = wave_sum(norm_box3<K>(r, hinv, i0, i1, j0, j1, k0, k1, ln, xt, rows));
+    // a cut footprint's full box is at most twice the grid's: int holds it
+    const int a0 = (int)A0, a1 = (int)A1, b0 = (int)B0, b1 = (int)B1;
+    const int c0 = (int)C0, c1 = (int)C1;
+    double out = 0.0;
+    for (int sl = cut ? 0 : 6; sl < (cut ? 6 : 7); ++sl) {
+        int ia = i0, ib = i1, ja = j0, jb = j1, ka = k0, kb = k1;
+        if (sl < 2) {
+            ja = b0;
+            jb = b1;
+            ka = c0;
+            kb = c1;
+            ia = sl == 0 ? a0 : i1 + 1;
+            ib = sl == 0 ? i0 - 1 : a1;
+        } else if (sl < 4) {
+            ka = c0;
+            kb = c1;
+            ja = sl == 2 ? b0 : j1 + 1;
+            jb = sl == 2 ? j0 - 1 : b1;
+        } else if (sl < 6) {
+            ka = sl == 4 ? c0 : k1 + 1;
+            kb = sl == 4 ? k0 - 1 : c1;
+        }
+        if (ia > ib || ja > jb || ka > kb)
+            continue;
+        out += norm_box3<K>(r, hinv, ia, ib, ja, jb, ka, kb, ln, xt, rows);
     }
+    out = wave_sum(out);
+    const double acc = cut ? mul_rn(mul_rn(r.h, r.h), r.h) * shape_ball_integral(K) - out : out;
     if (ln == 0)
         rec[p].s = acc != 0.0 ? r.s / acc : 0.0;
 }
