@@ -308,16 +308,16 @@ __device__ __forceinline__ float hp_dist(const RingTerms &t, int64_t k)
     return hp_angle((float)dphi, t.s1, t.s2);
 }
 
-// W (shape, or 1 in the fallback) and the area fraction (pixel_weights.jl:6-8,
-// without the constant 1/(ang_pix*dx)^2 that cancels)
+// W = shape(d / theta) and the area fraction (pixel_weights.jl:6-8, without
+// the constant 1/(ang_pix*dx)^2 that cancels):
+// A = max(0, min(ang, |theta - (d - ang / 2)|)) / ang, with thh = theta +
+// ang / 2 formed once per particle (the clamp is one v_med3_f32)
 template <int K>
-__device__ __forceinline__ void hp_weights(float d, float theta, float inv_theta, float ang,
+__device__ __forceinline__ void hp_weights(float d, float thh, float inv_theta, float ang,
                                            float inv_ang, float &W, float &A)
 {
-    const float u = d * inv_theta;
-    W = shape_of<K, float>(u);
-    const float a = fabsf(theta - (d - 0.5f * ang));
-    A = fmaxf(0.0f, fminf(ang, a)) * inv_ang;
+    W = shape_row<K>(d * inv_theta, 1.0f);
+    A = __builtin_amdgcn_fmed3f(fabsf(thh - d), 0.0f, ang) * inv_ang;
 }
 
 // One ring of a work unit, as the pixel loop reads it (per-wave LDS table).
@@ -352,7 +352,7 @@ template <int K, typename F>
 __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, int ui, int ln,
                                                float ang, HpRing *tab, F &&f)
 {
-    const float theta = (float)r.theta, inv_theta = (float)(1.0 / r.theta);
+    const float thh = (float)r.theta + 0.5f * ang, inv_theta = (float)(1.0 / r.theta);
     const float inv_ang = 1.0f / ang;
     const double cosr = cos(r.theta);
     const int64_t r0 = (int64_t)r.ra + (int64_t)ui * kHpRingsPerUnit;
@@ -424,7 +424,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
         dphi = dphi >= (float)kPi ? dphi - (float)(2.0 * kPi) : dphi;
         const float d = hp_angle(dphi, cur.s1, cur.s2);
         float W, A;
-        hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+        hp_weights<K>(d, thh, inv_theta, ang, inv_ang, W, A);
         f((int64_t)cur.start + k, W, A);
     }
     __builtin_amdgcn_wave_barrier();
@@ -437,7 +437,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
         const RingTerms t = ring_terms(nside, cring, r);
         const float d = hp_dist(t, ck);
         float W, A;
-        hp_weights<K>(d, theta, inv_theta, ang, inv_ang, W, A);
+        hp_weights<K>(d, thh, inv_theta, ang, inv_ang, W, A);
         f(r.cpix, W, A);
     }
 }
@@ -711,7 +711,7 @@ struct HpPair {
     int ck;     // ring-local index of the centre pixel, or -1 if not added
     int fb;     // fallback: area weights only
     float s1, s2, base, step;
-    float theta, inv_theta;
+    float theta, inv_theta;  // theta + ang / 2 (hp_weights's thh), 1 / theta
     double scale, bq;
 };
 
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
                 }
             }
             q.fb = r.fb;
-            q.theta = (float)r.theta;
+            q.theta = (float)r.theta + 0.5f * ang;  // theta + ang / 2, as the norm pass
             q.inv_theta = (float)(1.0 / r.theta);
             q.scale = r.s;
             q.bq = ld(binq, p);
