@@ -498,7 +498,10 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
 // block's LDS window of the tile + 34 pixels (ds_add_f64).  The window is
 // added to the image once per item (FP64 atomics, only non-zero cells).
 // ---------------------------------------------------------------------------
-constexpr int kSmallThreads = 512;
+// threads per block: 8 waves; 16 for three or four planes, whose window
+// (104 or 139 KB of LDS) leaves room for one block per CU, so that the CU
+// still runs four waves per SIMD
+constexpr int small_threads(int np) { return np > 2 ? 1024 : 512; }
 // particles whose records a block loads into LDS at a time
 constexpr int kSmallBatch = 64;
 // pixels per lane of the largest small footprint (34 x 34 over 64 lanes)
@@ -651,11 +654,12 @@ __device__ __forceinline__ bool small_particle(const Rec2 &r, int e, int ln, int
 }
 
 template <typename T, int K, int NP>
-__global__ __launch_bounds__(kSmallThreads) void k2d_small_tile(
+__global__ __launch_bounds__(small_threads(NP)) void k2d_small_tile(
     const int4 *__restrict__ items, const int64_t *__restrict__ nitems,
     const int32_t *__restrict__ plist, const Rec2 *__restrict__ rec,
     const Map2D *__restrict__ maps, int nq, int plane0)
 {
+    constexpr int kSmallThreads = small_threads(NP);
     __shared__ double acc[NP][kSW * kSW];
     // per-wave row / column terms of the current particle: (offset^2, overlap)
     __shared__ float2 s_rt[kSmallThreads / kWave][kSmallSide2D];
@@ -889,7 +893,7 @@ static void small_tile_k(int np, dim3 g, const int4 *items, const int64_t *nitem
                          const int32_t *plist, const Rec2 *rec, const Map2D *maps, int nq,
                          int plane0, hipStream_t s)
 {
-    const dim3 b(kSmallThreads);
+    const dim3 b(small_threads(np));
     switch (np) {
     case 1: hipLaunchKernelGGL((k2d_small_tile<T, K, 1>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
     case 2: hipLaunchKernelGGL((k2d_small_tile<T, K, 2>), g, b, 0, s, items, nitems, plist, rec, maps, nq, plane0); break;
