@@ -190,7 +190,10 @@ __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ 
 }
 
 // one wave per chunk: sequential over the chunk's particles, lanes over the
-// particle's tiles (distinct per lane), so every list is in particle order
+// particle's tiles (distinct per lane), so every list is in particle order.
+// The records of 64 particles are loaded at once (a lane each) and each
+// particle's tile rectangle is broadcast by readlane, so the loads' latency
+// is paid once per 64 particles.
 template <typename R>
 __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__ large,
                                                      const R *__restrict__ rec,
@@ -205,19 +208,28 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
     for (int t = threadIdx.x; t < tm.ntiles; t += kWave)
         cursor[t] = (int)off[tm.cbase + (int64_t)t * tm.nch + ch.k];
     __syncthreads();
-    for (int64_t l = ch.l0; l < ch.l1; ++l) {
-        const int32_t p = large[l];
-        const R r = rec[p];
-        int a0, a1, b0, b1, c0, c1;
-        tile_rect(r, a0, a1, b0, b1, c0, c1);
-        const int wb = b1 - b0 + 1, wc = c1 - c0 + 1;
-        const int ntr = (a1 - a0 + 1) * wb * wc;
-        for (int k = threadIdx.x; k < ntr; k += kWave) {
-            const int a = a0 + k / (wb * wc);
-            const int rem = k % (wb * wc);
-            const int b = b0 + rem / wc, c = c0 + rem % wc;
-            const int slot = atomicAdd(&cursor[(a * tm.nt + b) * tm.ntc + c], 1);
-            pairs[slot] = p;
+    for (int64_t lb = ch.l0; lb < ch.l1; lb += kWave) {
+        const int n = ch.l1 - lb < kWave ? (int)(ch.l1 - lb) : kWave;
+        int32_t pl = 0;
+        int ra0 = 0, ra1 = -1, rb0 = 0, rb1 = -1, rc0 = 0, rc1 = -1;
+        if ((int)threadIdx.x < n) {
+            pl = large[lb + threadIdx.x];
+            tile_rect(rec[pl], ra0, ra1, rb0, rb1, rc0, rc1);
+        }
+        for (int e = 0; e < n; ++e) {
+            const int32_t p = __builtin_amdgcn_readlane(pl, e);
+            const int a0 = __builtin_amdgcn_readlane(ra0, e), a1 = __builtin_amdgcn_readlane(ra1, e);
+            const int b0 = __builtin_amdgcn_readlane(rb0, e), b1 = __builtin_amdgcn_readlane(rb1, e);
+            const int c0 = __builtin_amdgcn_readlane(rc0, e), c1 = __builtin_amdgcn_readlane(rc1, e);
+            const int wb = b1 - b0 + 1, wc = c1 - c0 + 1;
+            const int ntr = (a1 - a0 + 1) * wb * wc;
+            for (int k = threadIdx.x; k < ntr; k += kWave) {
+                const int a = a0 + k / (wb * wc);
+                const int rem = k % (wb * wc);
+                const int b = b0 + rem / wc, c = c0 + rem % wc;
+                const int slot = atomicAdd(&cursor[(a * tm.nt + b) * tm.ntc + c], 1);
+                pairs[slot] = p;
+            }
         }
     }
 }
