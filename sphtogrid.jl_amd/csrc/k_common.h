@@ -7,7 +7,9 @@
 namespace s2g {
 
 constexpr int kPartBlock = 256;
-// large-list entries per counting-sort chunk (one LDS histogram per chunk)
+// largest usual count of large-list entries per counting-sort chunk (one LDS
+// histogram per chunk, one wave per chunk in the fill); the host sizes each
+// map's chunks from 256 up to this, larger only to bound the counters
 constexpr int kPairChunk = 4096;
 // per-workgroup LDS bound of the tile histograms (ints): tiles of one map
 constexpr int kMaxTiles = 16384;

@@ -133,7 +133,7 @@ void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *large
 
 // ---------------------------------------------------------------------------
 // (tile, particle) pair lists: a counting sort with deterministic order.
-// Chunks are map-local runs of at most kPairChunk large-list entries; map m's
+// Chunks are map-local runs of large-list entries (sized per map); map m's
 // counts are tile-major [tile][chunk] at cnt[cbase], so one exclusive scan
 // over all maps yields every list offset, ordered by (map, tile, particle).
 // Local tiles are t = (a * nt + b) * ntc + c over the map's tile grid

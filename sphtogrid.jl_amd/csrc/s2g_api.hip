@@ -138,13 +138,20 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
                                               std::to_string(kMaxTiles) + ")");
         max_tiles = tm[m].ntiles > max_tiles ? tm[m].ntiles : max_tiles;
         const int64_t l0 = lbase[m], l1 = lbase[m + 1];
-        const int nch = (int)((l1 - l0 + kPairChunk - 1) / kPairChunk);
+        // chunk size: about 32 chunks (one wave each in the fill) per CU,
+        // 256..kPairChunk entries, grown while the map's counters (tiles x
+        // chunks) would pass 2^25
+        int64_t csz = (l1 - l0 + 32 * (int64_t)c->ncu - 1) / (32 * (int64_t)c->ncu);
+        csz = csz < 256 ? 256 : csz > kPairChunk ? kPairChunk : (csz + 63) / 64 * 64;
+        while ((int64_t)tm[m].ntiles * ((l1 - l0 + csz - 1) / csz) > (1 << 25))
+            csz *= 2;
+        const int nch = (int)((l1 - l0 + csz - 1) / csz);
         tm[m].cbase = cb;
         tm[m].nch = nch;
         cb += (int64_t)tm[m].ntiles * nch;
         for (int k = 0; k < nch; ++k) {
-            const int64_t a = l0 + (int64_t)k * kPairChunk;
-            const int64_t e = a + kPairChunk < l1 ? a + kPairChunk : l1;
+            const int64_t a = l0 + (int64_t)k * csz;
+            const int64_t e = a + csz < l1 ? a + csz : l1;
             chunks.push_back(PairChunk{m, k, (int32_t)a, (int32_t)e});
         }
     }
