@@ -460,13 +460,18 @@ __global__ __launch_bounds__(256, 3) void k2d_gather_wide(const int4 *__restrict
 // fixed-order sum of a tile's item partials -> image planes (plain stores).
 // Every pixel of every tile of the pass is written (zero where no large
 // particle reaches it), which initialises the images for the small scatter.
+constexpr int kFinSplit = 16;
+
 template <int NP>
 __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ partial,
                                                     const int64_t *__restrict__ tile_items,
                                                     const Map2D *__restrict__ maps, int nmaps,
                                                     int plane0)
 {
-    const int t = blockIdx.x;
+    // kFinSplit blocks per tile, 4096 / kFinSplit pixels each, so that a pass
+    // of few tiles (a small map) still spreads the sums over the CUs
+    const int t = blockIdx.x / kFinSplit;
+    const int px0 = (blockIdx.x % kFinSplit) * (kGT * kGT / kFinSplit);
     const int mi = find_entry(maps, nmaps, t, [](const Map2D &e) { return e.tbase; });
     const int ntj = maps[mi].ntj;
     const int64_t npix = maps[mi].npix;
@@ -476,13 +481,14 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
     const int64_t b = tile_items ? tile_items[2 * t] : 0;
     const int64_t k = tile_items ? tile_items[2 * t + 1] : 0;
     const int64_t nd = npix * npix;
-    for (int px = threadIdx.x; px < kGT * kGT; px += 256) {
+    for (int px = px0 + threadIdx.x; px < px0 + kGT * kGT / kFinSplit; px += 256) {
         const int i = ti0 + px / kGT, j = tj0 + px % kGT;
         if (i >= npix || j >= npix)
             continue;
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
             double s = 0.0;
+#pragma unroll 4
             for (int64_t it = 0; it < k; ++it)
                 s += partial[((b + it) * NP + q) * kGT * kGT + px];
             image[(int64_t)(plane0 + q) * nd + (int64_t)i * npix + j] = s;
@@ -863,7 +869,7 @@ void launch_2d_gather(const GatherArgs &g, hipStream_t s)
 
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
 {
-    const dim3 grid((unsigned)g.ntiles), block(256);
+    const dim3 grid((unsigned)(g.ntiles * kFinSplit)), block(256);
     switch (g.np) {
     case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
     case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;

@@ -514,23 +514,29 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
 }
 
 // fixed-order sum of a brick's item partials -> image planes (plain stores)
+constexpr int kFinSplit3 = 16;
+
 template <int NP>
 __global__ __launch_bounds__(256) void k3d_finalize(const double *__restrict__ partial,
                                                     const int64_t *__restrict__ tile_items,
                                                     int nb, int64_t npix, int plane0,
                                                     double *__restrict__ image)
 {
-    const int t = blockIdx.x;
+    // kFinSplit3 blocks per brick, so that a grid of few bricks still
+    // spreads the sums over the CUs
+    const int t = blockIdx.x / kFinSplit3;
+    const int v0 = (blockIdx.x % kFinSplit3) * (kBT * kBT * kBT / kFinSplit3);
     const int bi0 = (t / (nb * nb)) * kBT, bj0 = ((t / nb) % nb) * kBT, bk0 = (t % nb) * kBT;
     const int64_t b = tile_items[2 * t], k = tile_items[2 * t + 1];
     const int64_t nd = npix * npix * npix;
-    for (int v = threadIdx.x; v < kBT * kBT * kBT; v += 256) {
+    for (int v = v0 + threadIdx.x; v < v0 + kBT * kBT * kBT / kFinSplit3; v += 256) {
         const int i = bi0 + (v >> 8), j = bj0 + ((v >> 4) & 15), kz = bk0 + (v & 15);
         if (i >= npix || j >= npix || kz >= npix)
             continue;
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
             double s = 0.0;
+#pragma unroll 4
             for (int64_t itm = 0; itm < k; ++itm)
                 s += partial[((b + itm) * NP + q) * (kBT * kBT * kBT) + v];
             image[(int64_t)(plane0 + q) * nd + ((int64_t)i * npix + j) * npix + kz] = s;
@@ -676,7 +682,7 @@ void launch_3d_gather(const Gather3Args &g, hipStream_t s)
 
 void launch_3d_finalize(const Gather3Args &g, hipStream_t s)
 {
-    const dim3 grid((unsigned)g.ntiles), blk(256);
+    const dim3 grid((unsigned)(g.ntiles * kFinSplit3)), blk(256);
     if (g.np == 1)
         hipLaunchKernelGGL(k3d_finalize<1>, grid, blk, 0, s, g.partial, g.tile_items, g.nb,
                            g.npix, g.plane0, g.image);
