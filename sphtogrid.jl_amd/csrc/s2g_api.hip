@@ -139,10 +139,10 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
         max_tiles = tm[m].ntiles > max_tiles ? tm[m].ntiles : max_tiles;
         const int64_t l0 = lbase[m], l1 = lbase[m + 1];
         // chunk size: about 32 chunks (one wave each in the fill) per CU,
-        // 256..kPairChunk entries, grown while the map's counters (tiles x
+        // 64..kPairChunk entries, grown while the map's counters (tiles x
         // chunks) would pass 2^25
         int64_t csz = (l1 - l0 + 32 * (int64_t)c->ncu - 1) / (32 * (int64_t)c->ncu);
-        csz = csz < 256 ? 256 : csz > kPairChunk ? kPairChunk : (csz + 63) / 64 * 64;
+        csz = csz < 64 ? 64 : csz > kPairChunk ? kPairChunk : (csz + 63) / 64 * 64;
         while ((int64_t)tm[m].ntiles * ((l1 - l0 + csz - 1) / csz) > (1 << 25))
             csz *= 2;
         const int nch = (int)((l1 - l0 + csz - 1) / csz);
@@ -404,7 +404,7 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         if (Gs < 256)
             Gs = 256;
         const int64_t max_items = nstiles + (ns + Gs - 1) / Gs;
-        if (max_items > ((int64_t)1 << 31) / 512)
+        if (max_items > ((int64_t)1 << 31) / 1024)  // blocks x threads (up to 1024)
             throw Error(S2G_EUNSUPPORTED, "too many small-footprint work items");
         int64_t *snit = (int64_t *)c->snitems.get(sizeof(int64_t) * (nstiles + 1));
         int4 *sitems = (int4 *)c->sitems.get(sizeof(int4) * max_items);
