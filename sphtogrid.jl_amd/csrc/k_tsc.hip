@@ -24,14 +24,60 @@ __device__ __forceinline__ double tsc_w(double d)
     return 0.0;
 }
 
+// nearest cell of particle p along each axis, as k_tsc forms it; false if
+// the particle is dropped (not finite, or far outside a non-periodic grid)
 template <typename T>
-__global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T *__restrict__ binq,
-                                             int64_t n, int64_t npix, double len2pix,
-                                             int periodic, double *__restrict__ grid)
+__device__ __forceinline__ bool tsc_cell(const T *__restrict__ pos, int64_t p, int64_t npix,
+                                         double len2pix, int periodic, int64_t c[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
+        const double fg = floor(g);
+        if (!(fg == fg) || ((fg < -2.0 || fg > (double)npix + 2.0) && !periodic))
+            return false;
+        c[d] = (int64_t)fg;
+    }
+    return true;
+}
+
+// Sort key: the 8^3 brick of the particle's nearest cell (row-major bricks;
+// dropped particles last), so that a wave's 27-cell stencils share cache
+// lines and the atomics stay in L2 instead of scattering over the grid.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tsc_keys(const T *__restrict__ pos, int64_t n,
+                                                  int64_t npix, double len2pix, int periodic,
+                                                  uint32_t *__restrict__ keys,
+                                                  int32_t *__restrict__ vals)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n)
         return;
+    const int64_t nb = (npix + 7) / 8;
+    int64_t c[3];
+    uint32_t key = (uint32_t)(nb * nb * nb);
+    if (tsc_cell(pos, p, npix, len2pix, periodic, c)) {
+        int64_t b[3];
+        for (int d = 0; d < 3; ++d) {
+            int64_t x = periodic ? ((c[d] % npix) + npix) % npix : c[d];
+            x = x < 0 ? 0 : x > npix - 1 ? npix - 1 : x;
+            b[d] = x / 8;
+        }
+        key = (uint32_t)((b[0] * nb + b[1]) * nb + b[2]);
+    }
+    keys[p] = key;
+    vals[p] = (int32_t)p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T *__restrict__ binq,
+                                             int64_t n, int64_t npix, double len2pix,
+                                             int periodic, const int32_t *__restrict__ order,
+                                             double *__restrict__ grid)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int64_t p = order ? (int64_t)order[t] : t;
     const int64_t nd = npix * npix * npix;
     int64_t c[3];
     double w[3][3];
@@ -80,18 +126,41 @@ __global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T 
     }
 }
 
+int tsc_key_bits(int64_t npix)
+{
+    const int64_t nb = (npix + 7) / 8;
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) <= nb * nb * nb)
+        ++bits;
+    return bits;
+}
+
+void launch_tsc_keys(int dtype, const void *pos, int64_t n, int64_t npix, double len2pix,
+                     int periodic, uint32_t *keys, int32_t *vals, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const dim3 g((unsigned)((n + 255) / 256)), blk(256);
+    if (dtype == S2G_F32)
+        hipLaunchKernelGGL(k_tsc_keys<float>, g, blk, 0, s, (const float *)pos, n, npix,
+                           len2pix, periodic, keys, vals);
+    else
+        hipLaunchKernelGGL(k_tsc_keys<double>, g, blk, 0, s, (const double *)pos, n, npix,
+                           len2pix, periodic, keys, vals);
+}
+
 void launch_tsc(int dtype, const void *pos, const void *binq, int64_t n, int64_t npix,
-                double len2pix, int periodic, double *grid, hipStream_t s)
+                double len2pix, int periodic, const int32_t *order, double *grid, hipStream_t s)
 {
     if (n == 0)
         return;
     const dim3 g((unsigned)((n + 255) / 256)), blk(256);
     if (dtype == S2G_F32)
         hipLaunchKernelGGL(k_tsc<float>, g, blk, 0, s, (const float *)pos, (const float *)binq,
-                           n, npix, len2pix, periodic, grid);
+                           n, npix, len2pix, periodic, order, grid);
     else
         hipLaunchKernelGGL(k_tsc<double>, g, blk, 0, s, (const double *)pos,
-                           (const double *)binq, n, npix, len2pix, periodic, grid);
+                           (const double *)binq, n, npix, len2pix, periodic, order, grid);
 }
 
 }  // namespace s2g

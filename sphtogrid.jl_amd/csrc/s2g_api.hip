@@ -1142,6 +1142,26 @@ int s2g_healpix_deposit_device(s2g_ctx *c, int dtype, const void *d_pos, const v
     });
 }
 
+// the particles ordered by the 8^3 brick of their nearest cell (stable radix
+// sort in the context's buffers), or null when there is nothing to deposit
+static const int32_t *tsc_order(s2g_ctx *c, int dtype, const void *d_pos, int64_t n,
+                                int64_t npix, double len2pix, uint32_t flags, hipStream_t s)
+{
+    if (n == 0)
+        return nullptr;
+    if (n >= (int64_t)INT32_MAX)
+        throw Error(S2G_EUNSUPPORTED, "more than 2^31 - 1 particles for the TSC deposit");
+    uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * n);
+    int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * n);
+    launch_tsc_keys(dtype, d_pos, n, npix, len2pix, (flags & S2G_FLAG_PERIODIC) ? 1 : 0, keys,
+                    vals, s);
+    const int end_bit = tsc_key_bits(npix);
+    const size_t tb = sort_pairs_tmp_bytes(n, end_bit);
+    void *stmp = c->sort_tmp.get(tb ? tb : 1);
+    sort_pairs(stmp, tb, keys, keys + n, vals, vals + n, n, end_bit, s);
+    return vals + n;
+}
+
 int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t n,
               const s2g_params *params, uint32_t flags, double *grid_out, s2g_stats *stats)
 {
@@ -1172,8 +1192,9 @@ int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t
         double *grid = (double *)c->image.get(sizeof(double) * 2 * nd);
         S2G_HIP(hipEventRecord(E[1], s));
         S2G_HIP(hipMemsetAsync(grid, 0, sizeof(double) * 2 * nd, s));
+        const int32_t *order = tsc_order(c, dtype, dpos, n, npix, params->len2pix, flags, s);
         launch_tsc(dtype, dpos, dq, n, npix, params->len2pix,
-                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, grid, s);
+                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, order, grid, s);
         S2G_HIP(hipGetLastError());
         S2G_HIP(hipEventRecord(E[2], s));
         S2G_HIP(hipEventSynchronize(E[2]));
@@ -1198,8 +1219,9 @@ int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin
         const int64_t npix = params->npixels;
         const int64_t nd = npix * npix * npix;
         S2G_HIP(hipMemsetAsync(d_grid_out, 0, sizeof(double) * 2 * nd, s));
+        const int32_t *order = tsc_order(c, dtype, d_pos, n, npix, params->len2pix, flags, s);
         launch_tsc(dtype, d_pos, d_bin_q, n, npix, params->len2pix,
-                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, d_grid_out, s);
+                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, order, d_grid_out, s);
         S2G_HIP(hipGetLastError());
         return S2G_OK;
     });
