@@ -51,8 +51,9 @@
  *     work queued there afterwards waits for it).  The deposits
  *     (s2g_deposit_device, s2g_healpix_deposit_device, s2g_deposit_batch)
  *     size their work lists from device-side counts, so they synchronise the
- *     stream and return with the result complete; s2g_reduce_device and
- *     s2g_tsc3d_device only enqueue.
+ *     stream and return with the result complete; so does s2g_tsc3d_device
+ *     (its particle order is sorted in the context's buffers);
+ *     s2g_reduce_device only enqueues.
  */
 #ifndef SPH2GRID_H
 #define SPH2GRID_H
@@ -196,7 +197,8 @@ int s2g_healpix_deposit_device(s2g_ctx *ctx, int dtype, const void *d_pos,
 int s2g_tsc3d(s2g_ctx *ctx, int dtype, const void *pos, const void *bin_q,
               int64_t n, const s2g_params *params, uint32_t flags,
               double *grid_out, s2g_stats *stats);
-/* Same on device pointers (asynchronous on `stream`). */
+/* Same on device pointers, on `stream`; returns with the grid complete (the
+ * particles are ordered by grid brick in the context's buffers first). */
 int s2g_tsc3d_device(s2g_ctx *ctx, int dtype, const void *d_pos,
                      const void *d_bin_q, int64_t n, const s2g_params *params,
                      uint32_t flags, double *d_grid_out, void *stream);

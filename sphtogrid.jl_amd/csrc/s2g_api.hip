@@ -1223,6 +1223,9 @@ int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin
         launch_tsc(dtype, d_pos, d_bin_q, n, npix, params->len2pix,
                    (flags & S2G_FLAG_PERIODIC) ? 1 : 0, order, d_grid_out, s);
         S2G_HIP(hipGetLastError());
+        // the particle order lives in the context's buffers: finish before
+        // another call (possibly on another stream) reuses them
+        S2G_HIP(hipStreamSynchronize(s));
         return S2G_OK;
     });
 }
