@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256) void k3d_setup(const T *__restrict__ pos,
 // Explicit normalisation: one wave per particle.  The voxel weights are
 // formed as the brick gather forms them (s = (x offset / h)^2 +
 // ((y offset / h)^2 + (z offset / h)^2) in the same FP32 operations,
-// shape_row with the x overlap's root, the y and z overlaps as an FP64
-// factor), so the sum and the deposited weights agree.
+// shape_row with the x overlap's root times the y overlap in FP32, the z
+// overlap as an FP64 factor), so the sum and the deposited weights agree.
 // ---------------------------------------------------------------------------
 
 constexpr int kNX3 = 128;  // x terms tabulated per pass (per wave, in LDS)
@@ -225,8 +225,8 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int ia, 
             NRow3 cur = {0, 0};
             for (int q0 = 0; q0 < total; q0 += kWave) {
                 const int q = q0 + ln;
-                float yz2 = 0.0f, wx = 0.0f;
-                double ayz = 0.0;
+                float yz2 = 0.0f, wx = 0.0f, ay = 0.0f;
+                double azd = 0.0;
                 if (q < total) {
                     while (q >= next) {
                         ++jr;
@@ -238,8 +238,8 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int ia, 
                     const float ys = (float)(coff1(r.y, fj) * hinv);
                     const float zs = (float)(coff1(r.z, fk) * hinv);
                     yz2 = ys * ys + zs * zs;
-                    ayz = (double)fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) *
-                          (double)fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
+                    ay = fmaxf((float)overlap1(r.y, r.h, fj), 0.0f);
+                    azd = (double)fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
                     const float dy = (float)jj + 0.5f - yc, dz = (float)kk + 0.5f - zc;
                     wx = sqrtf(fmaxf(hr * hr - dy * dy - dz * dz, 0.0f));
                 }
@@ -256,9 +256,9 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int ia, 
 #pragma unroll 4
                 for (int i = ra; i <= rb; ++i) {
                     const float2 t = xt[i - xa];
-                    col += (double)shape_row<K>(__builtin_amdgcn_sqrtf(t.x + yz2), t.y);
+                    col += (double)(shape_row<K>(__builtin_amdgcn_sqrtf(t.x + yz2), t.y) * ay);
                 }
-                acc += col * ayz;
+                acc += col * azd;
             }
         }
     }
@@ -353,9 +353,10 @@ __global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list
 // and overlaps, and one bit per sub-brick that can reach the kernel sphere
 // (voxel-centre box within h + 1 voxel of the centre, inside the footprint
 // box); a wave evaluates only its sub-bricks with that bit set.  The x
-// overlap enters shape_row as ax^(1/n), the y and z overlaps fold into the
-// FP64 plane multipliers, so per voxel the work equals the 2D gather's per
-// pixel.
+// overlap enters shape_row as ax^(1/n), the y overlap multiplies the FP32
+// weight and the z overlap (fixed per lane) folds into the FP64 plane
+// multipliers: one instruction per voxel more than the 2D gather's per
+// pixel, and a per-particle set-up of one FP64 multiplier per plane.
 // ---------------------------------------------------------------------------
 constexpr int kGBatch3 = 32;
 // sub-bricks per uniform test (1, 2 or 4: one bi per test)
@@ -463,21 +464,22 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
                 continue;
             const float2 zt = s_tab[2][b][4 * wv + dk];
             const double az = (double)zt.y;
-            // the y and z overlaps are common to a lane's voxels of one bj:
-            // fold them into the FP64 plane multipliers (for the bj pairs
-            // that a test of this particle will evaluate)
-            float yz[4];
-            double mult[4][NP];
+            // the z overlap is common to the lane's voxels: fold it into the
+            // FP64 plane multipliers; the y overlap of each bj multiplies the
+            // FP32 weight (for the bj pairs a test of this particle will
+            // evaluate)
+            double mult[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                mult[q] = s_mult[q][b] * az;
+            float yz[4], ay[4];
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 if (!(wm & (((1u << kG3) - 1u) * 0x1111u << (bj / kG3 * kG3))))
                     continue;
                 const float2 yt = s_tab[1][b][4 * bj + dj];
                 yz[bj] = yt.x + zt.x;
-                const double f = az * (double)yt.y;
-#pragma unroll
-                for (int q = 0; q < NP; ++q)
-                    mult[bj][q] = s_mult[q][b] * f;
+                ay[bj] = yt.y;
             }
 #pragma unroll
             for (int m = 0; m < 16; m += kG3) {
@@ -489,15 +491,14 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
 #pragma unroll
                 for (int g = 0; g < kG3; ++g) {
                     const int bj = (m + g) & 3;
-                    wk[g] = shape_row<K>(__builtin_amdgcn_sqrtf(xt.x + yz[bj]), xt.y);
+                    wk[g] = shape_row<K>(__builtin_amdgcn_sqrtf(xt.x + yz[bj]), xt.y) * ay[bj];
                 }
 #pragma unroll
                 for (int g = 0; g < kG3; ++g) {
-                    const int bj = (m + g) & 3;
                     const double f = (double)wk[g];
 #pragma unroll
                     for (int q = 0; q < NP; ++q)
-                        acc[m + g][q] = fma(f, mult[bj][q], acc[m + g][q]);
+                        acc[m + g][q] = fma(f, mult[q], acc[m + g][q]);
                 }
             }
         }
