@@ -25,7 +25,8 @@ __device__ __forceinline__ double tsc_w(double d)
 }
 
 // nearest cell of particle p along each axis, as k_tsc forms it; false if
-// the particle is dropped (not finite, or far outside a non-periodic grid)
+// the particle is dropped (a non-finite coordinate, or far outside a
+// non-periodic grid)
 template <typename T>
 __device__ __forceinline__ bool tsc_cell(const T *__restrict__ pos, int64_t p, int64_t npix,
                                          double len2pix, int periodic, int64_t c[3])
@@ -33,7 +34,7 @@ __device__ __forceinline__ bool tsc_cell(const T *__restrict__ pos, int64_t p, i
     for (int d = 0; d < 3; ++d) {
         const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
         const double fg = floor(g);
-        if (!(fg == fg) || ((fg < -2.0 || fg > (double)npix + 2.0) && !periodic))
+        if (!isfinite(fg) || ((fg < -2.0 || fg > (double)npix + 2.0) && !periodic))
             return false;
         c[d] = (int64_t)fg;
     }
@@ -85,8 +86,8 @@ __global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T 
     for (int d = 0; d < 3; ++d) {
         const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
         const double fg = floor(g);
-        if (!(fg == fg) || fg < -2.0 || fg > (double)npix + 2.0) {
-            if (!periodic || !(fg == fg))
+        if (!isfinite(fg) || fg < -2.0 || fg > (double)npix + 2.0) {
+            if (!periodic || !isfinite(fg))
                 bad = true;
         }
         c[d] = bad ? 0 : (int64_t)fg;

@@ -35,6 +35,21 @@ def test_tsc_matches_oracle(s2g, orc, periodic, dtype):
     np.testing.assert_allclose(g, ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tsc_odd_grid_and_nonfinite(s2g, orc, periodic):
+    # 20^3 is not a whole number of the 8^3 bricks the particles are ordered
+    # by; particles with a NaN or infinite coordinate are dropped
+    N = 20
+    r = np.random.default_rng(11)
+    pos = r.uniform(-0.7, 0.7, (4000, 3))
+    q = r.uniform(-1, 3, 4000)
+    bad = pos.copy()
+    bad[:3] = [[np.nan, 0.1, 0.2], [0.1, np.inf, 0.0], [-np.inf, 0.0, 0.3]]
+    g = _tsc_gpu(s2g, bad, q, N, periodic, np.float64)
+    ref = orc.tsc_3d(np.ascontiguousarray(pos[3:]), q[3:], N, float(N), periodic)
+    np.testing.assert_allclose(g, ref, rtol=1e-12, atol=1e-12)
+
+
 def test_tsc_cell_centre(s2g):
     N = 8
     c = np.array([3, 4, 5])
