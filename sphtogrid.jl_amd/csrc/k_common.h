@@ -42,7 +42,7 @@ struct PairChunk {
 };
 
 // fill == false: count + scan into cnt (ncnt int64, the last is the total)
-// fill == true : fill pairs (large-list indices) from the scanned cnt
+// fill == true : fill pairs (particle indices) from the scanned cnt
 template <typename R>
 void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
