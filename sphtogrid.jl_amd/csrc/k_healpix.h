@@ -47,10 +47,10 @@ void launch_hp_entry_offsets(const uint32_t *keys, int64_t ne, int64_t nside, in
 // count: 4 Nside + 1 int64 (entries touching each ring; ends 0)
 void launch_hp_ring_count(const uint32_t *keys, const int64_t *eoff, int64_t nside,
                           int64_t *count, hipStream_t s);
-// loff: exclusive scan of count; list: the particles per ring
+// loff: exclusive scan of count; vals: the sorted entries' particles; list:
+// the particles per ring
 void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_t *eoff,
-                         const int64_t *loff, const int2 *units, int64_t nu, int64_t nside,
-                         int32_t *list, hipStream_t s);
+                         const int64_t *loff, int64_t nside, int32_t *list, hipStream_t s);
 // writes every pixel of both maps
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
                             const HpRec *rec, double *map, double *wmap, hipStream_t s);

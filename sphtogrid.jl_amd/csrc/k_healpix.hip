@@ -621,7 +621,7 @@ __global__ void k_hp_entry_keys(const int2 *__restrict__ units, int64_t nu,
         }
     }
     keys[e] = key;
-    vals[e] = (int32_t)e;
+    vals[e] = p;  // the sort is stable: equal keys keep the entries' order
 }
 
 // off[t] = first sorted entry with first ring >= t, t in [0, 4 Nside]
@@ -675,13 +675,13 @@ __global__ __launch_bounds__(256) void k_hp_ring_count(const uint32_t *__restric
 }
 
 // list[loff[iz] ...] = the particles of the entries touching ring iz, in
-// sorted order (block-wide ordered compaction)
+// sorted order (block-wide ordered compaction; the sorted values are the
+// entries' particles)
 __global__ __launch_bounds__(256) void k_hp_ring_fill(const uint32_t *__restrict__ keys,
                                                       const int32_t *__restrict__ vals,
                                                       const int64_t *__restrict__ eoff,
                                                       const int64_t *__restrict__ loff,
-                                                      const int2 *__restrict__ units,
-                                                      int64_t nu, int64_t nside,
+                                                      int64_t nside,
                                                       int32_t *__restrict__ list)
 {
     __shared__ int s_wc[256 / kWave];
@@ -704,8 +704,7 @@ __global__ __launch_bounds__(256) void k_hp_ring_fill(const uint32_t *__restrict
             total += s_wc[w];
         }
         if (take)
-            list[out + before + (int)__popcll(bt & below)] =
-                hp_entry_particle(vals[i], nu, units);
+            list[out + before + (int)__popcll(bt & below)] = vals[i];
         out += total;
         __syncthreads();
     }
@@ -955,11 +954,10 @@ void launch_hp_ring_count(const uint32_t *keys, const int64_t *eoff, int64_t nsi
 }
 
 void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_t *eoff,
-                         const int64_t *loff, const int2 *units, int64_t nu, int64_t nside,
-                         int32_t *list, hipStream_t s)
+                         const int64_t *loff, int64_t nside, int32_t *list, hipStream_t s)
 {
     hipLaunchKernelGGL(k_hp_ring_fill, dim3((unsigned)(4 * nside - 1)), dim3(256), 0, s, keys,
-                       vals, eoff, loff, units, nu, nside, list);
+                       vals, eoff, loff, nside, list);
 }
 
 void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,

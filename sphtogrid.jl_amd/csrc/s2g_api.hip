@@ -1036,7 +1036,7 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
             if (npairs >= (int64_t)INT32_MAX)
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
             int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
-            launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, units, nu, nside, list, s);
+            launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, nside, list, s);
             launch_hp_ring_deposit(a, list, off, rec, d_map, d_wmap, s);
         } else {
             launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
