@@ -8,12 +8,14 @@ namespace s2g {
 // per-particle record of the all-sky deposit
 struct HpRec {
     double z0, phi0, theta0, sth0, xa;  // direction (cos, phi, colatitude, sin) and 1/sin
-    double theta;                       // projected hsml asin(h/dx), main.jl:182
+    double ch0, sh0;                    // cos, sin of theta0 / 2
+    double theta, cosr;                 // projected hsml asin(h/dx) (main.jl:182), its cos
     double s;                           // area*w*dz, then the final pixel scale
     int64_t cpix;                       // RING pixel containing the particle
     int64_t unit0;                      // first work unit
     int32_t ra, rb;                     // ring range incl. polar caps fully inside
     int32_t irmin, irmax;               // rings with a partial phi interval
+    int32_t cring, ck;                  // ring (1-based) and ring index of cpix
     int32_t fb, ok;
 };
 

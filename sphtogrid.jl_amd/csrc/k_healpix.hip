@@ -55,6 +55,28 @@ __device__ __forceinline__ double hp_ring_z(int64_t nside, int64_t ring)
     return -1.0 + (double)(r * r) * fact2;
 }
 
+// z, 1 - z and 1 + z of a ring, the small one of the last two formed
+// without cancellation (polar rings)
+__device__ __forceinline__ void hp_ring_zz(int64_t nside, int64_t ring, double &z, double &omz,
+                                           double &opz)
+{
+    const double fact2 = 4.0 / (double)(12 * nside * nside);
+    if (ring < nside) {
+        omz = (double)(ring * ring) * fact2;
+        z = 1.0 - omz;
+        opz = 2.0 - omz;
+    } else if (ring <= 3 * nside) {
+        z = (double)(2 * nside - ring) * (2.0 / (3.0 * (double)nside));
+        omz = 1.0 - z;
+        opz = 1.0 + z;
+    } else {
+        const int64_t r = 4 * nside - ring;
+        opz = (double)(r * r) * fact2;
+        z = -1.0 + opz;
+        omz = 2.0 - opz;
+    }
+}
+
 __device__ __forceinline__ int64_t hp_ring_above(int64_t nside, double z)
 {
     const double az = fabs(z);
@@ -170,6 +192,8 @@ __global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
         return;
     HpRec r;
     r.z0 = r.phi0 = r.theta0 = r.sth0 = r.xa = r.theta = r.s = 0.0;
+    r.ch0 = r.sh0 = r.cosr = 0.0;
+    r.cring = r.ck = 0;
     r.ra = 1;
     r.rb = 0;
     r.irmin = 1;
@@ -201,6 +225,9 @@ __global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
         if (ph0 < 0.0)
             ph0 += 2.0 * kPi;
         r.theta0 = th0;
+        r.ch0 = cos(0.5 * th0);
+        r.sh0 = sin(0.5 * th0);
+        r.cosr = cos(theta);
         r.phi0 = ph0;
         r.z0 = cos(th0);
         r.sth0 = sin(th0);
@@ -215,6 +242,10 @@ __global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
         r.ra = (rlat1 <= 0.0 && irmin > 1) ? 1 : (int32_t)irmin;
         r.rb = (rlat2 >= kPi && irmax + 1 < 4 * nside) ? (int32_t)(4 * nside - 1) : (int32_t)irmax;
         r.cpix = hp_ang2pix(nside, r.z0, ph0);
+        int64_t cring, ck;
+        hp_pix2ring(nside, r.cpix, cring, ck);
+        r.cring = (int32_t)cring;
+        r.ck = (int32_t)ck;
         r.ok = 1;
     }
     rec[p] = r;
@@ -254,11 +285,13 @@ __device__ __forceinline__ RingTerms ring_terms(int64_t nside, int64_t iz, const
     RingTerms t;
     double shift;
     hp_ring_info(nside, iz, t.start, t.nr, shift);
-    const double z = hp_ring_z(nside, iz);
-    const double th = acos(z);
-    const double sh = sin(0.5 * (th - r.theta0));
+    // sin((theta_ring - theta0) / 2) from the half-angle terms of both
+    // colatitudes: sin(theta_ring / 2) = sqrt((1 - z) / 2), cos = sqrt((1 + z) / 2)
+    double z, omz, opz;
+    hp_ring_zz(nside, iz, z, omz, opz);
+    const double sh = sqrt(0.5 * omz) * r.ch0 - sqrt(0.5 * opz) * r.sh0;
     t.s1 = (float)(sh * sh);
-    t.s2 = (float)(sqrt((1.0 - z) * (1.0 + z)) * r.sth0);
+    t.s2 = (float)(sqrt(omz * opz) * r.sth0);
     t.dphi_pix = 2.0 * kPi / (double)t.nr;
     t.phase = shift * t.dphi_pix - r.phi0;
     return t;
@@ -361,12 +394,11 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
 {
     const float thh = (float)r.theta + 0.5f * ang, inv_theta = (float)(1.0 / r.theta);
     const float inv_ang = 1.0f / ang;
-    const double cosr = cos(r.theta);
+    const double cosr = r.cosr;
     const int64_t r0 = (int64_t)r.ra + (int64_t)ui * kHpRingsPerUnit;
     const int64_t r1 = r0 + kHpRingsPerUnit - 1 < r.rb ? r0 + kHpRingsPerUnit - 1 : r.rb;
     const int nrings = r1 >= r0 ? (int)(r1 - r0 + 1) : 0;
-    int64_t cring = 0, ck = 0;
-    hp_pix2ring(nside, r.cpix, cring, ck);
+    const int64_t cring = r.cring, ck = r.ck;
 
     // phase 1: one ring per lane
     HpRing e;
@@ -576,7 +608,8 @@ __device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int6
 {
     if (!r.ok || r.s == 0.0)
         return false;
-    hp_pix2ring(nside, r.cpix, cring, ck);
+    cring = r.cring;
+    ck = r.ck;
     cring_extra = r.rb < r.ra || cring < r.ra || cring > r.rb;
     return true;
 }
@@ -769,7 +802,7 @@ __global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
                     bb = nr - 1;
                     any = true;
                 } else {
-                    any = hp_ring_interval(nside, iz, r.z0, r.xa, cos(r.theta), r.phi0, a, bb);
+                    any = hp_ring_interval(nside, iz, r.z0, r.xa, r.cosr, r.phi0, a, bb);
                 }
             }
             const RingTerms t = ring_terms(nside, iz, r);
