@@ -8,7 +8,12 @@
 //   k_hp_units      (particle, group of rings) work units from a scan
 //   k_hp_norm       per unit: partial sums of W*A and A (FP64, fixed order)
 //   k_hp_finish     per particle: ordered sum of its units -> scale, fallback
-//   k_hp_deposit    per unit: W*A*scale into the two maps (FP64 atomics)
+//   k_hp_entry_*    (Nside <= 1024) work units keyed by first and last ring,
+//   k_hp_ring_count radix-sorted; per ring the particles that touch it,
+//   k_hp_ring_fill  counted, scanned and written in sorted order (no atomics)
+//   k_hp_ring_deposit  one block per ring: W*A*scale of every particle on the
+//                   ring into both map rows held in LDS, then plain stores
+//   k_hp_deposit    (Nside > 1024) per unit, FP64 global atomics
 //
 // Pixel distances use the haversine form: the ring's colatitude difference
 // and the pixel's azimuth difference are formed in FP64, the remaining
