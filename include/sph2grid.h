@@ -136,7 +136,12 @@ const char *s2g_version(void);
 
 /* Deposit (cic_mapping_2D for dims == 2, cic_mapping_3D for dims == 3).
  * Host pointers.  image_out receives the flat unreduced image
- * (npix^dims x (nq+1), or x 2 in 3D where nq must be 1). */
+ * (npix^dims x (nq+1), or x 2 in 3D where nq must be 1).
+ * A particle with hsml <= 0 or a NaN coordinate or hsml deposits nothing.
+ * (The reference writes NaN into an h = 0 particle's pixel in 2D, where
+ * area_norm = 0 * Inf, cic_2D.jl:187-222; it deposits nothing for h = 0 in
+ * 3D, and it throws InexactError on NaN in floor(Integer, x - hsml),
+ * cic_shared.jl:46-52.) */
 int s2g_deposit(s2g_ctx *ctx, int dims, int dtype, const void *pos,
                 const void *hsml, const void *m, const void *rho,
                 const void *bin_q, int nq, const void *weights, int64_t n,

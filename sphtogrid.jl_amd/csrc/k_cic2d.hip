@@ -183,7 +183,7 @@ __device__ __forceinline__ double norm_rect(const Rec2 &r, double hinv, bool dis
         const int64_t j = jc + ln;
         const double fj = (double)j;
         const float ys = (float)(coff1(r.y, fj) * hinv);
-        const float yd2 = ys * ys;
+        const float yd2 = fmul_rn(ys, ys);  // rounded, as the gather's column table
         const float ay = j <= jb ? fmaxf((float)overlap1(r.y, r.h, fj), 0.0f) : 0.0f;
         double col = 0.0;
         for (int64_t ic = ra; ic <= rb; ic += kWave) {

@@ -237,7 +237,9 @@ __device__ __forceinline__ double norm_box3(const Rec3 &r, double hinv, int ia, 
                     const double fj = (double)jj, fk = (double)kk;
                     const float ys = (float)(coff1(r.y, fj) * hinv);
                     const float zs = (float)(coff1(r.z, fk) * hinv);
-                    yz2 = ys * ys + zs * zs;
+                    // squares rounded one by one, as the gather's tables hold
+                    // them: the sum is then the gather's to the bit
+                    yz2 = fmul_rn(ys, ys) + fmul_rn(zs, zs);
                     ay = fmaxf((float)overlap1(r.y, r.h, fj), 0.0f);
                     azd = (double)fmaxf((float)overlap1(r.z, r.h, fk), 0.0f);
                     const float dy = (float)jj + 0.5f - yc, dz = (float)kk + 0.5f - zc;

@@ -214,12 +214,34 @@ __host__ __device__ inline double closed_form_min_h3(int kernel)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 helpers that must not be contracted into FMAs (cic_shared.jl:85-100
-// rounds x*len2pix and then +0.5*Npix).
+// Products and sums rounded on their own (cic_shared.jl:85-100 rounds
+// x*len2pix and then +0.5*Npix).  HIP's __dmul_rn / __fmul_rn are plain
+// operators here and hipcc contracts them into FMAs; an operation emitted
+// under `fp contract(off)` carries no contract flag, so it stays separate
+// after inlining.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double sub_rn(double a, double b) { return __dsub_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double sub_rn(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+// an FP32 square summed with another must equal the sum of the squares held
+// in a table (no FMA between them)
+__device__ __forceinline__ float fmul_rn(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
 
 // overlap of [x-h, x+h] with pixel [i, i+1] (cic_shared.jl:60-62)
 __device__ __forceinline__ double overlap1(double x, double h, double fi)

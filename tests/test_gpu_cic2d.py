@@ -257,3 +257,99 @@ def test_cluster_scaled(s2g, orc):
                           return_both_maps=True, nworkers=8)
     print("cluster scaled:", st, flush=True)
     _check_flat(gpu, ref, "cluster 2^17")
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_degenerate_particles_deposit_nothing(s2g, orc, dims):
+    """h = 0, h < 0 and NaN h deposit nothing; the other particles' image is
+    unchanged.  In 3D the reference does the same for h = 0 (the footprint's
+    overlap volume is 0, so every pix_weight is 0, cic_3D.jl:168-188).  In 2D
+    it differs on purpose: area = (2h)^2 = 0 and dz = Inf make area_norm NaN,
+    which poisons the particle's pixel (cic_2D.jl:187-222, reproduced by the
+    oracle).  A NaN h throws InexactError in floor(Integer, x - hsml)
+    (cic_shared.jl:46-52)."""
+    npix, n = 48 if dims == 2 else 24, 400
+    x, h, m, rho, w = _random_particles(n, npix, seed=31 + dims, hmin_px=0.3, hmax_px=12.0)
+    q = np.random.default_rng(5).uniform(0.5, 2.0, n)
+    bad = np.array([3, 50, 51, 200])
+    h_bad = h.copy()
+    h_bad[bad] = [0.0, -1.0 / npix, np.nan, 0.0]
+    good = np.setdiff1d(np.arange(n), bad)
+    kw = dict(npix=npix, len2pix=float(npix), kernel=s2g.WendlandC4(dims), dimensions=dims)
+    gpu = s2g.deposit(x, h_bad, m, rho, q, w, **kw)
+    gpu_good = s2g.deposit(x[good], h[good], m[good], rho[good], q[good], w[good], **kw)
+    assert np.isfinite(gpu).all()
+    assert rel_l1(gpu, gpu_good) <= 1e-14
+    f = orc.cic_mapping_2D if dims == 2 else orc.cic_mapping_3D
+    ref = f(x[good], h[good], m[good], rho[good], q[good], w[good], npix, float(npix),
+            "wendlandc4", dims, True)
+    _check_flat(gpu, ref, f"degenerate {dims}D")
+    # the oracle follows the reference for h = 0 inside the image: NaN in
+    # every plane in 2D, nothing in 3D
+    ref_h0 = f(x[:4], np.zeros(4), m[:4], rho[:4], q[:4], w[:4], npix, float(npix),
+               "wendlandc4", dims, True)
+    if dims == 2:
+        assert np.isnan(ref_h0).any(axis=0).all()
+    else:
+        assert not ref_h0.any()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_lattice_aligned_ties(s2g, orc, dims):
+    """Particles on pixel centres, edges and corners with h a whole or half
+    number of pixels: footprint bounds floor(x -/+ h) land exactly on
+    integers, neighbouring centres sit at u = 1 exactly, and h = 0.5 on a
+    corner has every centre at u = 1 (the fallback).  Float64 inputs, so
+    every coordinate is exact."""
+    npix = 40 if dims == 2 else 20
+    offs = [0.0, 0.5, 0.25]  # pixel corner, pixel centre, quarter
+    hs = [0.5, 1.0, 1.5, 2.0, 4.0, 7.5] + ([16.0] if dims == 2 else [])
+    rows = []
+    r = np.random.default_rng(77 + dims)
+    for hp in hs:
+        for o in offs:
+            c = r.integers(2, npix - 2, size=3) + o  # pixel coordinates
+            rows.append((c, hp))
+    n = len(rows)
+    x = np.array([(c / npix) - 0.5 for c, _ in rows])
+    h = np.array([hp / npix for _, hp in rows])
+    m, rho, w = (r.uniform(0.5, 2.0, n) for _ in range(3))
+    q = r.uniform(0.5, 2.0, n)
+    for kname in ("cubic", "wendlandc6"):
+        gpu = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix),
+                          kernel=_kernel(s2g, kname, dims), dimensions=dims)
+        f = orc.cic_mapping_2D if dims == 2 else orc.cic_mapping_3D
+        ref = f(x, h, m, rho, q, w, npix, float(npix), kname, dims, True)
+        _check_flat(gpu, ref, f"ties {dims}D {kname}")
+        # with every footprint summed explicitly, the deposit conserves to
+        # FP64 rounding; the default closed-form sum for large unclipped 3D
+        # footprints is within RTOL_CONS (measured 2.6e-10 here)
+        ex = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix),
+                         kernel=_kernel(s2g, kname, dims), dimensions=dims, exact_norm=True)
+        _check_flat(ex, ref, f"ties {dims}D {kname} exact")
+        assert abs(ex[:, -1].sum() / ref[:, -1].sum() - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_explicit_norm_matches_deposit(s2g, dims):
+    """With every footprint summed explicitly (exact_norm), the normalisation
+    pass forms each weight with the gather's FP32 operations in the gather's
+    order, so every large particle deposits exactly w m / (rho / len2pix^3)
+    (x len2pix in 3D) up to FP64 rounding; clipped and unclipped footprints,
+    offsets / h not representable in FP32.  A different rounding of one FP32
+    term (an FMA in one pass and not the other) shows as ~1e-9."""
+    npix = 96 if dims == 2 else 40
+    n = 300
+    r = np.random.default_rng(900 + dims)
+    x = r.uniform(-0.5, 0.5, (n, 3))
+    h = (r.uniform(18.0, 45.0, n) if dims == 2 else r.uniform(4.6, 16.0, n)) / npix
+    m, rho, w = (r.uniform(0.5, 2.0, n) for _ in range(3))
+    q = r.uniform(0.5, 2.0, n)
+    st = {}
+    gpu = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix),
+                      kernel=s2g.WendlandC6(dims), dimensions=dims, exact_norm=True, stats=st)
+    assert st["n_large"] >= 0.9 * n  # a few clipped boxes fall to the small path
+    exp = (w * m / rho).sum() * float(npix) ** (3 if dims == 2 else 4)
+    cons = abs(gpu[:, -1].sum() / exp - 1)
+    print(f"{dims}D explicit-norm conservation {cons:.2e}", flush=True)
+    assert cons <= 1e-12
