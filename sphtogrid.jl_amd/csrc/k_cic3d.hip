@@ -548,63 +548,135 @@ __global__ __launch_bounds__(256) void k3d_finalize(const double *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Small footprints (box <= 8^3): one wave per particle, FP64 weights,
-// wave-reduced normalisation, FP64 atomics (cic_3D.jl:158-188).
+// Small footprints (box <= 8^3)
 // ---------------------------------------------------------------------------
+
+// Flat small scatter: a wave takes 64 consecutive particles of the list and
+// lays their footprint voxels end to end (exclusive prefix sum of the box
+// sizes), 64 voxels per step, so a particle of 1 - 8 voxels no longer holds
+// a whole wave.  In each 64-voxel window the particles that start there are
+// flagged in LDS; a ballot of the flags and a lane-prefix count (mbcnt) give
+// every lane its particle.  Pass 0 sums wk V per particle into LDS (one wave
+// sum when the window holds one particle, LDS FP64 atomics otherwise); pass 1
+// recomputes the same products and deposits wk V * vol w len2pix / sum with
+// FP64 atomics (cic_3D.jl:158-188).
+struct Small3Wave {
+    double x[kWave], y[kWave], z[kWave], h[kWave], hinv[kWave], s[kWave], bq[kWave];
+    double sum[kWave];
+    int off[kWave], i0[kWave], j0[kWave], k0[kWave], jk[kWave], kw[kWave], flag[kWave];
+    float invjk[kWave], invkw[kWave];
+};
+
 template <typename T, int K>
-__device__ __forceinline__ void k3d_small_atomic_one(int64_t gw, const int32_t *__restrict__ list, int64_t ns, const Rec3 *__restrict__ rec,
-        const T *__restrict__ binq, int64_t npix, double *__restrict__ image)
+__global__ __launch_bounds__(256) void k3d_small_flat(const int32_t *__restrict__ list, int64_t ns,
+                                                      const Rec3 *__restrict__ rec,
+                                                      const T *__restrict__ binq, int64_t npix,
+                                                      double *__restrict__ image)
 {
+    __shared__ Small3Wave s_w[256 / kWave];
     const int ln = threadIdx.x % kWave;
-    const int p = list[gw];
-    const Rec3 r = rec[p];
-    const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1, kw = r.k1 - r.k0 + 1;
-    const int jk = jw * kw, fp = iw * jk;
-    const double hinv = r.fb ? 0.0 : 1.0 / r.h;
-    const float invjk = 1.0f / (float)jk, invkw = 1.0f / (float)kw;
-    double sum = 0.0;
-    for (int f = ln; f < fp; f += kWave) {
-        const int di = (int)(((float)f + 0.5f) * invjk);
-        const int rem = f - di * jk;
-        const int dj = (int)(((float)rem + 0.5f) * invkw);
-        const int dk = rem - dj * kw;
-        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj), fk = (double)(r.k0 + dk);
-        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj), zd = coff1(r.z, fk);
-        const double V = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj) * overlap1(r.z, r.h, fk);
-        const double u = sqrt(xd * xd + yd * yd + zd * zd) * hinv;
-        sum += shape_of<K, double>(u) * V;
-    }
-    sum = wave_sum(sum);
-    if (!(sum != 0.0))
-        return;
-    const double sc = r.s / sum;
-    const double bq = ld(binq, p);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    Small3Wave &W = s_w[wv];
     const int64_t nd = npix * npix * npix;
-    for (int f = ln; f < fp; f += kWave) {
-        const int di = (int)(((float)f + 0.5f) * invjk);
-        const int rem = f - di * jk;
-        const int dj = (int)(((float)rem + 0.5f) * invkw);
-        const int dk = rem - dj * kw;
-        const double fi = (double)(r.i0 + di), fj = (double)(r.j0 + dj), fk = (double)(r.k0 + dk);
-        const double xd = coff1(r.x, fi), yd = coff1(r.y, fj), zd = coff1(r.z, fk);
-        const double V = overlap1(r.x, r.h, fi) * overlap1(r.y, r.h, fj) * overlap1(r.z, r.h, fk);
-        const double u = sqrt(xd * xd + yd * yd + zd * zd) * hinv;
-        const double pw = shape_of<K, double>(u) * V * sc;
-        if (pw != 0.0) {
-            const int64_t idx = ((int64_t)(r.i0 + di) * npix + (r.j0 + dj)) * npix + (r.k0 + dk);
-            unsafeAtomicAdd(&image[nd + idx], pw);
-            unsafeAtomicAdd(&image[idx], bq * pw);
+    const int64_t ngroups = (ns + kWave - 1) / kWave;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + wv; g < ngroups; g += nw) {
+        const int64_t first = g * kWave;
+        const int cnt = ns - first < kWave ? (int)(ns - first) : kWave;
+        int fp = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (ln < cnt) {
+            const int p = list[first + ln];
+            const Rec3 r = rec[p];
+            const int iw = r.i1 - r.i0 + 1, jw = r.j1 - r.j0 + 1, kw = r.k1 - r.k0 + 1;
+            fp = iw * jw * kw;
+            W.x[ln] = r.x;
+            W.y[ln] = r.y;
+            W.z[ln] = r.z;
+            W.h[ln] = r.h;
+            W.hinv[ln] = r.fb ? 0.0 : 1.0 / r.h;
+            W.s[ln] = r.s;
+            W.bq[ln] = (double)ld(binq, p);
+            W.sum[ln] = 0.0;
+            W.i0[ln] = r.i0;
+            W.j0[ln] = r.j0;
+            W.k0[ln] = r.k0;
+            W.jk[ln] = jw * kw;
+            W.kw[ln] = kw;
+            W.invjk[ln] = 1.0f / (float)(jw * kw);
+            W.invkw[ln] = 1.0f / (float)kw;
+        }
+        const int incl = norm_incl_scan(fp, ln);
+        W.off[ln] = incl - fp;
+        const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+        __builtin_amdgcn_wave_barrier();
+        for (int pass = 0; pass < 2; ++pass) {
+            int bprev = -1;  // the particle holding voxel e0 - 1
+            for (int e0 = 0; e0 < total; e0 += kWave) {
+                W.flag[ln] = 0;
+                __builtin_amdgcn_wave_barrier();
+                const int bj = bprev + 1 + ln;
+                if (bj < cnt) {
+                    const int pj = W.off[bj] - e0;  // >= 0: particle bj starts at or after e0
+                    if (pj < kWave)
+                        W.flag[pj] = 1;
+                }
+                __builtin_amdgcn_wave_barrier();
+                const uint64_t mask = __ballot(W.flag[ln] != 0);
+                const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                const int b = bprev + below + (int)((mask >> ln) & 1u);
+                const int e = e0 + ln;
+                double wvol = 0.0;
+                if (e < total) {
+                    const int f = e - W.off[b];
+                    const int jk = W.jk[b], kw = W.kw[b];
+                    const int di = (int)(((float)f + 0.5f) * W.invjk[b]);
+                    const int rem = f - di * jk;
+                    const int dj = (int)(((float)rem + 0.5f) * W.invkw[b]);
+                    const int dk = rem - dj * kw;
+                    const double x = W.x[b], y = W.y[b], z = W.z[b], h = W.h[b];
+                    const double fi = (double)(W.i0[b] + di), fj = (double)(W.j0[b] + dj),
+                                 fk = (double)(W.k0[b] + dk);
+                    const double xd = coff1(x, fi), yd = coff1(y, fj), zd = coff1(z, fk);
+                    const double V = overlap1(x, h, fi) * overlap1(y, h, fj) * overlap1(z, h, fk);
+                    const double u = sqrt(xd * xd + yd * yd + zd * zd) * W.hinv[b];
+                    wvol = shape_of<K, double>(u) * V;
+                    if (pass == 1) {
+                        const double pw = wvol * W.s[b];
+                        if (pw != 0.0) {
+                            const int64_t idx =
+                                ((int64_t)(W.i0[b] + di) * npix + (W.j0[b] + dj)) * npix +
+                                (W.k0[b] + dk);
+                            unsafeAtomicAdd(&image[nd + idx], pw);
+                            unsafeAtomicAdd(&image[idx], W.bq[b] * pw);
+                        }
+                    }
+                }
+                if (pass == 0) {
+                    if ((mask & ~1ull) == 0) {
+                        // one particle in the window
+                        const double t = wave_sum(wvol);
+                        if (ln == 0)
+                            W.sum[b] += t;
+                    } else if (wvol != 0.0) {
+                        unsafeAtomicAdd(&W.sum[b], wvol);
+                    }
+                }
+                bprev += __popcll(mask);
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (pass == 0) {
+                // the scale of pass 1: vol w len2pix / sum (0 when nothing
+                // in the box has weight)
+                if (ln < cnt) {
+                    const double sm = W.sum[ln];
+                    W.s[ln] = sm != 0.0 ? W.s[ln] / sm : 0.0;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
-}
-template <typename T, int K>
-__global__ __launch_bounds__(256) void k3d_small_atomic(const int32_t *__restrict__ list, int64_t ns, const Rec3 *__restrict__ rec,
-        const T *__restrict__ binq, int64_t npix, double *__restrict__ image)
-{
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ns;
-         gw += nw)
-        k3d_small_atomic_one<T, K>(gw, list, ns, rec, binq, npix, image);
 }
 
 
@@ -697,8 +769,8 @@ void launch_3d_finalize(const Gather3Args &g, hipStream_t s)
 template <typename T>
 static void small3_t(const Small3Args &a, hipStream_t s)
 {
-    const dim3 g((unsigned)wave_blocks(a.ns, 256)), blk(256);
-    S2G_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k3d_small_atomic<T, KK>), g, blk, 0, s,
+    const dim3 g((unsigned)wave_blocks((a.ns + kWave - 1) / kWave, 256)), blk(256);
+    S2G_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k3d_small_flat<T, KK>), g, blk, 0, s,
                                                    a.list, a.ns, a.rec, (const T *)a.binq,
                                                    a.npix, a.image));
 }
