@@ -353,3 +353,58 @@ def test_explicit_norm_matches_deposit(s2g, dims):
     cons = abs(gpu[:, -1].sum() / exp - 1)
     print(f"{dims}D explicit-norm conservation {cons:.2e}", flush=True)
     assert cons <= 1e-12
+
+
+def test_deposit_argument_errors(s2g):
+    """s2g_deposit / s2g_reduce reject bad arguments with the documented
+    codes and a message in s2g_last_error, and the context stays usable."""
+    import ctypes as C
+    from sph2grid import _lib
+    ctx = s2g.default_context()
+    L = ctx.lib
+    n, npix = 8, 16
+    r = np.random.default_rng(3)
+    x = np.ascontiguousarray(r.uniform(-0.4, 0.4, (n, 3)))
+    h = np.full(n, 2.0 / npix)
+    one = np.ones(n)
+    img = np.zeros(2 * npix ** 3)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+
+    def call(dims=2, dtype=_lib.S2G_F64, pos=True, nq=1, nn=n, par="ok", kernel=0,
+             kdim=2, npx=npix, l2p=float(npix)):
+        prm = _lib.Params(npixels=npx, len2pix=l2p) if par == "ok" else None
+        return L.s2g_deposit(ctx.handle, dims, dtype, P(x) if pos else None, P(h), P(one),
+                             P(one), P(one), nq, P(one), nn,
+                             C.byref(prm) if prm is not None else None, kernel, kdim, 0,
+                             P(img), None)
+
+    cases = [
+        (dict(dims=4), _lib.S2G_EINVAL, "dims"),
+        (dict(nq=0), _lib.S2G_EINVAL, "nq"),
+        (dict(nq=65), _lib.S2G_EINVAL, "nq"),
+        (dict(dims=3, kdim=3, nq=2), _lib.S2G_EINVAL, "scalar"),
+        (dict(dtype=7), _lib.S2G_EINVAL, "dtype"),
+        (dict(kernel=99), _lib.S2G_EUNSUPPORTED, "kernel"),
+        (dict(kdim=1), _lib.S2G_EINVAL, "kernel_dim"),
+        (dict(nn=-1), _lib.S2G_EINVAL, "particle count"),
+        (dict(par=None), _lib.S2G_EINVAL, "params"),
+        (dict(npx=0), _lib.S2G_EINVAL, "npixels"),
+        (dict(l2p=float("nan")), _lib.S2G_EINVAL, "len2pix"),
+        (dict(pos=False), _lib.S2G_EINVAL, "NULL"),
+    ]
+    for kw, code, word in cases:
+        rc = call(**kw)
+        msg = L.s2g_last_error(ctx.handle).decode()
+        print(kw, rc, msg, flush=True)
+        assert rc == code, (kw, rc, msg)
+        assert word in msg, (kw, msg)
+    # the reduce entry point: bad dims and NULL image
+    out = np.zeros(npix * npix)
+    assert L.s2g_reduce(ctx.handle, 5, npix, 1, 0, P(img), P(out)) == _lib.S2G_EINVAL
+    assert L.s2g_reduce(ctx.handle, 2, npix, 1, 0, None, P(out)) == _lib.S2G_EINVAL
+    # still usable: a valid call after the errors matches the Python surface
+    assert call() == _lib.S2G_OK
+    ref = s2g.deposit(x, h, one, one, one, one, npix=npix, len2pix=float(npix),
+                      kernel=s2g.Cubic(2))
+    np.testing.assert_allclose(img[:2 * npix * npix], ref.ravel(order="F"), rtol=1e-12,
+                               atol=0)
