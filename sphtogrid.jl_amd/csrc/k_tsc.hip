@@ -69,13 +69,20 @@ __global__ __launch_bounds__(256) void k_tsc_keys(const T *__restrict__ pos, int
     vals[p] = (int32_t)p;
 }
 
+constexpr int kTscRows = 3;  // threads per particle
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T *__restrict__ binq,
                                              int64_t n, int64_t npix, double len2pix,
                                              int periodic, const int32_t *__restrict__ order,
                                              double *__restrict__ grid)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // three threads per particle, one per x slab of the 27-cell stencil:
+    // more atomics in flight than one thread issuing all 54 (2^22 particles
+    // on 512^3: 8.9 ms with one thread, 8.4 with three, 8.5 with nine)
+    const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = tt / kTscRows;
+    const int slab = (int)(tt - kTscRows * t);
     if (t >= n)
         return;
     const int64_t p = order ? (int64_t)order[t] : t;
@@ -97,32 +104,30 @@ __global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T 
     if (bad)
         return;
     const double q = ld(binq, p);
-    for (int a = -1; a <= 1; ++a) {
-        int64_t i = c[0] + a;
+    int64_t i = c[0] + slab - 1;
+    if (periodic)
+        i = ((i % npix) + npix) % npix;
+    else if (i < 0 || i >= npix)
+        return;
+    for (int b = -1; b <= 1; ++b) {
+        int64_t j = c[1] + b;
         if (periodic)
-            i = ((i % npix) + npix) % npix;
-        else if (i < 0 || i >= npix)
+            j = ((j % npix) + npix) % npix;
+        else if (j < 0 || j >= npix)
             continue;
-        for (int b = -1; b <= 1; ++b) {
-            int64_t j = c[1] + b;
+        const double wab = w[0][slab] * w[1][b + 1];
+        for (int e = -1; e <= 1; ++e) {
+            int64_t k = c[2] + e;
             if (periodic)
-                j = ((j % npix) + npix) % npix;
-            else if (j < 0 || j >= npix)
+                k = ((k % npix) + npix) % npix;
+            else if (k < 0 || k >= npix)
                 continue;
-            const double wab = w[0][a + 1] * w[1][b + 1];
-            for (int e = -1; e <= 1; ++e) {
-                int64_t k = c[2] + e;
-                if (periodic)
-                    k = ((k % npix) + npix) % npix;
-                else if (k < 0 || k >= npix)
-                    continue;
-                const double ww = wab * w[2][e + 1];
-                if (ww == 0.0)
-                    continue;
-                const int64_t idx = (i * npix + j) * npix + k;
-                unsafeAtomicAdd(&grid[idx], q * ww);
-                unsafeAtomicAdd(&grid[nd + idx], ww);
-            }
+            const double ww = wab * w[2][e + 1];
+            if (ww == 0.0)
+                continue;
+            const int64_t idx = (i * npix + j) * npix + k;
+            unsafeAtomicAdd(&grid[idx], q * ww);
+            unsafeAtomicAdd(&grid[nd + idx], ww);
         }
     }
 }
@@ -155,7 +160,7 @@ void launch_tsc(int dtype, const void *pos, const void *binq, int64_t n, int64_t
 {
     if (n == 0)
         return;
-    const dim3 g((unsigned)((n + 255) / 256)), blk(256);
+    const dim3 g((unsigned)((kTscRows * n + 255) / 256)), blk(256);
     if (dtype == S2G_F32)
         hipLaunchKernelGGL(k_tsc<float>, g, blk, 0, s, (const float *)pos, (const float *)binq,
                            n, npix, len2pix, periodic, order, grid);
