@@ -232,6 +232,36 @@ typedef struct s2g_map_desc {
 int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
                       int nmaps, void *stream, s2g_stats *stats);
 
+/* Test export (no counterpart in the reference; for bit-exact checks of the
+ * deposit's integer work against a CPU restatement).  Runs the setup, class
+ * partition, (tile, particle) counting sort and small-footprint bucketing of
+ * s2g_deposit on the same host arguments and returns, in host arrays:
+ *   box        n x 4 (2D: i0, i1, j0, j1) or n x 6 (3D: + k0, k1) int32, the
+ *              footprint of pix_index_min_max (cic_shared.jl:46-52) clamped to
+ *              the image; (0, -1, ...) for a skipped particle
+ *   cls        n int32: 0 skipped (zero quantity without calc_mean, empty
+ *              footprint, hsml <= 0 or NaN), 1 small (box side <= 34 px in 2D,
+ *              <= 8 in 3D), 2 large with the closed-form normalisation, 3 large
+ *              with an explicit footprint sum
+ *   fb         n int32: 1 iff no footprint pixel centre has u < 1 (the
+ *              fallback of calculate_weights, cic_2D.jl:49-69, cic_3D.jl:55-75)
+ *   tile_off   ntiles + 1 int64 (ntiles = ceil(npix/64)^2 in 2D, ceil(npix/16)^3
+ *              in 3D, row-major): the large particles whose box meets tile t are
+ *              pairs[tile_off[t] .. tile_off[t+1]), in particle order
+ *   pairs      the pair lists, copied when their total is <= pair_cap
+ *   small_off  2D: ceil(npix/32)^2 + 1 int64, small_order[small_off[t] ..
+ *              small_off[t+1]) are the small particles whose box starts in
+ *              32 x 32 tile t, in particle order; 3D: {0, nsmall}
+ *   small_order n int32 (the first nsmall are written)
+ *   counts     6 int64: nsmall, nlarge, nlarge_explicit, npairs, ntiles,
+ *              small tiles */
+int s2g_footprints(s2g_ctx *ctx, int dims, int dtype, const void *pos, const void *hsml,
+                   const void *m, const void *rho, const void *bin_q, int nq,
+                   const void *weights, int64_t n, const s2g_params *params, int kernel,
+                   uint32_t flags, int32_t *box, int32_t *cls, int32_t *fb, int64_t *tile_off,
+                   int32_t *pairs, int64_t pair_cap, int64_t *small_off, int32_t *small_order,
+                   int64_t *counts);
+
 /* Subfile reductions of distributed_cic_map / distributed_allsky_map: add
  * one subfile's map into a resident sum, skipping NaN and Inf values
  * (distributed_mapping/cic.jl:63-73, distributed_mapping/healpix.jl:50-58).

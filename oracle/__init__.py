@@ -136,6 +136,13 @@ def lib():
             L.orc_hp_query_disc_ring.restype = _i64
             L.orc_hp_query_disc_ring.argtypes = [_i64, C.c_double, C.c_double,
                                                  C.c_double, C.POINTER(_i64), _i64]
+            L.orc_footprints.restype = C.c_int
+            L.orc_footprints.argtypes = [C.c_int, _d, _d, _d, C.c_int, _i64, _i64, C.c_double,
+                                         C.c_int, C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+            L.orc_hp_particle_weights.restype = _i64
+            L.orc_hp_particle_weights.argtypes = [_d, C.c_double, _i64, C.c_int, C.c_int, _i64,
+                                                  C.POINTER(_i64), _d, _d, _d]
             L.orc_healpix_map.restype = C.c_int
             L.orc_healpix_map.argtypes = [_d, _d, _d, _d, _d, _d, _i64, _d, _d, _i64,
                                           C.c_int, C.c_int, C.c_int, _d, _d]
@@ -324,6 +331,69 @@ def sph_mapping(Pos, HSML, M, Rho, Bin_Quant, Weights, par: Params, kernel,
     return reduce_image_3D(img, npix)
 
 
+# ---------------------------------------------------------------- integer work
+CLS_SKIP, CLS_SMALL, CLS_LARGE_CLOSED, CLS_LARGE_EXPLICIT = 0, 1, 2, 3
+
+
+def footprints(dims, pos, hsml, bin_q, npix, len2pix, kernel, calc_mean=True,
+               exact_norm=False):
+    """Per particle: footprint box (n x 2*dims int32), class and fallback flag
+    (orc_footprints).  pos is (n, 3) centred, bin_q n or (n, nq)."""
+    x = _f64(pos)
+    n = x.shape[0]
+    bq = _f64(bin_q)
+    nq = 1 if bq.ndim == 1 else bq.shape[1]
+    box = np.zeros((n, 2 * dims), dtype=np.int32)
+    cls = np.zeros(n, dtype=np.int32)
+    fb = np.zeros(n, dtype=np.int32)
+    flags = (1 if calc_mean else 0) | (2 if exact_norm else 0)
+    lib().orc_footprints(int(dims), _p(x), _p(_f64(hsml)), _p(bq), nq, n, int(npix),
+                         float(len2pix), _kid(kernel), flags, box.ctypes.data_as(C.c_void_p),
+                         cls.ctypes.data_as(C.c_void_p), fb.ctypes.data_as(C.c_void_p))
+    return box, cls, fb
+
+
+def tile_lists(dims, box, cls, npix):
+    """The deposit's (tile, particle) lists, restated from their definition:
+    tiles of 64^2 pixels (2D) or 16^3 voxels (3D) in row-major order; tile t
+    lists, in particle order, every large particle (class 2 or 3) whose box
+    meets it.  Returns (tile_off, pairs) as CSR arrays."""
+    T = 64 if dims == 2 else 16
+    nt = (npix + T - 1) // T
+    ntiles = nt ** dims
+    lists = [[] for _ in range(ntiles)]
+    for p in np.flatnonzero((cls == CLS_LARGE_CLOSED) | (cls == CLS_LARGE_EXPLICIT)):
+        r = [range(box[p, 2 * d] // T, box[p, 2 * d + 1] // T + 1) for d in range(dims)]
+        if dims == 2:
+            for a in r[0]:
+                for b in r[1]:
+                    lists[a * nt + b].append(p)
+        else:
+            for a in r[0]:
+                for b in r[1]:
+                    for c in r[2]:
+                        lists[(a * nt + b) * nt + c].append(p)
+    off = np.zeros(ntiles + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    pairs = np.array([p for x in lists for p in x], dtype=np.int32)
+    return off, pairs
+
+
+def small_buckets(dims, box, cls, npix):
+    """The small particles (class 1): in 2D bucketed by the 32 x 32 tile
+    holding their box's first pixel (row-major), particle order inside a
+    bucket; in 3D one bucket in particle order.  Returns (off, order)."""
+    small = np.flatnonzero(cls == CLS_SMALL).astype(np.int32)
+    if dims == 3:
+        return np.array([0, small.size], dtype=np.int64), small
+    S = 32
+    ns = (npix + S - 1) // S
+    key = (box[small, 0] // S) * ns + box[small, 2] // S
+    order = small[np.argsort(key, kind="stable")]
+    off = np.searchsorted(np.sort(key), np.arange(ns * ns + 1), side="left").astype(np.int64)
+    return off, order
+
+
 # ---------------------------------------------------------------- HEALPix
 def hp_npix(nside):
     return 12 * nside * nside
@@ -367,6 +437,31 @@ def hp_query_disc_ring(nside, theta, phi, radius):
         if cnt <= cap:
             return out[:cnt].copy()
         cap = int(cnt)
+
+
+def hp_particle_weights(pos, hsml, nside, kernel="wendlandc4", kernel_dim=2):
+    """One particle's pixel list, areas A and kernel weights wk
+    (pixel_weights.jl:6-140) and its info: dict with Dx, proj_hsml, ang_pix,
+    weight_per_pix, n_distr and fallback.  Empty arrays if skipped."""
+    p = _f64(pos)
+    info = np.zeros(6)
+    cap = 4096
+    while True:
+        pix = np.zeros(cap, dtype=np.int64)
+        A = np.zeros(cap)
+        wk = np.zeros(cap)
+        cnt = lib().orc_hp_particle_weights(_p(p), float(hsml), int(nside), _kid(kernel),
+                                            int(kernel_dim), cap,
+                                            pix.ctypes.data_as(C.POINTER(_i64)), _p(A), _p(wk),
+                                            _p(info))
+        if cnt < 0:
+            raise MemoryError("oracle hp_particle_weights failed")
+        if cnt <= cap:
+            break
+        cap = int(cnt)
+    d = {"Dx": info[0], "proj_hsml": info[1], "ang_pix": info[2], "weight_per_pix": info[3],
+         "n_distr": int(info[4]), "fallback": bool(info[5])}
+    return pix[:cnt].copy(), A[:cnt].copy(), wk[:cnt].copy(), d
 
 
 class OracleBoundsError(IndexError):

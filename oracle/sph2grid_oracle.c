@@ -497,6 +497,124 @@ int orc_cic_mapping_3D(const double *pos, const double *hsml,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Integer work of a deposit, for bit-exact checks of the GPU's               */
+/* ------------------------------------------------------------------------ */
+
+/* calculate_weights' fallback test (cic_2D.jl:49-69, cic_3D.jl:55-75):
+ * distr_weight == 0, i.e. no footprint pixel with u <= 1 has a non-zero
+ * kernel weight times overlap.  Every term is >= 0, so the loop stops at the
+ * first positive one. */
+static int fallback_2d(double x, double y, double h, int kernel, int64_t i0, int64_t i1,
+                       int64_t j0, int64_t j1)
+{
+    const double h_inv = 1.0 / h;
+    for (int64_t i = i0; i <= i1; ++i) {
+        double xd = centre_offset(x, i), dx = overlap_1d(x, h, i);
+        for (int64_t j = j0; j <= j1; ++j) {
+            double yd = centre_offset(y, j), dy = overlap_1d(y, h, j);
+            double u = sqrt(xd * xd + yd * yd) * h_inv;
+            if (u <= 1.0 && orc_kernel_value(kernel, 2, u, h_inv) * (dx * dy) != 0.0)
+                return 0;
+        }
+    }
+    return 1;
+}
+
+static int fallback_3d(const double c[3], double h, int kernel, const int64_t lo[3],
+                       const int64_t hi[3])
+{
+    const double h_inv = 1.0 / h;
+    for (int64_t i = lo[0]; i <= hi[0]; ++i) {
+        double xd = centre_offset(c[0], i), dx = overlap_1d(c[0], h, i);
+        for (int64_t j = lo[1]; j <= hi[1]; ++j) {
+            double yd = centre_offset(c[1], j), dy = overlap_1d(c[1], h, j);
+            for (int64_t k = lo[2]; k <= hi[2]; ++k) {
+                double zd = centre_offset(c[2], k), dz = overlap_1d(c[2], h, k);
+                double u = sqrt(xd * xd + yd * yd + zd * zd) * h_inv;
+                if (u <= 1.0 && orc_kernel_value(kernel, 3, u, h_inv) * (dx * dy * dz) != 0.0)
+                    return 0;
+            }
+        }
+    }
+    return 1;
+}
+
+/* Smallest hsml (pixels) for the closed-form normalisation of large
+ * unclipped footprints, per kernel (Cubic, Quintic, WendlandC2..C8). */
+static const double kMinH2[6] = {64.0, 24.0, 64.0, 28.0, 16.0, 12.0};
+static const double kMinH3[6] = {40.0, 24.0, 32.0, 20.0, 14.0, 11.0};
+
+int orc_footprints(int dims, const double *pos, const double *hsml, const double *bin_q,
+                   int nq, int64_t n, int64_t npix, double len2pix, int kernel,
+                   uint32_t flags, int32_t *box, int32_t *cls, int32_t *fb)
+{
+    const int nb = 2 * dims;
+    for (int64_t p = 0; p < n; ++p) {
+        int32_t *b = box + nb * p;
+        for (int d = 0; d < dims; ++d) {
+            b[2 * d] = 0;
+            b[2 * d + 1] = -1;
+        }
+        cls[p] = ORC_CLS_SKIP;
+        fb[p] = 0;
+        /* cic_2D.jl:155-168, cic_3D.jl:140-144: skip all-zero quantities
+         * unless calc_mean */
+        int allzero = 1;
+        for (int q = 0; q < nq; ++q)
+            if (bin_q[p * nq + q] != 0.0)
+                allzero = 0;
+        if (allzero && !(flags & ORC_FLAG_CALC_MEAN))
+            continue;
+        /* get_quantities_2D/3D and get_xyz (cic_shared.jl:85-100) */
+        const double h = hsml[p] * len2pix;
+        double c[3];
+        for (int d = 0; d < dims; ++d)
+            c[d] = pos[3 * p + d] * len2pix + 0.5 * (double)npix;
+        int bad = !(h > 0.0);
+        int64_t a[3] = {0, 0, 0}, e[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < dims && !bad; ++d) {
+            double fa = floor(c[d] - h), fe = floor(c[d] + h);
+            if (fa != fa || fe != fe) {
+                bad = 1;
+                break;
+            }
+            /* beyond the image by more than a pixel: the same footprint */
+            fa = fa < -1.0 ? -1.0 : fa > (double)npix + 1.0 ? (double)npix + 1.0 : fa;
+            fe = fe < -1.0 ? -1.0 : fe > (double)npix + 1.0 ? (double)npix + 1.0 : fe;
+            a[d] = (int64_t)fa;
+            e[d] = (int64_t)fe;
+            /* pix_index_min_max (cic_shared.jl:46-52) */
+            lo[d] = a[d] > 0 ? a[d] : 0;
+            hi[d] = e[d] < npix - 1 ? e[d] : npix - 1;
+            if (lo[d] > hi[d])
+                bad = 1;
+        }
+        if (bad) /* hsml <= 0, NaN, or an empty footprint: nothing deposited */
+            continue;
+        for (int d = 0; d < dims; ++d) {
+            b[2 * d] = (int32_t)lo[d];
+            b[2 * d + 1] = (int32_t)hi[d];
+        }
+        fb[p] = dims == 2 ? fallback_2d(c[0], c[1], h, kernel, lo[0], hi[0], lo[1], hi[1])
+                          : fallback_3d(c, h, kernel, lo, hi);
+        int small = 1, clipped = 0;
+        const int64_t side = dims == 2 ? ORC_SMALL_SIDE_2D : ORC_SMALL_SIDE_3D;
+        for (int d = 0; d < dims; ++d) {
+            small = small && hi[d] - lo[d] + 1 <= side;
+            clipped = clipped || a[d] < 0 || e[d] > npix - 1;
+        }
+        const double hmin = dims == 2 ? kMinH2[kernel] : kMinH3[kernel];
+        if (small)
+            cls[p] = ORC_CLS_SMALL;
+        else if (!(flags & ORC_FLAG_EXACT_NORM) && !clipped && !fb[p] && h >= hmin)
+            cls[p] = ORC_CLS_LARGE_CLOSED;
+        else
+            cls[p] = ORC_CLS_LARGE_EXPLICIT;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* parallel=true layout                                                      */
 /* ------------------------------------------------------------------------ */
 
@@ -910,22 +1028,28 @@ static int hp_grow(hp_scratch *s, int64_t need)
     return 0;
 }
 
-/* one particle of the loop at main.jl:158-213 */
-static int hp_deposit_one(const double pos[3], double hsml, double m,
-                          double rho, double bq, double w, int64_t nside,
-                          double ang_pix, int kernel, int kdim, int calc_mean,
-                          double *allsky, double *wmap, hp_scratch *s)
+/* The weights of one particle (constributing_pixels.jl:7-22 and
+ * pixel_weights.jl:6-140): its pixel list (the disc query plus the pixel
+ * holding the particle, deduplicated), per pixel the area A (with the
+ * /(ang_pix*Dx)^2 of weight_per_index) and the kernel weight wk (1 on every
+ * pixel in the fallback), the fallback flag, the pixel count n_distr and
+ * weight_per_pix.  Returns the pixel count, 0 for a skipped particle
+ * (Dx < hsml, main.jl:172-179), -1 if out of memory. */
+static int64_t hp_particle(const double pos[3], double hsml, int64_t nside, double ang_pix,
+                           int kernel, int kdim, hp_scratch *s, double *Dx_out,
+                           double *proj_out, double *wpp_out, int64_t *n_distr_out,
+                           int *fallback_out)
 {
-    if (!calc_mean && bq == 0.0)
-        return 0;
     /* shared.jl:1-10 */
     double dx2 = 0.0;
     for (int d = 0; d < 3; ++d)
         dx2 += pos[d] * pos[d];
     double Dx = sqrt(dx2);
+    *Dx_out = Dx;
     if (Dx < hsml) /* main.jl:172-179 */
         return 0;
     double proj_hsml = asin(hsml / Dx); /* main.jl:182 */
+    *proj_out = proj_hsml;
 
     /* constributing_pixels.jl:7-22 */
     double theta, phi;
@@ -943,12 +1067,7 @@ static int hp_deposit_one(const double pos[3], double hsml, double m,
     if (!found)
         s->pix[cnt++] = cpix; /* push! then unique! keeps first occurrences */
 
-    /* main.jl:56-63, 193 */
-    double dz = 2.0 * hsml;
-    double area = (m / rho) / dz;
     double ap = ang_pix * Dx;
-    dz /= ap * ap;
-
     /* pixel_weights.jl:87-140 */
     double h_inv = 1.0 / proj_hsml;
     wsum sm = {0.0, 0.0, 0, 0};
@@ -966,11 +1085,12 @@ static int hp_deposit_one(const double pos[3], double hsml, double m,
         a = a < ang_pix ? a : ang_pix;
         a = a > 0.0 ? a : 0.0;
         a = a / ang_pix;
-        a /= ap * ap;
+        a /= ap * ap; /* pixel_weights.jl:52 */
         weight_per_pixel(&sm, a, u, h_inv, kernel, kdim, &s->A[t], &s->wk[t]);
     }
     double wpp;
-    if (sm.distr_weight == 0.0) {
+    int fb = sm.distr_weight == 0.0;
+    if (fb) {
         sm.n_distr = sm.n_tot;
         for (int64_t t = 0; t < cnt; ++t)
             s->wk[t] = 1.0;
@@ -978,8 +1098,66 @@ static int hp_deposit_one(const double pos[3], double hsml, double m,
     } else {
         wpp = (double)sm.n_distr / sm.distr_weight;
     }
+    *wpp_out = wpp;
+    *n_distr_out = sm.n_distr;
+    *fallback_out = fb;
+    return cnt;
+}
+
+int64_t orc_hp_particle_weights(const double pos[3], double hsml, int64_t nside, int kernel,
+                                int kernel_dim, int64_t cap, int64_t *pix, double *A,
+                                double *wk, double *info)
+{
+    hp_scratch s = {NULL, NULL, NULL, 0};
+    if (hp_grow(&s, 4096))
+        return -1;
+    const double ang_pix = sqrt(4.0 * M_PI / (double)orc_hp_npix(nside));
+    double Dx = 0.0, proj = 0.0, wpp = 0.0;
+    int64_t nd = 0;
+    int fb = 0;
+    int64_t cnt = hp_particle(pos, hsml, nside, ang_pix, kernel, kernel_dim, &s, &Dx, &proj,
+                              &wpp, &nd, &fb);
+    if (cnt > 0 && cnt <= cap) {
+        memcpy(pix, s.pix, sizeof(int64_t) * (size_t)cnt);
+        memcpy(A, s.A, sizeof(double) * (size_t)cnt);
+        memcpy(wk, s.wk, sizeof(double) * (size_t)cnt);
+    }
+    if (info) {
+        info[0] = Dx;
+        info[1] = proj;
+        info[2] = ang_pix;
+        info[3] = wpp;
+        info[4] = (double)nd;
+        info[5] = (double)fb;
+    }
+    free(s.pix);
+    free(s.wk);
+    free(s.A);
+    return cnt;
+}
+
+/* one particle of the loop at main.jl:158-213 */
+static int hp_deposit_one(const double pos[3], double hsml, double m,
+                          double rho, double bq, double w, int64_t nside,
+                          double ang_pix, int kernel, int kdim, int calc_mean,
+                          double *allsky, double *wmap, hp_scratch *s)
+{
+    if (!calc_mean && bq == 0.0)
+        return 0;
+    double Dx = 0.0, proj = 0.0, wpp = 0.0;
+    int64_t n_distr = 0;
+    int fb = 0;
+    int64_t cnt = hp_particle(pos, hsml, nside, ang_pix, kernel, kdim, s, &Dx, &proj, &wpp,
+                              &n_distr, &fb);
+    if (cnt <= 0)
+        return cnt < 0 ? -1 : 0;
+    /* main.jl:56-63, 193 */
+    double dz = 2.0 * hsml;
+    double area = (m / rho) / dz;
+    double ap = ang_pix * Dx;
+    dz /= ap * ap;
     /* main.jl:25-45 */
-    double kernel_norm = area / (double)sm.n_distr;
+    double kernel_norm = area / (double)n_distr;
     double area_norm = kernel_norm * wpp * w * dz;
     for (int64_t t = 0; t < cnt; ++t) {
         double pw = area_norm * s->wk[t] * s->A[t];

@@ -126,6 +126,39 @@ int64_t orc_hp_ring_above(int64_t nside, double z);
 int64_t orc_hp_query_disc_ring(int64_t nside, double theta, double phi,
                                double radius, int64_t *out, int64_t cap);
 
+/* ---- integer work of a deposit (for bit-exact checks of the GPU's) ----
+ * Per particle: the footprint box of pix_index_min_max (cic_shared.jl:46-52)
+ * from get_xyz's two roundings (cic_shared.jl:85-100), (0, -1) per axis for a
+ * skipped particle; the fallback flag of calculate_weights (distr_weight ==
+ * 0, cic_2D.jl:49-69, cic_3D.jl:55-75) by the reference's own pixel loop; and
+ * the class of the GPU deposit's work split, which has no reference
+ * counterpart: skipped (zero quantity without calc_mean, hsml <= 0, NaN,
+ * empty footprint), small (every box side <= 34 px in 2D, <= 8 voxels in 3D),
+ * large with the closed-form normalisation (no ORC_FLAG_EXACT_NORM, footprint
+ * not clipped by the image border, no fallback, hsml >= a per-kernel
+ * threshold), or large with an explicit sum.  box is n x 2*dims. */
+#define ORC_CLS_SKIP 0
+#define ORC_CLS_SMALL 1
+#define ORC_CLS_LARGE_CLOSED 2
+#define ORC_CLS_LARGE_EXPLICIT 3
+#define ORC_SMALL_SIDE_2D 34
+#define ORC_SMALL_SIDE_3D 8
+#define ORC_FLAG_CALC_MEAN 0x1u
+#define ORC_FLAG_EXACT_NORM 0x2u
+int orc_footprints(int dims, const double *pos, const double *hsml, const double *bin_q,
+                   int nq, int64_t n, int64_t npix, double len2pix, int kernel,
+                   uint32_t flags, int32_t *box, int32_t *cls, int32_t *fb);
+
+/* The per-pixel weights of one particle (constributing_pixels.jl:7-22,
+ * pixel_weights.jl:6-140), without depositing: up to cap entries of the
+ * pixel list (RING ids, 0-based), the area A (including its /(ang_pix*Dx)^2)
+ * and the kernel weight wk (1 everywhere in the fallback).  info[0..5] =
+ * Dx, proj_hsml, ang_pix, weight_per_pix, n_distr_pix, fallback flag.
+ * Returns the pixel count (0: skipped, Dx < hsml; -1: out of memory). */
+int64_t orc_hp_particle_weights(const double pos[3], double hsml, int64_t nside, int kernel,
+                                int kernel_dim, int64_t cap, int64_t *pix, double *A,
+                                double *wk, double *info);
+
 /* healpix_interpolation/main.jl:92-227 + filter_particles.jl:17-54.
  * pos (3 x n, Float64) is mutated (centre subtracted) as in the reference.
  * Returns 0, or -1 where the reference throws a BoundsError (calc_mean=false

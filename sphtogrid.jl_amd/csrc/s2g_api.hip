@@ -260,21 +260,30 @@ struct MapIn {
     double *out;
 };
 
-void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_stats *st)
+// The pass's tables: per-map particle and tile offsets and the grids of the
+// small-footprint tiles (one bucket per tile), staged in pinned memory and
+// uploaded on `s`.
+struct PassTables {
+    const Map2D *d_maps = nullptr;
+    const int64_t *d_pbase = nullptr;
+    const TileMap *d_stm = nullptr;
+    std::vector<TileMap> tm;
+    int64_t ntot = 0;
+    int ntiles = 0, nstiles = 0;
+    bool any_out = false;
+};
+
+PassTables pass_tables(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s)
 {
-    const int dtype = mi[0].a.dtype, kernel = mi[0].a.kernel, nq = mi[0].a.nq;
-    const int nplanes = nq + 1;
-    // per-map table: particle and tile offsets of the pass, and the grids of
-    // the small-footprint tiles (one bucket per tile)
     const size_t btab = sizeof(Map2D) * nmaps, bpb = sizeof(int64_t) * (nmaps + 1);
     const size_t bst = sizeof(TileMap) * nmaps;
     char *h = (char *)c->h_tab.get(btab + bpb + bst);
     Map2D *tab = (Map2D *)h;
     int64_t *pb = (int64_t *)(h + btab);
     TileMap *stm = (TileMap *)(h + btab + bpb);
-    std::vector<TileMap> tm(nmaps);
+    PassTables pt;
+    pt.tm.resize(nmaps);
     int64_t ntot = 0, ttot = 0, stot = 0;
-    bool any_out = false;
     for (int m = 0; m < nmaps; ++m) {
         const Map2DArgs &a = mi[m].a;
         Map2D &e = tab[m];
@@ -299,29 +308,69 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         e.nts = nts;
         e.sbase = (int32_t)stot;
         pb[m] = ntot;
-        tm[m] = TileMap{0, (int32_t)ttot, ntj * ntj, ntj, 1, 0, 0};
+        pt.tm[m] = TileMap{0, (int32_t)ttot, ntj * ntj, ntj, 1, 0, 0};
         // bucket t of the small path spans sorted entries [off[t], off[t+1])
         stm[m] = TileMap{stot, (int32_t)stot, nts * nts, nts, 1, 1, 0};
         ntot += a.n;
         ttot += (int64_t)ntj * ntj;
         stot += (int64_t)nts * nts;
-        any_out = any_out || mi[m].out;
+        pt.any_out = pt.any_out || mi[m].out;
         if (ntot > (int64_t)INT32_MAX || stot > (int64_t)INT32_MAX)
             throw Error(S2G_EUNSUPPORTED, "deposit pass too large (2^31 particles or tiles)");
     }
     pb[nmaps] = ntot;
     char *d = (char *)c->tab.get(btab + bpb + bst);
-    const Map2D *d_maps = (const Map2D *)d;
-    const int64_t *d_pbase = (const int64_t *)(d + btab);
-    const TileMap *d_stm = (const TileMap *)(d + btab + bpb);
-    const int ntiles = (int)ttot, nstiles = (int)stot;
+    pt.d_maps = (const Map2D *)d;
+    pt.d_pbase = (const int64_t *)(d + btab);
+    pt.d_stm = (const TileMap *)(d + btab + bpb);
+    pt.ntot = ntot;
+    pt.ntiles = (int)ttot;
+    pt.nstiles = (int)stot;
+    S2G_HIP(hipMemcpyAsync(d, h, btab + bpb + bst, hipMemcpyHostToDevice, s));
+    return pt;
+}
+
+// The small list bucketed by the kST x kST tile holding each footprint's
+// first pixel: a stable radix sort of (tile key, particle), so particles stay
+// in particle order inside a tile.  Returns the sorted particles; off[t] is
+// the first entry of tile t (t in [0, nstiles]).
+const int32_t *bucket_small(s2g_ctx *c, const ClassLists &cl, const Rec2 *rec,
+                            const Map2D *d_maps, int nstiles, int64_t **off, hipStream_t s)
+{
+    const int64_t ns = cl.ns;
+    uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * ns);
+    int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * ns);
+    launch_2d_small_keys(cl.l0, ns, rec, d_maps, keys, vals, s);
+    int end_bit = 1;
+    while (end_bit < 32 && ((int64_t)1 << end_bit) < (int64_t)nstiles)
+        ++end_bit;
+    const size_t tb = sort_pairs_tmp_bytes(ns, end_bit);
+    void *stmp = c->sort_tmp.get(tb ? tb : 1);
+    sort_pairs(stmp, tb, keys, keys + ns, vals, vals + ns, ns, end_bit, s);
+    *off = (int64_t *)c->soff.get(sizeof(int64_t) * (nstiles + 1));
+    launch_tile_offsets(keys + ns, ns, nstiles, *off, s);
+    return vals + ns;
+}
+
+void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_stats *st)
+{
+    const int dtype = mi[0].a.dtype, kernel = mi[0].a.kernel, nq = mi[0].a.nq;
+    const int nplanes = nq + 1;
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        st->n_particles = ntot;
     }
     hipEvent_t *E = c->stage;
     S2G_HIP(hipEventRecord(E[0], s));
-    S2G_HIP(hipMemcpyAsync(d, h, btab + bpb + bst, hipMemcpyHostToDevice, s));
+    PassTables pt = pass_tables(c, mi, nmaps, s);
+    const Map2D *d_maps = pt.d_maps;
+    const int64_t *d_pbase = pt.d_pbase;
+    const TileMap *d_stm = pt.d_stm;
+    std::vector<TileMap> &tm = pt.tm;
+    const int64_t ntot = pt.ntot;
+    const int ntiles = pt.ntiles, nstiles = pt.nstiles;
+    const bool any_out = pt.any_out;
+    if (st)
+        st->n_particles = ntot;
 
     ClassLists cl;
     Rec2 *rec = nullptr;
@@ -385,19 +434,9 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     const int eS = s2g_ctx::kNumStageEvents - 3;
     S2G_HIP(hipEventRecord(E[eS], s));
     if (cl.ns > 0) {
-        // bucket the small list by tile: stable radix sort of (tile, particle)
         const int64_t ns = cl.ns;
-        uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * ns);
-        int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * ns);
-        launch_2d_small_keys(cl.l0, ns, rec, d_maps, keys, vals, s);
-        int end_bit = 1;
-        while (end_bit < 32 && ((int64_t)1 << end_bit) < (int64_t)nstiles)
-            ++end_bit;
-        const size_t tb = sort_pairs_tmp_bytes(ns, end_bit);
-        void *stmp = c->sort_tmp.get(tb ? tb : 1);
-        sort_pairs(stmp, tb, keys, keys + ns, vals, vals + ns, ns, end_bit, s);
-        int64_t *soff = (int64_t *)c->soff.get(sizeof(int64_t) * (nstiles + 1));
-        launch_tile_offsets(keys + ns, ns, nstiles, soff, s);
+        int64_t *soff = nullptr;
+        const int32_t *sorted = bucket_small(c, cl, rec, d_maps, nstiles, &soff, s);
         // work items of <= Gs particles; the item count stays on the device and
         // the grid is its upper bound nstiles + ns / Gs
         int64_t Gs = (ns + (int64_t)c->ncu * 8 - 1) / ((int64_t)c->ncu * 8);
@@ -414,7 +453,7 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         launch_items(soff, d_stm, nmaps, nstiles, Gs, snit, tmp, sitems, stile, true, s);
         for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
             const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
-            launch_2d_small_tile(dtype, kernel, np, sitems, snit + nstiles, max_items, vals + ns,
+            launch_2d_small_tile(dtype, kernel, np, sitems, snit + nstiles, max_items, sorted,
                                  rec, d_maps, nq, plane0, s);
         }
     }
@@ -1311,6 +1350,131 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
         }
         if (stats)
             *stats = total;
+        return S2G_OK;
+    });
+}
+
+int s2g_footprints(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml,
+                   const void *m, const void *rho, const void *bin_q, int nq,
+                   const void *weights, int64_t n, const s2g_params *params, int kernel,
+                   uint32_t flags, int32_t *box, int32_t *cls_out, int32_t *fb_out,
+                   int64_t *tile_off, int32_t *pairs_out, int64_t pair_cap, int64_t *small_off,
+                   int32_t *small_order, int64_t *counts)
+{
+    return guarded(c, [&]() {
+        check_common(dtype, kernel, dims == 3 ? 3 : 2, n, params);
+        check_dims_nq(dims, nq);
+        if (!box || !cls_out || !fb_out || !tile_off || !small_off || !small_order || !counts)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        if (pair_cap > 0 && !pairs_out)
+            throw Error(S2G_EINVAL, "pairs is NULL with pair_cap > 0");
+        hipStream_t s = c->stream;
+        const int64_t npix = params->npixels;
+        Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, nq, weights, n, params, kernel,
+                                flags);
+        {
+            s2g_stats tmp;
+            h = stage_inputs(c, h, s, &tmp);
+        }
+        ClassLists cl;
+        std::vector<TileMap> tm;
+        int ntiles = 0, nstiles = 1;
+        const void *rec = nullptr;
+        const int32_t *sorted = nullptr;
+        int64_t *soff = nullptr;
+        if (dims == 2) {
+            MapIn mi{h, nullptr, nullptr};
+            PassTables pt = pass_tables(c, &mi, 1, s);
+            tm = pt.tm;
+            ntiles = pt.ntiles;
+            nstiles = pt.nstiles;
+            if (n > 0) {
+                Rec2 *r2 = (Rec2 *)c->rec.get(sizeof(Rec2) * n);
+                int32_t *cl2 = (int32_t *)c->cls.get(sizeof(int32_t) * n);
+                launch_2d_setup(dtype, pt.d_maps, 1, n, nq, kernel, r2, cl2, s);
+                cl = partition_classes(c, cl2, n, pt.d_pbase, 1, s);
+                rec = r2;
+                if (cl.ns > 0)
+                    sorted = bucket_small(c, cl, r2, pt.d_maps, nstiles, &soff, s);
+            }
+        } else {
+            const int nb = (int)((npix + kBT - 1) / kBT);
+            ntiles = nb * nb * nb;
+            tm.push_back(TileMap{0, 0, ntiles, nb, nb, 0, 0});
+            if (n > 0) {
+                Rec3 *r3 = (Rec3 *)c->rec.get(sizeof(Rec3) * n);
+                int32_t *cl3 = (int32_t *)c->cls.get(sizeof(int32_t) * n);
+                int64_t *hpb = (int64_t *)c->h_tab.get(2 * sizeof(int64_t));
+                hpb[0] = 0;
+                hpb[1] = n;
+                int64_t *d_pbase = (int64_t *)c->tab.get(2 * sizeof(int64_t));
+                S2G_HIP(hipMemcpyAsync(d_pbase, hpb, 2 * sizeof(int64_t), hipMemcpyHostToDevice,
+                                       s));
+                launch_3d_setup(h, r3, cl3, s);
+                cl = partition_classes(c, cl3, n, d_pbase, 1, s);
+                rec = r3;
+                sorted = cl.l0;  // the 3D small path keeps particle order
+            }
+        }
+        // large particles: per-tile pair lists
+        int64_t npairs = 0;
+        std::vector<int64_t> cnt;
+        if (cl.nl > 0) {
+            double ms = 0.0;
+            const TilePlan tp = build_tile_lists(c, cl.l1, rec, dims == 3, tm, cl.lbase, ntiles,
+                                                 s, &ms);
+            npairs = tp.npairs;
+            const int64_t ncnt = (int64_t)ntiles * tm[0].nch + 1;
+            cnt.resize(ncnt);
+            d2h(c, cnt.data(), c->cnt.p, sizeof(int64_t) * ncnt, s, nullptr);
+            if (npairs > 0 && npairs <= pair_cap)
+                d2h(c, pairs_out, tp.pairs, sizeof(int32_t) * npairs, s, nullptr);
+        }
+        for (int t = 0; t < ntiles; ++t)
+            tile_off[t] = cl.nl > 0 ? cnt[(size_t)t * tm[0].nch] : 0;
+        tile_off[ntiles] = npairs;
+        // small particles: buckets (2D) or particle order (3D)
+        if (cl.ns > 0) {
+            d2h(c, small_order, sorted, sizeof(int32_t) * cl.ns, s, nullptr);
+            if (dims == 2) {
+                d2h(c, small_off, soff, sizeof(int64_t) * (nstiles + 1), s, nullptr);
+            } else {
+                small_off[0] = 0;
+                small_off[1] = cl.ns;
+            }
+        } else {
+            for (int t = 0; t <= nstiles; ++t)
+                small_off[t] = 0;
+        }
+        // per-particle records
+        if (n > 0) {
+            const int nb = dims == 2 ? 4 : 6;
+            if (dims == 2) {
+                std::vector<Rec2> r((size_t)n);
+                d2h(c, r.data(), rec, sizeof(Rec2) * n, s, nullptr);
+                for (int64_t p = 0; p < n; ++p) {
+                    const int32_t b[4] = {r[p].i0, r[p].i1, r[p].j0, r[p].j1};
+                    std::memcpy(box + nb * p, b, sizeof(b));
+                    cls_out[p] = r[p].cls;
+                    fb_out[p] = r[p].fb;
+                }
+            } else {
+                std::vector<Rec3> r((size_t)n);
+                d2h(c, r.data(), rec, sizeof(Rec3) * n, s, nullptr);
+                for (int64_t p = 0; p < n; ++p) {
+                    const int32_t b[6] = {r[p].i0, r[p].i1, r[p].j0, r[p].j1, r[p].k0, r[p].k1};
+                    std::memcpy(box + nb * p, b, sizeof(b));
+                    cls_out[p] = r[p].cls;
+                    fb_out[p] = r[p].fb;
+                }
+            }
+        }
+        counts[0] = cl.ns;
+        counts[1] = cl.nl;
+        counts[2] = cl.ne;
+        counts[3] = npairs;
+        counts[4] = ntiles;
+        counts[5] = dims == 2 ? nstiles : 1;
         return S2G_OK;
     });
 }

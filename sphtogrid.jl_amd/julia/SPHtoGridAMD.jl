@@ -14,12 +14,22 @@ Binding points in SPHtoGrid.jl (src/ paths):
   effects/mass_density.jl:30-99                            -> mass_density
   tsc_interpolation/tsc_interpolation.jl:131-133           -> TSCInterpolation
 
-The shared library path is taken from ENV["SPH2GRID_LIB"] or the default
-build location next to this file.
+`sphMapping` and `healpix_map` below keep the reference's surfaces
+(cic_interpolation.jl:35-273, main.jl:92-227) around those engines, using
+SPHtoGrid.jl's own centring, filtering and shell-sorting functions; with
+`parallel = true` they make the same single GPU call on the calling process
+instead of the reference's `@spawnat` fan-out (cic_interpolation.jl:171-215,
+236-271), since one call already uses the whole GPU.
+
+The module is included inside SPHtoGrid (it imports the parent module's
+functions).  The shared library path is taken from ENV["SPH2GRID_LIB"] or the
+default build location next to this file.
 """
 module SPHtoGridAMD
 
 using SPHKernels
+import ..mappingParameters, ..center_particles, ..filter_particles_in_image,
+       ..filter_sort_particles, ..HealpixMap, ..RingOrder, ..ustrip
 
 const libsph2grid = get(ENV, "SPH2GRID_LIB",
                         joinpath(@__DIR__, "..", "sph2grid", "libsph2grid.so"))
@@ -32,9 +42,27 @@ const S2G_FLAG_EXACT_NORM = UInt32(0x2)
 const S2G_FLAG_PERIODIC = UInt32(0x4)
 const S2G_FLAG_REDUCE = UInt32(0x8)
 
+# s2g_params of include/sph2grid.h
 struct S2GParams
     npixels::Int64
     len2pix::Float64
+end
+
+# s2g_map_desc of include/sph2grid.h (device pointers)
+struct S2GMapDesc
+    dims::Int32
+    nq::Int32
+    n::Int64
+    params::S2GParams
+    kernel::Int32
+    flags::UInt32
+    d_pos::Ptr{Cvoid}
+    d_hsml::Ptr{Cvoid}
+    d_m::Ptr{Cvoid}
+    d_rho::Ptr{Cvoid}
+    d_bin_q::Ptr{Cvoid}
+    d_weights::Ptr{Cvoid}
+    d_out::Ptr{Float64}
 end
 
 kernel_id(::Cubic) = Cint(0)
@@ -146,30 +174,183 @@ function cic_mapping_3D(Pos, HSML, M, Rho, Bin_Q, Weights; param, kernel,
 end
 
 """
+    reduce_image_2D(image, npix, reduce_image)
+
+SPHtoGrid.reduce_image_2D (reduce_image.jl:8-31) on the GPU: the flat image
+(N_distr x (Nq+1), deposit order) to the `Array{Float64,3}` (npix, npix, Nq)
+in Julia's [x, y, q] order, divided by the weights where they are > 0 iff
+`reduce_image`.
+"""
+function reduce_image_2D(image::Matrix{Float64}, npix::Integer, reduce_image::Bool)
+    nq = size(image, 2) - 1
+    out = Array{Float64}(undef, npix, npix, nq)
+    c = context()
+    GC.@preserve image out begin
+        rc = ccall((:s2g_reduce, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Int64, Cint, UInt32, Ptr{Float64}, Ptr{Float64}),
+                   c.handle, Cint(2), Int64(npix), Cint(nq),
+                   reduce_image ? S2G_FLAG_REDUCE : UInt32(0), image, out)
+    end
+    check(c, rc, "s2g_reduce")
+    return out
+end
+
+"""
+    reduce_image_3D(image, npix, reduce_image)
+
+SPHtoGrid.reduce_image_3D (reduce_image.jl:39-55) on the GPU, with the
+`image[:, 2] .= 1.0` of `reduce_image = false` (cic_interpolation.jl:230-232)
+folded in: the cube [z, y, x].
+"""
+function reduce_image_3D(image::Matrix{Float64}, npix::Integer, reduce_image::Bool)
+    out = Array{Float64}(undef, npix, npix, npix)
+    c = context()
+    GC.@preserve image out begin
+        rc = ccall((:s2g_reduce, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Int64, Cint, UInt32, Ptr{Float64}, Ptr{Float64}),
+                   c.handle, Cint(3), Int64(npix), Cint(1),
+                   reduce_image ? S2G_FLAG_REDUCE : UInt32(0), image, out)
+    end
+    check(c, rc, "s2g_reduce")
+    return out
+end
+
+_strip(a) = eltype(a) <: AbstractFloat ? a : ustrip.(a)
+
+"""
+    sphMapping(Pos, HSML, M, Rho, Bin_Quant, Weights=Rho, RM=nothing;
+               param, kernel, show_progress=true, parallel=false,
+               reduce_image=true, return_both_maps=false, dimensions=2,
+               calc_mean=false, stokes=false, sort_z=false)
+
+SPHtoGrid.sphMapping (cic_interpolation.jl:35-273) with the deposit and the
+reduce on the GPU.  The host steps are the reference's own: center_particles
+(mutates `Pos`), `stokes` implies `sort_z` (:74-81), filter_particles_in_image
+(with its sort_z selection), unit stripping (:95-111).  `parallel = true`
+takes the same single GPU call on this process (the reference's Distributed
+fan-out, :171-215 and :236-271, is not needed: one call uses the whole GPU).
+`RM` has no effect on the maps, as in the reference, where `stokes` is not
+passed on to cic_mapping_2D (:152-155) and the parallel path drops RM.
+"""
+function sphMapping(Pos::Array{<:Real}, HSML::Array{<:Real}, M::Array{<:Real},
+                    Rho::Array{<:Real}, Bin_Quant::Array{<:Real},
+                    Weights::Array{<:Real}=Rho,
+                    RM::Union{Array{<:Real},Nothing}=nothing;
+                    param::mappingParameters, kernel::AbstractSPHKernel,
+                    show_progress::Bool=true, parallel::Bool=false,
+                    reduce_image::Bool=true, return_both_maps::Bool=false,
+                    dimensions::Int=2, calc_mean::Bool=false,
+                    stokes::Bool=false, sort_z::Bool=false)
+    Pos, par = center_particles(Pos, param)
+    if stokes
+        sort_z = true
+        parallel && @warn "Mapping stokes parameters only works in serial!"
+    end
+    p_in_image = filter_particles_in_image(Pos, par, sort_z)
+    x = _strip(Pos[:, p_in_image])
+    hsml, m, rho, weights = (_strip(a[p_in_image]) for a in (HSML, M, Rho, Weights))
+    bin_q = _strip(ndims(Bin_Quant) == 1 ? Bin_Quant[p_in_image] : Bin_Quant[:, p_in_image])
+    if dimensions == 2
+        image = cic_mapping_2D(x, hsml, m, rho, bin_q, weights; param=par, kernel=kernel,
+                               calc_mean=calc_mean)
+        return_both_maps && return image
+        return reduce_image_2D(image, param.Npixels[1], reduce_image)
+    elseif dimensions == 3
+        # calc_mean is not forwarded to cic_mapping_3D (cic_interpolation.jl:219-221)
+        image = cic_mapping_3D(x, hsml, m, rho, bin_q, weights; param=par, kernel=kernel)
+        return reduce_image_3D(image, param.Npixels[1], reduce_image)
+    end
+    error("dimensions must be 2 or 3")
+end
+
+"""
+    deposit_batch!(descs::Vector{S2GMapDesc}; dtype=Float32, stream=C_NULL)
+
+Many independent 2D maps in one device pass (s2g_deposit_batch): inputs and
+outputs are device pointers (e.g. `pointer(::ROCArray)` from AMDGPU.jl), all
+of element type `dtype`; see include/sph2grid.h for the descriptor fields.
+"""
+function deposit_batch!(descs::Vector{S2GMapDesc}; dtype::Type=Float32,
+                        stream::Ptr{Cvoid}=C_NULL)
+    c = context()
+    GC.@preserve descs begin
+        rc = ccall((:s2g_deposit_batch, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{S2GMapDesc}, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+                   c.handle, _dtype(dtype), descs, Cint(length(descs)), stream, C_NULL)
+    end
+    check(c, rc, "s2g_deposit_batch")
+    return nothing
+end
+
+# the pixel vector of a map: a Vector{Float64} itself, or the `pixels` field
+# of the HealpixMap{Float64,RingOrder} the reference allocates (main.jl:110-111)
+_pixels(v::Vector{Float64}) = v
+function _pixels(v::AbstractVector{Float64})
+    hasproperty(v, :pixels) || error("expected a Vector{Float64} or a HealpixMap{Float64}")
+    p = getproperty(v, :pixels)
+    p isa Vector{Float64} || error("the map's pixels are not a Vector{Float64}")
+    return p
+end
+
+"""
     healpix_deposit!(allsky_map, weight_map, pos, hsml, m, rho, bin_q, weights,
                      nside, kernel, calc_mean)
 
 The particle loop of SPHtoGrid.healpix_map (main.jl:146-213) over the already
-filtered and sorted particles (filter_particles.jl:17-54); accumulates into
-two RING-ordered Float64 vectors of length 12*nside^2.
+filtered and sorted particles (filter_particles.jl:17-54).  The maps are the
+reference's two `HealpixMap{Float64,RingOrder}(Nside)` (main.jl:110-111) or
+plain `Vector{Float64}`s of 12*nside^2 RING-ordered pixels; both are
+overwritten with the deposit (the reference fills freshly allocated, zeroed
+maps).
 """
-function healpix_deposit!(allsky_map::Vector{Float64}, weight_map::Vector{Float64},
+function healpix_deposit!(allsky_map::AbstractVector{Float64},
+                          weight_map::AbstractVector{Float64},
                           pos, hsml, m, rho, bin_q, weights, nside::Integer, kernel,
                           calc_mean::Bool)
+    am, wm = _pixels(allsky_map), _pixels(weight_map)
+    length(am) == length(wm) == 12 * Int64(nside)^2 ||
+        error("maps must hold 12 * nside^2 = $(12 * Int64(nside)^2) pixels")
     T = Float64  # healpix_map is Float64-only in the reference (pixel_weights.jl:87-91)
     x = convert(Matrix{T}, pos)
     hs, mm, r, bq, w = (convert(Vector{T}, a) for a in (hsml, m, rho, bin_q, weights))
     c = context()
     flags = calc_mean ? S2G_FLAG_CALC_MEAN : UInt32(0)
-    GC.@preserve x hs mm r bq w allsky_map weight_map begin
+    GC.@preserve x hs mm r bq w am wm begin
         rc = ccall((:s2g_healpix_deposit, libsph2grid), Cint,
                    (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Int64,
                     Int64, Cint, Cint, UInt32, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}),
-                   c.handle, S2G_F64, x, hs, mm, r, bq, w, length(hs), Int64(nside),
+                   c.handle, S2G_F64, x, hs, mm, r, bq, w, Int64(length(hs)), Int64(nside),
                    kernel_id(kernel), Cint(kernel.dim in (2, 3) ? kernel.dim : 2), flags,
-                   allsky_map, weight_map, C_NULL)
+                   am, wm, C_NULL)
     end
     check(c, rc, "s2g_healpix_deposit")
+    return allsky_map, weight_map
+end
+
+"""
+    healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights; center=[0.0, 0.0, 0.0],
+                radius_limits=[0.0, Inf], Nside=1024, kernel, show_progress=true,
+                output_from_all_workers=false, calc_mean=true)
+
+SPHtoGrid.healpix_map (main.jl:92-227) with its particle loop on the GPU:
+the same early return (:114-121), `filter_sort_particles` (mutates `Pos`, as
+the reference does) and two `HealpixMap{Float64,RingOrder}` results.
+"""
+function healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights;
+                     center::Vector{<:Real}=[0.0, 0.0, 0.0],
+                     radius_limits::Vector{<:Real}=[0.0, Inf],
+                     Nside::Integer=1024, kernel::AbstractSPHKernel,
+                     show_progress::Bool=true, output_from_all_workers::Bool=false,
+                     calc_mean::Bool=true)
+    allsky_map = HealpixMap{Float64,RingOrder}(Nside)
+    weight_map = HealpixMap{Float64,RingOrder}(Nside)
+    if !calc_mean && iszero(sum(Bin_q))
+        return allsky_map, weight_map
+    end
+    pos, hsml, m, rho, bin_q, weights = filter_sort_particles(Pos, Hsml, M, Rho, Bin_q, Weights,
+                                                              center, radius_limits, calc_mean)
+    healpix_deposit!(allsky_map, weight_map, pos, hsml, m, rho, bin_q, weights, Nside, kernel,
+                     calc_mean)
     return allsky_map, weight_map
 end
 

@@ -178,3 +178,64 @@ def test_batch_argument_errors(s2g):
     assert rc == _lib.S2G_EINVAL
     rc = ctx.lib.s2g_deposit_batch(ctx.handle, _lib.S2G_F32, None, 2, None, None)
     assert rc == _lib.S2G_EINVAL
+
+
+def test_inputs_written_on_the_null_stream_just_before(s2g, orc):
+    """Ordering with the caller's stream: torch queues a long run of work and
+    then the writes of the inputs (a device copy from scratch values) on its
+    current (null) stream, and immediately calls s2g_deposit_batch and
+    s2g_deposit_device with stream NULL (the context's stream, which is
+    blocking, so it waits for that work).  The map is the nq = 3, 200^2,
+    reduced, calc_mean map of the batch tests; the result must be the
+    oracle's for the written inputs, on both entry points and on repeats."""
+    import torch
+    from sph2grid import _lib
+    dev = torch.device("cuda:0")
+    c = _calls(s2g, np.float32)[4]
+    npix, nq = 200, 3
+    pm_args = [c[k] for k in ("Pos", "HSML", "M", "Rho", "Bin_Quant", "Weights")]
+    from sph2grid.mapping import _prepare
+    pm = _prepare(*[a.copy() for a in pm_args], param=c["param"], kernel=c["kernel"],
+                  show_progress=False, reduce_image=True, calc_mean=True)
+    src = [torch.from_numpy(a).to(dev) for a in pm.arrays]
+    want = _oracle_result(orc, c)
+    ctx = s2g.default_context()
+    big = torch.randn(3072, 3072, device=dev)
+    for rep in range(3):
+        # wrong values first (zero hsml: every particle would be skipped)
+        dst = [torch.zeros_like(a) for a in src]
+        torch.cuda.synchronize()
+        for _ in range(6):
+            big = (big @ big).mul_(1e-3)  # keeps the null stream busy
+        for d, a in zip(dst, src):
+            d.copy_(a)  # queued behind the matmuls, on the null stream
+        out = torch.empty(npix * npix * nq, dtype=torch.float64, device=dev)
+        s2g.deposit_batch_device([dict(pos=dst[0], hsml=dst[1], m=dst[2], rho=dst[3],
+                                       bin_q=dst[4], weights=dst[5], out=out, n=pm.n, nq=nq,
+                                       npix=npix, len2pix=pm.params.len2pix, kernel=pm.kid,
+                                       calc_mean=True, reduce_image=True)], dtype="f32")
+        got = out.cpu().numpy().reshape((npix, npix, nq), order="F")
+        e = rel_l1(got, want)
+        print(f"rep {rep}: batch rel_l1 vs oracle {e:.2e}", flush=True)
+        assert e <= 1e-5
+        # the single-map device entry point, same ordering
+        dst = [torch.zeros_like(a) for a in src]
+        torch.cuda.synchronize()
+        for _ in range(6):
+            big = (big @ big).mul_(1e-3)
+        for d, a in zip(dst, src):
+            d.copy_(a)
+        img = torch.empty(npix * npix * (nq + 1), dtype=torch.float64, device=dev)
+        vp = lambda t: C.c_void_p(t.data_ptr())
+        ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F32, vp(dst[0]), vp(dst[1]),
+                                             vp(dst[2]), vp(dst[3]), vp(dst[4]), nq, vp(dst[5]),
+                                             pm.n, C.byref(pm.params), pm.kid, 2,
+                                             _lib.S2G_FLAG_CALC_MEAN, vp(img), None, None),
+                  "s2g_deposit_device")
+        red = torch.empty(npix * npix * nq, dtype=torch.float64, device=dev)
+        ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, npix, nq, _lib.S2G_FLAG_REDUCE,
+                                            vp(img), vp(red), None), "s2g_reduce_device")
+        got = red.cpu().numpy().reshape((npix, npix, nq), order="F")
+        e = rel_l1(got, want)
+        print(f"rep {rep}: device entry point rel_l1 vs oracle {e:.2e}", flush=True)
+        assert e <= 1e-5

@@ -125,3 +125,72 @@ def test_allsky_workload_scaled(s2g, orc):
     pos = d["Pos"].T.copy()
     _compare(s2g, orc, pos, d["HSML"], d["M"], d["Rho"], d["T"], d["Rho"], 128, "wendlandc4",
              tag="allsky")
+
+
+@pytest.mark.parametrize("kname", ["wendlandc4", "cubic"])
+def test_pole_centred_particle_closed_form(s2g, kname):
+    """One particle on the z axis, no oracle involved: the weight map equals
+    V * wk_i * A_i / sum(wk * A) (main.jl:25-45, 56-63, 193) with the ring
+    geometry of hp_geometry.py, A the piecewise contributing_area
+    (pixel_weights.jl:6-8) and wk the kernel shape at d / theta.  FP32 pixel
+    weights on the GPU: bound 1e-5 relative per pixel (above 1e-3 of the
+    maximum) and 1e-6 on the sum."""
+    import oracle
+    from hp_geometry import area_closed_form, ring_centres
+    nside, Dx = 64, 0.45
+    ang = np.sqrt(4 * np.pi / (12 * nside * nside))
+    centres, _ = ring_centres(nside)
+    d = np.arccos(np.clip(centres[:, 2], -1.0, 1.0))
+    for theta_pix in (1.7, 6.3, 40.5):
+        theta = theta_pix * ang
+        h = Dx * np.sin(theta)
+        m, rho, w, q = 1.5, 0.25, 2.0, 3.0
+        pos = np.array([[0.0], [0.0], [Dx]])
+        gm, gw = s2g.healpix_map(pos, np.array([h]), np.array([m]), np.array([rho]),
+                                 np.array([q]), np.array([w]), Nside=nside,
+                                 kernel=_kernel(s2g, kname), show_progress=False)
+        A = area_closed_form(d, theta, ang) * (d <= theta)
+        wk = np.array([oracle.kernel_shape(kname, x) if x <= 1 else 0.0 for x in d / theta])
+        V = w * m / rho / (ang * Dx) ** 2
+        expect = V * wk * A / (wk * A).sum()
+        big = expect > 1e-3 * expect.max()
+        err = np.abs(gw - expect)[big] / expect[big]
+        print(f"[pole {kname} theta={theta_pix} px] max rel {err.max():.2e}", flush=True)
+        assert err.max() <= 1e-5
+        assert np.all(gw[d > theta + 1e-9] == 0.0)
+        assert abs(gw.sum() / V - 1) <= 1e-6
+        np.testing.assert_allclose(gm, q * gw, rtol=1e-12)
+
+
+def test_fallback_particle_one_pixel(s2g):
+    """A particle whose disc holds no pixel centre deposits its whole weight
+    V into the pixel holding it (constributing_pixels.jl:16-19,
+    pixel_weights.jl:120-137)."""
+    from hp_geometry import ring_centres
+    nside, Dx = 16, 1.3
+    ang = np.sqrt(4 * np.pi / (12 * nside * nside))
+    centres, _ = ring_centres(nside)
+    c = centres[1000]
+    t = np.cross(c, [0.0, 0.0, 1.0])
+    t /= np.linalg.norm(t)
+    for d_pix in (0.2, 0.35, 0.6):
+        d = d_pix * ang
+        pos = (Dx * (np.cos(d) * c + np.sin(d) * t)).reshape(3, 1)
+        h = Dx * np.sin(0.12 * ang)
+        gm, gw = s2g.healpix_map(pos.copy(), np.array([h]), np.array([2.0]), np.array([0.5]),
+                                 np.array([7.0]), np.array([1.5]), Nside=nside,
+                                 kernel=s2g.WendlandC4(2), show_progress=False)
+        V = 1.5 * 2.0 / 0.5 / (ang * Dx) ** 2
+        assert np.flatnonzero(gw).tolist() == [1000]
+        assert abs(gw[1000] / V - 1) <= 1e-12 and abs(gm[1000] / (7.0 * V) - 1) <= 1e-12
+
+
+def test_allsky_nside1024_ring_lds(s2g, orc):
+    """The all-sky particle set at Nside 1024 (rings of up to 4096 pixels: the
+    largest rows of the ring-major LDS deposit), 2^12 particles against the
+    oracle."""
+    from sph2grid import workloads
+    d = workloads.allsky(n=1 << 12, nside=1024)
+    pos = d["Pos"].T.copy()
+    _compare(s2g, orc, pos, d["HSML"], d["M"], d["Rho"], d["T"], d["Rho"], 1024, "wendlandc4",
+             tag="allsky1024")

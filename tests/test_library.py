@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(s2g):
     L = _lib.load()
     for name in _declared():
         assert hasattr(L, name), f"{name} declared in include/sph2grid.h but not exported"
-    assert set(_declared()) <= set(_lib.EXPORTS) | {"s2g_footprints"}
+    assert set(_declared()) == set(_lib.EXPORTS)
 
 
 def test_version_string(s2g):
