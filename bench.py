@@ -25,6 +25,9 @@ import numpy as np  # noqa: E402
 
 PEAK_HBM_GBPS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+# the FP32 vector rate without packed (two-wide) FP32 instructions, which this
+# build does not emit: one FP32 FMA per lane and cycle
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
 # flops per pixel-update of the 2D deposit (roofline convention): 10 + 2*Nq + K
 K_FLOPS = {"cubic": 6, "quintic": 16, "wendlandc2": 6, "wendlandc4": 9, "wendlandc6": 11,
            "wendlandc8": 14}
@@ -67,10 +70,12 @@ def prepare(d, s2g):
     return out
 
 
-def cpu_baseline(p, target_s, threads):
+def cpu_baseline(p, threads, frac=1.0 / 16):
     """Oracle (FP64 C restatement, reference parallel=true layout: contiguous
-    batches, private images, summed in worker order) on a seeded particle
-    subsample; scaled to the full set by the ratio of footprint sums."""
+    batches, private images, summed in worker order) on a seeded uniform
+    particle subsample of `frac` of the set; the full-set time is a + b *
+    (footprint pixels), with a (the per-call cost of the full-size images of
+    the reference layout) from a second, tiny sample."""
     import oracle
     n = p["x"].shape[0]
     wi, wj = footprints(p["x"], p["hsml"], p["npix"], p["len2pix"])
@@ -78,9 +83,8 @@ def cpu_baseline(p, target_s, threads):
     fp_full = float(fp.sum())
     rng = np.random.default_rng(12345)
 
-    def timed(fs_target):
-        frac = min(1.0, fs_target / max(fp_full, 1.0))
-        idx = np.sort(rng.choice(n, size=max(1, int(n * frac)), replace=False))
+    def timed(m):
+        idx = np.sort(rng.choice(n, size=max(1, int(m)), replace=False))
         sub = {k: p[k][idx] for k in ("x", "hsml", "m", "rho", "w", "bq")}
         t0 = time.perf_counter()
         img = oracle.cic_mapping_2D(sub["x"], sub["hsml"], sub["m"], sub["rho"], sub["bq"],
@@ -90,26 +94,18 @@ def cpu_baseline(p, target_s, threads):
         dt = time.perf_counter() - t0
         fs = float(fp[idx].sum())
         log(f"cpu baseline sample: {idx.size} particles, {fs:.3g} footprint pixels, "
-            f"{dt:.2f} s")
+            f"{dt:.2f} s on {threads} threads")
         return idx.size, fs, dt
 
-    # wall time = a + b * (footprint pixels): a is the per-call cost of the
-    # reference layout (one full-size image + scratch per worker, summed in
-    # worker order), b the per-pixel cost; fit on two samples, the second
-    # sized to ~target_s, then extrapolate to the full set
-    _, f1, t1 = timed(min(fp_full, 3e7))
-    _, f2, t2 = timed(min(fp_full, 1.5e8))
+    _, f1, t1 = timed(max(64, n * 3e7 / max(fp_full, 1.0)))
+    n2, f2, t2 = timed(n * frac)
     b = max((t2 - t1) / max(f2 - f1, 1.0), 1e-15)
     a = max(t1 - b * f1, 0.0)
-    f3_target = min(fp_full, max(f2, (target_s - a) / b))
-    n3, f3, t3 = timed(f3_target)
-    b = max((t3 - t1) / max(f3 - f1, 1.0), 1e-15) if f3 > f1 else b
-    a = max(t1 - b * f1, 0.0)
     t_full = a + b * fp_full
-    sample = (f"{n3} of {n} particles (seeded uniform subsample, {f3:.3g} of {fp_full:.3g} "
-              f"footprint pixels) in {t3:.1f} s on {threads} threads; full set "
-              f"extrapolated as a + b*footprint (a = {a:.2f} s per call, b = {b:.3g} s per "
-              f"pixel-update) = {t_full:.1f} s")
+    sample = (f"{n2} of {n} particles (seeded uniform 1/{round(1 / frac)} subsample, {f2:.3g} "
+              f"of {fp_full:.3g} footprint pixels) in {t2:.1f} s on {threads} threads; full "
+              f"set = a + b*footprint with a = {a:.2f} s per call (from a {t1:.2f} s sample of "
+              f"{f1:.3g} pixels) and b = {b:.3g} s per pixel-update: {t_full:.1f} s")
     return {"value": n / t_full, "unit": "particles/s", "cores": threads, "kind": "port",
             "sample": sample}
 
@@ -531,7 +527,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the particle count")
     ap.add_argument("--npix", type=int, default=0)
     ap.add_argument("--exact-norm", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16,
+                    help="particle fraction of the timed CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other workloads (3D, all-sky, TSC, plain, sedov)")
@@ -643,11 +640,13 @@ def main():
     s_in = p["x"].dtype.itemsize
     b_alg = n * (6 + nq) * s_in + nd * nq * 8
     traffic, traffic_src = measured_traffic(args, n, npix)
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                 "traffic_source": traffic_src,
                 "kernel": "k2d_gather", "compute_unit": "FP32 vector ALU (no MFMA: "
-                "scatter/gather); peak = MI355X FP32 vector rate",
+                "scatter/gather); peak = MI355X FP32 vector rate with packed FP32",
+                "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+                "frac_nonpacked": achieved / PEAK_FP32_NONPACKED_TFLOPS,
                 "alg_flops_per_launch": fp_large * fk,
                 "alg_pixel_updates_per_launch": fp_large,
                 "ms_per_launch": stage["ms_gather"],
@@ -663,8 +662,13 @@ def main():
             aff = len(os.sched_getaffinity(0))
         except AttributeError:
             aff = os.cpu_count() or 1
+        # one GPU's share of the host is 16 cores, whatever the affinity mask
+        # shows; fewer if fewer are visible
         threads = max(1, min(16, aff))
-        cpu = cpu_baseline(p, args.cpu_seconds, threads)
+        cpu = cpu_baseline(p, threads, args.cpu_fraction)
+        cpu["cores_visible"] = aff
+        cpu["cores_note"] = ("threads = min(16, visible cores): 16 host cores are one GPU's "
+                             "share of this box")
 
     out = {
         "metric": "particles/s (2D 4096^2 WendlandC6 map, cluster 2^24)"

@@ -8,6 +8,20 @@
 //   W(d) = 3/4 - d^2          |d| < 1/2
 //        = (3/2 - |d|)^2 / 2  1/2 <= |d| < 3/2
 // and they sum to one.  Plane 0 accumulates q*W, plane 1 accumulates W.
+//
+// Pipeline (a gather: every cell is written once with a plain store, no
+// atomics and no memset, and the result is bit-reproducible):
+//   k_tsc_keys     brick of each particle's nearest cell (bricks of
+//                  kTX x kTY x kTZ cells, row-major; dropped particles last)
+//   (radix sort)   stable, so particles keep their input order in a brick
+//   k_tsc_offsets  first sorted particle of every brick
+//   k_tsc_prep     grid coordinates (FP64) and quantity in sorted order, and
+//                  each particle's nearest cell packed into one word
+//   k_tsc_gather   one wave per brick: scans the particles of the (up to) 27
+//                  neighbouring bricks, and deposits those whose 3^3 stencil
+//                  reaches the brick into LDS, one particle per step (27
+//                  lanes, one stencil cell each), in a fixed order; then
+//                  writes the brick's cells.
 #include "k_tsc.h"
 
 namespace s2g {
@@ -24,16 +38,21 @@ __device__ __forceinline__ double tsc_w(double d)
     return 0.0;
 }
 
-// nearest cell of particle p along each axis, as k_tsc forms it; false if
-// the particle is dropped (a non-finite coordinate, or far outside a
-// non-periodic grid)
+__host__ __device__ __forceinline__ int64_t tsc_wrap(int64_t c, int64_t npix)
+{
+    return ((c % npix) + npix) % npix;
+}
+
+// nearest cell of particle p along each axis; false if the particle is
+// dropped (a non-finite coordinate, or far outside a non-periodic grid)
 template <typename T>
 __device__ __forceinline__ bool tsc_cell(const T *__restrict__ pos, int64_t p, int64_t npix,
-                                         double len2pix, int periodic, int64_t c[3])
+                                         double len2pix, int periodic, double g[3],
+                                         int64_t c[3])
 {
     for (int d = 0; d < 3; ++d) {
-        const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
-        const double fg = floor(g);
+        g[d] = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
+        const double fg = floor(g[d]);
         if (!isfinite(fg) || ((fg < -2.0 || fg > (double)npix + 2.0) && !periodic))
             return false;
         c[d] = (int64_t)fg;
@@ -41,9 +60,23 @@ __device__ __forceinline__ bool tsc_cell(const T *__restrict__ pos, int64_t p, i
     return true;
 }
 
-// Sort key: the 8^3 brick of the particle's nearest cell (row-major bricks;
-// dropped particles last), so that a wave's 27-cell stencils share cache
-// lines and the atomics stay in L2 instead of scattering over the grid.
+// the brick grid of an npix^3 grid
+__host__ __device__ __forceinline__ void tsc_bricks(int64_t npix, int64_t nb[3])
+{
+    nb[0] = (npix + kTX - 1) / kTX;
+    nb[1] = (npix + kTY - 1) / kTY;
+    nb[2] = (npix + kTZ - 1) / kTZ;
+}
+
+int64_t tsc_nbricks(int64_t npix)
+{
+    int64_t nb[3];
+    tsc_bricks(npix, nb);
+    return nb[0] * nb[1] * nb[2];
+}
+
+// Sort key: the brick of the particle's nearest cell (clamped into the grid,
+// or wrapped when periodic); dropped particles get nbricks, after all bricks.
 template <typename T>
 __global__ __launch_bounds__(256) void k_tsc_keys(const T *__restrict__ pos, int64_t n,
                                                   int64_t npix, double len2pix, int periodic,
@@ -53,90 +86,202 @@ __global__ __launch_bounds__(256) void k_tsc_keys(const T *__restrict__ pos, int
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n)
         return;
-    const int64_t nb = (npix + 7) / 8;
+    int64_t nb[3];
+    tsc_bricks(npix, nb);
+    double g[3];
     int64_t c[3];
-    uint32_t key = (uint32_t)(nb * nb * nb);
-    if (tsc_cell(pos, p, npix, len2pix, periodic, c)) {
+    uint32_t key = (uint32_t)(nb[0] * nb[1] * nb[2]);
+    if (tsc_cell(pos, p, npix, len2pix, periodic, g, c)) {
+        const int64_t side[3] = {kTX, kTY, kTZ};
         int64_t b[3];
         for (int d = 0; d < 3; ++d) {
-            int64_t x = periodic ? ((c[d] % npix) + npix) % npix : c[d];
+            int64_t x = periodic ? tsc_wrap(c[d], npix) : c[d];
             x = x < 0 ? 0 : x > npix - 1 ? npix - 1 : x;
-            b[d] = x / 8;
+            b[d] = x / side[d];
         }
-        key = (uint32_t)((b[0] * nb + b[1]) * nb + b[2]);
+        key = (uint32_t)((b[0] * nb[1] + b[1]) * nb[2] + b[2]);
     }
     keys[p] = key;
     vals[p] = (int32_t)p;
 }
 
-constexpr int kTscRows = 3;  // threads per particle
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_tsc(const T *__restrict__ pos, const T *__restrict__ binq,
-                                             int64_t n, int64_t npix, double len2pix,
-                                             int periodic, const int32_t *__restrict__ order,
-                                             double *__restrict__ grid)
+// off[b] = first sorted entry with key >= b, b in [0, nbricks]
+__global__ __launch_bounds__(256) void k_tsc_offsets(const uint32_t *__restrict__ keys,
+                                                     int64_t n, int64_t nbricks,
+                                                     int64_t *__restrict__ off)
 {
-    // three threads per particle, one per x slab of the 27-cell stencil:
-    // more atomics in flight than one thread issuing all 54 (2^22 particles
-    // on 512^3: 8.9 ms with one thread, 8.4 with three, 8.5 with nine)
-    const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t t = tt / kTscRows;
-    const int slab = (int)(tt - kTscRows * t);
-    if (t >= n)
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nbricks)
         return;
-    const int64_t p = order ? (int64_t)order[t] : t;
-    const int64_t nd = npix * npix * npix;
-    int64_t c[3];
-    double w[3][3];
-    bool bad = false;
-    for (int d = 0; d < 3; ++d) {
-        const double g = add_rn(mul_rn(ld(pos, 3 * p + d), len2pix), 0.5 * (double)npix);
-        const double fg = floor(g);
-        if (!isfinite(fg) || fg < -2.0 || fg > (double)npix + 2.0) {
-            if (!periodic || !isfinite(fg))
-                bad = true;
-        }
-        c[d] = bad ? 0 : (int64_t)fg;
-        for (int o = -1; o <= 1; ++o)
-            w[d][o + 1] = tsc_w(sub_rn(g, (double)(c[d] + o) + 0.5));
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)keys[mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
-    if (bad)
+    off[b] = lo;
+}
+
+// packed nearest cell: 21 bits per axis, each stored as c + kCellBias (c is
+// in [-2, npix + 2] when not periodic, wrapped into [0, npix) when periodic)
+constexpr int64_t kCellBias = 4;
+__device__ __forceinline__ uint64_t tsc_pack(const int64_t c[3])
+{
+    return (uint64_t)(c[0] + kCellBias) << 42 | (uint64_t)(c[1] + kCellBias) << 21 |
+           (uint64_t)(c[2] + kCellBias);
+}
+__device__ __forceinline__ int64_t tsc_unpack(uint64_t v, int d)
+{
+    return (int64_t)((v >> (42 - 21 * d)) & 0x1FFFFF) - kCellBias;
+}
+
+// sorted grid coordinates and quantity (gq) and packed cells (cell) of the
+// particles that are not dropped (the first off[nbricks] sorted entries)
+template <typename T>
+__global__ __launch_bounds__(256) void k_tsc_prep(const T *__restrict__ pos,
+                                                  const T *__restrict__ binq,
+                                                  const int32_t *__restrict__ order,
+                                                  const int64_t *__restrict__ off,
+                                                  int64_t nbricks, int64_t npix, double len2pix,
+                                                  int periodic, double4 *__restrict__ gq,
+                                                  uint64_t *__restrict__ cell)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= off[nbricks])
         return;
-    const double q = ld(binq, p);
-    int64_t i = c[0] + slab - 1;
+    const int64_t p = order[k];
+    double g[3];
+    int64_t c[3];
+    tsc_cell(pos, p, npix, len2pix, periodic, g, c);
     if (periodic)
-        i = ((i % npix) + npix) % npix;
-    else if (i < 0 || i >= npix)
-        return;
-    for (int b = -1; b <= 1; ++b) {
-        int64_t j = c[1] + b;
+        for (int d = 0; d < 3; ++d)
+            c[d] = tsc_wrap(c[d], npix);
+    gq[k] = make_double4(g[0], g[1], g[2], ld(binq, p));
+    cell[k] = tsc_pack(c);
+}
+
+// bitmask of the stencil offsets o in {-1, 0, 1} (bit o + 1) whose cell
+// c + o (wrapped when periodic) lies in [b0, b0 + side) and in the grid
+__device__ __forceinline__ int tsc_reach(int64_t c, int64_t b0, int64_t side, int64_t npix,
+                                         int periodic)
+{
+    int m = 0;
+#pragma unroll
+    for (int o = -1; o <= 1; ++o) {
+        int64_t x = c + o;
         if (periodic)
-            j = ((j % npix) + npix) % npix;
-        else if (j < 0 || j >= npix)
-            continue;
-        const double wab = w[0][slab] * w[1][b + 1];
-        for (int e = -1; e <= 1; ++e) {
-            int64_t k = c[2] + e;
+            x = x < 0 ? x + npix : x >= npix ? x - npix : x;
+        const bool in = x >= b0 && x < b0 + side && x >= 0 && x < npix;
+        m |= in ? 1 << (o + 1) : 0;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict__ gq,
+                                                      const uint64_t *__restrict__ cell,
+                                                      const int64_t *__restrict__ off,
+                                                      int64_t npix, int periodic,
+                                                      double *__restrict__ grid)
+{
+    __shared__ double acc[2][kTX * kTY * kTZ];
+    int64_t nb[3];
+    tsc_bricks(npix, nb);
+    const int64_t side[3] = {kTX, kTY, kTZ};
+    const int64_t b = blockIdx.x;
+    const int64_t bi[3] = {b / (nb[1] * nb[2]), (b / nb[2]) % nb[1], b % nb[2]};
+    int64_t b0[3];
+    for (int d = 0; d < 3; ++d)
+        b0[d] = bi[d] * side[d];
+    const int ln = threadIdx.x;
+    for (int e = ln; e < 2 * kTX * kTY * kTZ; e += kWave)
+        (&acc[0][0])[e] = 0.0;
+    __syncthreads();
+    // this lane's stencil cell (lanes 0..26)
+    const int o0 = ln / 9 - 1, o1 = (ln / 3) % 3 - 1, o2 = ln % 3 - 1;
+    // neighbouring bricks, each once (periodic grids of one or two bricks
+    // per axis wrap onto the same brick)
+    int64_t nbr[3][3];
+    int nn[3];
+    for (int d = 0; d < 3; ++d) {
+        nn[d] = 0;
+        for (int o = -1; o <= 1; ++o) {
+            int64_t x = bi[d] + o;
             if (periodic)
-                k = ((k % npix) + npix) % npix;
-            else if (k < 0 || k >= npix)
+                x = x < 0 ? x + nb[d] : x >= nb[d] ? x - nb[d] : x;
+            else if (x < 0 || x >= nb[d])
                 continue;
-            const double ww = wab * w[2][e + 1];
-            if (ww == 0.0)
-                continue;
+            bool dup = false;
+            for (int k = 0; k < nn[d]; ++k)
+                dup = dup || nbr[d][k] == x;
+            if (!dup)
+                nbr[d][nn[d]++] = x;
+        }
+    }
+    for (int a = 0; a < nn[0]; ++a)
+        for (int bb = 0; bb < nn[1]; ++bb)
+            for (int cc = 0; cc < nn[2]; ++cc) {
+                const int64_t nbrick = (nbr[0][a] * nb[1] + nbr[1][bb]) * nb[2] + nbr[2][cc];
+                const int64_t s = off[nbrick], t = off[nbrick + 1];
+                for (int64_t k0 = s; k0 < t; k0 += kWave) {
+                    const int64_t k = k0 + ln;
+                    bool use = false;
+                    if (k < t) {
+                        const uint64_t v = cell[k];
+                        use = true;
+                        for (int d = 0; d < 3; ++d)
+                            use = use && tsc_reach(tsc_unpack(v, d), b0[d], side[d], npix,
+                                                   periodic) != 0;
+                    }
+                    uint64_t mask = __ballot(use);
+                    while (mask) {
+                        const int src = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const double4 p = gq[k0 + src];
+                        if (ln < 27) {
+                            const double g[3] = {p.x, p.y, p.z};
+                            const int o[3] = {o0, o1, o2};
+                            double w[3];
+                            int64_t loc[3];
+                            bool in = true;
+                            for (int d = 0; d < 3; ++d) {
+                                const int64_t c = (int64_t)floor(g[d]);
+                                w[d] = tsc_w(sub_rn(g[d], (double)(c + o[d]) + 0.5));
+                                int64_t x = c + o[d];
+                                if (periodic)
+                                    x = tsc_wrap(x, npix);
+                                loc[d] = x - b0[d];
+                                in = in && x >= 0 && x < npix && loc[d] >= 0 && loc[d] < side[d];
+                            }
+                            const double ww = (w[0] * w[1]) * w[2];
+                            if (in && ww != 0.0) {
+                                const int e = (int)((loc[0] * kTY + loc[1]) * kTZ + loc[2]);
+                                unsafeAtomicAdd(&acc[0][e], p.w * ww);
+                                unsafeAtomicAdd(&acc[1][e], ww);
+                            }
+                        }
+                    }
+                }
+            }
+    __syncthreads();
+    const int64_t nd = npix * npix * npix;
+    for (int e = ln; e < kTX * kTY * kTZ; e += kWave) {
+        const int64_t i = b0[0] + e / (kTY * kTZ), j = b0[1] + (e / kTZ) % kTY,
+                      k = b0[2] + e % kTZ;
+        if (i < npix && j < npix && k < npix) {
             const int64_t idx = (i * npix + j) * npix + k;
-            unsafeAtomicAdd(&grid[idx], q * ww);
-            unsafeAtomicAdd(&grid[nd + idx], ww);
+            grid[idx] = acc[0][e];
+            grid[nd + idx] = acc[1][e];
         }
     }
 }
 
 int tsc_key_bits(int64_t npix)
 {
-    const int64_t nb = (npix + 7) / 8;
+    const int64_t nbr = tsc_nbricks(npix);
     int bits = 1;
-    while (bits < 32 && ((int64_t)1 << bits) <= nb * nb * nb)
+    while (bits < 32 && ((int64_t)1 << bits) <= nbr)
         ++bits;
     return bits;
 }
@@ -155,18 +300,38 @@ void launch_tsc_keys(int dtype, const void *pos, int64_t n, int64_t npix, double
                            len2pix, periodic, keys, vals);
 }
 
-void launch_tsc(int dtype, const void *pos, const void *binq, int64_t n, int64_t npix,
-                double len2pix, int periodic, const int32_t *order, double *grid, hipStream_t s)
+void launch_tsc_offsets(const uint32_t *sorted_keys, int64_t n, int64_t npix, int64_t *off,
+                        hipStream_t s)
+{
+    const int64_t nbr = tsc_nbricks(npix);
+    hipLaunchKernelGGL(k_tsc_offsets, dim3((unsigned)((nbr + 1 + 255) / 256)), dim3(256), 0, s,
+                       sorted_keys, n, nbr, off);
+}
+
+void launch_tsc_prep(int dtype, const void *pos, const void *binq, const int32_t *order,
+                     const int64_t *off, int64_t n, int64_t npix, double len2pix, int periodic,
+                     double4 *gq, uint64_t *cell, hipStream_t s)
 {
     if (n == 0)
         return;
-    const dim3 g((unsigned)((kTscRows * n + 255) / 256)), blk(256);
+    const int64_t nbr = tsc_nbricks(npix);
+    const dim3 g((unsigned)((n + 255) / 256)), blk(256);
     if (dtype == S2G_F32)
-        hipLaunchKernelGGL(k_tsc<float>, g, blk, 0, s, (const float *)pos, (const float *)binq,
-                           n, npix, len2pix, periodic, order, grid);
+        hipLaunchKernelGGL(k_tsc_prep<float>, g, blk, 0, s, (const float *)pos,
+                           (const float *)binq, order, off, nbr, npix, len2pix, periodic, gq,
+                           cell);
     else
-        hipLaunchKernelGGL(k_tsc<double>, g, blk, 0, s, (const double *)pos,
-                           (const double *)binq, n, npix, len2pix, periodic, order, grid);
+        hipLaunchKernelGGL(k_tsc_prep<double>, g, blk, 0, s, (const double *)pos,
+                           (const double *)binq, order, off, nbr, npix, len2pix, periodic, gq,
+                           cell);
+}
+
+void launch_tsc_gather(const double4 *gq, const uint64_t *cell, const int64_t *off,
+                       int64_t npix, int periodic, double *grid, hipStream_t s)
+{
+    const int64_t nbr = tsc_nbricks(npix);
+    hipLaunchKernelGGL(k_tsc_gather, dim3((unsigned)nbr), dim3(kWave), 0, s, gq, cell, off,
+                       npix, periodic, grid);
 }
 
 }  // namespace s2g

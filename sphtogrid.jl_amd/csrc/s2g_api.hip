@@ -37,6 +37,8 @@ struct s2g_ctx {
     PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
     DevBuf hp_rec, hp_nunits, hp_units, hp_part, hp_ring, hp_list;
+    // TSC: brick offsets, sorted grid coordinates + quantity, packed cells
+    DevBuf tsc_off, tsc_gq, tsc_cell;
     // outputs
     DevBuf image, out;
     // pinned host scratch for small device->host reads
@@ -871,6 +873,9 @@ void s2g_destroy(s2g_ctx *c)
     c->hp_part.release();
     c->hp_ring.release();
     c->hp_list.release();
+    c->tsc_off.release();
+    c->tsc_gq.release();
+    c->tsc_cell.release();
     c->image.release();
     c->out.release();
     if (c->h_small)
@@ -1181,33 +1186,94 @@ int s2g_healpix_deposit_device(s2g_ctx *c, int dtype, const void *d_pos, const v
     });
 }
 
-// the particles ordered by the 8^3 brick of their nearest cell (stable radix
-// sort in the context's buffers), or null when there is nothing to deposit
-static const int32_t *tsc_order(s2g_ctx *c, int dtype, const void *d_pos, int64_t n,
-                                int64_t npix, double len2pix, uint32_t flags, hipStream_t s)
+// Work space of one TSC deposit of n particles on an npix^3 grid.
+struct TscWork {
+    uint32_t *keys;   // 2n: keys, sorted keys
+    int32_t *vals;    // 2n: particles, sorted particles
+    int64_t *off;     // nbricks + 1
+    double4 *gq;      // n
+    uint64_t *cell;   // n
+    void *sort_tmp;
+    size_t sort_bytes;
+};
+
+// Bytes of a TscWork laid out in one buffer (each part 256-byte aligned).
+static size_t tsc_work_bytes(int64_t n, int64_t npix, size_t sort_bytes)
 {
-    if (n == 0)
-        return nullptr;
-    if (n >= (int64_t)INT32_MAX)
-        throw Error(S2G_EUNSUPPORTED, "more than 2^31 - 1 particles for the TSC deposit");
-    uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * n);
-    int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * n);
-    launch_tsc_keys(dtype, d_pos, n, npix, len2pix, (flags & S2G_FLAG_PERIODIC) ? 1 : 0, keys,
-                    vals, s);
-    const int end_bit = tsc_key_bits(npix);
-    const size_t tb = sort_pairs_tmp_bytes(n, end_bit);
-    void *stmp = c->sort_tmp.get(tb ? tb : 1);
-    sort_pairs(stmp, tb, keys, keys + n, vals, vals + n, n, end_bit, s);
-    return vals + n;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    return al(8 * m) + al(8 * m) + al(sizeof(int64_t) * (size_t)(tsc_nbricks(npix) + 1)) +
+           al(sizeof(double4) * m) + al(8 * m) + al(sort_bytes ? sort_bytes : 1);
+}
+
+static TscWork tsc_work_in(void *base, int64_t n, int64_t npix, size_t sort_bytes)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    char *p = (char *)base;
+    TscWork w;
+    w.keys = (uint32_t *)p;
+    p += al(8 * m);
+    w.vals = (int32_t *)p;
+    p += al(8 * m);
+    w.off = (int64_t *)p;
+    p += al(sizeof(int64_t) * (size_t)(tsc_nbricks(npix) + 1));
+    w.gq = (double4 *)p;
+    p += al(sizeof(double4) * m);
+    w.cell = (uint64_t *)p;
+    p += al(8 * m);
+    w.sort_tmp = p;
+    w.sort_bytes = sort_bytes;
+    return w;
+}
+
+static void check_tsc(int dtype, int64_t n, const s2g_params *params)
+{
+    check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
+    if (params->npixels > 2048)
+        throw Error(S2G_EINVAL, "npixels out of range [1, 2048] for the TSC grid");
+}
+
+// The whole deposit on `s` (enqueues only): sort by brick, offsets, prep,
+// gather.  Writes every cell of d_grid.
+static void tsc_run(int dtype, const void *d_pos, const void *d_q, int64_t n, int64_t npix,
+             double len2pix, int periodic, const TscWork &w, double *d_grid, hipStream_t s)
+{
+    if (n > 0) {
+        launch_tsc_keys(dtype, d_pos, n, npix, len2pix, periodic, w.keys, w.vals, s);
+        sort_pairs(w.sort_tmp, w.sort_bytes, w.keys, w.keys + n, w.vals, w.vals + n, n,
+                   tsc_key_bits(npix), s);
+    }
+    // n == 0: every offset is 0 (no key is read)
+    launch_tsc_offsets(w.keys + n, n, npix, w.off, s);
+    launch_tsc_prep(dtype, d_pos, d_q, w.vals + n, w.off, n, npix, len2pix, periodic, w.gq,
+                    w.cell, s);
+    launch_tsc_gather(w.gq, w.cell, w.off, npix, periodic, d_grid, s);
+}
+
+// the same on the context's buffers (grown as needed)
+static void tsc_ctx(s2g_ctx *c, int dtype, const void *d_pos, const void *d_q, int64_t n,
+             const s2g_params *params, uint32_t flags, double *d_grid, hipStream_t s)
+{
+    const int64_t npix = params->npixels;
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    TscWork w;
+    w.keys = (uint32_t *)c->skeys.get(8 * m);
+    w.vals = (int32_t *)c->svals.get(8 * m);
+    w.off = (int64_t *)c->tsc_off.get(sizeof(int64_t) * (size_t)(tsc_nbricks(npix) + 1));
+    w.gq = (double4 *)c->tsc_gq.get(sizeof(double4) * m);
+    w.cell = (uint64_t *)c->tsc_cell.get(8 * m);
+    w.sort_bytes = n > 0 ? sort_pairs_tmp_bytes(n, tsc_key_bits(npix)) : 0;
+    w.sort_tmp = c->sort_tmp.get(w.sort_bytes ? w.sort_bytes : 1);
+    tsc_run(dtype, d_pos, d_q, n, npix, params->len2pix, (flags & S2G_FLAG_PERIODIC) ? 1 : 0, w,
+            d_grid, s);
 }
 
 int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t n,
               const s2g_params *params, uint32_t flags, double *grid_out, s2g_stats *stats)
 {
     return guarded(c, [&]() {
-        check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
-        if (params->npixels > 2048)
-            throw Error(S2G_EINVAL, "npixels out of range [1, 2048] for the TSC grid");
+        check_tsc(dtype, n, params);
         if (!grid_out)
             throw Error(S2G_EINVAL, "output pointer is NULL");
         hipStream_t s = c->stream;
@@ -1230,10 +1296,7 @@ int s2g_tsc3d(s2g_ctx *c, int dtype, const void *pos, const void *bin_q, int64_t
         const int64_t nd = npix * npix * npix;
         double *grid = (double *)c->image.get(sizeof(double) * 2 * nd);
         S2G_HIP(hipEventRecord(E[1], s));
-        S2G_HIP(hipMemsetAsync(grid, 0, sizeof(double) * 2 * nd, s));
-        const int32_t *order = tsc_order(c, dtype, dpos, n, npix, params->len2pix, flags, s);
-        launch_tsc(dtype, dpos, dq, n, npix, params->len2pix,
-                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, order, grid, s);
+        tsc_ctx(c, dtype, dpos, dq, n, params, flags, grid, s);
         S2G_HIP(hipGetLastError());
         S2G_HIP(hipEventRecord(E[2], s));
         S2G_HIP(hipEventSynchronize(E[2]));
@@ -1249,18 +1312,13 @@ int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin
                      const s2g_params *params, uint32_t flags, double *d_grid_out, void *stream)
 {
     return guarded(c, [&]() {
-        check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
-        if (params->npixels > 2048)
-            throw Error(S2G_EINVAL, "npixels out of range [1, 2048] for the TSC grid");
+        check_tsc(dtype, n, params);
         if (!d_grid_out)
             throw Error(S2G_EINVAL, "output pointer is NULL");
+        if (n > 0 && (!d_pos || !d_bin_q))
+            throw Error(S2G_EINVAL, "input array pointer is NULL");
         hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-        const int64_t npix = params->npixels;
-        const int64_t nd = npix * npix * npix;
-        S2G_HIP(hipMemsetAsync(d_grid_out, 0, sizeof(double) * 2 * nd, s));
-        const int32_t *order = tsc_order(c, dtype, d_pos, n, npix, params->len2pix, flags, s);
-        launch_tsc(dtype, d_pos, d_bin_q, n, npix, params->len2pix,
-                   (flags & S2G_FLAG_PERIODIC) ? 1 : 0, order, d_grid_out, s);
+        tsc_ctx(c, dtype, d_pos, d_bin_q, n, params, flags, d_grid_out, s);
         S2G_HIP(hipGetLastError());
         // the particle order lives in the context's buffers: finish before
         // another call (possibly on another stream) reuses them

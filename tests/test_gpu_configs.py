@@ -119,9 +119,10 @@ def test_tsc_allsky_512(s2g, orc):
     assert float(gw[1].min().item()) >= 0.0
     # a 2^14 subsample cell by cell against the oracle
     sub = np.sort(np.random.default_rng(9).choice(n, 1 << 14, replace=False))
-    ctx.check(ctx.lib.s2g_tsc3d_device(ctx.handle, _lib.S2G_F64, vp(dpos[sub].contiguous()),
-                                       vp(dq[sub].contiguous()), sub.size, C.byref(par), 0,
-                                       vp(grid), None), "s2g_tsc3d_device")
+    # named tensors: a temporary's memory could be reused before the kernel reads it
+    spos, sqq = dpos[sub].contiguous(), dq[sub].contiguous()
+    ctx.check(ctx.lib.s2g_tsc3d_device(ctx.handle, _lib.S2G_F64, vp(spos), vp(sqq), sub.size,
+                                       C.byref(par), 0, vp(grid), None), "s2g_tsc3d_device")
     ref = orc.tsc_3d(pos[sub], q[sub], N, L, False)
     nzr = np.flatnonzero(ref[:, 1])
     got = grid.view(2, -1).T

@@ -94,7 +94,7 @@ def test_cluster_2e17_512(s2g, orc):
     keep = s2g.filter_particles_in_image(pos, par2)
     _, cls, _ = compare(s2g, orc, 2, pos[:, keep].T, d["HSML"][keep], d["Bin_Quant"][keep], 512,
                         par2.len2pix, "wendlandc6", "cluster", calc_mean=False)
-    assert np.all(np.bincount(cls, minlength=4) > 0)
+    assert np.all(np.bincount(cls, minlength=4)[1:] > 0)  # T > 0: nothing is skipped
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
