@@ -208,6 +208,22 @@ int s2g_tsc3d_device(s2g_ctx *ctx, int dtype, const void *d_pos,
                      const void *d_bin_q, int64_t n, const s2g_params *params,
                      uint32_t flags, double *d_grid_out, void *stream);
 
+/* Allocation-free variant of s2g_tsc3d_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as s2g_tsc3d_device, same kernels, but its work
+ * space (sort keys, brick offsets, sorted particle data) is a fixed 80 MiB
+ * device array per GPU that the library holds until the process ends, so it
+ * calls no hipMalloc / hipFree; limits: n <= 2^20 particles and npixels <= 512
+ * (larger calls fail with S2G_EUNSUPPORTED before any device work).  It only
+ * enqueues on `stream` (NULL: the context's stream).  Every call on one GPU
+ * uses that GPU's one array: calls that overlap in time on one GPU (from
+ * other streams, contexts or threads) race; calls ordered on one stream do
+ * not. */
+int s2g_tsc3d_device_noalloc(s2g_ctx *ctx, int dtype, const void *d_pos, const void *d_bin_q,
+                             int64_t n, const s2g_params *params, uint32_t flags,
+                             double *d_grid_out, void *stream);
+
 /* One map of a batch (device pointers; inputs centred and filtered as for
  * s2g_deposit). */
 typedef struct s2g_map_desc {

@@ -2,6 +2,7 @@
 // staging of host arrays, orchestration of the device pipelines.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,14 @@
 #include "s2g_internal.h"
 
 using namespace s2g;
+
+// Fixed device work space of s2g_tsc3d_device_noalloc: one copy per GPU the
+// library runs on, held until the process ends.  Host code never uses this
+// name as an address (it is a host-side stand-in there); it reads the device
+// address of each GPU's copy with hipGetSymbolAddress once, in s2g_create,
+// and passes that address to the kernels.
+constexpr size_t kScratchBytes = (size_t)80 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 struct s2g_ctx {
     int device = 0;
@@ -43,11 +52,40 @@ struct s2g_ctx {
     DevBuf image, out;
     // pinned host scratch for small device->host reads
     int64_t *h_small = nullptr;
+    // device address of `scratch` on this context's GPU (null if unavailable)
+    void *scratch = nullptr;
+    std::string scratch_err;
 };
 
 namespace {
 
 const char *kVersion = "sph2grid 0.1.0 gfx950";
+
+// per-GPU device address of `scratch`, looked up once per device
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices];
+std::string g_scratch_err[kMaxDevices];
+
+// With `device` current: the device address of its copy of `scratch`.
+void *scratch_address(int device, std::string *err)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        *err = "device index beyond the work-space table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device] && g_scratch_err[device].empty()) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && ptr)
+            g_scratch_addr[device] = ptr;
+        else
+            g_scratch_err[device] = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+    }
+    *err = g_scratch_err[device];
+    return g_scratch_addr[device];
+}
 
 template <typename F>
 int guarded(s2g_ctx *c, F &&f)
@@ -831,6 +869,7 @@ int s2g_create(s2g_ctx **out, int device)
             s2g_destroy(c);
             return S2G_EHIP;
         }
+        c->scratch = scratch_address(device, &c->scratch_err);
     } catch (...) {
         delete c;
         return S2G_ENOMEM;
@@ -1227,6 +1266,10 @@ static TscWork tsc_work_in(void *base, int64_t n, int64_t npix, size_t sort_byte
     return w;
 }
 
+// limits of s2g_tsc3d_device_noalloc (its work space fits kScratchBytes)
+constexpr int64_t kNoallocMaxParticles = (int64_t)1 << 20;
+constexpr int64_t kNoallocMaxNpix = 512;
+
 static void check_tsc(int dtype, int64_t n, const s2g_params *params)
 {
     check_common(dtype, S2G_KERNEL_CUBIC, 3, n, params);
@@ -1327,6 +1370,42 @@ int s2g_tsc3d_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin
     });
 }
 
+int s2g_tsc3d_device_noalloc(s2g_ctx *c, int dtype, const void *d_pos, const void *d_bin_q,
+                             int64_t n, const s2g_params *params, uint32_t flags,
+                             double *d_grid_out, void *stream)
+{
+    return guarded(c, [&]() {
+        check_tsc(dtype, n, params);
+        if (!d_grid_out)
+            throw Error(S2G_EINVAL, "output pointer is NULL");
+        if (n > 0 && (!d_pos || !d_bin_q))
+            throw Error(S2G_EINVAL, "input array pointer is NULL");
+        if (n > kNoallocMaxParticles)
+            throw Error(S2G_EUNSUPPORTED,
+                        "s2g_tsc3d_device_noalloc takes at most 2^20 particles (its fixed work "
+                        "space); got " + std::to_string(n));
+        if (params->npixels > kNoallocMaxNpix)
+            throw Error(S2G_EUNSUPPORTED,
+                        "s2g_tsc3d_device_noalloc takes npixels <= 512 (its fixed work space); "
+                        "got " + std::to_string(params->npixels));
+        if (!c->scratch)
+            throw Error(S2G_EHIP, "fixed work space unavailable: " + c->scratch_err);
+        const int64_t npix = params->npixels;
+        const size_t sb = n > 0 ? sort_pairs_tmp_bytes(n, tsc_key_bits(npix)) : 0;
+        const size_t need = tsc_work_bytes(n, npix, sb);
+        if (need > kScratchBytes)
+            throw Error(S2G_EUNSUPPORTED, "work space of " + std::to_string(need) +
+                                              " bytes exceeds the fixed " +
+                                              std::to_string(kScratchBytes));
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        const TscWork w = tsc_work_in(c->scratch, n, npix, sb);
+        tsc_run(dtype, d_pos, d_bin_q, n, npix, params->len2pix,
+                (flags & S2G_FLAG_PERIODIC) ? 1 : 0, w, d_grid_out, s);
+        S2G_HIP(hipGetLastError());
+        return S2G_OK;
+    });
+}
+
 int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps, void *stream,
                       s2g_stats *stats)
 {
@@ -1367,11 +1446,11 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
                     done[j] = 1;
                 }
             // reduced maps deposit into context-owned flat images first
-            int64_t scratch = 0;
+            int64_t flat = 0;
             for (int j : grp)
                 if (maps[j].flags & S2G_FLAG_REDUCE)
-                    scratch += maps[j].params.npixels * maps[j].params.npixels * (maps[j].nq + 1);
-            double *img = scratch ? (double *)c->image.get(sizeof(double) * scratch) : nullptr;
+                    flat += maps[j].params.npixels * maps[j].params.npixels * (maps[j].nq + 1);
+            double *img = flat ? (double *)c->image.get(sizeof(double) * flat) : nullptr;
             std::vector<MapIn> in;
             int64_t off = 0;
             for (int j : grp) {
