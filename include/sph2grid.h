@@ -17,6 +17,8 @@
  *                           (cic_interpolation.jl:148-272)
  *   s2g_healpix_deposit  <- the particle loop of healpix_map
  *                           (healpix_interpolation/main.jl:146-213)
+ *   s2g_healpix_deposit_maps
+ *                        <- several healpix_map calls on one particle set
  *   s2g_tsc3d            <- TSCInterpolation(..., average=true)
  *                           (tsc_interpolation/tsc_interpolation.jl:131-133,
  *                           commented out in the reference)
@@ -194,6 +196,31 @@ int s2g_healpix_deposit_device(s2g_ctx *ctx, int dtype, const void *d_pos,
                                int kernel, int kernel_dim, uint32_t flags,
                                double *d_map_out, double *d_weight_out,
                                void *stream, s2g_stats *stats);
+
+/* Several all-sky maps of one particle set: map k deposits bin_q[k] weighted
+ * by weights[k] (both n values) into map_out[k] / weight_out[k], exactly as
+ * one s2g_healpix_deposit call per map would (the reference makes each with a
+ * healpix_map call on the same particles, main.jl:92-227, e.g. the density
+ * and the density-weighted temperature of config 5).  The disc queries, the
+ * normalisation sums (pixel_weights.jl:87-140, which depend only on positions
+ * and hsml) and the per-ring particle lists are computed once, and each
+ * pixel's weight once for every two maps.  1 <= nmaps <= 8.  Host pointers
+ * (arrays of nmaps pointers for the per-map arguments). */
+int s2g_healpix_deposit_maps(s2g_ctx *ctx, int dtype, const void *pos, const void *hsml,
+                             const void *m, const void *rho, int nmaps,
+                             const void *const *bin_q, const void *const *weights, int64_t n,
+                             int64_t nside, int kernel, int kernel_dim, uint32_t flags,
+                             double *const *map_out, double *const *weight_out,
+                             s2g_stats *stats);
+/* Same on device pointers (every pixel of every map written on `stream`;
+ * returns with the maps complete). */
+int s2g_healpix_deposit_maps_device(s2g_ctx *ctx, int dtype, const void *d_pos,
+                                    const void *d_hsml, const void *d_m, const void *d_rho,
+                                    int nmaps, const void *const *d_bin_q,
+                                    const void *const *d_weights, int64_t n, int64_t nside,
+                                    int kernel, int kernel_dim, uint32_t flags,
+                                    double *const *d_map_out, double *const *d_weight_out,
+                                    void *stream, s2g_stats *stats);
 
 /* Triangular-shaped-cloud deposit onto an npix^3 grid (positions centred as
  * for the 3D deposit, grid units x*len2pix + npix/2, cell c centred at c+0.5).

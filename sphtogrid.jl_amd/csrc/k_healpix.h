@@ -10,7 +10,8 @@ struct HpRec {
     double z0, phi0, theta0, sth0, xa;  // direction (cos, phi, colatitude, sin) and 1/sin
     double ch0, sh0;                    // cos, sin of theta0 / 2
     double theta, cosr;                 // projected hsml asin(h/dx) (main.jl:182), its cos
-    double s;                           // area*w*dz, then the final pixel scale
+    double s;                           // area*dz, then divided by the normalisation sum
+                                        // (a map's pixel scale is s times its weight)
     int64_t cpix;                       // RING pixel containing the particle
     int64_t unit0;                      // first work unit
     int32_t ra, rb;                     // ring range incl. polar caps fully inside
@@ -19,11 +20,24 @@ struct HpRec {
     int32_t fb, ok;
 };
 
+// maps of one particle set deposited together (the disc queries, the
+// normalisation and the ring lists depend only on positions and hsml)
+constexpr int kHpMaxMaps = 8;
+
 struct HpArgs {
     int dtype, kernel;
-    const void *pos, *hsml, *m, *rho, *binq, *w;
+    const void *pos, *hsml, *m, *rho;
+    int nmaps;
+    const void *binq[kHpMaxMaps], *w[kHpMaxMaps];  // per map
     int64_t n, nside;
     uint32_t flags;
+};
+
+// maps [m0, m0 + nm) of an HpArgs, nm <= 2, and their outputs
+struct HpGroup {
+    const void *binq[2], *w[2];
+    double *map[2], *wmap[2];
+    int nm;
 };
 
 // nunits: n+1 int64 (last = 0, the total after an exclusive scan)
@@ -33,8 +47,9 @@ void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec,
                     double4 *part, hipStream_t s);
 void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRec *rec,
                       hipStream_t s);
-void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpRec *rec,
-                       double *map, double *wmap, hipStream_t s);
+// per-unit atomic deposit of a group of maps (maps zeroed by the caller)
+void launch_hp_deposit(const HpArgs &a, const HpGroup &g, const int2 *units, int64_t nu,
+                       const HpRec *rec, hipStream_t s);
 
 // ring-major deposit (no global atomics) for 4 Nside <= its LDS ring size
 bool hp_ring_deposit_fits(int64_t nside);
@@ -53,8 +68,8 @@ void launch_hp_ring_count(const uint32_t *keys, const int64_t *eoff, int64_t nsi
 // the particles per ring
 void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_t *eoff,
                          const int64_t *loff, int64_t nside, int32_t *list, hipStream_t s);
-// writes every pixel of both maps
-void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
-                            const HpRec *rec, double *map, double *wmap, hipStream_t s);
+// writes every pixel of the group's maps
+void launch_hp_ring_deposit(const HpArgs &a, const HpGroup &g, const int32_t *list,
+                            const int64_t *off, const HpRec *rec, hipStream_t s);
 
 }  // namespace s2g

@@ -182,16 +182,14 @@ __device__ __forceinline__ bool hp_ring_interval(int64_t nside, int64_t iz, doub
 // setup
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
-                                                  const T *__restrict__ hsml,
-                                                  const T *__restrict__ m,
-                                                  const T *__restrict__ rho,
-                                                  const T *__restrict__ binq,
-                                                  const T *__restrict__ w, int64_t n,
-                                                  int64_t nside, uint32_t flags,
-                                                  HpRec *__restrict__ rec,
+__global__ __launch_bounds__(256) void k_hp_setup(const HpArgs a, HpRec *__restrict__ rec,
                                                   int64_t *__restrict__ nunits)
 {
+    const T *__restrict__ pos = (const T *)a.pos;
+    const T *__restrict__ hsml = (const T *)a.hsml;
+    const T *__restrict__ m = (const T *)a.m;
+    const T *__restrict__ rho = (const T *)a.rho;
+    const int64_t n = a.n, nside = a.nside;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n)
         return;
@@ -207,13 +205,16 @@ __global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
     r.fb = 0;
     r.ok = 0;
     r.unit0 = 0;
-    const double bq = ld(binq, p);
+    // main.jl:160-170: without calc_mean a zero quantity skips the particle;
+    // here a particle is left out only if it is skipped in every map
+    bool qskip = !(a.flags & S2G_FLAG_CALC_MEAN);
+    for (int k = 0; k < a.nmaps && qskip; ++k)
+        qskip = ld((const T *)a.binq[k], p) == 0.0;
     const double h = ld(hsml, p);
     const double px = ld(pos, 3 * p), py = ld(pos, 3 * p + 1), pz = ld(pos, 3 * p + 2);
     // shared.jl:1-10: sqrt((0 + x^2) + y^2) + z^2)
     const double dx = sqrt(add_rn(add_rn(mul_rn(px, px), mul_rn(py, py)), mul_rn(pz, pz)));
-    bool skip = (!(flags & S2G_FLAG_CALC_MEAN) && bq == 0.0) || dx < h || !(dx == dx) ||
-                !(h == h);
+    bool skip = qskip || dx < h || !(dx == dx) || !(h == h);
     if (!skip) {
         const double npix = (double)(12 * nside * nside);
         const double ang_pix = sqrt(4.0 * kPi / npix);  // main.jl:144
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(256) void k_hp_setup(const T *__restrict__ pos,
         const double area = (ld(m, p) / ld(rho, p)) / dz0;
         const double ap = ang_pix * dx;
         const double dz = dz0 / (ap * ap);  // main.jl:193
-        r.s = mul_rn(mul_rn(area, ld(w, p)), dz);
+        r.s = mul_rn(area, dz);
         r.theta = theta;
         // vec2ang
         const double th0 = atan2(sqrt(add_rn(mul_rn(px, px), mul_rn(py, py))), pz);
@@ -552,42 +553,56 @@ __global__ void k_hp_finish(const int64_t *__restrict__ ubase, int64_t n,
     rec[p] = r;
 }
 
-template <typename T, int K>
+// per-map pixel scale of particle p: s times the map's weight, 0 where the
+// map skips the particle (zero quantity without calc_mean, main.jl:160-170)
+template <typename T>
+__device__ __forceinline__ void hp_map_scale(const HpGroup &g, int k, int64_t p, double s,
+                                             uint32_t flags, double &sw, double &q)
+{
+    q = ld((const T *)g.binq[k], p);
+    const bool skip = !(flags & S2G_FLAG_CALC_MEAN) && q == 0.0;
+    sw = skip ? 0.0 : s * ld((const T *)g.w[k], p);
+}
+
+template <typename T, int K, int NM>
 __device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restrict__ units,
-                                                 const HpRec *__restrict__ rec,
-                                                 const T *__restrict__ binq, int64_t nside,
-                                                 HpRing *tab, double *__restrict__ map,
-                                                 double *__restrict__ wmap)
+                                                 const HpRec *__restrict__ rec, const HpGroup &g,
+                                                 uint32_t flags, int64_t nside, HpRing *tab)
 {
     const int ln = threadIdx.x % kWave;
     const int2 u = units[gw];
     const HpRec r = rec[u.x];
     if (r.s == 0.0)
         return;
-    const double bq = ld(binq, u.x);
+    double sw[NM], bq[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+        hp_map_scale<T>(g, k, u.x, r.s, flags, sw[k], bq[k]);
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     const bool fb = r.fb != 0;
     hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, [&](int64_t pix, float W, float A) {
-        const double pw = (double)((fb ? 1.0f : W) * A) * r.s;
-        if (pw != 0.0) {
-            unsafeAtomicAdd(&wmap[pix], pw);
-            unsafeAtomicAdd(&map[pix], bq * pw);
+        const double wa = (double)((fb ? 1.0f : W) * A);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            const double pw = wa * sw[k];
+            if (pw != 0.0) {
+                unsafeAtomicAdd(&g.wmap[k][pix], pw);
+                unsafeAtomicAdd(&g.map[k][pix], bq[k] * pw);
+            }
         }
     });
 }
-template <typename T, int K>
+template <typename T, int K, int NM>
 __global__ __launch_bounds__(256, 4) void k_hp_deposit(const int2 *__restrict__ units, int64_t nu,
                                                     const HpRec *__restrict__ rec,
-                                                    const T *__restrict__ binq, int64_t nside,
-                                                    double *__restrict__ map,
-                                                    double *__restrict__ wmap)
+                                                    const HpGroup g, uint32_t flags,
+                                                    int64_t nside)
 {
     __shared__ HpRing s_tab[256 / kWave][kWave];
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
          gw += nw)
-        k_hp_deposit_one<T, K>(gw, units, rec, binq, nside, s_tab[threadIdx.x / kWave], map,
-                               wmap);
+        k_hp_deposit_one<T, K, NM>(gw, units, rec, g, flags, nside, s_tab[threadIdx.x / kWave]);
 }
 
 
@@ -601,8 +616,13 @@ __global__ __launch_bounds__(256, 4) void k_hp_deposit(const int2 *__restrict__ 
 // arithmetic), so the deposited weights sum to the normalisation.
 // ---------------------------------------------------------------------------
 constexpr int kHpRingLds = 4096;  // pixels of the longest ring held in LDS
-constexpr int kHpRingThreads = 512;  // 8 waves per ring block
-constexpr int kHpRingPairs = 32;     // (particle, ring) chords per wave and round
+// One map: 8 waves per ring block, 32 chords per wave and round (two blocks
+// per CU); two maps (four rows, 128 KB): 16 waves, 28 chords per wave, one
+// block per CU.  Either way four waves per SIMD.
+template <int NM>
+constexpr int kHpRingThreads = NM == 1 ? 512 : 1024;
+template <int NM>
+constexpr int kHpRingPairs = NM == 1 ? 32 : 28;
 
 // The rings a particle deposits to: its disc's rings [ra, rb] and the ring of
 // its centre pixel (constributing_pixels.jl:16-19), which lies outside that
@@ -749,6 +769,7 @@ __global__ __launch_bounds__(256) void k_hp_ring_fill(const uint32_t *__restrict
 }
 
 // one (particle, ring) chord of the ring-major deposit (per-wave LDS table)
+template <int NM>
 struct HpPair {
     int off;    // first position of the chord in the wave's pixel list
     int lo;     // ring-local index of the chord's first pixel (< 0: wraps)
@@ -756,44 +777,46 @@ struct HpPair {
     int fb;     // fallback: area weights only
     float s1, s2, base, step;
     float theta, inv_theta;  // theta + ang / 2 (hp_weights's thh), 1 / theta
-    double scale, bq;
+    double sw[NM], bq[NM];   // per map: pixel scale and quantity
 };
 
-// Two blocks of 8 waves per CU (78 KB of LDS each), held to 128 VGPRs so
-// that four waves per SIMD run.
-template <typename T, int K>
-__global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
+template <typename T, int K, int NM>
+__global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
     const int32_t *__restrict__ list, const int64_t *__restrict__ off,
-    const HpRec *__restrict__ rec, const T *__restrict__ binq, int64_t nside,
-    double *__restrict__ map, double *__restrict__ wmap)
+    const HpRec *__restrict__ rec, const HpGroup g, uint32_t flags, int64_t nside)
 {
-    __shared__ double s_m[kHpRingLds], s_w[kHpRingLds];
-    __shared__ HpPair s_tab[kHpRingThreads / kWave][kHpRingPairs];
+    constexpr int kThreads = kHpRingThreads<NM>, kPairs = kHpRingPairs<NM>;
+    __shared__ double s_m[NM][kHpRingLds], s_w[NM][kHpRingLds];
+    __shared__ HpPair<NM> s_tab[kThreads / kWave][kPairs];
     const int64_t iz = (int64_t)blockIdx.x + 1;  // ring, 1-based
     int64_t start, nr;
     double shift;
     hp_ring_info(nside, iz, start, nr, shift);
-    for (int k = threadIdx.x; k < nr; k += kHpRingThreads) {
-        s_m[k] = 0.0;
-        s_w[k] = 0.0;
+    for (int k = threadIdx.x; k < nr; k += kThreads) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            s_m[q][k] = 0.0;
+            s_w[q][k] = 0.0;
+        }
     }
     __syncthreads();
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
-    HpPair *tab = s_tab[wv];
+    HpPair<NM> *tab = s_tab[wv];
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     const float inv_ang = 1.0f / ang;
     const int64_t b = off[iz], e = off[iz + 1];
-    for (int64_t c0 = b + (int64_t)wv * kHpRingPairs; c0 < e;
-         c0 += (kHpRingThreads / kWave) * kHpRingPairs) {
-        // phase 1: lane l < kHpRingPairs takes pair c0 + l: its chord on this ring
-        HpPair q;
+    for (int64_t c0 = b + (int64_t)wv * kPairs; c0 < e; c0 += (kThreads / kWave) * kPairs) {
+        // phase 1: lane l < kPairs takes pair c0 + l: its chord on this ring
+        HpPair<NM> q;
         q.lo = 0;
         q.ck = -1;
         q.fb = 0;
         q.s1 = q.s2 = q.base = q.step = q.theta = q.inv_theta = 0.0f;
-        q.scale = q.bq = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k)
+            q.sw[k] = q.bq[k] = 0.0;
         int len = 0;
-        if (ln < kHpRingPairs && c0 + ln < e) {
+        if (ln < kPairs && c0 + ln < e) {
             const int p = list[c0 + ln];
             const HpRec r = rec[p];
             int64_t cring, ck;
@@ -835,25 +858,26 @@ __global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
             q.fb = r.fb;
             q.theta = (float)r.theta + 0.5f * ang;  // theta + ang / 2, as the norm pass
             q.inv_theta = (float)(1.0 / r.theta);
-            q.scale = r.s;
-            q.bq = ld(binq, p);
+#pragma unroll
+            for (int k = 0; k < NM; ++k)
+                hp_map_scale<T>(g, k, p, r.s, flags, q.sw[k], q.bq[k]);
         }
         const int incl = wave_incl_scan(len, ln);
         const int total = __shfl(incl, kWave - 1, kWave);
         q.off = incl - len;
         __builtin_amdgcn_wave_barrier();
-        if (ln < kHpRingPairs)
+        if (ln < kPairs)
             tab[ln] = q;
         __builtin_amdgcn_wave_barrier();
         // phase 2: the chords' pixels, 64 list positions at a time
         int j = -1, next = 0;
-        HpPair cur = q;
+        HpPair<NM> cur = q;
         int clen = 0;
         for (int pos = ln; pos < total; pos += kWave) {
             while (pos >= next) {
                 ++j;
                 cur = tab[j];
-                next = j + 1 < kHpRingPairs ? tab[j + 1].off : total;
+                next = j + 1 < kPairs ? tab[j + 1].off : total;
                 clen = next - cur.off - (cur.ck >= 0 ? 1 : 0);
             }
             const int c = pos - cur.off;
@@ -873,18 +897,25 @@ __global__ __launch_bounds__(kHpRingThreads, 4) void k_hp_ring_deposit(
             }
             float W, A;
             hp_weights<K>(d, cur.theta, cur.inv_theta, ang, inv_ang, W, A);
-            const double pw = (double)((cur.fb ? 1.0f : W) * A) * cur.scale;
-            if (pw != 0.0) {
-                unsafeAtomicAdd(&s_w[k], pw);
-                unsafeAtomicAdd(&s_m[k], cur.bq * pw);
+            const double wa = (double)((cur.fb ? 1.0f : W) * A);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const double pw = wa * cur.sw[m];
+                if (pw != 0.0) {
+                    unsafeAtomicAdd(&s_w[m][k], pw);
+                    unsafeAtomicAdd(&s_m[m][k], cur.bq[m] * pw);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < nr; k += kHpRingThreads) {
-        map[start + k] = s_m[k];
-        wmap[start + k] = s_w[k];
+    for (int k = threadIdx.x; k < nr; k += kThreads) {
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            g.map[m][start + k] = s_m[m][k];
+            g.wmap[m][start + k] = s_w[m][k];
+        }
     }
 }
 
@@ -905,15 +936,9 @@ void launch_hp_setup(const HpArgs &a, HpRec *rec, int64_t *nunits, hipStream_t s
 {
     const dim3 g((unsigned)((a.n + 255) / 256)), blk(256);
     if (a.dtype == S2G_F32)
-        hipLaunchKernelGGL(k_hp_setup<float>, g, blk, 0, s, (const float *)a.pos,
-                           (const float *)a.hsml, (const float *)a.m, (const float *)a.rho,
-                           (const float *)a.binq, (const float *)a.w, a.n, a.nside, a.flags,
-                           rec, nunits);
+        hipLaunchKernelGGL(k_hp_setup<float>, g, blk, 0, s, a, rec, nunits);
     else
-        hipLaunchKernelGGL(k_hp_setup<double>, g, blk, 0, s, (const double *)a.pos,
-                           (const double *)a.hsml, (const double *)a.m, (const double *)a.rho,
-                           (const double *)a.binq, (const double *)a.w, a.n, a.nside, a.flags,
-                           rec, nunits);
+        hipLaunchKernelGGL(k_hp_setup<double>, g, blk, 0, s, a, rec, nunits);
 }
 
 void launch_hp_units(const int64_t *ubase, int64_t n, HpRec *rec, int2 *units, hipStream_t s)
@@ -939,21 +964,28 @@ void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRe
                        part, rec);
 }
 
-void launch_hp_deposit(const HpArgs &a, const int2 *units, int64_t nu, const HpRec *rec,
-                       double *map, double *wmap, hipStream_t s)
+template <typename T, int K>
+static void hp_deposit_k(const HpArgs &a, const HpGroup &g, const int2 *units, int64_t nu,
+                         const HpRec *rec, hipStream_t s)
+{
+    const dim3 grid((unsigned)wave_blocks(nu, 256)), blk(256);
+    if (g.nm == 1)
+        hipLaunchKernelGGL((k_hp_deposit<T, K, 1>), grid, blk, 0, s, units, nu, rec, g, a.flags,
+                           a.nside);
+    else
+        hipLaunchKernelGGL((k_hp_deposit<T, K, 2>), grid, blk, 0, s, units, nu, rec, g, a.flags,
+                           a.nside);
+}
+
+void launch_hp_deposit(const HpArgs &a, const HpGroup &g, const int2 *units, int64_t nu,
+                       const HpRec *rec, hipStream_t s)
 {
     if (nu == 0)
         return;
-    const dim3 g((unsigned)wave_blocks(nu, 256)), blk(256);
     if (a.dtype == S2G_F32) {
-        S2G_HP_KERNEL_SWITCH(a.kernel,
-                             hipLaunchKernelGGL((k_hp_deposit<float, KK>), g, blk, 0, s, units, nu,
-                                                rec, (const float *)a.binq, a.nside, map, wmap));
+        S2G_HP_KERNEL_SWITCH(a.kernel, (hp_deposit_k<float, KK>(a, g, units, nu, rec, s)));
     } else {
-        S2G_HP_KERNEL_SWITCH(a.kernel,
-                             hipLaunchKernelGGL((k_hp_deposit<double, KK>), g, blk, 0, s, units,
-                                                nu, rec, (const double *)a.binq, a.nside, map,
-                                                wmap));
+        S2G_HP_KERNEL_SWITCH(a.kernel, (hp_deposit_k<double, KK>(a, g, units, nu, rec, s)));
     }
 }
 
@@ -998,20 +1030,26 @@ void launch_hp_ring_fill(const uint32_t *keys, const int32_t *vals, const int64_
                        vals, eoff, loff, nside, list);
 }
 
-void launch_hp_ring_deposit(const HpArgs &a, const int32_t *list, const int64_t *off,
-                            const HpRec *rec, double *map, double *wmap, hipStream_t s)
+template <typename T, int K>
+static void hp_ring_deposit_k(const HpArgs &a, const HpGroup &g, const int32_t *list,
+                              const int64_t *off, const HpRec *rec, hipStream_t s)
 {
-    const dim3 g((unsigned)(4 * a.nside - 1)), blk(kHpRingThreads);
+    const dim3 grid((unsigned)(4 * a.nside - 1));
+    if (g.nm == 1)
+        hipLaunchKernelGGL((k_hp_ring_deposit<T, K, 1>), grid, dim3(kHpRingThreads<1>), 0, s,
+                           list, off, rec, g, a.flags, a.nside);
+    else
+        hipLaunchKernelGGL((k_hp_ring_deposit<T, K, 2>), grid, dim3(kHpRingThreads<2>), 0, s,
+                           list, off, rec, g, a.flags, a.nside);
+}
+
+void launch_hp_ring_deposit(const HpArgs &a, const HpGroup &g, const int32_t *list,
+                            const int64_t *off, const HpRec *rec, hipStream_t s)
+{
     if (a.dtype == S2G_F32) {
-        S2G_HP_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k_hp_ring_deposit<float, KK>), g, blk,
-                                                          0, s, list, off, rec,
-                                                          (const float *)a.binq, a.nside, map,
-                                                          wmap));
+        S2G_HP_KERNEL_SWITCH(a.kernel, (hp_ring_deposit_k<float, KK>(a, g, list, off, rec, s)));
     } else {
-        S2G_HP_KERNEL_SWITCH(a.kernel, hipLaunchKernelGGL((k_hp_ring_deposit<double, KK>), g,
-                                                          blk, 0, s, list, off, rec,
-                                                          (const double *)a.binq, a.nside, map,
-                                                          wmap));
+        S2G_HP_KERNEL_SWITCH(a.kernel, (hp_ring_deposit_k<double, KK>(a, g, list, off, rec, s)));
     }
 }
 

@@ -179,6 +179,20 @@ __device__ __forceinline__ int tsc_reach(int64_t c, int64_t b0, int64_t side, in
     return m;
 }
 
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// One wave per brick.  Lane j < 27 takes neighbouring brick j (offsets
+// (j/9 - 1, (j/3)%3 - 1, j%3 - 1), each brick once) and reads its particle
+// range; the ranges are laid end to end (wave prefix sum) and scanned 64
+// candidates at a time, the next chunk's loads issued before the current
+// chunk is deposited.  A candidate whose stencil reaches the brick is
+// deposited by lanes 0..26 (one stencil cell each) from its lane's registers
+// (readlane), in neighbour order and particle order: a fixed order, so the
+// sums are reproducible.
 __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict__ gq,
                                                       const uint64_t *__restrict__ cell,
                                                       const int64_t *__restrict__ off,
@@ -186,6 +200,8 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
                                                       double *__restrict__ grid)
 {
     __shared__ double acc[2][kTX * kTY * kTZ];
+    __shared__ int64_t s_start[32];
+    __shared__ int s_base[32];
     int64_t nb[3];
     tsc_bricks(npix, nb);
     const int64_t side[3] = {kTX, kTY, kTZ};
@@ -197,73 +213,111 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
     const int ln = threadIdx.x;
     for (int e = ln; e < 2 * kTX * kTY * kTZ; e += kWave)
         (&acc[0][0])[e] = 0.0;
-    __syncthreads();
-    // this lane's stencil cell (lanes 0..26)
-    const int o0 = ln / 9 - 1, o1 = (ln / 3) % 3 - 1, o2 = ln % 3 - 1;
-    // neighbouring bricks, each once (periodic grids of one or two bricks
-    // per axis wrap onto the same brick)
-    int64_t nbr[3][3];
-    int nn[3];
-    for (int d = 0; d < 3; ++d) {
-        nn[d] = 0;
-        for (int o = -1; o <= 1; ++o) {
-            int64_t x = bi[d] + o;
-            if (periodic)
-                x = x < 0 ? x + nb[d] : x >= nb[d] ? x - nb[d] : x;
-            else if (x < 0 || x >= nb[d])
-                continue;
-            bool dup = false;
-            for (int k = 0; k < nn[d]; ++k)
-                dup = dup || nbr[d][k] == x;
-            if (!dup)
-                nbr[d][nn[d]++] = x;
+    // this lane's stencil cell / neighbour offsets (lanes 0..26)
+    const int o[3] = {ln / 9 - 1, (ln / 3) % 3 - 1, ln % 3 - 1};
+    // neighbour brick of lane ln, if it exists and is not a repeat of an
+    // earlier offset on some axis (periodic grids of one or two bricks)
+    int len = 0;
+    int64_t start = 0;
+    if (ln < 27) {
+        bool valid = true;
+        int64_t x[3];
+        for (int d = 0; d < 3; ++d) {
+            auto nbr = [&](int oo) {
+                int64_t v = bi[d] + oo;
+                if (periodic)
+                    v = v < 0 ? v + nb[d] : v >= nb[d] ? v - nb[d] : v;
+                return v;
+            };
+            x[d] = nbr(o[d]);
+            if (!periodic && (x[d] < 0 || x[d] >= nb[d]))
+                valid = false;
+            for (int oo = -1; oo < o[d]; ++oo)
+                valid = valid && nbr(oo) != x[d];
+        }
+        if (valid) {
+            const int64_t nbrick = (x[0] * nb[1] + x[1]) * nb[2] + x[2];
+            start = off[nbrick];
+            len = (int)(off[nbrick + 1] - start);
         }
     }
-    for (int a = 0; a < nn[0]; ++a)
-        for (int bb = 0; bb < nn[1]; ++bb)
-            for (int cc = 0; cc < nn[2]; ++cc) {
-                const int64_t nbrick = (nbr[0][a] * nb[1] + nbr[1][bb]) * nb[2] + nbr[2][cc];
-                const int64_t s = off[nbrick], t = off[nbrick + 1];
-                for (int64_t k0 = s; k0 < t; k0 += kWave) {
-                    const int64_t k = k0 + ln;
-                    bool use = false;
-                    if (k < t) {
-                        const uint64_t v = cell[k];
-                        use = true;
-                        for (int d = 0; d < 3; ++d)
-                            use = use && tsc_reach(tsc_unpack(v, d), b0[d], side[d], npix,
-                                                   periodic) != 0;
-                    }
-                    uint64_t mask = __ballot(use);
-                    while (mask) {
-                        const int src = __builtin_ctzll(mask);
-                        mask &= mask - 1;
-                        const double4 p = gq[k0 + src];
-                        if (ln < 27) {
-                            const double g[3] = {p.x, p.y, p.z};
-                            const int o[3] = {o0, o1, o2};
-                            double w[3];
-                            int64_t loc[3];
-                            bool in = true;
-                            for (int d = 0; d < 3; ++d) {
-                                const int64_t c = (int64_t)floor(g[d]);
-                                w[d] = tsc_w(sub_rn(g[d], (double)(c + o[d]) + 0.5));
-                                int64_t x = c + o[d];
-                                if (periodic)
-                                    x = tsc_wrap(x, npix);
-                                loc[d] = x - b0[d];
-                                in = in && x >= 0 && x < npix && loc[d] >= 0 && loc[d] < side[d];
-                            }
-                            const double ww = (w[0] * w[1]) * w[2];
-                            if (in && ww != 0.0) {
-                                const int e = (int)((loc[0] * kTY + loc[1]) * kTZ + loc[2]);
-                                unsafeAtomicAdd(&acc[0][e], p.w * ww);
-                                unsafeAtomicAdd(&acc[1][e], ww);
-                            }
-                        }
-                    }
+    // exclusive prefix of the range lengths over the lanes
+    int incl = len;
+#pragma unroll
+    for (int sh = 1; sh < kWave; sh <<= 1) {
+        const int t = __shfl_up(incl, sh);
+        incl += ln >= sh ? t : 0;
+    }
+    const int total = __shfl(incl, 26);
+    if (ln < 27) {
+        s_start[ln] = start;
+        s_base[ln] = incl - len;
+    }
+    __syncthreads();
+
+    // candidate v of the concatenated ranges -> sorted particle index: the
+    // last range whose base is <= v (bases are exclusive prefix sums, so an
+    // empty range after the one holding v has a base > v)
+    auto locate = [&](int v) {
+        int lo = 0;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1)
+            if (lo + step < 27 && s_base[lo + step] <= v)
+                lo += step;
+        return s_start[lo] + (int64_t)(v - s_base[lo]);
+    };
+    bool have = false;
+    uint64_t cv = 0;
+    double4 pv = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (ln < total) {
+        const int64_t k = locate(ln);
+        cv = cell[k];
+        pv = gq[k];
+        have = true;
+    }
+    for (int v0 = 0; v0 < total; v0 += kWave) {
+        bool use = have;
+        if (use)
+            for (int d = 0; d < 3; ++d)
+                use = use && tsc_reach(tsc_unpack(cv, d), b0[d], side[d], npix, periodic) != 0;
+        const double4 cur = pv;
+        // prefetch the next chunk
+        have = false;
+        if (v0 + kWave + ln < total) {
+            const int64_t k = locate(v0 + kWave + ln);
+            cv = cell[k];
+            pv = gq[k];
+            have = true;
+        }
+        uint64_t mask = __ballot(use);
+        while (mask) {
+            const int src = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const double g[3] = {readlane_d(cur.x, src), readlane_d(cur.y, src),
+                                 readlane_d(cur.z, src)};
+            const double q = readlane_d(cur.w, src);
+            if (ln < 27) {
+                double w[3];
+                int64_t loc[3];
+                bool in = true;
+                for (int d = 0; d < 3; ++d) {
+                    const int64_t c = (int64_t)floor(g[d]);
+                    w[d] = tsc_w(sub_rn(g[d], (double)(c + o[d]) + 0.5));
+                    int64_t x = c + o[d];
+                    if (periodic)
+                        x = tsc_wrap(x, npix);
+                    loc[d] = x - b0[d];
+                    in = in && x >= 0 && x < npix && loc[d] >= 0 && loc[d] < side[d];
+                }
+                const double ww = (w[0] * w[1]) * w[2];
+                if (in && ww != 0.0) {
+                    const int e = (int)((loc[0] * kTY + loc[1]) * kTZ + loc[2]);
+                    unsafeAtomicAdd(&acc[0][e], q * ww);
+                    unsafeAtomicAdd(&acc[1][e], ww);
                 }
             }
+        }
+    }
     __syncthreads();
     const int64_t nd = npix * npix * npix;
     for (int e = ln; e < kTX * kTY * kTZ; e += kWave) {

@@ -1062,9 +1062,11 @@ int s2g_map(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml, 
     });
 }
 
-// all-sky deposit on device pointers into d_map / d_wmap (zeroed here)
-static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d_wmap,
-                           hipStream_t s, s2g_stats *st)
+// all-sky deposit of a.nmaps maps of one particle set on device pointers:
+// map k into maps[k] / wmaps[k] (every pixel written).  The disc queries, the
+// normalisation sums and the per-ring lists are computed once for all maps.
+static void healpix_device(s2g_ctx *c, const HpArgs &a, double *const *maps,
+                           double *const *wmaps, hipStream_t s, s2g_stats *st)
 {
     const int64_t n = a.n, nside = a.nside;
     const int64_t npx = 12 * nside * nside;
@@ -1072,10 +1074,23 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
     S2G_HIP(hipEventRecord(E[0], s));
     // the ring-major deposit writes every pixel; the atomic one adds
     const bool rings = n > 0 && hp_ring_deposit_fits(nside);
-    if (!rings) {
-        S2G_HIP(hipMemsetAsync(d_map, 0, sizeof(double) * npx, s));
-        S2G_HIP(hipMemsetAsync(d_wmap, 0, sizeof(double) * npx, s));
-    }
+    if (!rings)
+        for (int k = 0; k < a.nmaps; ++k) {
+            S2G_HIP(hipMemsetAsync(maps[k], 0, sizeof(double) * npx, s));
+            S2G_HIP(hipMemsetAsync(wmaps[k], 0, sizeof(double) * npx, s));
+        }
+    auto group = [&](int m0) {
+        HpGroup g;
+        g.nm = a.nmaps - m0 < 2 ? a.nmaps - m0 : 2;
+        for (int k = 0; k < 2; ++k) {
+            const int mk = m0 + (k < g.nm ? k : 0);
+            g.binq[k] = a.binq[mk];
+            g.w[k] = a.w[mk];
+            g.map[k] = maps[mk];
+            g.wmap[k] = wmaps[mk];
+        }
+        return g;
+    };
     if (n > 0) {
         HpRec *rec = (HpRec *)c->hp_rec.get(sizeof(HpRec) * n);
         int64_t *nun = (int64_t *)c->hp_nunits.get(sizeof(int64_t) * (n + 1));
@@ -1120,13 +1135,18 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
             int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
             launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, nside, list, s);
-            launch_hp_ring_deposit(a, list, off, rec, d_map, d_wmap, s);
+            for (int m0 = 0; m0 < a.nmaps; m0 += 2)
+                launch_hp_ring_deposit(a, group(m0), list, off, rec, s);
         } else {
-            launch_hp_deposit(a, units, nu, rec, d_map, d_wmap, s);
+            for (int m0 = 0; m0 < a.nmaps; m0 += 2)
+                launch_hp_deposit(a, group(m0), units, nu, rec, s);
         }
         S2G_HIP(hipGetLastError());
         if (st)
             st->n_pairs = nu;
+    } else {
+        // nothing to deposit: every map is zero (written by the memset above
+        // unless the ring path was chosen, which needs particles)
     }
     S2G_HIP(hipEventRecord(E[4], s));
     S2G_HIP(hipEventSynchronize(E[4]));
@@ -1141,8 +1161,8 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *d_map, double *d
 }
 
 static HpArgs hp_args(int dtype, int kernel, const void *pos, const void *hsml, const void *m,
-                      const void *rho, const void *bin_q, const void *w, int64_t n,
-                      int64_t nside, uint32_t flags)
+                      const void *rho, int nmaps, const void *const *bin_q,
+                      const void *const *w, int64_t n, int64_t nside, uint32_t flags)
 {
     HpArgs a;
     a.dtype = dtype;
@@ -1151,51 +1171,56 @@ static HpArgs hp_args(int dtype, int kernel, const void *pos, const void *hsml, 
     a.hsml = hsml;
     a.m = m;
     a.rho = rho;
-    a.binq = bin_q;
-    a.w = w;
+    a.nmaps = nmaps;
+    for (int k = 0; k < kHpMaxMaps; ++k) {
+        a.binq[k] = k < nmaps ? bin_q[k] : nullptr;
+        a.w[k] = k < nmaps ? w[k] : nullptr;
+    }
     a.n = n;
     a.nside = nside;
     a.flags = flags;
     return a;
 }
 
-int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml,
-                        const void *m, const void *rho, const void *bin_q,
-                        const void *weights, int64_t n, int64_t nside, int kernel,
-                        int kernel_dim, uint32_t flags, double *map_out, double *weight_out,
-                        s2g_stats *stats)
+static void check_healpix(int dtype, int kernel, int kernel_dim, int64_t n, int64_t nside,
+                          int nmaps)
+{
+    s2g_params dummy{1, 1.0};
+    check_common(dtype, kernel, kernel_dim, n, &dummy);
+    if (nside < 1 || nside > 8192)
+        throw Error(S2G_EINVAL, "nside out of range [1, 8192]");
+    if (nmaps < 1 || nmaps > kHpMaxMaps)
+        throw Error(S2G_EINVAL, "nmaps out of range [1, " + std::to_string(kHpMaxMaps) + "]");
+}
+
+int s2g_healpix_deposit_maps_device(s2g_ctx *c, int dtype, const void *d_pos,
+                                    const void *d_hsml, const void *d_m, const void *d_rho,
+                                    int nmaps, const void *const *d_bin_q,
+                                    const void *const *d_weights, int64_t n, int64_t nside,
+                                    int kernel, int kernel_dim, uint32_t flags,
+                                    double *const *d_map_out, double *const *d_weight_out,
+                                    void *stream, s2g_stats *stats)
 {
     return guarded(c, [&]() {
-        s2g_params dummy{1, 1.0};
-        check_common(dtype, kernel, kernel_dim, n, &dummy);
-        if (nside < 1 || nside > 8192)
-            throw Error(S2G_EINVAL, "nside out of range [1, 8192]");
-        if (!map_out || !weight_out)
-            throw Error(S2G_EINVAL, "output pointer is NULL");
-        hipStream_t s = c->stream;
-        s2g_stats local;
-        s2g_stats *st = stats ? stats : &local;
-        std::memset(st, 0, sizeof(*st));
-        st->n_particles = n;
-        Map2DArgs h = make_args(dtype, pos, hsml, m, rho, bin_q, 1, weights, n, &dummy, kernel,
-                                flags);
-        double h2d = 0.0;
-        {
-            s2g_stats tmp;
-            h = stage_inputs(c, h, s, &tmp);
-            h2d = tmp.ms_h2d;
+        check_healpix(dtype, kernel, kernel_dim, n, nside, nmaps);
+        if (!d_bin_q || !d_weights || !d_map_out || !d_weight_out)
+            throw Error(S2G_EINVAL, "per-map pointer array is NULL");
+        for (int k = 0; k < nmaps; ++k) {
+            if (!d_map_out[k] || !d_weight_out[k])
+                throw Error(S2G_EINVAL, "map " + std::to_string(k) + ": output pointer is NULL");
+            if (n > 0 && (!d_bin_q[k] || !d_weights[k]))
+                throw Error(S2G_EINVAL, "map " + std::to_string(k) + ": input pointer is NULL");
         }
-        const int64_t npx = 12 * nside * nside;
-        double *maps = (double *)c->image.get(sizeof(double) * 2 * npx);
-        healpix_device(c, hp_args(dtype, kernel, h.pos, h.hsml, h.m, h.rho, h.binq, h.w, n,
-                                  nside, flags),
-                       maps + npx, maps, s, st);
-        st->n_particles = n;
-        st->ms_h2d = h2d;
-        double t1 = 0.0, t2 = 0.0;
-        d2h(c, map_out, maps + npx, sizeof(double) * npx, s, &t1);
-        d2h(c, weight_out, maps, sizeof(double) * npx, s, &t2);
-        st->ms_d2h = t1 + t2;
+        if (n > 0 && (!d_pos || !d_hsml || !d_m || !d_rho))
+            throw Error(S2G_EINVAL, "input array pointer is NULL");
+        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->n_particles = n;
+        }
+        healpix_device(c, hp_args(dtype, kernel, d_pos, d_hsml, d_m, d_rho, nmaps, d_bin_q,
+                                  d_weights, n, nside, flags),
+                       d_map_out, d_weight_out, s, stats);
         return S2G_OK;
     });
 }
@@ -1206,23 +1231,92 @@ int s2g_healpix_deposit_device(s2g_ctx *c, int dtype, const void *d_pos, const v
                                int kernel_dim, uint32_t flags, double *d_map_out,
                                double *d_weight_out, void *stream, s2g_stats *stats)
 {
+    return s2g_healpix_deposit_maps_device(c, dtype, d_pos, d_hsml, d_m, d_rho, 1, &d_bin_q,
+                                           &d_weights, n, nside, kernel, kernel_dim, flags,
+                                           &d_map_out, &d_weight_out, stream, stats);
+}
+
+int s2g_healpix_deposit_maps(s2g_ctx *c, int dtype, const void *pos, const void *hsml,
+                             const void *m, const void *rho, int nmaps,
+                             const void *const *bin_q, const void *const *weights, int64_t n,
+                             int64_t nside, int kernel, int kernel_dim, uint32_t flags,
+                             double *const *map_out, double *const *weight_out,
+                             s2g_stats *stats)
+{
     return guarded(c, [&]() {
-        s2g_params dummy{1, 1.0};
-        check_common(dtype, kernel, kernel_dim, n, &dummy);
-        if (nside < 1 || nside > 8192)
-            throw Error(S2G_EINVAL, "nside out of range [1, 8192]");
-        if (!d_map_out || !d_weight_out)
-            throw Error(S2G_EINVAL, "output pointer is NULL");
-        hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-        if (stats) {
-            std::memset(stats, 0, sizeof(*stats));
-            stats->n_particles = n;
+        check_healpix(dtype, kernel, kernel_dim, n, nside, nmaps);
+        if (!bin_q || !weights || !map_out || !weight_out)
+            throw Error(S2G_EINVAL, "per-map pointer array is NULL");
+        for (int k = 0; k < nmaps; ++k)
+            if (!map_out[k] || !weight_out[k] || (n > 0 && (!bin_q[k] || !weights[k])))
+                throw Error(S2G_EINVAL, "map " + std::to_string(k) + ": pointer is NULL");
+        if (n > 0 && (!pos || !hsml || !m || !rho))
+            throw Error(S2G_EINVAL, "input array pointer is NULL");
+        hipStream_t s = c->stream;
+        s2g_stats local;
+        s2g_stats *st = stats ? stats : &local;
+        std::memset(st, 0, sizeof(*st));
+        st->n_particles = n;
+        const size_t es = elem_size(dtype);
+        const size_t cnt = (size_t)(n > 0 ? n : 1);
+        hipEvent_t e0 = c->ev0, e1 = c->ev1;
+        S2G_HIP(hipEventRecord(e0, s));
+        // staging: pos, hsml, m, rho in in[0..3]; per map bin_q and weights in
+        // one buffer (in[4])
+        const void *src[4] = {pos, hsml, m, rho};
+        void *dst[4];
+        for (int k = 0; k < 4; ++k) {
+            const size_t b = es * cnt * (k == 0 ? 3 : 1);
+            dst[k] = c->in[k].get(b);
+            if (n > 0)
+                S2G_HIP(hipMemcpyAsync(dst[k], src[k], b, hipMemcpyHostToDevice, s));
         }
-        healpix_device(c, hp_args(dtype, kernel, d_pos, d_hsml, d_m, d_rho, d_bin_q, d_weights, n,
+        char *qw = (char *)c->in[4].get(es * cnt * 2 * nmaps);
+        const void *dq[kHpMaxMaps], *dw[kHpMaxMaps];
+        for (int k = 0; k < nmaps; ++k) {
+            dq[k] = qw + es * cnt * (2 * k);
+            dw[k] = qw + es * cnt * (2 * k + 1);
+            if (n > 0) {
+                S2G_HIP(hipMemcpyAsync((void *)dq[k], bin_q[k], es * n, hipMemcpyHostToDevice, s));
+                S2G_HIP(hipMemcpyAsync((void *)dw[k], weights[k], es * n, hipMemcpyHostToDevice,
+                                       s));
+            }
+        }
+        S2G_HIP(hipEventRecord(e1, s));
+        S2G_HIP(hipEventSynchronize(e1));
+        const double h2d = elapsed(e0, e1);
+        const int64_t npx = 12 * nside * nside;
+        double *out = (double *)c->image.get(sizeof(double) * 2 * npx * nmaps);
+        double *dm[kHpMaxMaps], *dwm[kHpMaxMaps];
+        for (int k = 0; k < nmaps; ++k) {
+            dm[k] = out + 2 * npx * k;
+            dwm[k] = out + 2 * npx * k + npx;
+        }
+        healpix_device(c, hp_args(dtype, kernel, dst[0], dst[1], dst[2], dst[3], nmaps, dq, dw, n,
                                   nside, flags),
-                       d_map_out, d_weight_out, s, stats);
+                       dm, dwm, s, st);
+        st->n_particles = n;
+        st->ms_h2d = h2d;
+        double d2h_ms = 0.0;
+        for (int k = 0; k < nmaps; ++k) {
+            double t1 = 0.0, t2 = 0.0;
+            d2h(c, map_out[k], dm[k], sizeof(double) * npx, s, &t1);
+            d2h(c, weight_out[k], dwm[k], sizeof(double) * npx, s, &t2);
+            d2h_ms += t1 + t2;
+        }
+        st->ms_d2h = d2h_ms;
         return S2G_OK;
     });
+}
+
+int s2g_healpix_deposit(s2g_ctx *c, int dtype, const void *pos, const void *hsml,
+                        const void *m, const void *rho, const void *bin_q,
+                        const void *weights, int64_t n, int64_t nside, int kernel,
+                        int kernel_dim, uint32_t flags, double *map_out, double *weight_out,
+                        s2g_stats *stats)
+{
+    return s2g_healpix_deposit_maps(c, dtype, pos, hsml, m, rho, 1, &bin_q, &weights, n, nside,
+                                    kernel, kernel_dim, flags, &map_out, &weight_out, stats);
 }
 
 // Work space of one TSC deposit of n particles on an npix^3 grid.

@@ -15,7 +15,7 @@ from .distributed import (AllskySubfile, Subfile, distributed_allsky_map,  # noq
 from .io import (get_map_grid_3D, read_allsky_fits_image, read_fits_image,  # noqa: F401
                  read_healpix_fits, read_vtk_image, save_healpix_fits, write_fits_image,
                  write_vtk_image)
-from .healpix import BoundsError, filter_sort_particles, healpix_map  # noqa: F401
+from .healpix import BoundsError, filter_sort_particles, healpix_map, healpix_maps  # noqa: F401
 from .kernels import (AbstractSPHKernel, Cubic, Quintic, WendlandC2,  # noqa: F401
                       WendlandC4, WendlandC6, WendlandC8)
 from .mapping import (center_particles, deposit, filter_particles_in_image,  # noqa: F401
@@ -32,7 +32,8 @@ __all__ = [
     "part_weight_physical", "part_weight_emission", "part_weight_spectroscopic",
     "part_weight_XrayBand", "AbstractSPHKernel", "Cubic", "Quintic", "WendlandC2",
     "WendlandC4", "WendlandC6", "WendlandC8", "Context", "LibraryError",
-    "default_context", "version", "healpix_map", "filter_sort_particles", "BoundsError",
+    "default_context", "version", "healpix_map", "healpix_maps", "filter_sort_particles",
+    "BoundsError",
     "tsc_mapping", "map_batch", "deposit_batch_device", "distributed_cic_map",
     "distributed_allsky_map", "Subfile", "AllskySubfile", "write_fits_image",
     "read_fits_image", "read_allsky_fits_image", "save_healpix_fits", "read_healpix_fits",

@@ -35,7 +35,8 @@ EXPORTS = (
     "s2g_map", "s2g_healpix_deposit", "s2g_healpix_deposit_device", "s2g_tsc3d",
     "s2g_tsc3d_device", "s2g_deposit_batch", "s2g_accumulate_device",
     "s2g_healpix_reduce_device", "s2g_mass_density", "s2g_mass_density_device",
-    "s2g_footprints", "s2g_tsc3d_device_noalloc",
+    "s2g_footprints", "s2g_tsc3d_device_noalloc", "s2g_healpix_deposit_maps",
+    "s2g_healpix_deposit_maps_device",
 )
 
 
@@ -152,6 +153,15 @@ def load(path: str | None = None):
         L.s2g_mass_density_device.restype = C.c_int
         L.s2g_mass_density_device.argtypes = [vp, C.c_int, vp, vp, i64, C.c_int, C.c_int, i64,
                                               vp, vp, vp, vp, C.POINTER(Stats)]
+        pp = C.POINTER(vp)
+        L.s2g_healpix_deposit_maps.restype = C.c_int
+        L.s2g_healpix_deposit_maps.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, pp, pp, i64,
+                                               i64, C.c_int, C.c_int, C.c_uint32, pp, pp,
+                                               C.POINTER(Stats)]
+        L.s2g_healpix_deposit_maps_device.restype = C.c_int
+        L.s2g_healpix_deposit_maps_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, pp,
+                                                      pp, i64, i64, C.c_int, C.c_int, C.c_uint32,
+                                                      pp, pp, vp, C.POINTER(Stats)]
         L.s2g_tsc3d_device_noalloc.restype = C.c_int
         L.s2g_tsc3d_device_noalloc.argtypes = [vp, C.c_int, vp, vp, i64, C.POINTER(Params),
                                                C.c_uint32, vp, vp]

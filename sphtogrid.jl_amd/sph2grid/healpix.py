@@ -111,3 +111,66 @@ def healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights, *, center=(0.0, 0.0, 0.0),
         stats.update(st.as_dict())
         stats["wall_s"] = time.perf_counter() - t0
     return allsky, weight
+
+
+def healpix_maps(Pos, Hsml, M, Rho, quantities, *, center=(0.0, 0.0, 0.0),
+                 radius_limits=(0.0, np.inf), Nside: int = 1024, kernel,
+                 calc_mean: bool = True, context=None, stats: dict | None = None):
+    """Several ``healpix_map`` calls on one particle set in one deposit.
+
+    ``quantities`` is a sequence of ``(Bin_q, Weights)`` pairs; returns the
+    list of ``(map, weight_map)`` that ``healpix_map(Pos, Hsml, M, Rho, Bin_q,
+    Weights, ...)`` returns for each (main.jl:92-227), with the same checks
+    and early returns per map.  ``Pos`` is shifted by ``center`` once, as
+    each reference call would shift its own copy.  The disc queries, the
+    normalisation sums and the per-ring lists depend only on positions and
+    hsml, so the GPU computes them once (``s2g_healpix_deposit_maps``)."""
+    npix = 12 * int(Nside) ** 2
+    Pos = np.asarray(Pos)
+    arrays = [Pos] + [np.asarray(a) for a in (Hsml, M, Rho)]
+    qs = [(np.asarray(q), np.asarray(w)) for q, w in quantities]
+    if any(a.dtype != np.float64 for a in arrays + [x for p in qs for x in p]):
+        raise TypeError("healpix_map takes Float64 arrays (MethodError in the reference "
+                        "for other element types)")
+    if Pos.ndim != 2 or Pos.shape[0] != 3:
+        raise ValueError("Pos must be a 3 x N array")
+    kid, kdim = kernel_args(kernel)
+    out = [(np.zeros(npix), np.zeros(npix)) for _ in qs]
+    # main.jl:114-121 per map; filter_sort_particles' checks per map (the
+    # selection itself depends only on the positions)
+    live = [k for k, (q, _) in enumerate(qs) if calc_mean or np.sum(q) != 0]
+    if not live:
+        return out
+    for k in live:
+        if not calc_mean:
+            q = qs[k][0]
+            dx = np.sqrt(((Pos - np.asarray(center, dtype=np.float64)[:, None]) ** 2).sum(0))
+            if not find_in_shell(dx, radius_limits).all():
+                raise BoundsError("BoundsError: sel[Bin_q[sel] .> 0.0] (filter_particles.jl:28-30)")
+            if not np.all(q > 0.0):
+                raise BoundsError("BoundsError: sorted[sel] (filter_particles.jl:36)")
+    pos, hsml, m, rho, *rest = filter_sort_particles(
+        Pos, Hsml, M, Rho, np.stack([qs[k][0] for k in live]).T,
+        np.stack([qs[k][1] for k in live]).T, np.asarray(center, dtype=np.float64),
+        radius_limits, True)
+    bq, w = rest
+    n = hsml.shape[0]
+    cv = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    x_il, hsml, m, rho = np.ascontiguousarray(pos.T), cv(hsml), cv(m), cv(rho)
+    qk = [cv(bq[:, j]) for j in range(len(live))]
+    wk = [cv(w[:, j]) for j in range(len(live))]
+    ctx = context or _lib.default_context()
+    st = _lib.Stats()
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    arr = lambda xs: (C.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+    maps = [out[k][0] for k in live]
+    wmaps = [out[k][1] for k in live]
+    flags = _lib.S2G_FLAG_CALC_MEAN if calc_mean else 0
+    rc = ctx.lib.s2g_healpix_deposit_maps(ctx.handle, _lib.S2G_F64, vp(x_il), vp(hsml), vp(m),
+                                          vp(rho), len(live), arr(qk), arr(wk), n, int(Nside),
+                                          kid, kdim, flags, arr(maps), arr(wmaps),
+                                          C.byref(st))
+    ctx.check(rc, "s2g_healpix_deposit_maps")
+    if stats is not None:
+        stats.update(st.as_dict())
+    return out

@@ -194,3 +194,35 @@ def test_allsky_nside1024_ring_lds(s2g, orc):
     pos = d["Pos"].T.copy()
     _compare(s2g, orc, pos, d["HSML"], d["M"], d["Rho"], d["T"], d["Rho"], 1024, "wendlandc4",
              tag="allsky1024")
+
+
+@pytest.mark.parametrize("nside", [64, 2048])
+def test_maps_of_one_particle_set(s2g, orc, nside):
+    """healpix_maps: three maps of one particle set (rho with unit weights, T
+    with rho weights, a third with zero quantities on some particles) in one
+    deposit (shared disc queries, normalisation and ring lists; two-map and
+    one-map groups; the ring-major path at Nside 64, the atomic one at 2048)
+    equal the separate healpix_map calls (<= 1e-12 relative) and the oracle
+    (rel-L1 <= 1e-5)."""
+    pos, h, m, rho, q, w = _shell(500, 91, hmin=2e-3, hmax=0.25 if nside == 64 else 2e-2,
+                                  rmin=0.05 if nside == 64 else 0.3)
+    q3 = q.copy()
+    q3[::4] = 0.0
+    quant = [(rho, np.ones_like(rho)), (q, rho), (q3, w)]
+    for cm in (True, False):
+        if not cm:
+            quant = [(rho, np.ones_like(rho)), (q, rho)]  # calc_mean=false needs Bin_q > 0
+        got = s2g.healpix_maps(pos.T.copy(), h, m, rho, quant, Nside=nside,
+                               kernel=s2g.WendlandC4(2), calc_mean=cm)
+        for k, (bq, ww) in enumerate(quant):
+            gm, gw = s2g.healpix_map(pos.T.copy(), h, m, rho, bq, ww, Nside=nside,
+                                     kernel=s2g.WendlandC4(2), calc_mean=cm,
+                                     show_progress=False)
+            tol = lambda a: 1e-300 + 1e-13 * np.abs(a).max()
+            np.testing.assert_allclose(got[k][0], gm, rtol=1e-12, atol=tol(gm))
+            np.testing.assert_allclose(got[k][1], gw, rtol=1e-12, atol=tol(gw))
+            om, ow = orc.healpix_map(pos.copy(), h, m, rho, bq, ww, Nside=nside,
+                                     kernel="wendlandc4", calc_mean=cm)
+            e1, e2 = rel_l1(got[k][0], om), rel_l1(got[k][1], ow)
+            print(f"[maps nside={nside} cm={cm} map {k}] rel_l1 {e1:.2e} {e2:.2e}", flush=True)
+            assert e1 <= 1e-5 and e2 <= 1e-5
