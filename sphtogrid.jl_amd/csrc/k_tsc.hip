@@ -18,10 +18,10 @@
 //   k_tsc_prep     grid coordinates (FP64) and quantity in sorted order, and
 //                  each particle's nearest cell packed into one word
 //   k_tsc_gather   one wave per brick: scans the particles of the (up to) 27
-//                  neighbouring bricks, and deposits those whose 3^3 stencil
-//                  reaches the brick into LDS, one particle per step (27
-//                  lanes, one stencil cell each), in a fixed order; then
-//                  writes the brick's cells.
+//                  neighbouring bricks, queues those whose 3^3 stencil
+//                  reaches the brick and deposits them into LDS seven at a
+//                  time (9 lanes per particle, one stencil row each), in scan
+//                  order; then writes the brick's cells.
 #include "k_tsc.h"
 
 namespace s2g {
@@ -179,20 +179,65 @@ __device__ __forceinline__ int tsc_reach(int64_t c, int64_t b0, int64_t side, in
     return m;
 }
 
-__device__ __forceinline__ double readlane_d(double v, int lane)
+// Deposit queue of the gather: the candidates whose stencil reaches the
+// brick, in scan order (a ring of LDS slots: grid coordinates + quantity).
+constexpr int kTscQueue = 128;
+// particles per deposit step: 9 lanes each, one stencil row (o0, o1) per
+// lane and its three cells along z
+constexpr int kTscPerStep = kWave / 9;
+
+// One deposit step: particles slots [h, h + cnt) of the queue.  The
+// weights are formed as the oracle forms them: per axis W(g - (c + o) - 1/2)
+// with c = floor(g), and the product (wx * wy) * wz.
+__device__ __forceinline__ void tsc_step(const double4 *queue, int h, int cnt, int ln,
+                                         const int64_t b0[3], int64_t npix, int periodic,
+                                         double (*acc)[kTX * kTY * kTZ])
 {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+    const int slot = ln / 9, row = ln - 9 * slot;
+    if (slot >= cnt)
+        return;
+    const double4 e = queue[(h + slot) % kTscQueue];
+    const double g[3] = {e.x, e.y, e.z};
+    int64_t c[3];
+    for (int d = 0; d < 3; ++d)
+        c[d] = (int64_t)floor(g[d]);
+    const int o0 = row / 3 - 1, o1 = row % 3 - 1;
+    const double wx = tsc_w(sub_rn(g[0], (double)(c[0] + o0) + 0.5));
+    const double wy = tsc_w(sub_rn(g[1], (double)(c[1] + o1) + 0.5));
+    int64_t x = c[0] + o0, y = c[1] + o1;
+    if (periodic) {
+        x = tsc_wrap(x, npix);
+        y = tsc_wrap(y, npix);
+    }
+    const int64_t lx = x - b0[0], ly = y - b0[1];
+    if (x < 0 || x >= npix || lx < 0 || lx >= kTX || y < 0 || y >= npix || ly < 0 ||
+        ly >= kTY)
+        return;
+    const double wxy = wx * wy;
+#pragma unroll
+    for (int o2 = -1; o2 <= 1; ++o2) {
+        int64_t z = c[2] + o2;
+        if (periodic)
+            z = tsc_wrap(z, npix);
+        const int64_t lz = z - b0[2];
+        const double ww = wxy * tsc_w(sub_rn(g[2], (double)(c[2] + o2) + 0.5));
+        if (z >= 0 && z < npix && lz >= 0 && lz < kTZ && ww != 0.0) {
+            const int cell = (int)((lx * kTY + ly) * kTZ + lz);
+            unsafeAtomicAdd(&acc[0][cell], e.w * ww);
+            unsafeAtomicAdd(&acc[1][cell], ww);
+        }
+    }
 }
 
 // One wave per brick.  Lane j < 27 takes neighbouring brick j (offsets
 // (j/9 - 1, (j/3)%3 - 1, j%3 - 1), each brick once) and reads its particle
 // range; the ranges are laid end to end (wave prefix sum) and scanned 64
-// candidates at a time, the next chunk's loads issued before the current
-// chunk is deposited.  A candidate whose stencil reaches the brick is
-// deposited by lanes 0..26 (one stencil cell each) from its lane's registers
-// (readlane), in neighbour order and particle order: a fixed order, so the
-// sums are reproducible.
+// candidates at a time (packed cells only; the next chunk's cells are loaded
+// before the current chunk is handled).  The candidates whose stencil
+// reaches the brick load their coordinates into the LDS queue in scan order,
+// and the queue is deposited kTscPerStep particles per step.  The order of
+// the additions to a cell follows the scan order, so the sums are
+// reproducible.
 __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict__ gq,
                                                       const uint64_t *__restrict__ cell,
                                                       const int64_t *__restrict__ off,
@@ -200,6 +245,7 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
                                                       double *__restrict__ grid)
 {
     __shared__ double acc[2][kTX * kTY * kTZ];
+    __shared__ double4 queue[kTscQueue];
     __shared__ int64_t s_start[32];
     __shared__ int s_base[32];
     int64_t nb[3];
@@ -213,13 +259,12 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
     const int ln = threadIdx.x;
     for (int e = ln; e < 2 * kTX * kTY * kTZ; e += kWave)
         (&acc[0][0])[e] = 0.0;
-    // this lane's stencil cell / neighbour offsets (lanes 0..26)
-    const int o[3] = {ln / 9 - 1, (ln / 3) % 3 - 1, ln % 3 - 1};
     // neighbour brick of lane ln, if it exists and is not a repeat of an
     // earlier offset on some axis (periodic grids of one or two bricks)
     int len = 0;
     int64_t start = 0;
     if (ln < 27) {
+        const int o[3] = {ln / 9 - 1, (ln / 3) % 3 - 1, ln % 3 - 1};
         bool valid = true;
         int64_t x[3];
         for (int d = 0; d < 3; ++d) {
@@ -266,58 +311,39 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
                 lo += step;
         return s_start[lo] + (int64_t)(v - s_base[lo]);
     };
-    bool have = false;
+    const uint64_t below = ln == 0 ? 0ull : (~0ull >> (kWave - ln));
+    int head = 0, tail = 0;  // queue slots consumed / filled (wave-uniform)
+    int64_t kc = -1;
     uint64_t cv = 0;
-    double4 pv = make_double4(0.0, 0.0, 0.0, 0.0);
     if (ln < total) {
-        const int64_t k = locate(ln);
-        cv = cell[k];
-        pv = gq[k];
-        have = true;
+        kc = locate(ln);
+        cv = cell[kc];
     }
     for (int v0 = 0; v0 < total; v0 += kWave) {
-        bool use = have;
+        bool use = kc >= 0;
         if (use)
             for (int d = 0; d < 3; ++d)
                 use = use && tsc_reach(tsc_unpack(cv, d), b0[d], side[d], npix, periodic) != 0;
-        const double4 cur = pv;
-        // prefetch the next chunk
-        have = false;
+        const int64_t kuse = kc;
+        // the next chunk's cells
+        kc = -1;
         if (v0 + kWave + ln < total) {
-            const int64_t k = locate(v0 + kWave + ln);
-            cv = cell[k];
-            pv = gq[k];
-            have = true;
+            kc = locate(v0 + kWave + ln);
+            cv = cell[kc];
         }
-        uint64_t mask = __ballot(use);
-        while (mask) {
-            const int src = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const double g[3] = {readlane_d(cur.x, src), readlane_d(cur.y, src),
-                                 readlane_d(cur.z, src)};
-            const double q = readlane_d(cur.w, src);
-            if (ln < 27) {
-                double w[3];
-                int64_t loc[3];
-                bool in = true;
-                for (int d = 0; d < 3; ++d) {
-                    const int64_t c = (int64_t)floor(g[d]);
-                    w[d] = tsc_w(sub_rn(g[d], (double)(c + o[d]) + 0.5));
-                    int64_t x = c + o[d];
-                    if (periodic)
-                        x = tsc_wrap(x, npix);
-                    loc[d] = x - b0[d];
-                    in = in && x >= 0 && x < npix && loc[d] >= 0 && loc[d] < side[d];
-                }
-                const double ww = (w[0] * w[1]) * w[2];
-                if (in && ww != 0.0) {
-                    const int e = (int)((loc[0] * kTY + loc[1]) * kTZ + loc[2]);
-                    unsafeAtomicAdd(&acc[0][e], q * ww);
-                    unsafeAtomicAdd(&acc[1][e], ww);
-                }
-            }
+        const uint64_t mask = __ballot(use);
+        if (use)
+            queue[(tail + (int)__popcll(mask & below)) % kTscQueue] = gq[kuse];
+        tail += (int)__popcll(mask);
+        __builtin_amdgcn_wave_barrier();
+        while (tail - head >= kTscPerStep) {
+            tsc_step(queue, head, kTscPerStep, ln, b0, npix, periodic, acc);
+            head += kTscPerStep;
         }
+        __builtin_amdgcn_wave_barrier();
     }
+    if (tail > head)
+        tsc_step(queue, head, tail - head, ln, b0, npix, periodic, acc);
     __syncthreads();
     const int64_t nd = npix * npix * npix;
     for (int e = ln; e < kTX * kTY * kTZ; e += kWave) {
