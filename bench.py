@@ -35,6 +35,11 @@ KIDS = {"cubic": 0, "quintic": 1, "wendlandc2": 2, "wendlandc4": 3, "wendlandc6"
         "wendlandc8": 5}
 
 
+# workloads that --only can select
+ONLY = ("cluster3d", "cluster2d_oracle", "plain", "sedov", "allsky", "tsc", "batched", "density",
+        "streaming")
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -131,17 +136,37 @@ def _errs(gpu, ref):
 
 
 def extras(args, p, dev_arrays, torch, s2g, _lib):
-    """Other workloads and oracle comparisons (reported beside the headline)."""
-    import oracle
-    from sph2grid import workloads
+    """Other workloads and oracle comparisons (reported beside the headline);
+    with --only, the selected ones."""
     out = {}
+    want = lambda name: not args.only or name in args.only
     ctx = s2g.default_context()
-    vp = lambda t: C.c_void_p(t.data_ptr())
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    dev = torch.device("cuda:0")
     st = _lib.Stats()
 
     # --- cluster 3D: the same resident particles on a 256^3 grid (WendlandC6)
+    if want("cluster3d"):
+        out.update(cluster3d_extra(ctx, p, dev_arrays, torch, _lib, st, stream))
+    if want("cluster2d_oracle") and not args.no_oracle:
+        out.update(cluster2d_oracle_extra(s2g))
+    if want("plain"):
+        out.update(plain_extra(args, ctx, torch, s2g, _lib, st, stream))
+    if want("sedov"):
+        out.update(sedov_extra(args, ctx, torch, s2g, _lib, st, stream))
+    if want("allsky") or want("tsc"):
+        out.update(allsky_extra(args, ctx, torch, s2g, _lib, st, stream, want))
+    if want("batched"):
+        out["batched_64maps"] = batched_extra(args, torch, s2g, _lib)
+    if want("density"):
+        out["mass_density_2e20_k64"] = density_extra(args, torch, s2g, _lib)
+    return out
+
+
+def cluster3d_extra(ctx, p, dev_arrays, torch, _lib, st, stream):
+    """The headline's resident particles deposited on a 256^3 grid."""
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    dev = torch.device("cuda:0")
+    out = {}
     dx, dh, dm, drho, dw, dbq = dev_arrays
     n = p["x"].shape[0]
     N3 = 256
@@ -168,9 +193,15 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
                                         "exact": st.n_large_exact, "pairs": st.n_pairs},
                              "conservation_rel_err": abs(w3 / w3_exp - 1)}
     log(f"extra cluster 3D: {ms3:.1f} ms")
+    return out
 
-    # --- cluster 2D subsample against the FP64 oracle (same distribution, 2^18)
-    if not args.no_oracle:
+
+def cluster2d_oracle_extra(s2g):
+    """Cluster 2D subsample against the FP64 oracle (same distribution, 2^18)."""
+    import oracle
+    from sph2grid import workloads
+    out = {}
+    if True:
         d = workloads.cluster(n=1 << 18, npix=4096)
         ps = prepare(d, s2g)
         gpu = s2g.deposit(ps["x"], ps["hsml"], ps["m"], ps["rho"], ps["bq"], ps["w"], npix=4096,
@@ -182,8 +213,16 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
         e["conservation_rel_err"] = abs(gpu[:, 1].sum() / ref[:, 1].sum() - 1)
         out["cluster_2d_4096_subsample_2e18_vs_oracle"] = e
         log(f"extra cluster 2D oracle check: {e}")
+    return out
 
-    # --- plain workload (65,536 particles, 256^2, Cubic, Float64)
+
+def plain_extra(args, ctx, torch, s2g, _lib, st, stream):
+    """The plain workload (65,536 particles, 256^2, Cubic, Float64)."""
+    import oracle
+    from sph2grid import workloads
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    dev = torch.device("cuda:0")
+    out = {}
     d = workloads.plain()
     pp = prepare(d, s2g)
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
@@ -212,8 +251,17 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
         rec["conservation_rel_err"] = abs(g[:, 1].sum() / ref[:, 1].sum() - 1)
     out["plain_65536_256"] = rec
     log(f"extra plain: {rec}")
+    return out
 
-    # --- sedov-like (2^20 particles, 1024^2, WendlandC6, [rho; T], calc_mean)
+
+def sedov_extra(args, ctx, torch, s2g, _lib, st, stream):
+    """Sedov-like blast (2^20 particles, 1024^2, WendlandC6, [rho; T], calc_mean)."""
+    import oracle
+    from sph2grid import workloads
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = {}
     d = workloads.sedov()
     ps = prepare(d, s2g)
     ns = ps["x"].shape[0]
@@ -242,9 +290,18 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
         rec["conservation_rel_err"] = abs(g[:, 2].sum() / ref[:, 2].sum() - 1)
     out["sedov_2e20_1024"] = rec
     log(f"extra sedov: {rec}")
+    return out
 
-    # --- all-sky: 2^22 particles, Nside 1024, WendlandC4: rho (Weights = 1)
-    #     and T (Weights = Rho), plus TSC 512^3 of the same particles
+
+def allsky_extra(args, ctx, torch, s2g, _lib, st, stream, want):
+    """All-sky: 2^22 particles, Nside 1024, WendlandC4: rho (Weights = 1) and
+    T (Weights = Rho), plus TSC 512^3 of the same particles."""
+    import oracle
+    from sph2grid import workloads
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = {}
     d = workloads.allsky()
     na = d["HSML"].shape[0]
     nside = d["Nside"]
@@ -265,6 +322,23 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
             nside, 3, 2, _lib.S2G_FLAG_CALC_MEAN, C.c_void_p(maps.data_ptr() + 16 * npx),
             C.c_void_p(maps.data_ptr() + 24 * npx), stream, C.byref(st2)), "allsky T")
 
+    if want("allsky"):
+        out.update(_allsky_maps(args, ctx, s2g, _lib, st, stream, d, apos, ah, am, arho, aT,
+                                stepa_separate, ones, maps, torch))
+    if want("tsc"):
+        out.update(_tsc512(ctx, _lib, stream, apos, aT, na, torch))
+    return out
+
+
+def _allsky_maps(args, ctx, s2g, _lib, st, stream, d, apos, ah, am, arho, aT, stepa_separate,
+                 ones, maps, torch):
+    import oracle
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    out = {}
+    na = d["HSML"].shape[0]
+    nside = d["Nside"]
+    npx = 12 * nside * nside
+    st2 = _lib.Stats()
     # both maps in one s2g_healpix_deposit_maps_device call: one disc query,
     # normalisation and ring-list pass for the particle set
     pa = lambda *xs: (C.c_void_p * len(xs))(*[C.c_void_p(x) for x in xs])
@@ -305,7 +379,13 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
         rec["subsample_2e16_vs_oracle"] = e
     out["allsky_2e22_nside1024"] = rec
     log(f"extra allsky: {rec}")
+    return out
 
+
+def _tsc512(ctx, _lib, stream, apos, aT, na, torch):
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    dev = torch.device("cuda:0")
+    out = {}
     N5 = 512
     grid = torch.empty(2 * N5 ** 3, dtype=torch.float64, device=dev)
     part = _lib.Params(npixels=N5, len2pix=N5 / 2.0)
@@ -318,8 +398,6 @@ def extras(args, p, dev_arrays, torch, s2g, _lib):
     out["tsc_2e22_512"] = {"ms_per_step": mst, "particles_per_s": na / (mst / 1e3),
                            "weight_sum_over_n": wsum / na}
     log(f"extra tsc: {out['tsc_2e22_512']}")
-    out["batched_64maps"] = batched_extra(args, torch, s2g, _lib)
-    out["mass_density_2e20_k64"] = density_extra(args, torch, s2g, _lib)
     return out
 
 
@@ -533,6 +611,16 @@ def measured_traffic(args, n, npix):
     return None, None
 
 
+def calib_copy(torch, dev):
+    """A plain copy of known size (reads 1 GiB, writes 1 GiB): the reference
+    dispatch for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -551,6 +639,9 @@ def main():
                     help="skip the oracle comparisons inside the extras")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
+    ap.add_argument("--only", nargs="+", default=None, choices=ONLY,
+                    help="run only these extras (no headline step, no CPU baseline) and print "
+                         "their records as one JSON line; for profiling one workload")
     args = ap.parse_args()
     if args.gpus != 1:
         msg = ("bench.py runs on one GPU only: the deposit is measured on a single MI355X "
@@ -563,6 +654,14 @@ def main():
 
     import sph2grid as s2g
     from sph2grid import _lib, workloads
+
+    if args.only and not {"cluster3d", "streaming"} & set(args.only):
+        # the selected extras generate their own particle sets
+        dev = torch.device("cuda:0")
+        if args.calib_copy:
+            calib_copy(torch, dev)
+        print(json.dumps({"extra": extras(args, None, None, torch, s2g, _lib)}), flush=True)
+        return
 
     t0 = time.perf_counter()
     gen = {"cluster": workloads.cluster, "plain": workloads.plain, "sedov": workloads.sedov}
@@ -581,6 +680,14 @@ def main():
     tdt = torch.float32 if p["x"].dtype == np.float32 else torch.float64
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype=tdt)
     dx, dh, dm, drho, dw, dbq = (T(p[k]) for k in ("x", "hsml", "m", "rho", "w", "bq"))
+    if args.only:
+        if args.calib_copy:
+            calib_copy(torch, dev)
+        ex = extras(args, p, (dx, dh, dm, drho, dw, dbq), torch, s2g, _lib)
+        if "streaming" in args.only:
+            ex["subfiles_cluster_16x2e20"] = streaming_extra(args, d, torch, s2g, _lib)
+        print(json.dumps({"extra": ex}), flush=True)
+        return
     npix, nq = p["npix"], p["nq"]
     nd = npix * npix
     d_img = torch.empty(nd * (nq + 1), dtype=torch.float64, device=dev)
@@ -622,12 +729,7 @@ def main():
     t2 = time.perf_counter()
     ms_step = (t2 - t1) * 1e3 / args.steps
     if args.calib_copy:
-        # a plain copy of known size: reads 1 GiB and writes 1 GiB
-        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calib_copy(torch, dev)
     for k in stage:
         stage[k] /= args.steps
     value = n / (ms_step / 1e3)

@@ -46,6 +46,41 @@ def test_footprint_box_brute_force(orc, dims):
             assert [tuple(box[p, 2 * d:2 * d + 2]) for d in range(dims)] == axes, p
 
 
+@pytest.mark.parametrize("L", [40.0, 4096.0 / 3.0])
+def test_footprint_box_near_ties(orc, L):
+    """Positions and hsml a few ulps either side of values that put x - h or
+    x + h on a pixel edge: the box then depends on every rounding of
+    get_xyz (x * len2pix, then + Npix / 2, then -/+ h, each in FP64).  The
+    brute force evaluates the same expression in numpy's IEEE FP64, so a
+    fused multiply-add or a reordered sum in the oracle changes boxes here."""
+    npix = 48
+    r = np.random.default_rng(int(L))
+    n = 4000
+    k = r.integers(-20, 20, (n, 3)) + r.choice([0.0, 0.5], (n, 3))
+    x = k / L
+    h = r.choice([0.5, 1.0, 1.5, 2.0, 3.0], n) / L
+    for a, ulps in ((x, r.integers(-3, 4, (n, 3))), (h, r.integers(-3, 4, n))):
+        step = np.where(ulps < 0, -np.inf, np.inf)
+        for _ in range(3):
+            move = np.abs(ulps) > 0
+            a[move] = np.nextafter(a[move], step[move])
+            ulps = ulps - np.sign(ulps)
+    box, cls, _ = orc.footprints(2, x, h, np.ones(n), npix, L, "cubic")
+    changed = 0
+    for p in range(n):
+        axes = []
+        for d in range(2):
+            c = x[p, d] * L + 0.5 * npix
+            axes.append(_brute_axis(c, h[p] * L, npix))
+            exact = (k[p, d] + 0.5 * npix)
+            changed += np.floor(c - h[p] * L) != np.floor(exact - h[p] * L)
+        if any(a is None for a in axes):
+            assert cls[p] == orc.CLS_SKIP, p
+        else:
+            assert [tuple(box[p, 2 * d:2 * d + 2]) for d in range(2)] == axes, p
+    assert changed > 100  # the nudges do move boxes across pixel edges
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_fallback_brute_force(orc, dims):
     npix, L = 32, 32.0
@@ -86,6 +121,8 @@ def test_class_thresholds(orc):
     assert one(2, 16.6, kernel="wendlandc4")[0] == orc.CLS_LARGE_EXPLICIT  # below 28 px
     assert one(2, 28.0, kernel="wendlandc4")[0] == orc.CLS_LARGE_CLOSED
     assert one(2, 40.0, shift=-100.0 / L)[0] == orc.CLS_LARGE_EXPLICIT  # clipped at x = 0
+    assert one(2, 40.0, shift=100.0 / L)[0] == orc.CLS_LARGE_EXPLICIT  # clipped at x = Npix
+    assert one(2, 40.0, shift=80.0 / L)[0] == orc.CLS_LARGE_CLOSED  # 168.5 + 40 < 256
     assert one(2, 5.0, q=0.0, cm=False)[0] == orc.CLS_SKIP
     assert one(2, 5.0, q=0.0, cm=True)[0] == orc.CLS_SMALL
     # 3D: side 8 small, 9 large; threshold 14 voxels for WendlandC6
