@@ -328,6 +328,50 @@ function healpix_deposit!(allsky_map::AbstractVector{Float64},
 end
 
 """
+    healpix_deposit_maps!(allsky_maps, weight_maps, pos, hsml, m, rho, bin_qs, weightss,
+                          nside, kernel, calc_mean)
+
+Several all-sky maps of one particle set in one call (s2g_healpix_deposit_maps,
+at most 8): map k is what `healpix_deposit!` would give for `bin_qs[k]` and
+`weightss[k]` (e.g. the ρ map with weights 1 and the ρ-weighted T map), but
+the disc queries, the kernel normalisation and the ring lists are computed
+once for the particle set.  Every map is overwritten.
+"""
+function healpix_deposit_maps!(allsky_maps::AbstractVector, weight_maps::AbstractVector,
+                               pos, hsml, m, rho, bin_qs::AbstractVector,
+                               weightss::AbstractVector, nside::Integer, kernel,
+                               calc_mean::Bool)
+    nm = length(allsky_maps)
+    1 <= nm <= 8 || error("1 to 8 maps per call")
+    length(weight_maps) == length(bin_qs) == length(weightss) == nm ||
+        error("one weight map, quantity and weight vector per map")
+    am = [_pixels(v) for v in allsky_maps]
+    wm = [_pixels(v) for v in weight_maps]
+    all(length(v) == 12 * Int64(nside)^2 for v in vcat(am, wm)) ||
+        error("maps must hold 12 * nside^2 = $(12 * Int64(nside)^2) pixels")
+    T = Float64
+    x = convert(Matrix{T}, pos)
+    hs, mm, r = (convert(Vector{T}, a) for a in (hsml, m, rho))
+    bq = [convert(Vector{T}, a) for a in bin_qs]
+    w = [convert(Vector{T}, a) for a in weightss]
+    c = context()
+    flags = calc_mean ? S2G_FLAG_CALC_MEAN : UInt32(0)
+    GC.@preserve x hs mm r bq w am wm begin
+        bqp, wp = [pointer(a) for a in bq], [pointer(a) for a in w]
+        amp, wmp = [pointer(a) for a in am], [pointer(a) for a in wm]
+        rc = ccall((:s2g_healpix_deposit_maps, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{T}, Ptr{T}, Ptr{T}, Ptr{T}, Cint, Ptr{Ptr{T}},
+                    Ptr{Ptr{T}}, Int64, Int64, Cint, Cint, UInt32, Ptr{Ptr{Float64}},
+                    Ptr{Ptr{Float64}}, Ptr{Cvoid}),
+                   c.handle, S2G_F64, x, hs, mm, r, Cint(nm), bqp, wp, Int64(length(hs)),
+                   Int64(nside), kernel_id(kernel), Cint(kernel.dim in (2, 3) ? kernel.dim : 2),
+                   flags, amp, wmp, C_NULL)
+    end
+    check(c, rc, "s2g_healpix_deposit_maps")
+    return allsky_maps, weight_maps
+end
+
+"""
     healpix_map(Pos, Hsml, M, Rho, Bin_q, Weights; center=[0.0, 0.0, 0.0],
                 radius_limits=[0.0, Inf], Nside=1024, kernel, show_progress=true,
                 output_from_all_workers=false, calc_mean=true)

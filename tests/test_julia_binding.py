@@ -25,6 +25,8 @@ C2JL = {
     "const void*": {"Ptr{T}", "Ptr{Cvoid}", "Ptr{Float32}", "Ptr{Float64}"},
     "void*": {"Ptr{Cvoid}"},
     "double*": {"Ptr{Float64}"},
+    "const void*const*": {"Ptr{Ptr{T}}", "Ptr{Ptr{Cvoid}}"},
+    "double*const*": {"Ptr{Ptr{Float64}}"},
     "const double*": {"Ptr{Float64}"},
     "const s2g_params*": {"Ref{S2GParams}", "Ptr{S2GParams}"},
     "s2g_stats*": {"Ptr{Cvoid}"},
@@ -160,6 +162,7 @@ def test_binding_covers_the_drop_in_surface():
     src = open(JL).read()
     for fn in ("function sphMapping(", "function healpix_map(", "function healpix_deposit!(",
                "function cic_mapping_2D(", "function cic_mapping_3D(",
+               "function healpix_deposit_maps!(",
                "function reduce_image_2D(", "function reduce_image_3D(",
                "function mass_density(", "function TSCInterpolation(",
                "function deposit_batch!("):
