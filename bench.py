@@ -249,6 +249,24 @@ def plain_extra(args, ctx, torch, s2g, _lib, st, stream):
         rec.update(_errs(oracle.reduce_image_2D(np.asfortranarray(g), 256, True),
                          oracle.reduce_image_2D(ref, 256, True)))
         rec["conservation_rel_err"] = abs(g[:, 1].sum() / ref[:, 1].sum() - 1)
+    if not args.no_tune:
+        # work split tuned for this call (sph2grid.autotune; tuning not timed)
+        from sph2grid import autotune
+        ref_map = mp.clone()
+
+        def run():
+            stepp()
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tp, _, _ = autotune.tune(run, ctx, nplanes=2, reps=5)
+        tune_s = time.perf_counter() - t0
+        mst = _time_steps(stepp, torch, steps=20, warmup=2)
+        ctx.set_tuning(None)
+        rec["tuned"] = {"tuning": tp, "ms_per_step": mst,
+                        "particles_per_s": pp["x"].shape[0] / (mst / 1e3),
+                        "max_rel_diff_vs_default": float((mp - ref_map).abs().max().item() /
+                                                         ref_map.abs().max().item()),
+                        "tuning_s": tune_s}
     out["plain_65536_256"] = rec
     log(f"extra plain: {rec}")
     return out
@@ -280,6 +298,22 @@ def sedov_extra(args, ctx, torch, s2g, _lib, st, stream):
     mss = _time_steps(steps, torch)
     rec = {"ms_per_step": mss, "particles_per_s": ns / (mss / 1e3),
            "stage_ms": {"gather": st.ms_gather, "small": st.ms_small, "sort": st.ms_sort}}
+    # the same with S2G_FLAG_FP32_ACCUM (three planes: the FP32-accumulator gather)
+    mp64 = mp.clone()
+
+    def steps32():
+        ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F32, vp(arr[0]), vp(arr[1]),
+                                             vp(arr[2]), vp(arr[3]), vp(arr[5]), 2, vp(arr[4]),
+                                             ns, C.byref(pars), 4, 2,
+                                             _lib.S2G_FLAG_CALC_MEAN | _lib.S2G_FLAG_FP32_ACCUM,
+                                             vp(img), stream, C.byref(st)), "sedov fp32")
+        ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, 1024, 2, _lib.S2G_FLAG_REDUCE,
+                                            vp(img), vp(mp), stream), "sedov reduce")
+    mss32 = _time_steps(steps32, torch)
+    rec["fp32_accum"] = {"ms_per_step": mss32, "gather_ms": st.ms_gather,
+                         "rel_l1_vs_fp64": float((mp - mp64).abs().sum().item() /
+                                                 mp64.abs().sum().item())}
+    steps()  # leave the FP64 result in img for the oracle check below
     if not args.no_oracle:
         ref = oracle.cic_mapping_2D(ps["x"], ps["hsml"], ps["m"], ps["rho"], ps["bq"], ps["w"],
                                     1024, ps["len2pix"], "wendlandc6", 2, True,
@@ -459,11 +493,15 @@ def batched_extra(args, torch, s2g, _lib):
 
     ms_plain = _time_steps(step_plain, torch, steps=3, warmup=1)
     ms_batch = _time_steps(step_batch, torch, steps=3, warmup=1)
-    diff = 0.0
-    for r in res:
-        a, b = r["o_batch"], r["o_plain"]
-        diff = max(diff, float((a - b).abs().max().item()) / max(float(b.abs().max().item()),
-                                                                  1e-300))
+
+    def max_diff():
+        diff = 0.0
+        for r in res:
+            a, b = r["o_batch"], r["o_plain"]
+            diff = max(diff, float((a - b).abs().max().item()) /
+                       max(float(b.abs().max().item()), 1e-300))
+        return diff
+    diff = max_diff()
     rec = {"maps": len(res), "particles": ntot, "sum_footprint": sum(fps),
            "plain_ms_per_step": ms_plain, "batched_ms_per_step": ms_batch,
            "plain_maps_per_s": len(res) / (ms_plain / 1e3),
@@ -475,6 +513,98 @@ def batched_extra(args, torch, s2g, _lib):
                                 "small": st.ms_small, "device": st.ms_device},
            "counts": {"small": st.n_small, "large": st.n_large, "pairs": st.n_pairs},
            "max_rel_diff_batched_vs_plain": diff}
+    if not args.no_tune:
+        # work split tuned (sph2grid.autotune) for the batch pass and, for the
+        # plain loop, for each map's own call; tuning runs are not timed
+        from sph2grid import autotune
+
+        def sync(fn):
+            def run():
+                fn()
+                torch.cuda.synchronize()
+            return run
+        t0 = time.perf_counter()
+        tb, _, _ = autotune.tune(sync(step_batch), ctx, nplanes=4, reps=2)
+        tplain = []
+        for r in res:
+            def one(r=r):
+                x, h, m, rho, bq, w = r["t"]
+                ctx.check(ctx.lib.s2g_deposit_device(
+                    ctx.handle, 2, _lib.S2G_F32, vp(x), vp(h), vp(m), vp(rho), vp(bq), r["nq"],
+                    vp(w), r["n"], C.byref(r["par"]), kid, 2, flags, vp(r["img"]), stream,
+                    None), "deposit")
+                ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, r["npix"], r["nq"], flags,
+                                                    vp(r["img"]), vp(r["o_plain"]), stream),
+                          "reduce")
+            t, _, _ = autotune.tune(sync(one), ctx, nplanes=4, reps=2)
+            tplain.append(t)
+        tune_s = time.perf_counter() - t0
+
+        def step_plain_tuned():
+            for r, t in zip(res, tplain):
+                ctx.set_tuning(t)
+                x, h, m, rho, bq, w = r["t"]
+                ctx.check(ctx.lib.s2g_deposit_device(
+                    ctx.handle, 2, _lib.S2G_F32, vp(x), vp(h), vp(m), vp(rho), vp(bq), r["nq"],
+                    vp(w), r["n"], C.byref(r["par"]), kid, 2, flags, vp(r["img"]), stream,
+                    None), "deposit")
+                ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, r["npix"], r["nq"], flags,
+                                                    vp(r["img"]), vp(r["o_plain"]), stream),
+                          "reduce")
+        ms_plain_t = _time_steps(step_plain_tuned, torch, steps=3, warmup=1)
+        ctx.set_tuning(tb)
+        ms_batch_t = _time_steps(step_batch, torch, steps=3, warmup=1)
+        ctx.set_tuning(None)
+        rec["tuned"] = {"batch_tuning": tb, "plain_ms_per_step": ms_plain_t,
+                        "batched_ms_per_step": ms_batch_t,
+                        "speedup_batched_vs_plain": ms_plain_t / ms_batch_t,
+                        "speedup_batched_tuned_vs_plain_default": ms_plain / ms_batch_t,
+                        "batched_stage_ms": {"gather": st.ms_gather, "small": st.ms_small,
+                                             "device": st.ms_device},
+                        "max_rel_diff_batched_vs_plain": max_diff(),
+                        "tuning_s": tune_s}
+    # S2G_FLAG_FP32_ACCUM on every map (FP32 per-item pixel sums in the gather),
+    # both sides; errors against the FP64 default's maps
+    ref64 = [r["o_batch"].clone() for r in res]
+    f32 = flags | _lib.S2G_FLAG_FP32_ACCUM
+    fdescs = (_lib.MapDesc * len(res))()
+    for k in range(len(res)):
+        fdescs[k] = descs[k]
+        fdescs[k].flags = f32
+    st32 = _lib.Stats()
+
+    def step_batch32():
+        ctx.check(ctx.lib.s2g_deposit_batch(ctx.handle, _lib.S2G_F32, fdescs, len(res), stream,
+                                            C.byref(st32)), "batch fp32 accumulation")
+
+    def step_plain32():
+        for r in res:
+            x, h, m, rho, bq, w = r["t"]
+            ctx.check(ctx.lib.s2g_deposit_device(ctx.handle, 2, _lib.S2G_F32, vp(x), vp(h), vp(m),
+                                                 vp(rho), vp(bq), r["nq"], vp(w), r["n"],
+                                                 C.byref(r["par"]), kid, 2, f32, vp(r["img"]),
+                                                 stream, None), "deposit")
+            ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, r["npix"], r["nq"], flags,
+                                                vp(r["img"]), vp(r["o_plain"]), stream), "reduce")
+    ms_plain32 = _time_steps(step_plain32, torch, steps=3, warmup=1)
+    ms_batch32 = _time_steps(step_batch32, torch, steps=3, warmup=1)
+    num = den = 0.0
+    worst = 0.0
+    for r, a in zip(res, ref64):
+        d = (r["o_batch"] - a).abs()
+        num += float(d.sum().item())
+        den += float(a.abs().sum().item())
+        m = a.abs() > 1e-3 * float(a.abs().max().item())
+        if bool(m.any()):
+            worst = max(worst, float((d[m] / a.abs()[m]).max().item()))
+    rec["fp32_accum"] = {"plain_ms_per_step": ms_plain32, "batched_ms_per_step": ms_batch32,
+                         "speedup_batched_vs_plain": ms_plain32 / ms_batch32,
+                         "speedup_vs_fp64_plain": ms_plain / ms_batch32,
+                         "batched_stage_ms": {"gather": st32.ms_gather, "small": st32.ms_small,
+                                              "device": st32.ms_device},
+                         "rel_l1_vs_fp64_all_maps": num / den if den else 0.0,
+                         "max_pixel_rel_vs_fp64": worst,
+                         "max_rel_diff_batched_vs_plain": max_diff()}
     if not args.no_oracle:
         # the three maps with the fewest footprint pixels against the oracle
         errs = {}
@@ -635,6 +765,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other workloads (3D, all-sky, TSC, plain, sedov)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuned timings of the plain and batched extras")
     ap.add_argument("--no-oracle", action="store_true",
                     help="skip the oracle comparisons inside the extras")
     ap.add_argument("--calib-copy", action="store_true",

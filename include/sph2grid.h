@@ -95,6 +95,11 @@ enum s2g_dtype { S2G_F32 = 0, S2G_F64 = 1 };
                                     large unclipped footprints) */
 #define S2G_FLAG_PERIODIC 0x4u   /* TSC only: wrap cells around the grid */
 #define S2G_FLAG_REDUCE 0x8u     /* s2g_map / s2g_reduce: reduce_image=true */
+#define S2G_FLAG_FP32_ACCUM 0x10u /* 2D deposit: the tile gather accumulates each work
+                                     item's pixels in FP32 (the items' partials are
+                                     still summed in FP64); faster with three or four
+                                     planes, rel-L1 about 1e-6 instead of 1e-8.  No
+                                     counterpart in the reference (Float64 there) */
 
 typedef struct s2g_ctx s2g_ctx;
 
@@ -128,6 +133,28 @@ typedef struct s2g_stats {
     int64_t n_gather_launches;
 } s2g_stats;
 
+/* Work split of the 2D deposit (s2g_deposit, s2g_map, s2g_deposit_batch in 2D).
+ * No counterpart in the reference: these choose which kernel deposits a
+ * particle and how the work is cut into blocks, not the arithmetic of a pixel
+ * weight, so every setting gives the same maps up to rounding.  Set per
+ * context; sph2grid/autotune.py picks them per shape class. */
+typedef struct s2g_tuning {
+    int32_t small_side;          /* footprint box side (px) up to which a particle
+                                    takes the small-footprint scatter; 5..34, default 34
+                                    (>= 5 keeps every particle of hsml < 2 px, whose
+                                    weights need FP64, in the scatter) */
+    int32_t small_item_min;      /* least particles per small-footprint work item;
+                                    16..4096, default 256 */
+    int32_t small_items_per_cu;  /* small-footprint work items per CU aimed at;
+                                    1..64, default 8 */
+    int32_t gather_item_min;     /* least (tile, particle) pairs per gather work item;
+                                    8..4096, default 64 */
+    int32_t gather_items_per_cu; /* gather work items per CU aimed at; 1..256, default 16 */
+    int32_t gather_planes;       /* image planes per gather launch; 1..4, default 4 */
+    int32_t small_planes;        /* image planes per small-footprint launch; 1..4, default 4 */
+    int32_t reserved;            /* 0 */
+} s2g_tuning;
+
 /* Context: owns the device buffers (grown monotonically) and one blocking
  * stream (default priority) on `device`. */
 int s2g_create(s2g_ctx **out, int device);
@@ -135,6 +162,11 @@ void s2g_destroy(s2g_ctx *ctx);
 const char *s2g_last_error(const s2g_ctx *ctx);
 /* library version string, e.g. "sph2grid 0.1.0 gfx950" */
 const char *s2g_version(void);
+/* Work split of later 2D deposits on this context (NULL: the defaults);
+ * out-of-range fields give S2G_EINVAL and leave the setting unchanged.  No
+ * counterpart in the reference (its deposit is one loop per worker). */
+int s2g_set_tuning(s2g_ctx *ctx, const s2g_tuning *tuning);
+int s2g_get_tuning(s2g_ctx *ctx, s2g_tuning *tuning);
 
 /* Deposit (cic_mapping_2D for dims == 2, cic_mapping_3D for dims == 3).
  * Host pointers.  image_out receives the flat unreduced image
@@ -269,7 +301,7 @@ typedef struct s2g_map_desc {
  * sphMapping calls (cic_interpolation.jl:35-273, one call per map, as in
  * distributed_mapping/cic.jl:24-110).  Maps with the same kernel and nq share
  * one device pass: their particles and tiles are concatenated, and the host
- * synchronises a fixed number of times per pass, not per map.  Synchronous
+ * reads back twice per pass, not per map.  Synchronous
  * on return; `stream` may be NULL (the context's stream).  Errors name the
  * offending map. */
 int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
@@ -284,7 +316,8 @@ int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
  *              the image; (0, -1, ...) for a skipped particle
  *   cls        n int32: 0 skipped (zero quantity without calc_mean, empty
  *              footprint, hsml <= 0 or NaN), 1 small (box side <= 34 px in 2D,
- *              <= 8 in 3D), 2 large with the closed-form normalisation, 3 large
+ *              or the context's s2g_tuning.small_side; <= 8 in 3D), 2 large
+ *              with the closed-form normalisation, 3 large
  *              with an explicit footprint sum
  *   fb         n int32: 1 iff no footprint pixel centre has u < 1 (the
  *              fallback of calculate_weights, cic_2D.jl:49-69, cic_3D.jl:55-75)

@@ -22,7 +22,7 @@ struct Map2D {
     int32_t sbase;    // first small-footprint tile of the map in the pass
     int32_t nts;      // small-footprint tiles per side
     uint32_t flags;
-    int32_t pad;
+    int32_t small_side;  // box side up to which a particle is small (<= kSmallSide2D)
 };
 
 // small-footprint path: particles are bucketed by the kST x kST tile holding
@@ -44,6 +44,7 @@ struct Map2DArgs {
 
 struct GatherArgs {
     int dtype, kernel, np, plane0, nq, nmaps;
+    int fp32acc;  // accumulate each item's pixels in FP32 (S2G_FLAG_FP32_ACCUM)
     int64_t ntiles, nitems;
     const int4 *items;
     const int32_t *pairs;  // particle index per (tile, particle) pair

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k2d_setup(const Map2D *__restrict__ maps,
     r.j0 = (int32_t)j0;
     r.j1 = (int32_t)j1;
     const double num = mul_rn(mul_rn(area, wl), dz);
-    const bool small = (i1 - i0 + 1) <= kSmallSide2D && (j1 - j0 + 1) <= kSmallSide2D;
+    const bool small = (i1 - i0 + 1) <= md.small_side && (j1 - j0 + 1) <= md.small_side;
     if (small) {
         r.cls = kClsSmall;
         r.s = num;
@@ -280,7 +280,9 @@ __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restri
 constexpr int kGBatch = 32;
 constexpr int kGStep = 4;  // row groups per skip test (measured best of 1, 2, 4, 8, 16)
 
-template <typename T, int K, int NP>
+// A: the accumulator type, double, or float with S2G_FLAG_FP32_ACCUM (an
+// item's pixel sums in FP32, written to its FP64 partials)
+template <typename T, int K, int NP, typename A>
 __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
                                             const int32_t *__restrict__ pairs,
                                             const Rec2 *__restrict__ rec,
@@ -305,12 +307,12 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
     const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
     const int rg = ln >> 4;
 
-    double acc[kGRows][NP];
+    A acc[kGRows][NP];
 #pragma unroll
     for (int k = 0; k < kGRows; ++k)
 #pragma unroll
         for (int q = 0; q < NP; ++q)
-            acc[k][q] = 0.0;
+            acc[k][q] = A(0);
 
     for (int base = 0; base < it.z; base += kGBatch) {
         const int cnt = it.z - base < kGBatch ? it.z - base : kGBatch;
@@ -394,10 +396,10 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
             // the lane's column overlap is common to its 16 pixels: fold it
             // into the FP64 plane multipliers
             const double ayd = (double)cv.y;
-            double mult[NP];
+            A mult[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q)
-                mult[q] = s_mult[q][b] * ayd;
+                mult[q] = (A)(s_mult[q][b] * ayd);
             // kGStep row groups per wave-uniform test, so their LDS reads and
             // dependency chains overlap.  A group outside [klo, khi] adds
             // exact zeros (a = 0 outside the footprint, u >= 1 outside the
@@ -417,7 +419,7 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
                 }
 #pragma unroll
                 for (int d = 0; d < kGStep; ++d) {
-                    const double f = (double)wk[d];
+                    const A f = (A)wk[d];
 #pragma unroll
                     for (int q = 0; q < NP; ++q)
                         acc[k0 + d][q] = fma(f, mult[q], acc[k0 + d][q]);
@@ -431,13 +433,16 @@ __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
     for (int q = 0; q < NP; ++q)
 #pragma unroll
         for (int k = 0; k < kGRows; ++k)
-            dst[(q * kGT + 4 * k + rg) * kGT + 16 * wv + (ln & 15)] = acc[k][q];
+            dst[(q * kGT + 4 * k + rg) * kGT + 16 * wv + (ln & 15)] = (double)acc[k][q];
 }
 
 // up to two planes: 128 VGPRs, four waves per SIMD (the accumulators need
 // 64); three or four planes: 168 VGPRs, three waves per SIMD (four planes
 // spill a few registers, which costs less than the lost wave: the batched
-// 64-map set's gather 85 -> 73 ms)
+// 64-map set's gather 85 -> 73 ms).  Tried and dropped: eight waves per
+// block with half the tile's rows per wave, so that four planes fit 128
+// VGPRs at four waves per SIMD: the batched set's gather 70.6 -> 81.5 ms
+// (each wave repeats the per-particle set-up for half the pixels).
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256, 4) void k2d_gather(const int4 *__restrict__ items,
                                                      const int32_t *__restrict__ pairs,
@@ -445,7 +450,7 @@ __global__ __launch_bounds__(256, 4) void k2d_gather(const int4 *__restrict__ it
                                                      const Map2D *__restrict__ maps, int nq,
                                                      int plane0, double *__restrict__ partial)
 {
-    gather_tile<T, K, NP>(items, pairs, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial);
 }
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256, 3) void k2d_gather_wide(const int4 *__restrict__ items,
@@ -454,7 +459,17 @@ __global__ __launch_bounds__(256, 3) void k2d_gather_wide(const int4 *__restrict
                                                        const Map2D *__restrict__ maps, int nq,
                                                        int plane0, double *__restrict__ partial)
 {
-    gather_tile<T, K, NP>(items, pairs, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial);
+}
+// FP32 accumulators: at most 64 VGPRs for four planes, four waves per SIMD
+template <typename T, int K, int NP>
+__global__ __launch_bounds__(256, 4) void k2d_gather_f32acc(const int4 *__restrict__ items,
+                                                         const int32_t *__restrict__ pairs,
+                                                         const Rec2 *__restrict__ rec,
+                                                         const Map2D *__restrict__ maps, int nq,
+                                                         int plane0, double *__restrict__ partial)
+{
+    gather_tile<T, K, NP, float>(items, pairs, rec, maps, nq, plane0, partial);
 }
 
 // fixed-order sum of a tile's item partials -> image planes (plain stores).
@@ -659,6 +674,9 @@ __device__ __forceinline__ bool small_particle(const Rec2 &r, int e, int ln, int
     return true;
 }
 
+// Tried and dropped: FP32 windows (ds_add_f32, half the LDS, two 512-thread
+// blocks per CU) with S2G_FLAG_FP32_ACCUM: the batched set's four-plane
+// scatter went from 30 to 140 ms.
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(small_threads(NP)) void k2d_small_tile(
     const int4 *__restrict__ items, const int64_t *__restrict__ nitems,
@@ -838,7 +856,10 @@ void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *ma
 template <typename T, int K, int NP>
 static void gather_knp(const GatherArgs &g, hipStream_t s)
 {
-    if constexpr (NP <= 2)
+    if (g.fp32acc)
+        hipLaunchKernelGGL((k2d_gather_f32acc<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0,
+                           s, g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial);
+    else if constexpr (NP <= 2)
         hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
                            g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial);
     else

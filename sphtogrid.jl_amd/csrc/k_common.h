@@ -48,10 +48,13 @@ void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, i
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
                   int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s);
 // fill == false: nitems (ntiles+1, last = total) ; fill == true: items + tile_items.
-// items[i] = (global tile, first pair, count, map)
-void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles, int64_t G,
-                  int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items,
-                  bool fill, hipStream_t s);
+// items[i] = (global tile, first pair, count, map).  A tile's list is cut into
+// items of at most G = max(gmin, ceil(*total / target)) entries (G = gmin if
+// total is null); *total is read on the device.
+void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles,
+                  int64_t gmin, int64_t target, const int64_t *total, int64_t *nitems,
+                  int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items, bool fill,
+                  hipStream_t s);
 
 // index of the last table entry whose key is <= v (keys ascending, key[0] <= v)
 template <typename E, typename K, typename F>

@@ -273,14 +273,28 @@ __device__ __forceinline__ void tile_list(const int64_t *__restrict__ off,
     e = off[tm.cbase + (tl + 1) * tm.nch];
 }
 
+// item size: max(gmin, ceil(total / target)) with the total read on the
+// device (so no host round trip is needed between the count and the items),
+// or gmin when total is null
+__device__ __forceinline__ int64_t item_size(int64_t gmin, int64_t target,
+                                             const int64_t *__restrict__ total)
+{
+    if (!total)
+        return gmin;
+    const int64_t g = (*total + target - 1) / target;
+    return g > gmin ? g : gmin;
+}
+
 __global__ void k_item_count(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
-                             int nmaps, int ntiles, int64_t G, int64_t *__restrict__ nitems)
+                             int nmaps, int ntiles, int64_t gmin, int64_t target,
+                             const int64_t *__restrict__ total, int64_t *__restrict__ nitems)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == ntiles)
         nitems[t] = 0;  // becomes the item total after the scan
     if (t >= ntiles)
         return;
+    const int64_t G = item_size(gmin, target, total);
     int m;
     int64_t s, e;
     tile_list(off, tmaps, nmaps, t, m, s, e);
@@ -288,12 +302,14 @@ __global__ void k_item_count(const int64_t *__restrict__ off, const TileMap *__r
 }
 
 __global__ void k_item_fill(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
-                            int nmaps, int ntiles, int64_t G, const int64_t *__restrict__ ibase,
+                            int nmaps, int ntiles, int64_t gmin, int64_t target,
+                            const int64_t *__restrict__ total, const int64_t *__restrict__ ibase,
                             int4 *__restrict__ items, int64_t *__restrict__ tile_items)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles)
         return;
+    const int64_t G = item_size(gmin, target, total);
     int m;
     int64_t s, e;
     tile_list(off, tmaps, nmaps, t, m, s, e);
@@ -307,18 +323,19 @@ __global__ void k_item_fill(const int64_t *__restrict__ off, const TileMap *__re
     tile_items[2 * t + 1] = k;
 }
 
-void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles, int64_t G,
-                  int64_t *nitems, int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items,
-                  bool fill, hipStream_t s)
+void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles,
+                  int64_t gmin, int64_t target, const int64_t *total, int64_t *nitems,
+                  int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items, bool fill,
+                  hipStream_t s)
 {
     const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
     if (!fill) {
         hipLaunchKernelGGL(k_item_count, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles,
-                           G, nitems);
+                           gmin, target, total, nitems);
         exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
     } else {
-        hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles, G,
-                           nitems, items_or_null, tile_items);
+        hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles,
+                           gmin, target, total, nitems, items_or_null, tile_items);
     }
 }
 

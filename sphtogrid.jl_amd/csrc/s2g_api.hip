@@ -24,6 +24,8 @@ using namespace s2g;
 constexpr size_t kScratchBytes = (size_t)80 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
+constexpr s2g_tuning kDefaultTuning = {kSmallSide2D, 256, 8, 64, 16, 4, 4, 0};
+
 struct s2g_ctx {
     int device = 0;
     int ncu = 256;
@@ -55,6 +57,8 @@ struct s2g_ctx {
     // device address of `scratch` on this context's GPU (null if unavailable)
     void *scratch = nullptr;
     std::string scratch_err;
+    // work split of the 2D deposit
+    s2g_tuning tune = kDefaultTuning;
 };
 
 namespace {
@@ -161,9 +165,12 @@ struct TilePlan {
 
 // tm: tile grid of every map of the pass (cbase / nch filled here from the
 // per-map large-list bounds lbase[0..nmaps])
+// Work items of <= G pairs, G = max(gmin, pairs / (ncu * per_cu)): enough
+// items to fill every CU several times.
 TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, bool rec3,
                           std::vector<TileMap> &tm, const std::vector<int64_t> &lbase,
-                          int ntiles, hipStream_t s, double *ms_sort)
+                          int ntiles, hipStream_t s, double *ms_sort, int64_t gmin = 64,
+                          int64_t per_cu = 16)
 {
     hipEvent_t *E = c->stage;
     TilePlan tp;
@@ -208,7 +215,10 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     const PairChunk *d_ch = (const PairChunk *)(d + btm);
 
     int64_t *cnt = (int64_t *)c->cnt.get(sizeof(int64_t) * ncnt);
-    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(ncnt));
+    // the scans of the pair counts (ncnt) and of the item counts (ntiles + 1;
+    // larger when maps of the pass have no large particle) share it
+    const int64_t nscan = ncnt > (int64_t)ntiles + 1 ? ncnt : (int64_t)ntiles + 1;
+    int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nscan));
     S2G_HIP(hipEventRecord(E[4], s));
     if (rec3)
         launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
@@ -216,17 +226,23 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     else
         launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
                            tmp, nullptr, false, s);
+    // item counts from the scanned pair counts, the item size from the pair
+    // total on the device: one read-back gives both totals
+    const int64_t target = (int64_t)c->ncu * per_cu;
+    const int64_t *d_npairs = cnt + ncnt - 1;
+    int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
+    launch_items(cnt, d_tm, nmaps, ntiles, gmin, target, d_npairs, nit, tmp, nullptr, nullptr,
+                 false, s);
     S2G_HIP(hipEventRecord(E[5], s));
-    read_i64(c, s, {cnt + ncnt - 1}, &tp.npairs);
+    int64_t tot[2];
+    read_i64(c, s, {d_npairs, nit + ntiles}, tot);
+    tp.npairs = tot[0];
+    tp.nitems = tot[1];
     if (tp.npairs >= (int64_t)INT32_MAX)
         throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
     tp.pairs = (int32_t *)c->pairs.get(sizeof(int32_t) * (tp.npairs ? tp.npairs : 1));
-    // work items of <= G pairs: enough items to fill every CU several times
-    const int64_t target = (int64_t)c->ncu * 16;
-    int64_t G = (tp.npairs + target - 1) / target;
-    if (G < 64)
-        G = 64;
-    int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
+    tp.items = (int4 *)c->items.get(sizeof(int4) * (tp.nitems ? tp.nitems : 1));
+    tp.tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
     S2G_HIP(hipEventRecord(E[6], s));
     if (rec3)
         launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
@@ -234,16 +250,11 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     else
         launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
                            tmp, tp.pairs, true, s);
-    launch_items(cnt, d_tm, nmaps, ntiles, G, nit, tmp, nullptr, nullptr, false, s);
+    launch_items(cnt, d_tm, nmaps, ntiles, gmin, target, d_npairs, nit, tmp, tp.items,
+                 tp.tile_items, true, s);
     S2G_HIP(hipEventRecord(E[7], s));
-    read_i64(c, s, {nit + ntiles}, &tp.nitems);
-    tp.items = (int4 *)c->items.get(sizeof(int4) * (tp.nitems ? tp.nitems : 1));
-    tp.tile_items = (int64_t *)c->tile_items.get(sizeof(int64_t) * 2 * ntiles);
-    S2G_HIP(hipEventRecord(E[8], s));
-    launch_items(cnt, d_tm, nmaps, ntiles, G, nit, tmp, tp.items, tp.tile_items, true, s);
-    S2G_HIP(hipEventRecord(E[9], s));
-    S2G_HIP(hipEventSynchronize(E[9]));
-    *ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]) + elapsed(E[8], E[9]);
+    S2G_HIP(hipEventSynchronize(E[7]));
+    *ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]);
     return tp;
 }
 
@@ -340,7 +351,7 @@ PassTables pass_tables(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s)
         e.n = a.n;
         e.pbase = ntot;
         e.flags = a.flags;
-        e.pad = 0;
+        e.small_side = c->tune.small_side;
         const int ntj = (int)((a.npix + kGT - 1) / kGT);
         e.ntj = ntj;
         e.tbase = (int32_t)ttot;
@@ -396,6 +407,8 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
 {
     const int dtype = mi[0].a.dtype, kernel = mi[0].a.kernel, nq = mi[0].a.nq;
     const int nplanes = nq + 1;
+    // FP32 per-item sums in the gather (all maps of a pass share the flag)
+    const int fp32acc = (mi[0].a.flags & S2G_FLAG_FP32_ACCUM) ? 1 : 0;
     if (st) {
         std::memset(st, 0, sizeof(*st));
     }
@@ -431,16 +444,19 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     int64_t launches = 0;
     TilePlan tp;
     if (cl.nl > 0) {
-        tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, ntiles, s, &ms_sort);
+        tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, ntiles, s, &ms_sort,
+                              c->tune.gather_item_min, c->tune.gather_items_per_cu);
         if (st)
             st->n_pairs = tp.npairs;
     }
     int ev = 10;
-    for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
-        const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
+    const int gstep = c->tune.gather_planes;
+    for (int plane0 = 0; plane0 < nplanes; plane0 += gstep) {
+        const int np = nplanes - plane0 < gstep ? nplanes - plane0 : gstep;
         GatherArgs g;
         g.dtype = dtype;
         g.kernel = kernel;
+        g.fp32acc = fp32acc;
         g.np = np;
         g.plane0 = plane0;
         g.nq = nq;
@@ -479,9 +495,10 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         const int32_t *sorted = bucket_small(c, cl, rec, d_maps, nstiles, &soff, s);
         // work items of <= Gs particles; the item count stays on the device and
         // the grid is its upper bound nstiles + ns / Gs
-        int64_t Gs = (ns + (int64_t)c->ncu * 8 - 1) / ((int64_t)c->ncu * 8);
-        if (Gs < 256)
-            Gs = 256;
+        const int64_t starget = (int64_t)c->ncu * c->tune.small_items_per_cu;
+        int64_t Gs = (ns + starget - 1) / starget;
+        if (Gs < c->tune.small_item_min)
+            Gs = c->tune.small_item_min;
         const int64_t max_items = nstiles + (ns + Gs - 1) / Gs;
         if (max_items > ((int64_t)1 << 31) / 1024)  // blocks x threads (up to 1024)
             throw Error(S2G_EUNSUPPORTED, "too many small-footprint work items");
@@ -489,10 +506,13 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
         int4 *sitems = (int4 *)c->sitems.get(sizeof(int4) * max_items);
         int64_t *stile = (int64_t *)c->stile.get(sizeof(int64_t) * 2 * nstiles);
         int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nstiles + 1));
-        launch_items(soff, d_stm, nmaps, nstiles, Gs, snit, tmp, nullptr, nullptr, false, s);
-        launch_items(soff, d_stm, nmaps, nstiles, Gs, snit, tmp, sitems, stile, true, s);
-        for (int plane0 = 0; plane0 < nplanes; plane0 += 4) {
-            const int np = nplanes - plane0 < 4 ? nplanes - plane0 : 4;
+        launch_items(soff, d_stm, nmaps, nstiles, Gs, 1, nullptr, snit, tmp, nullptr, nullptr,
+                     false, s);
+        launch_items(soff, d_stm, nmaps, nstiles, Gs, 1, nullptr, snit, tmp, sitems, stile,
+                     true, s);
+        const int sstep = c->tune.small_planes;
+        for (int plane0 = 0; plane0 < nplanes; plane0 += sstep) {
+            const int np = nplanes - plane0 < sstep ? nplanes - plane0 : sstep;
             launch_2d_small_tile(dtype, kernel, np, sitems, snit + nstiles, max_items, sorted,
                                  rec, d_maps, nq, plane0, s);
         }
@@ -833,6 +853,44 @@ void mass_density_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d
 extern "C" {
 
 const char *s2g_version(void) { return kVersion; }
+
+int s2g_set_tuning(s2g_ctx *c, const s2g_tuning *t)
+{
+    return guarded(c, [&] {
+        if (!t) {
+            c->tune = kDefaultTuning;
+            return S2G_OK;
+        }
+        auto in = [](int32_t v, int32_t lo, int32_t hi) { return v >= lo && v <= hi; };
+        // >= 5: a footprint of hsml < 2 px spans at most 5 pixels per axis, and
+        // only the scatter has the FP64 weights such footprints need
+        if (!in(t->small_side, 5, kSmallSide2D))
+            throw Error(S2G_EINVAL, "small_side out of range [5, " +
+                                        std::to_string(kSmallSide2D) + "]");
+        if (!in(t->small_item_min, 16, 4096) || !in(t->small_items_per_cu, 1, 64))
+            throw Error(S2G_EINVAL, "small_item_min / small_items_per_cu out of range "
+                                    "[16, 4096] / [1, 64]");
+        if (!in(t->gather_item_min, 8, 4096) || !in(t->gather_items_per_cu, 1, 256))
+            throw Error(S2G_EINVAL, "gather_item_min / gather_items_per_cu out of range "
+                                    "[8, 4096] / [1, 256]");
+        if (!in(t->gather_planes, 1, 4) || !in(t->small_planes, 1, 4))
+            throw Error(S2G_EINVAL, "gather_planes / small_planes out of range [1, 4]");
+        if (t->reserved != 0)
+            throw Error(S2G_EINVAL, "reserved must be 0");
+        c->tune = *t;
+        return S2G_OK;
+    });
+}
+
+int s2g_get_tuning(s2g_ctx *c, s2g_tuning *t)
+{
+    return guarded(c, [&] {
+        if (!t)
+            throw Error(S2G_EINVAL, "tuning is NULL");
+        *t = c->tune;
+        return S2G_OK;
+    });
+}
 
 int s2g_create(s2g_ctx **out, int device)
 {
@@ -1528,14 +1586,16 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
         hipStream_t s = stream ? (hipStream_t)stream : c->stream;
         s2g_stats total;
         std::memset(&total, 0, sizeof(total));
-        // one pass per (kernel, nq) group, maps in their given order
+        // one pass per (kernel, nq, accumulator) group, maps in their given order
         std::vector<char> done(nmaps, 0);
         for (int k = 0; k < nmaps; ++k) {
             if (done[k])
                 continue;
             std::vector<int> grp;
             for (int j = k; j < nmaps; ++j)
-                if (!done[j] && maps[j].kernel == maps[k].kernel && maps[j].nq == maps[k].nq) {
+                if (!done[j] && maps[j].kernel == maps[k].kernel && maps[j].nq == maps[k].nq &&
+                    (maps[j].flags & S2G_FLAG_FP32_ACCUM) ==
+                        (maps[k].flags & S2G_FLAG_FP32_ACCUM)) {
                     grp.push_back(j);
                     done[j] = 1;
                 }

@@ -65,6 +65,22 @@ struct S2GMapDesc
     d_out::Ptr{Float64}
 end
 
+# s2g_tuning of include/sph2grid.h (the 2D deposit's work split)
+struct S2GTuning
+    small_side::Int32
+    small_item_min::Int32
+    small_items_per_cu::Int32
+    gather_item_min::Int32
+    gather_items_per_cu::Int32
+    gather_planes::Int32
+    small_planes::Int32
+    reserved::Int32
+end
+S2GTuning(; small_side=34, small_item_min=256, small_items_per_cu=8, gather_item_min=64,
+          gather_items_per_cu=16, gather_planes=4, small_planes=4) =
+    S2GTuning(small_side, small_item_min, small_items_per_cu, gather_item_min,
+              gather_items_per_cu, gather_planes, small_planes, 0)
+
 kernel_id(::Cubic) = Cint(0)
 kernel_id(::Quintic) = Cint(1)
 kernel_id(::WendlandC2) = Cint(2)
@@ -261,6 +277,24 @@ function sphMapping(Pos::Array{<:Real}, HSML::Array{<:Real}, M::Array{<:Real},
         return reduce_image_3D(image, param.Npixels[1], reduce_image)
     end
     error("dimensions must be 2 or 3")
+end
+
+"""
+    set_tuning!(t::Union{S2GTuning,Nothing}=nothing)
+
+The work split of later 2D deposits (s2g_set_tuning); `nothing` restores the
+library defaults.  It never changes a pixel weight's arithmetic.
+"""
+function set_tuning!(t::Union{S2GTuning,Nothing}=nothing)
+    c = context()
+    rc = if t === nothing
+        ccall((:s2g_set_tuning, libsph2grid), Cint, (Ptr{Cvoid}, Ptr{S2GTuning}), c.handle,
+              C_NULL)
+    else
+        ccall((:s2g_set_tuning, libsph2grid), Cint, (Ptr{Cvoid}, Ref{S2GTuning}), c.handle, t)
+    end
+    check(c, rc, "s2g_set_tuning")
+    return nothing
 end
 
 """

@@ -27,6 +27,7 @@ S2G_FLAG_CALC_MEAN = 0x1
 S2G_FLAG_EXACT_NORM = 0x2
 S2G_FLAG_PERIODIC = 0x4
 S2G_FLAG_REDUCE = 0x8
+S2G_FLAG_FP32_ACCUM = 0x10
 
 # every symbol include/sph2grid.h declares
 EXPORTS = (
@@ -36,7 +37,7 @@ EXPORTS = (
     "s2g_tsc3d_device", "s2g_deposit_batch", "s2g_accumulate_device",
     "s2g_healpix_reduce_device", "s2g_mass_density", "s2g_mass_density_device",
     "s2g_footprints", "s2g_tsc3d_device_noalloc", "s2g_healpix_deposit_maps",
-    "s2g_healpix_deposit_maps_device",
+    "s2g_healpix_deposit_maps_device", "s2g_set_tuning", "s2g_get_tuning",
 )
 
 
@@ -64,6 +65,23 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    """s2g_tuning: the 2D deposit's work split (no effect on the arithmetic)."""
+    _fields_ = [("small_side", C.c_int32), ("small_item_min", C.c_int32),
+                ("small_items_per_cu", C.c_int32), ("gather_item_min", C.c_int32),
+                ("gather_items_per_cu", C.c_int32), ("gather_planes", C.c_int32),
+                ("small_planes", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# the library's defaults (kDefaultTuning in csrc/s2g_api.hip)
+DEFAULT_TUNING = dict(small_side=34, small_item_min=256, small_items_per_cu=8,
+                      gather_item_min=64, gather_items_per_cu=16, gather_planes=4,
+                      small_planes=4)
 
 
 class MapDesc(C.Structure):
@@ -169,6 +187,10 @@ def load(path: str | None = None):
         L.s2g_footprints.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, i64,
                                      C.POINTER(Params), C.c_int, C.c_uint32, vp, vp, vp, vp, vp,
                                      i64, vp, vp, vp]
+        L.s2g_set_tuning.restype = C.c_int
+        L.s2g_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.s2g_get_tuning.restype = C.c_int
+        L.s2g_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         if path is None:
             _lib = L
         return L
@@ -192,6 +214,23 @@ class Context:
             msg = self.lib.s2g_last_error(self.handle)
             msg = msg.decode() if msg else ""
             raise LibraryError(f"{what} failed ({rc}): {msg}", rc)
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self.check(self.lib.s2g_get_tuning(self.handle, C.byref(t)), "s2g_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, tuning: dict | None = None):
+        """Set the 2D work split (a dict of s2g_tuning fields; missing fields
+        keep the library defaults); None restores the defaults."""
+        if tuning is None:
+            self.check(self.lib.s2g_set_tuning(self.handle, None), "s2g_set_tuning")
+            return
+        unknown = set(tuning) - set(DEFAULT_TUNING)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        t = Tuning(**{**DEFAULT_TUNING, **tuning})
+        self.check(self.lib.s2g_set_tuning(self.handle, C.byref(t)), "s2g_set_tuning")
 
     def close(self):
         if getattr(self, "handle", None):

@@ -42,7 +42,7 @@ def deposit_batch_device(maps, *, dtype: str = "f32", context=None, stream=None,
     ``out`` (flat image of (nq+1) npix^2 doubles, or with ``reduce_image`` the
     reduced map of nq npix^2 doubles), ``n``, ``nq``, ``npix``, ``len2pix``,
     ``kernel`` (kernel object or id) and optionally ``calc_mean``,
-    ``reduce_image``, ``exact_norm``.  Arrays are torch CUDA tensors or device
+    ``reduce_image``, ``exact_norm``, ``fp32_accum``.  Arrays are torch CUDA tensors or device
     addresses.  Particles must already be centred and filtered."""
     ctx = context or _lib.default_context()
     descs = (_lib.MapDesc * max(len(maps), 1))()
@@ -50,7 +50,8 @@ def deposit_batch_device(maps, *, dtype: str = "f32", context=None, stream=None,
         kid = d["kernel"] if isinstance(d["kernel"], int) else kernel_args(d["kernel"])[0]
         flags = ((_lib.S2G_FLAG_CALC_MEAN if d.get("calc_mean", False) else 0) |
                  (_lib.S2G_FLAG_REDUCE if d.get("reduce_image", False) else 0) |
-                 (_lib.S2G_FLAG_EXACT_NORM if d.get("exact_norm", False) else 0))
+                 (_lib.S2G_FLAG_EXACT_NORM if d.get("exact_norm", False) else 0) |
+                 (_lib.S2G_FLAG_FP32_ACCUM if d.get("fp32_accum", False) else 0))
         descs[k] = _lib.MapDesc(
             dims=2, nq=int(d["nq"]), n=int(d["n"]),
             params=_lib.Params(npixels=int(d["npix"]), len2pix=float(d["len2pix"])),

@@ -408,3 +408,26 @@ def test_deposit_argument_errors(s2g):
                       kernel=s2g.Cubic(2))
     np.testing.assert_allclose(img[:2 * npix * npix], ref.ravel(order="F"), rtol=1e-12,
                                atol=0)
+
+
+@pytest.mark.parametrize("nq", [1, 3])
+def test_fp32_accumulation(s2g, orc, nq):
+    """S2G_FLAG_FP32_ACCUM: the gather's per-item pixel sums in FP32 stay
+    within the deposit's tolerances of the FP64 oracle."""
+    npix, n = 256, 6000
+    x, h, m, rho, w = _random_particles(n, npix, seed=77 + nq, hmin_px=5.0, hmax_px=120.0)
+    q = np.random.default_rng(3).lognormal(0.0, 1.0, (n, nq))
+    kern = _kernel(s2g, "wendlandc6")
+    st = {}
+    got = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix), kernel=kern,
+                      calc_mean=True, fp32_accum=True, stats=st)
+    f64 = s2g.deposit(x, h, m, rho, q, w, npix=npix, len2pix=float(npix), kernel=kern,
+                      calc_mean=True)
+    ref = orc.cic_mapping_2D(x, h, m, rho, q, w, npix, float(npix), "wendlandc6", 2, True)
+    assert st["n_large"] > n // 2
+    print("fp32 accumulation rel_l1 vs oracle:",
+          [f"{rel_l1(got[:, k], ref[:, k]):.2e}" for k in range(nq + 1)],
+          "fp64:", [f"{rel_l1(f64[:, k], ref[:, k]):.2e}" for k in range(nq + 1)], flush=True)
+    _check_flat(got, ref, f"fp32acc/nq{nq}")
+    for k in range(nq + 1):
+        assert rel_l1(got[:, k], f64[:, k]) <= 1e-5

@@ -31,6 +31,7 @@ C2JL = {
     "const s2g_params*": {"Ref{S2GParams}", "Ptr{S2GParams}"},
     "s2g_stats*": {"Ptr{Cvoid}"},
     "const s2g_map_desc*": {"Ptr{S2GMapDesc}"},
+    "const s2g_tuning*": {"Ref{S2GTuning}", "Ptr{S2GTuning}"},
 }
 CRET = {"int": "Cint", "void": "Cvoid", "const char*": "Cstring"}
 # C struct field type -> Julia field type
@@ -149,7 +150,8 @@ def test_every_ccall_matches_its_prototype():
 
 
 @pytest.mark.parametrize("cname, jname", [("s2g_params", "S2GParams"),
-                                          ("s2g_map_desc", "S2GMapDesc")])
+                                          ("s2g_map_desc", "S2GMapDesc"),
+                                          ("s2g_tuning", "S2GTuning")])
 def test_struct_mirrors_match(cname, jname):
     c = c_struct_fields(cname)
     j = jl_struct_fields(jname)
