@@ -6,9 +6,12 @@
 namespace s2g {
 
 // brick of the gather: kTX x kTY x kTZ cells (z fastest in memory), one wave
-// per brick; a 3-cell stencil meets on average (1 + 2/8)^2 (1 + 2/16) = 1.76
-// bricks
-constexpr int kTX = 8, kTY = 8, kTZ = 16;
+// per brick; a 3-cell stencil meets on average (1 + 2/4)(1 + 2/8)^2 = 2.34
+// bricks.  Small bricks keep the wave's LDS (two FP64 planes of 256 cells and
+// the queue, 8 KB) small enough for 20 waves per CU: on 2^22 particles at
+// 512^3, 8x8x16 bricks took 3.80 ms, 8x8x8 2.17, 8x8x4 2.19, 4x8x8 2.05 and
+// 4x8x16 2.87 ms.
+constexpr int kTX = 4, kTY = 8, kTZ = 8;
 
 // bricks of an npix^3 grid (the sort keys run over [0, nbricks])
 int64_t tsc_nbricks(int64_t npix);
