@@ -126,6 +126,11 @@ def _time_steps(fn, torch, steps=3, warmup=1):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def _errs_max(a, b):
+    """max |a - b| / max |b| of two device tensors."""
+    return float((a - b).abs().max().item()) / max(float(b.abs().max().item()), 1e-300)
+
+
 def _errs(gpu, ref):
     gpu = np.asarray(gpu, dtype=np.float64).ravel(order="F")
     ref = np.asarray(ref, dtype=np.float64).ravel(order="F")
@@ -249,10 +254,32 @@ def plain_extra(args, ctx, torch, s2g, _lib, st, stream):
         rec.update(_errs(oracle.reduce_image_2D(np.asfortranarray(g), 256, True),
                          oracle.reduce_image_2D(ref, 256, True)))
         rec["conservation_rel_err"] = abs(g[:, 1].sum() / ref[:, 1].sum() - 1)
+    # the same call as a replayed plan (s2g_plan_*: one hipGraph, no host
+    # read-back inside); its output against the eager call's
+    ref_map = mp.clone()
+    mp_plan = torch.empty_like(mp)
+    pdesc = (_lib.MapDesc * 1)()
+    pdesc[0] = _lib.MapDesc(dims=2, nq=1, n=pp["x"].shape[0], params=parp, kernel=0,
+                            flags=_lib.S2G_FLAG_REDUCE, d_pos=arr[0].data_ptr(),
+                            d_hsml=arr[1].data_ptr(), d_m=arr[2].data_ptr(),
+                            d_rho=arr[3].data_ptr(), d_bin_q=arr[5].data_ptr(),
+                            d_weights=arr[4].data_ptr(), d_out=mp_plan.data_ptr())
+    ph = C.c_void_p()
+    ctx.check(ctx.lib.s2g_plan_create(ctx.handle, _lib.S2G_F64, pdesc, 1, 1.25, C.byref(ph)),
+              "s2g_plan_create")
+    try:
+        def stepplan():
+            ctx.check(ctx.lib.s2g_plan_run(ph, stream, None), "s2g_plan_run")
+        msplan = _time_steps(stepplan, torch, steps=50, warmup=3)
+        rec["plan"] = {"ms_per_step": msplan,
+                       "particles_per_s": pp["x"].shape[0] / (msplan / 1e3),
+                       "max_rel_diff_vs_eager": float((mp_plan - ref_map).abs().max().item() /
+                                                      ref_map.abs().max().item())}
+    finally:
+        ctx.lib.s2g_plan_destroy(ph)
     if not args.no_tune:
         # work split tuned for this call (sph2grid.autotune; tuning not timed)
         from sph2grid import autotune
-        ref_map = mp.clone()
 
         def run():
             stepp()
@@ -563,6 +590,32 @@ def batched_extra(args, torch, s2g, _lib):
                                              "device": st.ms_device},
                         "max_rel_diff_batched_vs_plain": max_diff(),
                         "tuning_s": tune_s}
+    # the batch as a replayed plan (s2g_plan_*: one hipGraph, sized at creation)
+    pdescs = (_lib.MapDesc * len(res))()
+    for k, r in enumerate(res):
+        r["o_plan"] = torch.empty_like(r["o_batch"])
+        pdescs[k] = descs[k]
+        pdescs[k].d_out = r["o_plan"].data_ptr()
+    ph = C.c_void_p()
+    t0 = time.perf_counter()
+    ctx.check(ctx.lib.s2g_plan_create(ctx.handle, _lib.S2G_F32, pdescs, len(res), 1.25,
+                                      C.byref(ph)), "s2g_plan_create")
+    create_s = time.perf_counter() - t0
+    stp = _lib.Stats()
+    try:
+        def step_plan():
+            ctx.check(ctx.lib.s2g_plan_run(ph, stream, C.byref(stp)), "s2g_plan_run")
+        ms_plan = _time_steps(step_plan, torch, steps=3, warmup=1)
+        step_batch()
+        torch.cuda.synchronize()
+        pdiff = max(_errs_max(r["o_plan"], r["o_batch"]) for r in res)
+    finally:
+        ctx.lib.s2g_plan_destroy(ph)
+    rec["plan"] = {"ms_per_step": ms_plan, "create_s": create_s,
+                   "speedup_vs_plain": ms_plain / ms_plan, "device_ms": stp.ms_device,
+                   "max_rel_diff_vs_batch": pdiff}
+    for r in res:
+        del r["o_plan"]
     # S2G_FLAG_FP32_ACCUM on every map (FP32 per-item pixel sums in the gather),
     # both sides; errors against the FP64 default's maps
     ref64 = [r["o_batch"].clone() for r in res]

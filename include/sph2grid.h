@@ -71,6 +71,7 @@ extern "C" {
 #define S2G_EHIP (-2)
 #define S2G_ENOMEM (-3)
 #define S2G_EUNSUPPORTED (-4)
+#define S2G_ECAPACITY (-5)  /* s2g_plan_run: the inputs outgrew the plan's capacities */
 
 /* SPHKernels.jl kernel types (the `kernel::AbstractSPHKernel` argument of
  * sphMapping, cic_interpolation.jl:40).  Their normalisation and h^-dim
@@ -306,6 +307,29 @@ typedef struct s2g_map_desc {
  * offending map. */
 int s2g_deposit_batch(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps,
                       int nmaps, void *stream, s2g_stats *stats);
+
+/* Replayable batch (no counterpart in the reference; for callers that
+ * deposit the same maps again and again, e.g. every snapshot of a run or a
+ * serving loop, and for callers that cannot afford host round trips inside
+ * a deposit).  s2g_plan_create fixes the descriptors (device pointers,
+ * particle counts, image sizes, kernels, flags) and the context's current
+ * s2g_tuning, sizes every work list from one run of the class and pair-count
+ * stages on the current inputs (capacities = that run's counts times
+ * `headroom`, 1..64), and captures the whole batch in one hipGraph.
+ * s2g_plan_run replays it on `stream` (NULL: the context's stream) with the
+ * inputs' current contents and returns when the maps are written; nothing in
+ * the replay reads back to the host, and the run reads back only the
+ * capacity flags (and the counts for `stats`) after it.  When the inputs
+ * need more large particles or (tile, particle) pairs than the capacities,
+ * the run writes nothing past them and returns S2G_ECAPACITY (the maps are
+ * then incomplete): create a new plan.  Results equal s2g_deposit_batch's up
+ * to the order of the small-footprint additions.  Plans hold their own
+ * device memory; a plan and its context are used from one thread. */
+typedef struct s2g_plan s2g_plan;
+int s2g_plan_create(s2g_ctx *ctx, int dtype, const s2g_map_desc *maps, int nmaps,
+                    double headroom, s2g_plan **out);
+int s2g_plan_run(s2g_plan *plan, void *stream, s2g_stats *stats);
+void s2g_plan_destroy(s2g_plan *plan);
 
 /* Test export (no counterpart in the reference; for bit-exact checks of the
  * deposit's integer work against a CPU restatement).  Runs the setup, class

@@ -45,7 +45,9 @@ struct Map2DArgs {
 struct GatherArgs {
     int dtype, kernel, np, plane0, nq, nmaps;
     int fp32acc;  // accumulate each item's pixels in FP32 (S2G_FLAG_FP32_ACCUM)
-    int64_t ntiles, nitems;
+    int64_t ntiles, nitems;  // nitems: the item count, or its upper bound with d_nitems
+    const int64_t *d_nitems = nullptr;  // the item count on the device (replayed passes)
+    const int *d_flag = nullptr;        // nonzero: skip the gather (capacity exceeded)
     const int4 *items;
     const int32_t *pairs;  // particle index per (tile, particle) pair
     const Rec2 *rec;
@@ -57,8 +59,10 @@ struct GatherArgs {
 // setup over all particles of the pass: records and the class array
 void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int nq, int kernel,
                      Rec2 *rec, int32_t *cls, hipStream_t s);
+// ne: the count, or with d_bounds its upper bound (the count then being
+// d_bounds[3] - d_bounds[2] on the device)
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *maps, Rec2 *rec,
-                    hipStream_t s);
+                    hipStream_t s, const int64_t *d_bounds = nullptr);
 void launch_2d_gather(const GatherArgs &g, hipStream_t s);
 void launch_2d_finalize(const GatherArgs &g, hipStream_t s);
 // keys (global small tile) and values (particle) of the small list
@@ -71,6 +75,14 @@ void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *
 void launch_2d_small_tile(int dtype, int kernel, int np, const int4 *items,
                           const int64_t *nitems, int64_t max_items, const int32_t *plist,
                           const Rec2 *rec, const Map2D *maps, int nq, int plane0, hipStream_t s);
+// small-footprint buckets on the device (no library sort, no memset): off
+// (nstiles + 1; off[nstiles] = ns) and the particles of tile t at
+// sorted[off[t] .. off[t+1]); ns = *d_ns <= ns_cap; cursor: nstiles int64
+void launch_2d_small_buckets(const int32_t *list, const int64_t *d_ns, int64_t ns_cap,
+                             const Rec2 *rec, const Map2D *maps, int nstiles, int64_t *off,
+                             int64_t *cursor, int64_t *scan_tmp, int32_t *sorted, hipStream_t s);
+// d[0 .. n) = v (a kernel, so that it can be recorded in a graph)
+void launch_fill_i64(int64_t *d, int64_t v, int64_t n, hipStream_t s);
 // reduce_image_2D of every map of the pass that has an output
 void launch_2d_reduce_maps(const Map2D *maps, int nmaps, int ntiles, int nq, hipStream_t s);
 // reduce_image_2D of one flat image

@@ -253,11 +253,16 @@ __device__ __forceinline__ void k2d_norm_one(int64_t gw, const int32_t *__restri
 }
 
 template <int K>
+// d_bounds (null: ne as given): the class bounds on the device, ne =
+// d_bounds[3] - d_bounds[2]
 __global__ __launch_bounds__(kNormThreads) void k2d_norm(const int32_t *__restrict__ list,
                                                          int64_t ne,
+                                                         const int64_t *__restrict__ d_bounds,
                                                          const Map2D *__restrict__ maps,
                                                          Rec2 *__restrict__ rec)
 {
+    if (d_bounds)
+        ne = d_bounds[3] - d_bounds[2];
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < ne;
          gw += nw)
@@ -282,13 +287,20 @@ constexpr int kGStep = 4;  // row groups per skip test (measured best of 1, 2, 4
 
 // A: the accumulator type, double, or float with S2G_FLAG_FP32_ACCUM (an
 // item's pixel sums in FP32, written to its FP64 partials)
+// d_nitems (null in a call sized on the host): the item count on the device,
+// the grid being its upper bound; d_flag != 0 (a replay whose pair lists
+// outgrew their capacity): no item runs
 template <typename T, int K, int NP, typename A>
 __device__ __forceinline__ void gather_tile(const int4 *__restrict__ items,
                                             const int32_t *__restrict__ pairs,
                                             const Rec2 *__restrict__ rec,
                                             const Map2D *__restrict__ maps, int nq, int plane0,
-                                            double *__restrict__ partial)
+                                            double *__restrict__ partial,
+                                            const int64_t *__restrict__ d_nitems,
+                                            const int *__restrict__ d_flag)
 {
+    if (d_nitems && ((int64_t)blockIdx.x >= *d_nitems || *d_flag))
+        return;  // the whole block
     __shared__ float2 s_row[kGBatch][kGT];
     __shared__ float2 s_col[kGBatch][kGT];
     __shared__ double s_x[kGBatch], s_y[kGBatch], s_h[kGBatch];
@@ -448,18 +460,24 @@ __global__ __launch_bounds__(256, 4) void k2d_gather(const int4 *__restrict__ it
                                                      const int32_t *__restrict__ pairs,
                                                      const Rec2 *__restrict__ rec,
                                                      const Map2D *__restrict__ maps, int nq,
-                                                     int plane0, double *__restrict__ partial)
+                                                     int plane0, double *__restrict__ partial,
+                                                     const int64_t *__restrict__ d_nitems,
+                                                     const int *__restrict__ d_flag)
 {
-    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial, d_nitems,
+                                  d_flag);
 }
 template <typename T, int K, int NP>
 __global__ __launch_bounds__(256, 3) void k2d_gather_wide(const int4 *__restrict__ items,
                                                        const int32_t *__restrict__ pairs,
                                                        const Rec2 *__restrict__ rec,
                                                        const Map2D *__restrict__ maps, int nq,
-                                                       int plane0, double *__restrict__ partial)
+                                                       int plane0, double *__restrict__ partial,
+                                                     const int64_t *__restrict__ d_nitems,
+                                                     const int *__restrict__ d_flag)
 {
-    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP, double>(items, pairs, rec, maps, nq, plane0, partial, d_nitems,
+                                  d_flag);
 }
 // FP32 accumulators: at most 64 VGPRs for four planes, four waves per SIMD
 template <typename T, int K, int NP>
@@ -467,9 +485,12 @@ __global__ __launch_bounds__(256, 4) void k2d_gather_f32acc(const int4 *__restri
                                                          const int32_t *__restrict__ pairs,
                                                          const Rec2 *__restrict__ rec,
                                                          const Map2D *__restrict__ maps, int nq,
-                                                         int plane0, double *__restrict__ partial)
+                                                         int plane0, double *__restrict__ partial,
+                                                         const int64_t *__restrict__ d_nitems,
+                                                         const int *__restrict__ d_flag)
 {
-    gather_tile<T, K, NP, float>(items, pairs, rec, maps, nq, plane0, partial);
+    gather_tile<T, K, NP, float>(items, pairs, rec, maps, nq, plane0, partial, d_nitems,
+                                 d_flag);
 }
 
 // fixed-order sum of a tile's item partials -> image planes (plain stores).
@@ -540,6 +561,61 @@ __global__ void k2d_small_keys(const int32_t *__restrict__ list, int64_t ns,
     const Map2D &md = maps[r.map];
     keys[e] = (uint32_t)(md.sbase + (r.i0 / kST) * md.nts + r.j0 / kST);
     vals[e] = p;
+}
+
+// Small-footprint buckets without a library sort (replayed passes, whose
+// graphs hold kernels only): a histogram of the anchor tiles (hist zeroed by
+// k_fill_i64, ntiles + 1 entries), an exclusive scan, then each particle
+// placed at its tile's cursor.  The order inside a tile depends on the
+// atomics, which changes only the order of the scatter's LDS additions (not
+// reproducible in any case).  ns = *d_ns on the device; grid-stride loops.
+__device__ __forceinline__ int64_t small_key(const Rec2 &r, const Map2D *__restrict__ maps)
+{
+    const Map2D &md = maps[r.map];
+    return md.sbase + (int64_t)(r.i0 / kST) * md.nts + r.j0 / kST;
+}
+
+__global__ __launch_bounds__(256) void k2d_small_hist(const int32_t *__restrict__ list,
+                                                      const int64_t *__restrict__ d_ns,
+                                                      const Rec2 *__restrict__ rec,
+                                                      const Map2D *__restrict__ maps,
+                                                      unsigned long long *__restrict__ hist)
+{
+    const int64_t ns = *d_ns;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns;
+         e += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&hist[small_key(rec[list[e]], maps)], 1ull);
+}
+
+__global__ __launch_bounds__(256) void k2d_small_place(const int32_t *__restrict__ list,
+                                                       const int64_t *__restrict__ d_ns,
+                                                       const Rec2 *__restrict__ rec,
+                                                       const Map2D *__restrict__ maps,
+                                                       unsigned long long *__restrict__ cursor,
+                                                       int32_t *__restrict__ sorted)
+{
+    const int64_t ns = *d_ns;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = list[e];
+        const unsigned long long slot = atomicAdd(&cursor[small_key(rec[p], maps)], 1ull);
+        sorted[slot] = p;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_i64(int64_t *__restrict__ d, int64_t v, int64_t n)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        d[e] = v;
+}
+
+__global__ __launch_bounds__(256) void k_copy_i64(const int64_t *__restrict__ src,
+                                                  int64_t *__restrict__ dst, int64_t n)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
 }
 
 __global__ void k_tile_offsets(const uint32_t *__restrict__ keys, int64_t ns, int ntiles,
@@ -845,12 +921,13 @@ void launch_2d_setup(int dtype, const Map2D *maps, int nmaps, int64_t ntot, int 
 }
 
 void launch_2d_norm(int kernel, const int32_t *list, int64_t ne, const Map2D *maps, Rec2 *rec,
-                    hipStream_t s)
+                    hipStream_t s, const int64_t *d_bounds)
 {
     if (ne == 0)
         return;
     const dim3 g((unsigned)wave_blocks(ne, kNormThreads)), b(kNormThreads);
-    S2G_K2D_SWITCH(kernel, hipLaunchKernelGGL(k2d_norm<KK>, g, b, 0, s, list, ne, maps, rec));
+    S2G_K2D_SWITCH(kernel,
+                   hipLaunchKernelGGL(k2d_norm<KK>, g, b, 0, s, list, ne, d_bounds, maps, rec));
 }
 
 template <typename T, int K, int NP>
@@ -858,14 +935,16 @@ static void gather_knp(const GatherArgs &g, hipStream_t s)
 {
     if (g.fp32acc)
         hipLaunchKernelGGL((k2d_gather_f32acc<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0,
-                           s, g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial);
+                           s, g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial,
+                           g.d_nitems, g.d_flag);
     else if constexpr (NP <= 2)
         hipLaunchKernelGGL((k2d_gather<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0, s,
-                           g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial);
+                           g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0, g.partial,
+                           g.d_nitems, g.d_flag);
     else
         hipLaunchKernelGGL((k2d_gather_wide<T, K, NP>), dim3((unsigned)g.nitems), dim3(256), 0,
                            s, g.items, g.pairs, g.rec, g.maps, g.nq, g.plane0,
-                           g.partial);
+                           g.partial, g.d_nitems, g.d_flag);
 }
 
 template <typename T, int K>
@@ -906,6 +985,32 @@ void launch_2d_small_keys(const int32_t *list, int64_t ns, const Rec2 *rec, cons
         return;
     hipLaunchKernelGGL(k2d_small_keys, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, list,
                        ns, rec, maps, keys, vals);
+}
+
+static unsigned stride_blocks(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+void launch_fill_i64(int64_t *d, int64_t v, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_i64, dim3(stride_blocks(n)), dim3(256), 0, s, d, v, n);
+}
+
+void launch_2d_small_buckets(const int32_t *list, const int64_t *d_ns, int64_t ns_cap,
+                             const Rec2 *rec, const Map2D *maps, int nstiles, int64_t *off,
+                             int64_t *cursor, int64_t *scan_tmp, int32_t *sorted, hipStream_t s)
+{
+    launch_fill_i64(off, 0, nstiles + 1, s);
+    const unsigned nb = stride_blocks(ns_cap);
+    hipLaunchKernelGGL(k2d_small_hist, dim3(nb), dim3(256), 0, s, list, d_ns, rec, maps,
+                       (unsigned long long *)off);
+    exclusive_scan_i64(off, nstiles + 1, scan_tmp, s);
+    hipLaunchKernelGGL(k_copy_i64, dim3(stride_blocks(nstiles)), dim3(256), 0, s, off, cursor,
+                       (int64_t)nstiles);
+    hipLaunchKernelGGL(k2d_small_place, dim3(nb), dim3(256), 0, s, list, d_ns, rec, maps,
+                       (unsigned long long *)cursor, sorted);
 }
 
 void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *off,

@@ -34,7 +34,8 @@ struct TileMap {
     int64_t cbase;
     int32_t tbase, ntiles;
     int32_t nt, ntc;
-    int32_t nch, pad;
+    int32_t nch;
+    int32_t csz;  // entries per chunk when the chunk ranges are taken on the device
 };
 // chunk k of map `map`: large-list entries [l0, l1)
 struct PairChunk {
@@ -43,10 +44,18 @@ struct PairChunk {
 
 // fill == false: count + scan into cnt (ncnt int64, the last is the total)
 // fill == true : fill pairs (particle indices) from the scanned cnt
+// lbase (null: the chunk table's fixed ranges): per-map large-list starts on
+// the device, chunk k of map m then covering lbase[m] + k csz .. (clipped);
+// cap: entries of pairs (slots past it are not written)
 template <typename R>
 void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
-                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s);
+                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s,
+                  const int64_t *lbase = nullptr, int64_t cap = INT64_MAX);
+// flag = (a map's large list outgrew nch * csz entries) | 2 * (pair total >
+// pair_cap), for a replayed pass sized at creation
+void launch_plan_check(const int64_t *lbase, int nmaps, const TileMap *tmaps,
+                       const int64_t *pair_total, int64_t pair_cap, int *flag, hipStream_t s);
 // fill == false: nitems (ntiles+1, last = total) ; fill == true: items + tile_items.
 // items[i] = (global tile, first pair, count, map).  A tile's list is cut into
 // items of at most G = max(gmin, ceil(*total / target)) entries (G = gmin if

@@ -160,20 +160,43 @@ __device__ __forceinline__ void tile_rect(const Rec3 &r, int &a0, int &a1, int &
     c1 = r.k1 / kBT;
 }
 
+// A chunk's large-list entries [l0, l1): fixed in the chunk table, or, with
+// lbase (the per-map list starts, on the device), entries lbase[m] + k csz ..
+// of map m (csz = TileMap::csz), clipped to the map's list, so that the
+// table does not depend on the class counts.
+__device__ __forceinline__ void chunk_range(const PairChunk &ch, const TileMap &tm,
+                                            const int64_t *__restrict__ lbase, int64_t &l0,
+                                            int64_t &l1)
+{
+    if (!lbase) {
+        l0 = ch.l0;
+        l1 = ch.l1;
+        return;
+    }
+    const int64_t a = lbase[ch.map] + (int64_t)ch.k * tm.csz, e = lbase[ch.map + 1];
+    l0 = a;
+    l1 = a + tm.csz < e ? a + tm.csz : e;
+    if (l1 < l0)
+        l1 = l0;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ large,
                                                     const R *__restrict__ rec,
                                                     const PairChunk *__restrict__ chunks,
                                                     const TileMap *__restrict__ tmaps,
+                                                    const int64_t *__restrict__ lbase,
                                                     int64_t ncnt, int64_t *__restrict__ cnt)
 {
     extern __shared__ int hist[];
     const PairChunk ch = chunks[blockIdx.x];
     const TileMap tm = tmaps[ch.map];
+    int64_t cl0, cl1;
+    chunk_range(ch, tm, lbase, cl0, cl1);
     for (int t = threadIdx.x; t < tm.ntiles; t += blockDim.x)
         hist[t] = 0;
     __syncthreads();
-    for (int64_t l = ch.l0 + threadIdx.x; l < ch.l1; l += blockDim.x) {
+    for (int64_t l = cl0 + threadIdx.x; l < cl1; l += blockDim.x) {
         const R r = rec[large[l]];
         int a0, a1, b0, b1, c0, c1;
         tile_rect(r, a0, a1, b0, b1, c0, c1);
@@ -194,22 +217,27 @@ __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ 
 // The records of 64 particles are loaded at once (a lane each) and each
 // particle's tile rectangle is broadcast by readlane, so the loads' latency
 // is paid once per 64 particles.
+// pairs: cap entries; a slot at or past cap is not written (a replayed pass
+// whose pair total outgrew its capacity; the caller sees the total)
 template <typename R>
 __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__ large,
                                                      const R *__restrict__ rec,
                                                      const PairChunk *__restrict__ chunks,
                                                      const TileMap *__restrict__ tmaps,
+                                                     const int64_t *__restrict__ lbase,
                                                      const int64_t *__restrict__ off,
-                                                     int32_t *__restrict__ pairs)
+                                                     int32_t *__restrict__ pairs, int64_t cap)
 {
     extern __shared__ int cursor[];
     const PairChunk ch = chunks[blockIdx.x];
     const TileMap tm = tmaps[ch.map];
+    int64_t cl0, cl1;
+    chunk_range(ch, tm, lbase, cl0, cl1);
     for (int t = threadIdx.x; t < tm.ntiles; t += kWave)
         cursor[t] = (int)off[tm.cbase + (int64_t)t * tm.nch + ch.k];
     __syncthreads();
-    for (int64_t lb = ch.l0; lb < ch.l1; lb += kWave) {
-        const int n = ch.l1 - lb < kWave ? (int)(ch.l1 - lb) : kWave;
+    for (int64_t lb = cl0; lb < cl1; lb += kWave) {
+        const int n = cl1 - lb < kWave ? (int)(cl1 - lb) : kWave;
         int32_t pl = 0;
         int ra0 = 0, ra1 = -1, rb0 = 0, rb1 = -1, rc0 = 0, rc1 = -1;
         if ((int)threadIdx.x < n) {
@@ -228,7 +256,8 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
                 const int rem = k % (wb * wc);
                 const int b = b0 + rem / wc, c = c0 + rem % wc;
                 const int slot = atomicAdd(&cursor[(a * tm.nt + b) * tm.ntc + c], 1);
-                pairs[slot] = p;
+                if (slot < cap)
+                    pairs[slot] = p;
             }
         }
     }
@@ -237,25 +266,54 @@ __global__ __launch_bounds__(kWave) void k_pair_fill(const int32_t *__restrict__
 template <typename R>
 void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
-                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s)
+                  int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s,
+                  const int64_t *lbase, int64_t cap)
 {
     const size_t lds = sizeof(int) * (size_t)(max_tiles > 0 ? max_tiles : 1);
     if (!fill) {
         hipLaunchKernelGGL(k_pair_count<R>, dim3((unsigned)nchunks), dim3(256), lds, s, large,
-                           rec, chunks, tmaps, ncnt, cnt);
+                           rec, chunks, tmaps, lbase, ncnt, cnt);
         exclusive_scan_i64(cnt, ncnt, scan_tmp, s);
     } else {
         hipLaunchKernelGGL(k_pair_fill<R>, dim3((unsigned)nchunks), dim3(kWave), lds, s, large,
-                           rec, chunks, tmaps, cnt, pairs_or_null);
+                           rec, chunks, tmaps, lbase, cnt, pairs_or_null, cap);
     }
 }
 
 template void launch_pairs<Rec2>(const int32_t *, const Rec2 *, const PairChunk *, int,
                                  const TileMap *, int, int64_t *, int64_t, int64_t *, int32_t *,
-                                 bool, hipStream_t);
+                                 bool, hipStream_t, const int64_t *, int64_t);
 template void launch_pairs<Rec3>(const int32_t *, const Rec3 *, const PairChunk *, int,
                                  const TileMap *, int, int64_t *, int64_t, int64_t *, int32_t *,
-                                 bool, hipStream_t);
+                                 bool, hipStream_t, const int64_t *, int64_t);
+
+// Capacity flag of a replayed pass (written whole, one block): bit 0, a
+// map's large list is longer than its chunks cover (nch * csz entries; the
+// rest of the list was neither counted nor placed); bit 1, the pair total
+// passed the pair capacity (the pairs past it were not written).
+__global__ __launch_bounds__(256) void k_plan_check(const int64_t *__restrict__ lbase, int nmaps,
+                                                    const TileMap *__restrict__ tmaps,
+                                                    const int64_t *__restrict__ pair_total,
+                                                    int64_t pair_cap, int *__restrict__ flag)
+{
+    __shared__ int f;
+    if (threadIdx.x == 0)
+        f = 0;
+    __syncthreads();
+    for (int m = threadIdx.x; m < nmaps; m += blockDim.x)
+        if (lbase[m + 1] - lbase[m] > (int64_t)tmaps[m].nch * tmaps[m].csz)
+            atomicOr(&f, 1);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *flag = f | (*pair_total > pair_cap ? 2 : 0);
+}
+
+void launch_plan_check(const int64_t *lbase, int nmaps, const TileMap *tmaps,
+                       const int64_t *pair_total, int64_t pair_cap, int *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_plan_check, dim3(1), dim3(256), 0, s, lbase, nmaps, tmaps, pair_total,
+                       pair_cap, flag);
+}
 
 // ---------------------------------------------------------------------------
 // Work items: tile t's list [start, start+count) is split into ceil(count/G)

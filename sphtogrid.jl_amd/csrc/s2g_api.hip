@@ -2,6 +2,7 @@
 // staging of host arrays, orchestration of the device pipelines.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -848,6 +849,427 @@ void mass_density_device(s2g_ctx *c, int dtype, const void *d_pos, const void *d
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batches: argument checks, pass groups, per-pass inputs.
+// ---------------------------------------------------------------------------
+void check_batch(int dtype, const s2g_map_desc *maps, int nmaps)
+{
+    if (nmaps < 0 || (nmaps > 0 && !maps))
+        throw Error(S2G_EINVAL, "maps is NULL or nmaps < 0");
+    if (dtype != S2G_F32 && dtype != S2G_F64)
+        throw Error(S2G_EINVAL, "dtype must be S2G_F32 or S2G_F64");
+    for (int k = 0; k < nmaps; ++k) {
+        const s2g_map_desc &d = maps[k];
+        const std::string at = "map " + std::to_string(k) + ": ";
+        if (d.dims != 2)
+            throw Error(S2G_EUNSUPPORTED, at + "batched deposits are 2D (dims == 2)");
+        try {
+            check_common(dtype, d.kernel, 2, d.n, &d.params);
+            check_dims_nq(2, d.nq);
+        } catch (const Error &e) {
+            throw Error(e.code, at + e.msg);
+        }
+        if (!d.d_out)
+            throw Error(S2G_EINVAL, at + "output pointer is NULL");
+        if (d.n > 0 && (!d.d_pos || !d.d_hsml || !d.d_m || !d.d_rho || !d.d_bin_q ||
+                        !d.d_weights))
+            throw Error(S2G_EINVAL, at + "input array pointer is NULL");
+    }
+}
+
+// one pass per (kernel, nq, accumulator) group, maps in their given order
+std::vector<std::vector<int>> batch_groups(const s2g_map_desc *maps, int nmaps)
+{
+    std::vector<std::vector<int>> groups;
+    std::vector<char> done(nmaps, 0);
+    for (int k = 0; k < nmaps; ++k) {
+        if (done[k])
+            continue;
+        std::vector<int> grp;
+        for (int j = k; j < nmaps; ++j)
+            if (!done[j] && maps[j].kernel == maps[k].kernel && maps[j].nq == maps[k].nq &&
+                (maps[j].flags & S2G_FLAG_FP32_ACCUM) == (maps[k].flags & S2G_FLAG_FP32_ACCUM)) {
+                grp.push_back(j);
+                done[j] = 1;
+            }
+        groups.push_back(grp);
+    }
+    return groups;
+}
+
+// doubles of the flat images of a group's reduced maps
+int64_t batch_flat_elems(const s2g_map_desc *maps, const std::vector<int> &grp)
+{
+    int64_t flat = 0;
+    for (int j : grp)
+        if (maps[j].flags & S2G_FLAG_REDUCE)
+            flat += maps[j].params.npixels * maps[j].params.npixels * (maps[j].nq + 1);
+    return flat;
+}
+
+// the group's pass inputs; reduced maps deposit into img (batch_flat_elems)
+std::vector<MapIn> batch_inputs(int dtype, const s2g_map_desc *maps, const std::vector<int> &grp,
+                                double *img)
+{
+    std::vector<MapIn> in;
+    int64_t off = 0;
+    for (int j : grp) {
+        const s2g_map_desc &d = maps[j];
+        MapIn m;
+        m.a = make_args(dtype, d.d_pos, d.d_hsml, d.d_m, d.d_rho, d.d_bin_q, d.nq, d.d_weights,
+                        d.n, &d.params, d.kernel, d.flags);
+        if (d.flags & S2G_FLAG_REDUCE) {
+            m.image = img + off;
+            m.out = d.d_out;
+            off += d.params.npixels * d.params.npixels * (d.nq + 1);
+        } else {
+            m.image = d.d_out;
+            m.out = nullptr;
+        }
+        in.push_back(m);
+    }
+    return in;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Replayable batch (s2g_plan_*): the passes of s2g_deposit_batch with every
+// size fixed at creation and every count kept on the device, captured once
+// in a hipGraph and replayed.  Differences from the eager pass, all to keep
+// the host out of the replay:
+//  * the explicit-norm and gather grids are upper bounds, the counts read on
+//    the device (class bounds, item totals);
+//  * the pair-list chunks are ranges of each map's large list computed on the
+//    device from the class bounds (nch chunks of csz entries per map, sized
+//    from the creation run's large count times the headroom);
+//  * pairs have a capacity (the creation run's total times the headroom);
+//    k_plan_check raises a flag when either capacity is passed, the pair
+//    fill writes nothing past it and the gather does not run;
+//  * small-footprint buckets come from a histogram, a scan and an atomic
+//    placement (kernels only: no library sort, no memset node).
+// ---------------------------------------------------------------------------
+struct PlanPass {
+    std::vector<MapIn> mi;
+    int dtype = 0, kernel = 0, nq = 0, nplanes = 0, fp32acc = 0, nmaps = 0;
+    int64_t ntot = 0, nb = 0, ncnt = 0, pair_cap = 0, item_cap = 0, sitem_cap = 0;
+    int ntiles = 0, nstiles = 0, max_tiles = 1, nchunks = 0, gstep = 4, sstep = 4;
+    int64_t gmin = 64, gtarget = 1, smin = 256, starget = 1;
+    bool any_out = false;
+    DevBuf tab, img, rec, cls, counts, scan_tmp, l0, l1, l2, bounds, tiles, cnt, pairs, nitems,
+        items, tile_items, partial, soff, scur, sorted, snitems, sitems, stile, flag;
+    const Map2D *d_maps = nullptr;
+    const int64_t *d_pbase = nullptr;
+    const TileMap *d_stm = nullptr, *d_tm = nullptr;
+    const PairChunk *d_ch = nullptr;
+    void release()
+    {
+        for (DevBuf *b : {&tab, &img, &rec, &cls, &counts, &scan_tmp, &l0, &l1, &l2, &bounds,
+                          &tiles, &cnt, &pairs, &nitems, &items, &tile_items, &partial, &soff,
+                          &scur, &sorted, &snitems, &sitems, &stile, &flag})
+            b->release();
+    }
+};
+
+struct s2g_plan {
+    s2g_ctx *c = nullptr;
+    std::vector<PlanPass> passes;
+    hipStream_t cs = nullptr;  // capture stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t *h_read = nullptr;  // pinned: per pass flag + counts after a run
+};
+
+namespace {
+
+void plan_release(s2g_plan *pl)
+{
+    if (pl->exec)
+        (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph)
+        (void)hipGraphDestroy(pl->graph);
+    if (pl->cs)
+        (void)hipStreamDestroy(pl->cs);
+    if (pl->h_read)
+        (void)hipHostFree(pl->h_read);
+    for (PlanPass &pp : pl->passes)
+        pp.release();
+    pl->exec = nullptr;
+    pl->graph = nullptr;
+    pl->cs = nullptr;
+    pl->h_read = nullptr;
+}
+
+// Everything of one pass on `s`, sized by the pass's capacities, with no
+// host read-back (the sequence that is captured).
+void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
+{
+    Rec2 *rec = (Rec2 *)pp.rec.p;
+    int32_t *cls = (int32_t *)pp.cls.p, *l0 = (int32_t *)pp.l0.p, *l1 = (int32_t *)pp.l1.p,
+            *l2 = (int32_t *)pp.l2.p;
+    int64_t *tmp = (int64_t *)pp.scan_tmp.p, *bounds = (int64_t *)pp.bounds.p;
+    launch_2d_setup(pp.dtype, pp.d_maps, pp.nmaps, pp.ntot, pp.nq, pp.kernel, rec, cls, s);
+    launch_partition(cls, pp.ntot, (int64_t *)pp.counts.p, tmp, l0, l1, l2, s);
+    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, l1, pp.d_pbase, pp.nmaps, bounds, s);
+    launch_2d_norm(pp.kernel, l2, pp.ntot, pp.d_maps, rec, s, bounds);
+    // large particles: pair lists over device-side chunk ranges
+    int64_t *cnt = (int64_t *)pp.cnt.p, *nit = (int64_t *)pp.nitems.p;
+    const int64_t *lbase = bounds + 4, *d_npairs = cnt + pp.ncnt - 1;
+    launch_pairs<Rec2>(l1, rec, pp.d_ch, pp.nchunks, pp.d_tm, pp.max_tiles, cnt, pp.ncnt, tmp,
+                       nullptr, false, s, lbase);
+    launch_items(cnt, pp.d_tm, pp.nmaps, pp.ntiles, pp.gmin, pp.gtarget, d_npairs, nit, tmp,
+                 nullptr, nullptr, false, s);
+    launch_plan_check(lbase, pp.nmaps, pp.d_tm, d_npairs, pp.pair_cap, (int *)pp.flag.p, s);
+    launch_pairs<Rec2>(l1, rec, pp.d_ch, pp.nchunks, pp.d_tm, pp.max_tiles, cnt, pp.ncnt, tmp,
+                       (int32_t *)pp.pairs.p, true, s, lbase, pp.pair_cap);
+    launch_items(cnt, pp.d_tm, pp.nmaps, pp.ntiles, pp.gmin, pp.gtarget, d_npairs, nit, tmp,
+                 (int4 *)pp.items.p, (int64_t *)pp.tile_items.p, true, s);
+    for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.gstep) {
+        const int np = pp.nplanes - plane0 < pp.gstep ? pp.nplanes - plane0 : pp.gstep;
+        GatherArgs g;
+        g.dtype = pp.dtype;
+        g.kernel = pp.kernel;
+        g.fp32acc = pp.fp32acc;
+        g.np = np;
+        g.plane0 = plane0;
+        g.nq = pp.nq;
+        g.nmaps = pp.nmaps;
+        g.ntiles = pp.ntiles;
+        g.nitems = pp.item_cap;
+        g.d_nitems = nit + pp.ntiles;
+        g.d_flag = (const int *)pp.flag.p;
+        g.items = (const int4 *)pp.items.p;
+        g.pairs = (const int32_t *)pp.pairs.p;
+        g.rec = rec;
+        g.maps = pp.d_maps;
+        g.partial = (double *)pp.partial.p;
+        g.tile_items = (const int64_t *)pp.tile_items.p;
+        launch_2d_gather(g, s);
+        launch_2d_finalize(g, s);
+    }
+    // small particles: device-side buckets and items (bounds[1] = the small count)
+    int64_t *soff = (int64_t *)pp.soff.p, *snit = (int64_t *)pp.snitems.p;
+    launch_2d_small_buckets(l0, bounds + 1, pp.ntot, rec, pp.d_maps, pp.nstiles, soff,
+                            (int64_t *)pp.scur.p, tmp, (int32_t *)pp.sorted.p, s);
+    launch_items(soff, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
+                 nullptr, nullptr, false, s);
+    launch_items(soff, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
+                 (int4 *)pp.sitems.p, (int64_t *)pp.stile.p, true, s);
+    for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.sstep) {
+        const int np = pp.nplanes - plane0 < pp.sstep ? pp.nplanes - plane0 : pp.sstep;
+        launch_2d_small_tile(pp.dtype, pp.kernel, np, (const int4 *)pp.sitems.p, snit + pp.nstiles,
+                             pp.sitem_cap, (const int32_t *)pp.sorted.p, rec, pp.d_maps, pp.nq,
+                             plane0, s);
+    }
+    if (pp.any_out)
+        launch_2d_reduce_maps(pp.d_maps, pp.nmaps, pp.ntiles, pp.nq, s);
+    (void)c;
+}
+
+// Sizes one pass from an eager run of its first stages, allocates it.
+void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
+{
+    const int nmaps = pp.nmaps;
+    // tables into plan-owned memory (the eager tables live in the context)
+    PassTables pt = pass_tables(c, pp.mi.data(), nmaps, s);
+    const size_t btab = sizeof(Map2D) * nmaps, bpb = sizeof(int64_t) * (nmaps + 1);
+    const size_t bst = sizeof(TileMap) * nmaps;
+    char *d = (char *)pp.tab.get(btab + bpb + bst);
+    S2G_HIP(hipMemcpyAsync(d, pt.d_maps, btab + bpb + bst, hipMemcpyDeviceToDevice, s));
+    pp.d_maps = (const Map2D *)d;
+    pp.d_pbase = (const int64_t *)(d + btab);
+    pp.d_stm = (const TileMap *)(d + btab + bpb);
+    pp.ntot = pt.ntot;
+    pp.ntiles = pt.ntiles;
+    pp.nstiles = pt.nstiles;
+    pp.any_out = pt.any_out;
+    if (pp.ntot == 0)  // the class stages need at least one particle per pass
+        throw Error(S2G_EINVAL, "every pass of a plan needs particles (maps with the same "
+                                "kernel, nq and accumulator have none)");
+    const int64_t n = pp.ntot > 0 ? pp.ntot : 1;
+    pp.nb = partition_blocks(n);
+    pp.rec.get(sizeof(Rec2) * n);
+    pp.cls.get(sizeof(int32_t) * n);
+    pp.counts.get(sizeof(int64_t) * (3 * pp.nb + 1));
+    pp.l0.get(sizeof(int32_t) * n);
+    pp.l1.get(sizeof(int32_t) * n);
+    pp.l2.get(sizeof(int32_t) * n);
+    pp.bounds.get(sizeof(int64_t) * (nmaps + 5));
+    pp.flag.get(sizeof(int) * 2);
+    // creation run of the class stages: the per-map large counts
+    launch_2d_setup(pp.dtype, pp.d_maps, nmaps, pp.ntot, pp.nq, pp.kernel, (Rec2 *)pp.rec.p,
+                    (int32_t *)pp.cls.p, s);
+    pp.scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * pp.nb + 1));
+    launch_partition((const int32_t *)pp.cls.p, pp.ntot, (int64_t *)pp.counts.p,
+                     (int64_t *)pp.scan_tmp.p, (int32_t *)pp.l0.p, (int32_t *)pp.l1.p,
+                     (int32_t *)pp.l2.p, s);
+    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, (const int32_t *)pp.l1.p, pp.d_pbase,
+                        nmaps, (int64_t *)pp.bounds.p, s);
+    std::vector<int64_t> hb(nmaps + 5);
+    S2G_HIP(hipMemcpyAsync(hb.data(), pp.bounds.p, sizeof(int64_t) * (nmaps + 5),
+                           hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipStreamSynchronize(s));
+    // chunks: csz per map from its large count (as the eager pass), nch
+    // covering headroom times that count, at least one chunk
+    std::vector<TileMap> tm = pt.tm;
+    std::vector<PairChunk> chunks;
+    int64_t cb = 0;
+    pp.max_tiles = 1;
+    for (int m = 0; m < nmaps; ++m) {
+        if (tm[m].ntiles > kMaxTiles)
+            throw Error(S2G_EUNSUPPORTED, "grid too large for the tile lists");
+        pp.max_tiles = tm[m].ntiles > pp.max_tiles ? tm[m].ntiles : pp.max_tiles;
+        const int64_t nl = hb[4 + m + 1] - hb[4 + m];
+        const int64_t cap = (int64_t)std::ceil((double)nl * headroom) + 64;
+        int64_t csz = (cap + 32 * (int64_t)c->ncu - 1) / (32 * (int64_t)c->ncu);
+        csz = csz < 64 ? 64 : csz > kPairChunk ? kPairChunk : (csz + 63) / 64 * 64;
+        while ((int64_t)tm[m].ntiles * ((cap + csz - 1) / csz) > (1 << 25))
+            csz *= 2;
+        const int nch = (int)((cap + csz - 1) / csz);
+        tm[m].cbase = cb;
+        tm[m].nch = nch;
+        tm[m].csz = (int32_t)csz;
+        cb += (int64_t)tm[m].ntiles * nch;
+        for (int k = 0; k < nch; ++k)
+            chunks.push_back(PairChunk{m, k, 0, 0});
+    }
+    pp.nchunks = (int)chunks.size();
+    pp.ncnt = cb + 1;
+    const size_t btm = sizeof(TileMap) * nmaps, bch = sizeof(PairChunk) * chunks.size();
+    char *dt = (char *)pp.tiles.get(btm + bch);
+    S2G_HIP(hipMemcpy(dt, tm.data(), btm, hipMemcpyHostToDevice));
+    S2G_HIP(hipMemcpy(dt + btm, chunks.data(), bch, hipMemcpyHostToDevice));
+    pp.d_tm = (const TileMap *)dt;
+    pp.d_ch = (const PairChunk *)(dt + btm);
+    // scans: class counts, pair counts, item counts, small buckets and items
+    int64_t nscan = 3 * pp.nb + 1;
+    for (int64_t v : {pp.ncnt, (int64_t)pp.ntiles + 1, (int64_t)pp.nstiles + 1})
+        nscan = v > nscan ? v : nscan;
+    pp.scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nscan));
+    pp.cnt.get(sizeof(int64_t) * pp.ncnt);
+    // creation run of the pair count: the pair capacity
+    launch_pairs<Rec2>((const int32_t *)pp.l1.p, (const Rec2 *)pp.rec.p, pp.d_ch, pp.nchunks,
+                       pp.d_tm, pp.max_tiles, (int64_t *)pp.cnt.p, pp.ncnt,
+                       (int64_t *)pp.scan_tmp.p, nullptr, false, s, (const int64_t *)pp.bounds.p + 4);
+    int64_t npairs = 0;
+    S2G_HIP(hipMemcpyAsync(&npairs, (int64_t *)pp.cnt.p + pp.ncnt - 1, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s));
+    S2G_HIP(hipStreamSynchronize(s));
+    pp.pair_cap = (int64_t)std::ceil((double)npairs * headroom) + 1024;
+    if (pp.pair_cap >= (int64_t)INT32_MAX)
+        throw Error(S2G_EUNSUPPORTED, "more than 2^31 (tile, particle) pairs");
+    pp.pairs.get(sizeof(int32_t) * pp.pair_cap);
+    pp.gmin = c->tune.gather_item_min;
+    pp.gtarget = (int64_t)c->ncu * c->tune.gather_items_per_cu;
+    // items <= sum over tiles of (count / G + 1) <= target + ntiles
+    pp.item_cap = pp.ntiles + pp.gtarget + 1;
+    pp.nitems.get(sizeof(int64_t) * (pp.ntiles + 1));
+    pp.items.get(sizeof(int4) * pp.item_cap);
+    pp.tile_items.get(sizeof(int64_t) * 2 * pp.ntiles);
+    pp.gstep = c->tune.gather_planes;
+    pp.sstep = c->tune.small_planes;
+    const int gnp = pp.nplanes < pp.gstep ? pp.nplanes : pp.gstep;
+    pp.partial.get(sizeof(double) * pp.item_cap * gnp * kGT * kGT);
+    pp.smin = c->tune.small_item_min;
+    pp.starget = (int64_t)c->ncu * c->tune.small_items_per_cu;
+    pp.sitem_cap = pp.nstiles + pp.starget + 1;
+    if (pp.sitem_cap > ((int64_t)1 << 31) / 1024)
+        throw Error(S2G_EUNSUPPORTED, "too many small-footprint work items");
+    pp.soff.get(sizeof(int64_t) * (pp.nstiles + 1));
+    pp.scur.get(sizeof(int64_t) * (pp.nstiles > 0 ? pp.nstiles : 1));
+    pp.sorted.get(sizeof(int32_t) * n);
+    pp.snitems.get(sizeof(int64_t) * (pp.nstiles + 1));
+    pp.sitems.get(sizeof(int4) * pp.sitem_cap);
+    pp.stile.get(sizeof(int64_t) * 2 * (pp.nstiles > 0 ? pp.nstiles : 1));
+}
+
+void plan_build(s2g_ctx *c, s2g_plan *pl, int dtype, const s2g_map_desc *maps, int nmaps,
+                double headroom)
+{
+    hipStream_t s = c->stream;
+    for (const std::vector<int> &grp : batch_groups(maps, nmaps)) {
+        pl->passes.emplace_back();
+        PlanPass &pp = pl->passes.back();
+        const int64_t flat = batch_flat_elems(maps, grp);
+        double *img = flat ? (double *)pp.img.get(sizeof(double) * flat) : nullptr;
+        pp.mi = batch_inputs(dtype, maps, grp, img);
+        pp.nmaps = (int)pp.mi.size();
+        pp.dtype = dtype;
+        pp.kernel = maps[grp[0]].kernel;
+        pp.nq = maps[grp[0]].nq;
+        pp.nplanes = pp.nq + 1;
+        pp.fp32acc = (maps[grp[0]].flags & S2G_FLAG_FP32_ACCUM) ? 1 : 0;
+        plan_pass_build(c, pp, headroom, s);
+    }
+    S2G_HIP(hipStreamSynchronize(s));
+    S2G_HIP(hipHostMalloc((void **)&pl->h_read, sizeof(int64_t) * 8 * pl->passes.size(),
+                          hipHostMallocDefault));
+    // the replayed sequence, captured on a stream of its own
+    S2G_HIP(hipStreamCreateWithFlags(&pl->cs, hipStreamNonBlocking));
+    S2G_HIP(hipStreamBeginCapture(pl->cs, hipStreamCaptureModeThreadLocal));
+    try {
+        for (PlanPass &pp : pl->passes)
+            plan_enqueue(c, pp, pl->cs);
+        S2G_HIP(hipGetLastError());
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(pl->cs, &g);
+        if (g)
+            (void)hipGraphDestroy(g);
+        throw;
+    }
+    S2G_HIP(hipStreamEndCapture(pl->cs, &pl->graph));
+    S2G_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+}
+
+int plan_run(s2g_plan *pl, hipStream_t s, s2g_stats *st)
+{
+    s2g_ctx *c = pl->c;
+    hipEvent_t *E = c->stage;
+    if (st)
+        std::memset(st, 0, sizeof(*st));
+    S2G_HIP(hipEventRecord(E[0], s));
+    S2G_HIP(hipGraphLaunch(pl->exec, s));
+    S2G_HIP(hipEventRecord(E[1], s));
+    // per pass: the capacity flag, then the class bounds and the pair total
+    for (size_t k = 0; k < pl->passes.size(); ++k) {
+        const PlanPass &pp = pl->passes[k];
+        int64_t *h = pl->h_read + 8 * k;
+        S2G_HIP(hipMemcpyAsync(h, pp.flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        S2G_HIP(hipMemcpyAsync(h + 1, pp.bounds.p, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, s));
+        S2G_HIP(hipMemcpyAsync(h + 5, (const int64_t *)pp.cnt.p + pp.ncnt - 1, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+    }
+    S2G_HIP(hipStreamSynchronize(s));
+    std::string over;
+    for (size_t k = 0; k < pl->passes.size(); ++k) {
+        const PlanPass &pp = pl->passes[k];
+        const int64_t *h = pl->h_read + 8 * k;
+        const int flag = *(const int *)h;
+        if (st) {
+            st->n_particles += pp.ntot;
+            st->n_small += h[2] - h[1];
+            st->n_large += h[3] - h[2];
+            st->n_large_exact += h[4] - h[3];
+            st->n_pairs += h[5];
+        }
+        if (flag)
+            over += " pass " + std::to_string(k) + ":" +
+                    ((flag & 1) ? " large particles past the chunk capacity" : "") +
+                    ((flag & 2) ? " " + std::to_string(h[5]) + " pairs for a capacity of " +
+                                      std::to_string(pp.pair_cap)
+                                : "");
+    }
+    if (st) {
+        st->ms_device = elapsed(E[0], E[1]);
+        st->n_skipped = st->n_particles - st->n_small - st->n_large;
+    }
+    if (!over.empty())
+        throw Error(S2G_ECAPACITY, "the inputs outgrew the plan's capacities (outputs "
+                                   "incomplete; create a new plan):" + over);
+    return S2G_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1562,66 +1984,17 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
                       s2g_stats *stats)
 {
     return guarded(c, [&]() {
-        if (nmaps < 0 || (nmaps > 0 && !maps))
-            throw Error(S2G_EINVAL, "maps is NULL or nmaps < 0");
-        if (dtype != S2G_F32 && dtype != S2G_F64)
-            throw Error(S2G_EINVAL, "dtype must be S2G_F32 or S2G_F64");
-        for (int k = 0; k < nmaps; ++k) {
-            const s2g_map_desc &d = maps[k];
-            const std::string at = "map " + std::to_string(k) + ": ";
-            if (d.dims != 2)
-                throw Error(S2G_EUNSUPPORTED, at + "batched deposits are 2D (dims == 2)");
-            try {
-                check_common(dtype, d.kernel, 2, d.n, &d.params);
-                check_dims_nq(2, d.nq);
-            } catch (const Error &e) {
-                throw Error(e.code, at + e.msg);
-            }
-            if (!d.d_out)
-                throw Error(S2G_EINVAL, at + "output pointer is NULL");
-            if (d.n > 0 && (!d.d_pos || !d.d_hsml || !d.d_m || !d.d_rho || !d.d_bin_q ||
-                            !d.d_weights))
-                throw Error(S2G_EINVAL, at + "input array pointer is NULL");
-        }
+        check_batch(dtype, maps, nmaps);
         hipStream_t s = stream ? (hipStream_t)stream : c->stream;
         s2g_stats total;
         std::memset(&total, 0, sizeof(total));
-        // one pass per (kernel, nq, accumulator) group, maps in their given order
-        std::vector<char> done(nmaps, 0);
-        for (int k = 0; k < nmaps; ++k) {
-            if (done[k])
-                continue;
-            std::vector<int> grp;
-            for (int j = k; j < nmaps; ++j)
-                if (!done[j] && maps[j].kernel == maps[k].kernel && maps[j].nq == maps[k].nq &&
-                    (maps[j].flags & S2G_FLAG_FP32_ACCUM) ==
-                        (maps[k].flags & S2G_FLAG_FP32_ACCUM)) {
-                    grp.push_back(j);
-                    done[j] = 1;
-                }
+        for (const std::vector<int> &grp : batch_groups(maps, nmaps)) {
             // reduced maps deposit into context-owned flat images first
-            int64_t flat = 0;
-            for (int j : grp)
-                if (maps[j].flags & S2G_FLAG_REDUCE)
-                    flat += maps[j].params.npixels * maps[j].params.npixels * (maps[j].nq + 1);
-            double *img = flat ? (double *)c->image.get(sizeof(double) * flat) : nullptr;
-            std::vector<MapIn> in;
-            int64_t off = 0;
-            for (int j : grp) {
-                const s2g_map_desc &d = maps[j];
-                MapIn m;
-                m.a = make_args(dtype, d.d_pos, d.d_hsml, d.d_m, d.d_rho, d.d_bin_q, d.nq,
-                                d.d_weights, d.n, &d.params, d.kernel, d.flags);
-                if (d.flags & S2G_FLAG_REDUCE) {
-                    m.image = img + off;
-                    m.out = d.d_out;
-                    off += d.params.npixels * d.params.npixels * (d.nq + 1);
-                } else {
-                    m.image = d.d_out;
-                    m.out = nullptr;
-                }
-                in.push_back(m);
-            }
+            double *img = nullptr;
+            const int64_t flat = batch_flat_elems(maps, grp);
+            if (flat)
+                img = (double *)c->image.get(sizeof(double) * flat);
+            std::vector<MapIn> in = batch_inputs(dtype, maps, grp, img);
             s2g_stats st;
             deposit2d_pass(c, in.data(), (int)in.size(), s, &st);
             total.ms_device += st.ms_device;
@@ -1643,6 +2016,49 @@ int s2g_deposit_batch(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps
             *stats = total;
         return S2G_OK;
     });
+}
+
+int s2g_plan_create(s2g_ctx *c, int dtype, const s2g_map_desc *maps, int nmaps, double headroom,
+                    s2g_plan **out)
+{
+    return guarded(c, [&]() {
+        if (!out)
+            throw Error(S2G_EINVAL, "out is NULL");
+        *out = nullptr;
+        check_batch(dtype, maps, nmaps);
+        if (nmaps < 1)
+            throw Error(S2G_EINVAL, "a plan needs at least one map");
+        if (!(headroom >= 1.0 && headroom <= 64.0))
+            throw Error(S2G_EINVAL, "headroom out of range [1, 64]");
+        std::unique_ptr<s2g_plan> pl(new s2g_plan());
+        pl->c = c;
+        try {
+            plan_build(c, pl.get(), dtype, maps, nmaps, headroom);
+        } catch (...) {
+            plan_release(pl.get());
+            throw;
+        }
+        *out = pl.release();
+        return S2G_OK;
+    });
+}
+
+int s2g_plan_run(s2g_plan *pl, void *stream, s2g_stats *stats)
+{
+    if (!pl)
+        return S2G_EINVAL;
+    return guarded(pl->c, [&]() { return plan_run(pl, stream ? (hipStream_t)stream : pl->c->stream, stats); });
+}
+
+void s2g_plan_destroy(s2g_plan *pl)
+{
+    if (!pl)
+        return;
+    int dev = -1;
+    if (hipGetDevice(&dev) == hipSuccess && dev != pl->c->device)
+        (void)hipSetDevice(pl->c->device);
+    plan_release(pl);
+    delete pl;
 }
 
 int s2g_footprints(s2g_ctx *c, int dims, int dtype, const void *pos, const void *hsml,

@@ -8,7 +8,7 @@ types.  All deposits run as HIP kernels through libsph2grid's C ABI
 (include/sph2grid.h); there is no CPU fallback.
 """
 from ._lib import Context, LibraryError, default_context, version  # noqa: F401
-from .batch import deposit_batch_device, map_batch  # noqa: F401
+from .batch import BatchPlan, deposit_batch_device, map_batch  # noqa: F401
 from .density import mass_density  # noqa: F401
 from .distributed import (AllskySubfile, Subfile, distributed_allsky_map,  # noqa: F401
                           distributed_cic_map)
@@ -34,7 +34,7 @@ __all__ = [
     "WendlandC4", "WendlandC6", "WendlandC8", "Context", "LibraryError",
     "default_context", "version", "healpix_map", "healpix_maps", "filter_sort_particles",
     "BoundsError",
-    "tsc_mapping", "map_batch", "deposit_batch_device", "distributed_cic_map",
+    "tsc_mapping", "map_batch", "deposit_batch_device", "BatchPlan", "distributed_cic_map",
     "distributed_allsky_map", "Subfile", "AllskySubfile", "write_fits_image",
     "read_fits_image", "read_allsky_fits_image", "save_healpix_fits", "read_healpix_fits",
     "write_vtk_image", "read_vtk_image", "get_map_grid_3D", "mass_density",

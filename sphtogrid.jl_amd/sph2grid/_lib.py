@@ -19,6 +19,7 @@ S2G_EINVAL = -1
 S2G_EHIP = -2
 S2G_ENOMEM = -3
 S2G_EUNSUPPORTED = -4
+S2G_ECAPACITY = -5
 
 S2G_F32 = 0
 S2G_F64 = 1
@@ -38,6 +39,7 @@ EXPORTS = (
     "s2g_healpix_reduce_device", "s2g_mass_density", "s2g_mass_density_device",
     "s2g_footprints", "s2g_tsc3d_device_noalloc", "s2g_healpix_deposit_maps",
     "s2g_healpix_deposit_maps_device", "s2g_set_tuning", "s2g_get_tuning",
+    "s2g_plan_create", "s2g_plan_run", "s2g_plan_destroy",
 )
 
 
@@ -187,6 +189,13 @@ def load(path: str | None = None):
         L.s2g_footprints.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, i64,
                                      C.POINTER(Params), C.c_int, C.c_uint32, vp, vp, vp, vp, vp,
                                      i64, vp, vp, vp]
+        L.s2g_plan_create.restype = C.c_int
+        L.s2g_plan_create.argtypes = [vp, C.c_int, C.POINTER(MapDesc), C.c_int, C.c_double,
+                                      C.POINTER(vp)]
+        L.s2g_plan_run.restype = C.c_int
+        L.s2g_plan_run.argtypes = [vp, vp, C.POINTER(Stats)]
+        L.s2g_plan_destroy.restype = None
+        L.s2g_plan_destroy.argtypes = [vp]
         L.s2g_set_tuning.restype = C.c_int
         L.s2g_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.s2g_get_tuning.restype = C.c_int

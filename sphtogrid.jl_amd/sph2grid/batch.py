@@ -45,6 +45,21 @@ def deposit_batch_device(maps, *, dtype: str = "f32", context=None, stream=None,
     ``reduce_image``, ``exact_norm``, ``fp32_accum``.  Arrays are torch CUDA tensors or device
     addresses.  Particles must already be centred and filtered."""
     ctx = context or _lib.default_context()
+    descs = _descs(maps)
+    st = _lib.Stats()
+    rc = ctx.lib.s2g_deposit_batch(ctx.handle, _dtype_id(dtype), descs, len(maps),
+                                   C.c_void_p(stream) if stream else None, C.byref(st))
+    ctx.check(rc, "s2g_deposit_batch")
+    if stats is not None:
+        stats.update(st.as_dict())
+
+
+def _dtype_id(dtype) -> int:
+    return _lib.S2G_F32 if dtype in ("f32", "float32", np.float32) else _lib.S2G_F64
+
+
+def _descs(maps):
+    """s2g_map_desc array of deposit_batch_device's map dicts."""
     descs = (_lib.MapDesc * max(len(maps), 1))()
     for k, d in enumerate(maps):
         kid = d["kernel"] if isinstance(d["kernel"], int) else kernel_args(d["kernel"])[0]
@@ -59,13 +74,53 @@ def deposit_batch_device(maps, *, dtype: str = "f32", context=None, stream=None,
             d_pos=_addr(d["pos"]), d_hsml=_addr(d["hsml"]), d_m=_addr(d["m"]),
             d_rho=_addr(d["rho"]), d_bin_q=_addr(d["bin_q"]), d_weights=_addr(d["weights"]),
             d_out=_addr(d["out"]))
-    st = _lib.Stats()
-    dt = _lib.S2G_F32 if dtype in ("f32", "float32", np.float32) else _lib.S2G_F64
-    rc = ctx.lib.s2g_deposit_batch(ctx.handle, dt, descs, len(maps),
-                                   C.c_void_p(stream) if stream else None, C.byref(st))
-    ctx.check(rc, "s2g_deposit_batch")
-    if stats is not None:
-        stats.update(st.as_dict())
+    return descs
+
+
+class BatchPlan:
+    """A batch captured once in a hipGraph and replayed (``s2g_plan_*``).
+
+    ``maps`` as for ``deposit_batch_device``; the device arrays must stay
+    allocated (and at the same addresses) for the plan's lifetime, their
+    contents may change between runs.  Capacities are the creation run's
+    counts times ``headroom``; ``run`` raises ``LibraryError`` with code
+    ``S2G_ECAPACITY`` when the inputs outgrow them (the outputs are then
+    incomplete; build a new plan)."""
+
+    def __init__(self, maps, *, dtype: str = "f32", headroom: float = 1.25, context=None):
+        self.ctx = context or _lib.default_context()
+        self._maps = list(maps)  # keeps the tensors alive
+        self._descs = _descs(self._maps)
+        h = C.c_void_p()
+        rc = self.ctx.lib.s2g_plan_create(self.ctx.handle, _dtype_id(dtype), self._descs,
+                                          len(self._maps), float(headroom), C.byref(h))
+        self.ctx.check(rc, "s2g_plan_create")
+        self.handle = h
+
+    def run(self, stream=None, stats: dict | None = None):
+        st = _lib.Stats()
+        rc = self.ctx.lib.s2g_plan_run(self.handle, C.c_void_p(stream) if stream else None,
+                                       C.byref(st))
+        self.ctx.check(rc, "s2g_plan_run")
+        if stats is not None:
+            stats.update(st.as_dict())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.s2g_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def map_batch(calls, *, context=None, stats: dict | None = None):

@@ -1,0 +1,116 @@
+"""Replayable batch (s2g_plan_*): the whole batch captured once in a hipGraph
+and replayed.  Each replay is checked against the eager s2g_deposit_batch on
+the same inputs (the large-footprint part is the same computation; the
+small-footprint additions may come in another order, hence 1e-12), with
+eager work and synchronisations between replays, with the inputs changed in
+place between replays, and against the FP64 oracle on small maps.  The
+capacity path: inputs that outgrow the plan give S2G_ECAPACITY, not a fault."""
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_maps(s2g, nmaps=6, scale=1.0 / 400, seed_scale=1.0):
+    import torch
+    from sph2grid import workloads
+    from sph2grid.mapping import _prepare
+    dev = torch.device("cuda:0")
+    out = []
+    for d in workloads.batched(nmaps=nmaps, scale=scale):
+        par = s2g.mappingParameters(x_lim=d["lims"][0], y_lim=d["lims"][1], z_lim=d["lims"][2],
+                                    Npixels=d["Npixels"])
+        pm = _prepare(d["Pos"].copy(), d["HSML"], d["M"], d["Rho"], d["Bin_Quant"],
+                      d["Weights"], param=par, kernel=s2g.WendlandC6(2), show_progress=False,
+                      reduce_image=True, calc_mean=True)
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in pm.arrays]
+        out.append(dict(t=t, pm=pm, npix=pm.npix, nq=pm.nq, host=[a.copy() for a in pm.arrays]))
+    return out
+
+
+def _descs(maps, key):
+    import torch
+    descs = []
+    for m in maps:
+        if key not in m:
+            m[key] = torch.empty(m["npix"] ** 2 * m["nq"], dtype=torch.float64,
+                                 device=m["t"][0].device)
+        x, h, mm, rho, bq, w = m["t"]
+        descs.append(dict(pos=x, hsml=h, m=mm, rho=rho, bin_q=bq, weights=w, out=m[key],
+                          n=m["pm"].n, nq=m["nq"], npix=m["npix"],
+                          len2pix=m["pm"].params.len2pix, kernel=m["pm"].kid,
+                          calc_mean=True, reduce_image=True))
+    return descs
+
+
+def _max_rel(a, b):
+    return float((a - b).abs().max().item()) / max(float(b.abs().max().item()), 1e-300)
+
+
+def test_plan_replays_match_the_eager_batch(s2g, orc):
+    import torch
+    maps = _device_maps(s2g)
+    eager, plan_d = _descs(maps, "o_eager"), _descs(maps, "o_plan")
+    st = {}
+    with s2g.BatchPlan(plan_d, dtype="f32") as plan:
+        big = torch.randn(2048, 2048, device="cuda:0")
+        for rep in range(4):
+            if rep == 2:  # inputs changed in place: quantities doubled, hsml 10 % smaller
+                for m in maps:
+                    m["t"][4].mul_(2.0)
+                    m["t"][1].mul_(0.9)
+            s2g.deposit_batch_device(eager, dtype="f32")
+            for _ in range(3):
+                big = (big @ big).mul_(1e-3)  # eager work between replays
+            torch.cuda.synchronize()
+            plan.run(stats=st)
+            torch.cuda.synchronize()
+            errs = [_max_rel(m["o_plan"], m["o_eager"]) for m in maps]
+            print(f"replay {rep}: max rel diff vs eager per map {['%.1e' % e for e in errs]}",
+                  {k: st[k] for k in ("n_small", "n_large", "n_pairs")}, flush=True)
+            assert max(errs) <= 1e-12, (rep, errs)
+            assert st["n_small"] > 0 and st["n_large"] > 0 and st["n_pairs"] > 0
+    # the two smallest maps of the last replay against the oracle
+    for m in sorted(maps, key=lambda m: m["pm"].n)[:2]:
+        x, h, mm, rho, bq, w = [a.double().cpu().numpy() for a in m["t"]]
+        ref = orc.reduce_image_2D(orc.cic_mapping_2D(x.reshape(-1, 3), h, mm, rho,
+                                                     bq.reshape(-1, m["nq"]), w, m["npix"],
+                                                     m["pm"].params.len2pix, "wendlandc6", 2,
+                                                     True), m["npix"], True)
+        got = m["o_plan"].cpu().numpy().reshape(ref.shape, order="F")
+        e = rel_l1(got, ref)
+        print(f"map n={m['pm'].n} npix={m['npix']}: rel_l1 vs oracle {e:.2e}", flush=True)
+        assert e <= 1e-5
+
+
+def test_plan_capacity_error_and_rebuild(s2g):
+    from sph2grid import _lib
+    maps = _device_maps(s2g, nmaps=3, scale=1.0 / 800)
+    plan = s2g.BatchPlan(_descs(maps, "o_plan"), dtype="f32", headroom=1.0)
+    plan.run()
+    for m in maps:
+        m["t"][1].mul_(3.0)  # every footprint grows: more large particles and pairs
+    with pytest.raises(_lib.LibraryError) as e:
+        plan.run()
+    print(e.value, flush=True)
+    assert e.value.code == _lib.S2G_ECAPACITY and "capacit" in str(e.value)
+    plan.close()
+    # a new plan on the grown inputs matches the eager batch
+    eager = _descs(maps, "o_eager")
+    s2g.deposit_batch_device(eager, dtype="f32")
+    with s2g.BatchPlan(_descs(maps, "o_plan"), dtype="f32") as plan:
+        plan.run()
+    for m in maps:
+        assert _max_rel(m["o_plan"], m["o_eager"]) <= 1e-12
+
+
+def test_plan_argument_errors(s2g):
+    from sph2grid import _lib
+    maps = _device_maps(s2g, nmaps=1, scale=1.0 / 800)
+    for bad in (0.5, 100.0):
+        with pytest.raises(_lib.LibraryError, match="headroom"):
+            s2g.BatchPlan(_descs(maps, "o"), headroom=bad)
+    with pytest.raises(_lib.LibraryError, match="at least one map"):
+        s2g.BatchPlan([])
