@@ -264,19 +264,21 @@ def plain_extra(args, ctx, torch, s2g, _lib, st, stream):
                             d_hsml=arr[1].data_ptr(), d_m=arr[2].data_ptr(),
                             d_rho=arr[3].data_ptr(), d_bin_q=arr[5].data_ptr(),
                             d_weights=arr[4].data_ptr(), d_out=mp_plan.data_ptr())
-    ph = C.c_void_p()
-    ctx.check(ctx.lib.s2g_plan_create(ctx.handle, _lib.S2G_F64, pdesc, 1, 1.25, C.byref(ph)),
-              "s2g_plan_create")
-    try:
-        def stepplan():
-            ctx.check(ctx.lib.s2g_plan_run(ph, stream, None), "s2g_plan_run")
-        msplan = _time_steps(stepplan, torch, steps=50, warmup=3)
-        rec["plan"] = {"ms_per_step": msplan,
-                       "particles_per_s": pp["x"].shape[0] / (msplan / 1e3),
-                       "max_rel_diff_vs_eager": float((mp_plan - ref_map).abs().max().item() /
-                                                      ref_map.abs().max().item())}
-    finally:
-        ctx.lib.s2g_plan_destroy(ph)
+
+    def time_plan():
+        ph = C.c_void_p()
+        ctx.check(ctx.lib.s2g_plan_create(ctx.handle, _lib.S2G_F64, pdesc, 1, 1.25,
+                                          C.byref(ph)), "s2g_plan_create")
+        try:
+            def stepplan():
+                ctx.check(ctx.lib.s2g_plan_run(ph, stream, None), "s2g_plan_run")
+            msplan = _time_steps(stepplan, torch, steps=50, warmup=3)
+        finally:
+            ctx.lib.s2g_plan_destroy(ph)
+        return {"ms_per_step": msplan, "particles_per_s": pp["x"].shape[0] / (msplan / 1e3),
+                "max_rel_diff_vs_eager": float((mp_plan - ref_map).abs().max().item() /
+                                               ref_map.abs().max().item())}
+    rec["plan"] = time_plan()
     if not args.no_tune:
         # work split tuned for this call (sph2grid.autotune; tuning not timed)
         from sph2grid import autotune
@@ -288,8 +290,9 @@ def plain_extra(args, ctx, torch, s2g, _lib, st, stream):
         tp, _, _ = autotune.tune(run, ctx, nplanes=2, reps=5)
         tune_s = time.perf_counter() - t0
         mst = _time_steps(stepp, torch, steps=20, warmup=2)
+        plan_tuned = time_plan()  # a plan takes the context's tuning at creation
         ctx.set_tuning(None)
-        rec["tuned"] = {"tuning": tp, "ms_per_step": mst,
+        rec["tuned"] = {"tuning": tp, "ms_per_step": mst, "plan": plan_tuned,
                         "particles_per_s": pp["x"].shape[0] / (mst / 1e3),
                         "max_rel_diff_vs_default": float((mp - ref_map).abs().max().item() /
                                                          ref_map.abs().max().item()),
@@ -922,6 +925,27 @@ def main():
     # full-size property checks: conservation of the weight plane
     img = d_img.view(nq + 1, nd)
     w_sum = float(img[nq].sum().item())
+    # the same step with S2G_FLAG_FP32_ACCUM (opt-in FP32 per-item sums in the
+    # gather), reported beside the FP64 default; not the headline value
+    fp32 = None
+    if not args.no_extras:
+        map64 = d_map.clone()
+        flags_keep = flags
+        flags |= _lib.S2G_FLAG_FP32_ACCUM
+        step()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        ms32 = (time.perf_counter() - t3) * 1e3 / 3
+        fp32 = {"ms_per_step": ms32, "particles_per_s": n / (ms32 / 1e3),
+                "gather_ms": st.ms_gather,
+                "rel_l1_vs_fp64": float((d_map - map64).abs().sum().item() /
+                                        map64.abs().sum().item())}
+        flags = flags_keep
+        step()
+        torch.cuda.synchronize()
     x64 = {k: p[k].astype(np.float64) for k in ("hsml", "m", "rho", "w")}
     wi, wj = footprints(p["x"], p["hsml"], npix, p["len2pix"])
     dep = (wi > 0) & (wj > 0)
@@ -992,6 +1016,7 @@ def main():
         "conservation_rel_err": cons,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32_accum": fp32,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]

@@ -28,7 +28,6 @@ struct Map2D {
 // small-footprint path: particles are bucketed by the kST x kST tile holding
 // their footprint's first pixel; a work item accumulates its particles in a
 // kSW x kSW LDS window (the tile plus the largest footprint) per plane
-constexpr int kST = 32;
 constexpr int kSW = kST + kSmallSide2D;
 
 // host-side arguments of one map (Map2D without the pass offsets)
@@ -75,14 +74,6 @@ void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *
 void launch_2d_small_tile(int dtype, int kernel, int np, const int4 *items,
                           const int64_t *nitems, int64_t max_items, const int32_t *plist,
                           const Rec2 *rec, const Map2D *maps, int nq, int plane0, hipStream_t s);
-// small-footprint buckets on the device (no library sort, no memset): off
-// (nstiles + 1; off[nstiles] = ns) and the particles of tile t at
-// sorted[off[t] .. off[t+1]); ns = *d_ns <= ns_cap; cursor: nstiles int64
-void launch_2d_small_buckets(const int32_t *list, const int64_t *d_ns, int64_t ns_cap,
-                             const Rec2 *rec, const Map2D *maps, int nstiles, int64_t *off,
-                             int64_t *cursor, int64_t *scan_tmp, int32_t *sorted, hipStream_t s);
-// d[0 .. n) = v (a kernel, so that it can be recorded in a graph)
-void launch_fill_i64(int64_t *d, int64_t v, int64_t n, hipStream_t s);
 // reduce_image_2D of every map of the pass that has an output
 void launch_2d_reduce_maps(const Map2D *maps, int nmaps, int ntiles, int nq, hipStream_t s);
 // reduce_image_2D of one flat image

@@ -563,61 +563,6 @@ __global__ void k2d_small_keys(const int32_t *__restrict__ list, int64_t ns,
     vals[e] = p;
 }
 
-// Small-footprint buckets without a library sort (replayed passes, whose
-// graphs hold kernels only): a histogram of the anchor tiles (hist zeroed by
-// k_fill_i64, ntiles + 1 entries), an exclusive scan, then each particle
-// placed at its tile's cursor.  The order inside a tile depends on the
-// atomics, which changes only the order of the scatter's LDS additions (not
-// reproducible in any case).  ns = *d_ns on the device; grid-stride loops.
-__device__ __forceinline__ int64_t small_key(const Rec2 &r, const Map2D *__restrict__ maps)
-{
-    const Map2D &md = maps[r.map];
-    return md.sbase + (int64_t)(r.i0 / kST) * md.nts + r.j0 / kST;
-}
-
-__global__ __launch_bounds__(256) void k2d_small_hist(const int32_t *__restrict__ list,
-                                                      const int64_t *__restrict__ d_ns,
-                                                      const Rec2 *__restrict__ rec,
-                                                      const Map2D *__restrict__ maps,
-                                                      unsigned long long *__restrict__ hist)
-{
-    const int64_t ns = *d_ns;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns;
-         e += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&hist[small_key(rec[list[e]], maps)], 1ull);
-}
-
-__global__ __launch_bounds__(256) void k2d_small_place(const int32_t *__restrict__ list,
-                                                       const int64_t *__restrict__ d_ns,
-                                                       const Rec2 *__restrict__ rec,
-                                                       const Map2D *__restrict__ maps,
-                                                       unsigned long long *__restrict__ cursor,
-                                                       int32_t *__restrict__ sorted)
-{
-    const int64_t ns = *d_ns;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t p = list[e];
-        const unsigned long long slot = atomicAdd(&cursor[small_key(rec[p], maps)], 1ull);
-        sorted[slot] = p;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_fill_i64(int64_t *__restrict__ d, int64_t v, int64_t n)
-{
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x)
-        d[e] = v;
-}
-
-__global__ __launch_bounds__(256) void k_copy_i64(const int64_t *__restrict__ src,
-                                                  int64_t *__restrict__ dst, int64_t n)
-{
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-         e += (int64_t)gridDim.x * blockDim.x)
-        dst[e] = src[e];
-}
-
 __global__ void k_tile_offsets(const uint32_t *__restrict__ keys, int64_t ns, int ntiles,
                                int64_t *__restrict__ off)
 {
@@ -985,32 +930,6 @@ void launch_2d_small_keys(const int32_t *list, int64_t ns, const Rec2 *rec, cons
         return;
     hipLaunchKernelGGL(k2d_small_keys, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, list,
                        ns, rec, maps, keys, vals);
-}
-
-static unsigned stride_blocks(int64_t n)
-{
-    int64_t b = (n + 255) / 256;
-    return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
-}
-
-void launch_fill_i64(int64_t *d, int64_t v, int64_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fill_i64, dim3(stride_blocks(n)), dim3(256), 0, s, d, v, n);
-}
-
-void launch_2d_small_buckets(const int32_t *list, const int64_t *d_ns, int64_t ns_cap,
-                             const Rec2 *rec, const Map2D *maps, int nstiles, int64_t *off,
-                             int64_t *cursor, int64_t *scan_tmp, int32_t *sorted, hipStream_t s)
-{
-    launch_fill_i64(off, 0, nstiles + 1, s);
-    const unsigned nb = stride_blocks(ns_cap);
-    hipLaunchKernelGGL(k2d_small_hist, dim3(nb), dim3(256), 0, s, list, d_ns, rec, maps,
-                       (unsigned long long *)off);
-    exclusive_scan_i64(off, nstiles + 1, scan_tmp, s);
-    hipLaunchKernelGGL(k_copy_i64, dim3(stride_blocks(nstiles)), dim3(256), 0, s, off, cursor,
-                       (int64_t)nstiles);
-    hipLaunchKernelGGL(k2d_small_place, dim3(nb), dim3(256), 0, s, list, d_ns, rec, maps,
-                       (unsigned long long *)cursor, sorted);
 }
 
 void launch_tile_offsets(const uint32_t *keys, int64_t ns, int ntiles, int64_t *off,

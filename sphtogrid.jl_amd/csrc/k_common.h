@@ -23,9 +23,11 @@ void launch_partition(const int32_t *cls, int64_t n, int64_t *counts, int64_t *s
 // class offsets counts[0], counts[nb], counts[2nb], counts[3nb]; out[4 + m] =
 // first entry of map m in the large list (particles of map m are
 // [pbase[m], pbase[m+1]), and the partition is stable), out[4 + nmaps] = size
-// of the large list.
-void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *large,
-                         const int64_t *pbase, int nmaps, int64_t *out, hipStream_t s);
+// of the large list; out[5 + nmaps + m]: the same for the small list, and
+// out[5 + 2 nmaps] its size (out: 2 nmaps + 6 entries).
+void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *small,
+                         const int32_t *large, const int64_t *pbase, int nmaps, int64_t *out,
+                         hipStream_t s);
 
 // Tile grid of one map in the (tile, particle) counting sort.  Local tile id
 // t = (a * nt + b) * ntc + c; global tile id tbase + t.  The map's count block
@@ -52,6 +54,13 @@ void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, i
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
                   int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s,
                   const int64_t *lbase = nullptr, int64_t cap = INT64_MAX);
+// small-footprint buckets as per-tile lists (fill == false: counts per
+// (tile, chunk) + scan into cnt; fill == true: out = the particles, tile t of
+// map m at cnt[cbase + t nch] ..), chunk ranges from lbase as for the pairs
+void launch_anchor_lists(const int32_t *list, const Rec2 *rec, const PairChunk *chunks,
+                         int nchunks, const TileMap *tmaps, int max_tiles, int64_t *cnt,
+                         int64_t ncnt, int64_t *scan_tmp, int32_t *out, bool fill,
+                         const int64_t *lbase, hipStream_t s);
 // flag = (a map's large list outgrew nch * csz entries) | 2 * (pair total >
 // pair_cap), for a replayed pass sized at creation
 void launch_plan_check(const int64_t *lbase, int nmaps, const TileMap *tmaps,

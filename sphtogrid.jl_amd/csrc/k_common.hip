@@ -93,42 +93,52 @@ void launch_partition(const int32_t *cls, int64_t n, int64_t *counts, int64_t *s
                        l1, l2);
 }
 
-// class offsets and the per-map ranges of the large list
+// first entry of a particle-order list (len entries) whose particle index is
+// >= key
+__device__ __forceinline__ int64_t list_lower_bound(const int32_t *__restrict__ list, int64_t len,
+                                                    int64_t key)
+{
+    int64_t lo = 0, hi = len;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)list[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// class offsets and the per-map ranges of the large and small lists
 __global__ void k_class_bounds(const int64_t *__restrict__ counts, int64_t nb,
+                               const int32_t *__restrict__ small,
                                const int32_t *__restrict__ large,
                                const int64_t *__restrict__ pbase, int nmaps,
                                int64_t *__restrict__ out)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t l_lo = counts[nb], l_hi = counts[2 * nb];
-    const int64_t nl = l_hi - l_lo;
+    const int64_t ns = counts[nb] - counts[0];
+    const int64_t nl = counts[2 * nb] - counts[nb];
     if (m < 4)
         out[m] = counts[(int64_t)m * nb];
     if (m > nmaps)
         return;
     if (m == nmaps) {
         out[4 + m] = nl;
+        out[5 + nmaps + m] = ns;
         return;
     }
-    // first large entry whose particle index is >= pbase[m]
-    const int64_t key = pbase[m];
-    int64_t lo = 0, hi = nl;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)large[mid] < key)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    out[4 + m] = lo;
+    out[4 + m] = list_lower_bound(large, nl, pbase[m]);
+    out[5 + nmaps + m] = list_lower_bound(small, ns, pbase[m]);
 }
 
-void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *large,
-                         const int64_t *pbase, int nmaps, int64_t *out, hipStream_t s)
+void launch_class_bounds(const int64_t *counts, int64_t nb, const int32_t *small,
+                         const int32_t *large, const int64_t *pbase, int nmaps, int64_t *out,
+                         hipStream_t s)
 {
     const int nt = nmaps + 1 > 4 ? nmaps + 1 : 4;
     hipLaunchKernelGGL(k_class_bounds, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
-                       counts, nb, large, pbase, nmaps, out);
+                       counts, nb, small, large, pbase, nmaps, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,7 +190,30 @@ __device__ __forceinline__ void chunk_range(const PairChunk &ch, const TileMap &
         l1 = l0;
 }
 
-template <typename R>
+// the small-footprint bucket of a particle: the kST x kST tile holding the
+// first pixel of its footprint (one tile, in the map's small-tile grid)
+struct AnchorTile {
+    const Rec2 &r;
+};
+__device__ __forceinline__ void tile_rect(const AnchorTile &t, int &a0, int &a1, int &b0,
+                                          int &b1, int &c0, int &c1)
+{
+    a0 = a1 = t.r.i0 / kST;
+    b0 = b1 = t.r.j0 / kST;
+    c0 = c1 = 0;
+}
+
+template <typename R, bool Anchor>
+__device__ __forceinline__ void list_rect(const R &r, int &a0, int &a1, int &b0, int &b1, int &c0,
+                                          int &c1)
+{
+    if constexpr (Anchor)
+        tile_rect(AnchorTile{r}, a0, a1, b0, b1, c0, c1);
+    else
+        tile_rect(r, a0, a1, b0, b1, c0, c1);
+}
+
+template <typename R, bool Anchor = false>
 __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ large,
                                                     const R *__restrict__ rec,
                                                     const PairChunk *__restrict__ chunks,
@@ -199,7 +232,7 @@ __global__ __launch_bounds__(256) void k_pair_count(const int32_t *__restrict__ 
     for (int64_t l = cl0 + threadIdx.x; l < cl1; l += blockDim.x) {
         const R r = rec[large[l]];
         int a0, a1, b0, b1, c0, c1;
-        tile_rect(r, a0, a1, b0, b1, c0, c1);
+        list_rect<R, Anchor>(r, a0, a1, b0, b1, c0, c1);
         for (int a = a0; a <= a1; ++a)
             for (int b = b0; b <= b1; ++b)
                 for (int c = c0; c <= c1; ++c)
@@ -271,7 +304,7 @@ void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, i
 {
     const size_t lds = sizeof(int) * (size_t)(max_tiles > 0 ? max_tiles : 1);
     if (!fill) {
-        hipLaunchKernelGGL(k_pair_count<R>, dim3((unsigned)nchunks), dim3(256), lds, s, large,
+        hipLaunchKernelGGL((k_pair_count<R, false>), dim3((unsigned)nchunks), dim3(256), lds, s, large,
                            rec, chunks, tmaps, lbase, ncnt, cnt);
         exclusive_scan_i64(cnt, ncnt, scan_tmp, s);
     } else {
@@ -286,6 +319,51 @@ template void launch_pairs<Rec2>(const int32_t *, const Rec2 *, const PairChunk 
 template void launch_pairs<Rec3>(const int32_t *, const Rec3 *, const PairChunk *, int,
                                  const TileMap *, int, int64_t *, int64_t, int64_t *, int32_t *,
                                  bool, hipStream_t, const int64_t *, int64_t);
+
+// Small-footprint buckets as lists over the maps' small-tile grids: counts
+// per (tile, chunk) as for the pair lists (k_pair_count with the anchor
+// tile), one exclusive scan, then one block per chunk placing its particles
+// at LDS cursors.  Within a chunk the order in a tile follows the LDS
+// atomics, which only reorders the small scatter's additions.
+__global__ __launch_bounds__(256) void k_anchor_fill(const int32_t *__restrict__ list,
+                                                     const Rec2 *__restrict__ rec,
+                                                     const PairChunk *__restrict__ chunks,
+                                                     const TileMap *__restrict__ tmaps,
+                                                     const int64_t *__restrict__ lbase,
+                                                     const int64_t *__restrict__ off,
+                                                     int32_t *__restrict__ out)
+{
+    extern __shared__ int cursor[];
+    const PairChunk ch = chunks[blockIdx.x];
+    const TileMap tm = tmaps[ch.map];
+    int64_t cl0, cl1;
+    chunk_range(ch, tm, lbase, cl0, cl1);
+    for (int t = threadIdx.x; t < tm.ntiles; t += blockDim.x)
+        cursor[t] = (int)off[tm.cbase + (int64_t)t * tm.nch + ch.k];
+    __syncthreads();
+    for (int64_t l = cl0 + threadIdx.x; l < cl1; l += blockDim.x) {
+        const int32_t p = list[l];
+        const Rec2 r = rec[p];
+        const int slot = atomicAdd(&cursor[(r.i0 / kST) * tm.nt + r.j0 / kST], 1);
+        out[slot] = p;
+    }
+}
+
+void launch_anchor_lists(const int32_t *list, const Rec2 *rec, const PairChunk *chunks,
+                         int nchunks, const TileMap *tmaps, int max_tiles, int64_t *cnt,
+                         int64_t ncnt, int64_t *scan_tmp, int32_t *out, bool fill,
+                         const int64_t *lbase, hipStream_t s)
+{
+    const size_t lds = sizeof(int) * (size_t)(max_tiles > 0 ? max_tiles : 1);
+    if (!fill) {
+        hipLaunchKernelGGL((k_pair_count<Rec2, true>), dim3((unsigned)nchunks), dim3(256), lds, s,
+                           list, rec, chunks, tmaps, lbase, ncnt, cnt);
+        exclusive_scan_i64(cnt, ncnt, scan_tmp, s);
+    } else {
+        hipLaunchKernelGGL(k_anchor_fill, dim3((unsigned)nchunks), dim3(256), lds, s, list, rec,
+                           chunks, tmaps, lbase, cnt, out);
+    }
+}
 
 // Capacity flag of a replayed pass (written whole, one block): bit 0, a
 // map's large list is longer than its chunks cover (nch * csz entries; the

@@ -278,8 +278,8 @@ ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, const in
     cl.l2 = (int32_t *)c->lists[2].get(sizeof(int32_t) * n);
     int64_t *tmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(3 * nb + 1));
     launch_partition(cls, n, counts, tmp, cl.l0, cl.l1, cl.l2, s);
-    int64_t *bounds = (int64_t *)c->bounds.get(sizeof(int64_t) * (nmaps + 5));
-    launch_class_bounds(counts, nb, cl.l1, d_pbase, nmaps, bounds, s);
+    int64_t *bounds = (int64_t *)c->bounds.get(sizeof(int64_t) * (2 * nmaps + 6));
+    launch_class_bounds(counts, nb, cl.l0, cl.l1, d_pbase, nmaps, bounds, s);
     S2G_HIP(hipEventRecord(c->stage[1], s));
     int64_t *h = (int64_t *)c->h_bounds.get(sizeof(int64_t) * (nmaps + 5));
     S2G_HIP(hipMemcpyAsync(h, bounds, sizeof(int64_t) * (nmaps + 5), hipMemcpyDeviceToHost, s));
@@ -946,27 +946,29 @@ std::vector<MapIn> batch_inputs(int dtype, const s2g_map_desc *maps, const std::
 //  * pairs have a capacity (the creation run's total times the headroom);
 //    k_plan_check raises a flag when either capacity is passed, the pair
 //    fill writes nothing past it and the gather does not run;
-//  * small-footprint buckets come from a histogram, a scan and an atomic
-//    placement (kernels only: no library sort, no memset node).
+//  * small-footprint buckets are per-tile lists built like the pair lists
+//    (LDS counts per chunk of each map's small list, a scan, LDS cursors):
+//    kernels only, no library sort and no memset node.
 // ---------------------------------------------------------------------------
 struct PlanPass {
     std::vector<MapIn> mi;
     int dtype = 0, kernel = 0, nq = 0, nplanes = 0, fp32acc = 0, nmaps = 0;
-    int64_t ntot = 0, nb = 0, ncnt = 0, pair_cap = 0, item_cap = 0, sitem_cap = 0;
-    int ntiles = 0, nstiles = 0, max_tiles = 1, nchunks = 0, gstep = 4, sstep = 4;
+    int64_t ntot = 0, nb = 0, ncnt = 0, sncnt = 0, pair_cap = 0, item_cap = 0, sitem_cap = 0;
+    int ntiles = 0, nstiles = 0, max_tiles = 1, smax_tiles = 1, nchunks = 0, nschunks = 0;
+    int gstep = 4, sstep = 4;
     int64_t gmin = 64, gtarget = 1, smin = 256, starget = 1;
     bool any_out = false;
     DevBuf tab, img, rec, cls, counts, scan_tmp, l0, l1, l2, bounds, tiles, cnt, pairs, nitems,
-        items, tile_items, partial, soff, scur, sorted, snitems, sitems, stile, flag;
+        items, tile_items, partial, stiles, scnt, sorted, snitems, sitems, stile, flag;
     const Map2D *d_maps = nullptr;
     const int64_t *d_pbase = nullptr;
-    const TileMap *d_stm = nullptr, *d_tm = nullptr;
-    const PairChunk *d_ch = nullptr;
+    const TileMap *d_tm = nullptr, *d_stm = nullptr;  // large / small lists' tile grids
+    const PairChunk *d_ch = nullptr, *d_sch = nullptr;
     void release()
     {
         for (DevBuf *b : {&tab, &img, &rec, &cls, &counts, &scan_tmp, &l0, &l1, &l2, &bounds,
-                          &tiles, &cnt, &pairs, &nitems, &items, &tile_items, &partial, &soff,
-                          &scur, &sorted, &snitems, &sitems, &stile, &flag})
+                          &tiles, &cnt, &pairs, &nitems, &items, &tile_items, &partial, &stiles,
+                          &scnt, &sorted, &snitems, &sitems, &stile, &flag})
             b->release();
     }
 };
@@ -1010,7 +1012,8 @@ void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
     int64_t *tmp = (int64_t *)pp.scan_tmp.p, *bounds = (int64_t *)pp.bounds.p;
     launch_2d_setup(pp.dtype, pp.d_maps, pp.nmaps, pp.ntot, pp.nq, pp.kernel, rec, cls, s);
     launch_partition(cls, pp.ntot, (int64_t *)pp.counts.p, tmp, l0, l1, l2, s);
-    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, l1, pp.d_pbase, pp.nmaps, bounds, s);
+    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, l0, l1, pp.d_pbase, pp.nmaps, bounds,
+                        s);
     launch_2d_norm(pp.kernel, l2, pp.ntot, pp.d_maps, rec, s, bounds);
     // large particles: pair lists over device-side chunk ranges
     int64_t *cnt = (int64_t *)pp.cnt.p, *nit = (int64_t *)pp.nitems.p;
@@ -1047,13 +1050,18 @@ void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
         launch_2d_gather(g, s);
         launch_2d_finalize(g, s);
     }
-    // small particles: device-side buckets and items (bounds[1] = the small count)
-    int64_t *soff = (int64_t *)pp.soff.p, *snit = (int64_t *)pp.snitems.p;
-    launch_2d_small_buckets(l0, bounds + 1, pp.ntot, rec, pp.d_maps, pp.nstiles, soff,
-                            (int64_t *)pp.scur.p, tmp, (int32_t *)pp.sorted.p, s);
-    launch_items(soff, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
+    // small particles: per-tile lists over device-side chunk ranges of each
+    // map's small list (bounds[5 + nmaps + m]), then items (bounds[1] = the
+    // small count)
+    int64_t *scnt = (int64_t *)pp.scnt.p, *snit = (int64_t *)pp.snitems.p;
+    const int64_t *sbase = bounds + 5 + pp.nmaps;
+    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
+                        tmp, nullptr, false, sbase, s);
+    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
+                        tmp, (int32_t *)pp.sorted.p, true, sbase, s);
+    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
                  nullptr, nullptr, false, s);
-    launch_items(soff, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
+    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
                  (int4 *)pp.sitems.p, (int64_t *)pp.stile.p, true, s);
     for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.sstep) {
         const int np = pp.nplanes - plane0 < pp.sstep ? pp.nplanes - plane0 : pp.sstep;
@@ -1078,7 +1086,6 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     S2G_HIP(hipMemcpyAsync(d, pt.d_maps, btab + bpb + bst, hipMemcpyDeviceToDevice, s));
     pp.d_maps = (const Map2D *)d;
     pp.d_pbase = (const int64_t *)(d + btab);
-    pp.d_stm = (const TileMap *)(d + btab + bpb);
     pp.ntot = pt.ntot;
     pp.ntiles = pt.ntiles;
     pp.nstiles = pt.nstiles;
@@ -1094,7 +1101,7 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     pp.l0.get(sizeof(int32_t) * n);
     pp.l1.get(sizeof(int32_t) * n);
     pp.l2.get(sizeof(int32_t) * n);
-    pp.bounds.get(sizeof(int64_t) * (nmaps + 5));
+    pp.bounds.get(sizeof(int64_t) * (2 * nmaps + 6));
     pp.flag.get(sizeof(int) * 2);
     // creation run of the class stages: the per-map large counts
     launch_2d_setup(pp.dtype, pp.d_maps, nmaps, pp.ntot, pp.nq, pp.kernel, (Rec2 *)pp.rec.p,
@@ -1103,7 +1110,8 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     launch_partition((const int32_t *)pp.cls.p, pp.ntot, (int64_t *)pp.counts.p,
                      (int64_t *)pp.scan_tmp.p, (int32_t *)pp.l0.p, (int32_t *)pp.l1.p,
                      (int32_t *)pp.l2.p, s);
-    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, (const int32_t *)pp.l1.p, pp.d_pbase,
+    launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, (const int32_t *)pp.l0.p,
+                        (const int32_t *)pp.l1.p, pp.d_pbase,
                         nmaps, (int64_t *)pp.bounds.p, s);
     std::vector<int64_t> hb(nmaps + 5);
     S2G_HIP(hipMemcpyAsync(hb.data(), pp.bounds.p, sizeof(int64_t) * (nmaps + 5),
@@ -1141,9 +1149,44 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     S2G_HIP(hipMemcpy(dt + btm, chunks.data(), bch, hipMemcpyHostToDevice));
     pp.d_tm = (const TileMap *)dt;
     pp.d_ch = (const PairChunk *)(dt + btm);
-    // scans: class counts, pair counts, item counts, small buckets and items
+    // small lists: chunks of each map's small list over its small-tile grid;
+    // capacity: all of the map's particles (a true bound)
+    std::vector<TileMap> stm;
+    std::vector<PairChunk> schunks;
+    int64_t scb = 0, stot = 0;
+    pp.smax_tiles = 1;
+    for (int m = 0; m < nmaps; ++m) {
+        const int nts = (int)((pp.mi[m].a.npix + kST - 1) / kST);
+        const int64_t ntl = (int64_t)nts * nts;
+        if (ntl > kMaxTiles)
+            throw Error(S2G_EUNSUPPORTED, "map " + std::to_string(m) + ": npixels above " +
+                                              std::to_string(kST * 128) +
+                                              " is not supported in a plan");
+        pp.smax_tiles = (int)ntl > pp.smax_tiles ? (int)ntl : pp.smax_tiles;
+        const int64_t cap = pp.mi[m].a.n > 0 ? pp.mi[m].a.n : 1;
+        int64_t csz = (cap + 8 * (int64_t)c->ncu - 1) / (8 * (int64_t)c->ncu);
+        csz = csz < kPairChunk ? kPairChunk : (csz + 63) / 64 * 64;
+        while (ntl * ((cap + csz - 1) / csz) > (1 << 25))
+            csz *= 2;
+        const int nch = (int)((cap + csz - 1) / csz);
+        stm.push_back(TileMap{scb, (int32_t)stot, (int32_t)ntl, nts, 1, nch, (int32_t)csz});
+        scb += ntl * nch;
+        stot += ntl;
+        for (int k = 0; k < nch; ++k)
+            schunks.push_back(PairChunk{m, k, 0, 0});
+    }
+    pp.nschunks = (int)schunks.size();
+    pp.sncnt = scb + 1;
+    const size_t bstm = sizeof(TileMap) * nmaps, bsch = sizeof(PairChunk) * schunks.size();
+    char *ds = (char *)pp.stiles.get(bstm + bsch);
+    S2G_HIP(hipMemcpy(ds, stm.data(), bstm, hipMemcpyHostToDevice));
+    S2G_HIP(hipMemcpy(ds + bstm, schunks.data(), bsch, hipMemcpyHostToDevice));
+    pp.d_stm = (const TileMap *)ds;
+    pp.d_sch = (const PairChunk *)(ds + bstm);
+    pp.scnt.get(sizeof(int64_t) * pp.sncnt);
+    // scans: class counts, pair counts, item counts, small lists and items
     int64_t nscan = 3 * pp.nb + 1;
-    for (int64_t v : {pp.ncnt, (int64_t)pp.ntiles + 1, (int64_t)pp.nstiles + 1})
+    for (int64_t v : {pp.ncnt, pp.sncnt, (int64_t)pp.ntiles + 1, (int64_t)pp.nstiles + 1})
         nscan = v > nscan ? v : nscan;
     pp.scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nscan));
     pp.cnt.get(sizeof(int64_t) * pp.ncnt);
@@ -1175,8 +1218,6 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     pp.sitem_cap = pp.nstiles + pp.starget + 1;
     if (pp.sitem_cap > ((int64_t)1 << 31) / 1024)
         throw Error(S2G_EUNSUPPORTED, "too many small-footprint work items");
-    pp.soff.get(sizeof(int64_t) * (pp.nstiles + 1));
-    pp.scur.get(sizeof(int64_t) * (pp.nstiles > 0 ? pp.nstiles : 1));
     pp.sorted.get(sizeof(int32_t) * n);
     pp.snitems.get(sizeof(int64_t) * (pp.nstiles + 1));
     pp.sitems.get(sizeof(int4) * pp.sitem_cap);

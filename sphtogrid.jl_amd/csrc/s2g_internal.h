@@ -26,6 +26,9 @@ enum : int32_t {
 // Largest footprint side (pixels) of the scatter path.  A box of side <= 34
 // corresponds to hsml <= 16 pixels (side = floor(x+h) - floor(x-h) + 1).
 constexpr int kSmallSide2D = 34;
+// side of the small-footprint tiles (a particle's bucket is the tile holding
+// its footprint's first pixel)
+constexpr int kST = 32;
 // 2D gather tile: 64 x 64 pixels, 4 waves x 16 rows x 64 columns.
 constexpr int kGT = 64;
 constexpr int kGRows = 16;
