@@ -316,6 +316,49 @@ function deposit_batch!(descs::Vector{S2GMapDesc}; dtype::Type=Float32,
     return nothing
 end
 
+"""
+    BatchPlan(descs::Vector{S2GMapDesc}; dtype=Float32, headroom=1.25)
+
+A batch captured once in a hipGraph (s2g_plan_create); `run!(plan)` replays it
+on the inputs' current contents (same device arrays), `close!(plan)` frees it.
+A replay whose inputs outgrew the plan's capacities raises an error
+(S2G_ECAPACITY; the maps are then incomplete).
+"""
+mutable struct BatchPlan
+    handle::Ptr{Cvoid}
+    descs::Vector{S2GMapDesc}
+end
+
+function BatchPlan(descs::Vector{S2GMapDesc}; dtype::Type=Float32, headroom::Real=1.25)
+    c = context()
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve descs begin
+        rc = ccall((:s2g_plan_create, libsph2grid), Cint,
+                   (Ptr{Cvoid}, Cint, Ptr{S2GMapDesc}, Cint, Cdouble, Ptr{Ptr{Cvoid}}),
+                   c.handle, _dtype(dtype), descs, Cint(length(descs)), Float64(headroom), h)
+    end
+    check(c, rc, "s2g_plan_create")
+    plan = BatchPlan(h[], descs)
+    finalizer(close!, plan)
+    return plan
+end
+
+function run!(plan::BatchPlan; stream::Ptr{Cvoid}=C_NULL)
+    plan.handle == C_NULL && error("the plan is closed")
+    rc = ccall((:s2g_plan_run, libsph2grid), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+               plan.handle, stream, C_NULL)
+    check(context(), rc, "s2g_plan_run")
+    return nothing
+end
+
+function close!(plan::BatchPlan)
+    if plan.handle != C_NULL
+        ccall((:s2g_plan_destroy, libsph2grid), Cvoid, (Ptr{Cvoid},), plan.handle)
+        plan.handle = C_NULL
+    end
+    return nothing
+end
+
 # the pixel vector of a map: a Vector{Float64} itself, or the `pixels` field
 # of the HealpixMap{Float64,RingOrder} the reference allocates (main.jl:110-111)
 _pixels(v::Vector{Float64}) = v

@@ -32,6 +32,9 @@ C2JL = {
     "s2g_stats*": {"Ptr{Cvoid}"},
     "const s2g_map_desc*": {"Ptr{S2GMapDesc}"},
     "const s2g_tuning*": {"Ref{S2GTuning}", "Ptr{S2GTuning}"},
+    "double": {"Cdouble", "Float64"},
+    "s2g_plan**": {"Ptr{Ptr{Cvoid}}"},
+    "s2g_plan*": {"Ptr{Cvoid}"},
 }
 CRET = {"int": "Cint", "void": "Cvoid", "const char*": "Cstring"}
 # C struct field type -> Julia field type
