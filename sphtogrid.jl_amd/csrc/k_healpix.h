@@ -38,13 +38,17 @@ struct HpGroup {
     const void *binq[2], *w[2];
     double *map[2], *wmap[2];
     int nm;
+    const double4 *rtab;  // per-ring terms (launch_hp_ring_table)
 };
 
 // nunits: n+1 int64 (last = 0, the total after an exclusive scan)
 void launch_hp_setup(const HpArgs &a, HpRec *rec, int64_t *nunits, hipStream_t s);
 void launch_hp_units(const int64_t *ubase, int64_t n, HpRec *rec, int2 *units, hipStream_t s);
+// rtab: 4 Nside double4, ring iz at rtab[iz] = (sin(theta/2), cos(theta/2),
+// sin(theta), cos(theta)) of the ring's colatitude, for iz in [1, 4 Nside - 1]
+void launch_hp_ring_table(int64_t nside, double4 *rtab, hipStream_t s);
 void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
-                    double4 *part, hipStream_t s);
+                    const double4 *rtab, double4 *part, hipStream_t s);
 void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRec *rec,
                       hipStream_t s);
 // per-unit atomic deposit of a group of maps (maps zeroed by the caller)

@@ -286,18 +286,37 @@ struct RingTerms {
     int64_t nr;
 };
 
-__device__ __forceinline__ RingTerms ring_terms(int64_t nside, int64_t iz, const HpRec &r)
+// Per-ring half-angle terms (ring iz at rtab[iz]): sin(theta_ring / 2) =
+// sqrt((1 - z) / 2), cos(theta_ring / 2) = sqrt((1 + z) / 2) and sin(theta_ring)
+// = sqrt((1 - z)(1 + z)), z = cos(theta_ring); formed once per call instead of
+// once per (particle, ring), with the same FP64 operations.
+__global__ __launch_bounds__(256) void k_hp_ring_table(int64_t nside, double4 *__restrict__ rtab)
+{
+    const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz < 1 || iz > 4 * nside - 1)
+        return;
+    double z, omz, opz;
+    hp_ring_zz(nside, iz, z, omz, opz);
+    rtab[iz] = make_double4(sqrt(0.5 * omz), sqrt(0.5 * opz), sqrt(omz * opz), z);
+}
+
+void launch_hp_ring_table(int64_t nside, double4 *rtab, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hp_ring_table, dim3((unsigned)((4 * nside + 255) / 256)), dim3(256), 0,
+                       s, nside, rtab);
+}
+
+__device__ __forceinline__ RingTerms ring_terms(int64_t nside, int64_t iz, const HpRec &r,
+                                                const double4 *__restrict__ rtab)
 {
     RingTerms t;
     double shift;
     hp_ring_info(nside, iz, t.start, t.nr, shift);
-    // sin((theta_ring - theta0) / 2) from the half-angle terms of both
-    // colatitudes: sin(theta_ring / 2) = sqrt((1 - z) / 2), cos = sqrt((1 + z) / 2)
-    double z, omz, opz;
-    hp_ring_zz(nside, iz, z, omz, opz);
-    const double sh = sqrt(0.5 * omz) * r.ch0 - sqrt(0.5 * opz) * r.sh0;
+    // sin((theta_ring - theta0) / 2) from the half-angle terms of both colatitudes
+    const double4 q = rtab[iz];
+    const double sh = q.x * r.ch0 - q.y * r.sh0;
     t.s1 = (float)(sh * sh);
-    t.s2 = (float)(sqrt(omz * opz) * r.sth0);
+    t.s2 = (float)(q.z * r.sth0);
     t.dphi_pix = 2.0 * kPi / (double)t.nr;
     t.phase = shift * t.dphi_pix - r.phi0;
     return t;
@@ -396,7 +415,8 @@ __device__ __forceinline__ int wave_incl_scan(int v, int ln)
 // F(pix, W, A) is called per pixel on every lane that holds a pixel.
 template <int K, typename F>
 __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, int ui, int ln,
-                                               float ang, HpRing *tab, F &&f)
+                                               float ang, HpRing *tab,
+                                               const double4 *__restrict__ rtab, F &&f)
 {
     const float thh = (float)r.theta + 0.5f * ang, inv_theta = (float)(1.0 / r.theta);
     const float inv_ang = 1.0f / ang;
@@ -429,7 +449,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
             any = hp_ring_interval(nside, iz, r.z0, r.xa, cosr, r.phi0, a, b);
         }
         if (any) {
-            const RingTerms t = ring_terms(nside, iz, r);
+            const RingTerms t = ring_terms(nside, iz, r, rtab);
             e.lo = (int)a;
             len = (int)(b - a + 1);
             e.start = (int)t.start;
@@ -479,7 +499,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
     const bool own = (cring >= r0 && cring <= r1) ||
                      (ui == 0 && (cring < r.ra || cring > r.rb || r.rb < r.ra));
     if (own && !covered_cp && ln == 0) {
-        const RingTerms t = ring_terms(nside, cring, r);
+        const RingTerms t = ring_terms(nside, cring, r, rtab);
         const float d = hp_dist(t, ck);
         float W, A;
         hp_weights<K>(d, thh, inv_theta, ang, inv_ang, W, A);
@@ -493,7 +513,8 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
 template <int K>
 __device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict__ units,
                                               const HpRec *__restrict__ rec, int64_t nside,
-                                              HpRing *tab, double4 *__restrict__ part)
+                                              HpRing *tab, const double4 *__restrict__ rtab,
+                                              double4 *__restrict__ part)
 {
     const int ln = threadIdx.x % kWave;
     const int2 u = units[gw];
@@ -501,7 +522,7 @@ __device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     double swa = 0.0;
     float sa = 0.0f;
-    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, [&](int64_t, float W, float A) {
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, rtab, [&](int64_t, float W, float A) {
         swa += (double)(W * A);
         sa += A;
     });
@@ -516,13 +537,14 @@ __device__ __forceinline__ void k_hp_norm_one(int64_t gw, const int2 *__restrict
 template <int K>
 __global__ __launch_bounds__(256, 4) void k_hp_norm(const int2 *__restrict__ units, int64_t nu,
                                                  const HpRec *__restrict__ rec, int64_t nside,
+                                                 const double4 *__restrict__ rtab,
                                                  double4 *__restrict__ part)
 {
     __shared__ HpRing s_tab[256 / kWave][kWave];
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; gw < nu;
          gw += nw)
-        k_hp_norm_one<K>(gw, units, rec, nside, s_tab[threadIdx.x / kWave], part);
+        k_hp_norm_one<K>(gw, units, rec, nside, s_tab[threadIdx.x / kWave], rtab, part);
 }
 
 
@@ -580,7 +602,7 @@ __device__ __forceinline__ void k_hp_deposit_one(int64_t gw, const int2 *__restr
         hp_map_scale<T>(g, k, u.x, r.s, flags, sw[k], bq[k]);
     const float ang = (float)sqrt(4.0 * kPi / (double)(12 * nside * nside));
     const bool fb = r.fb != 0;
-    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, [&](int64_t pix, float W, float A) {
+    hp_unit_pixels<K>(nside, r, u.y, ln, ang, tab, g.rtab, [&](int64_t pix, float W, float A) {
         const double wa = (double)((fb ? 1.0f : W) * A);
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
@@ -833,7 +855,7 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
                     any = hp_ring_interval(nside, iz, r.z0, r.xa, r.cosr, r.phi0, a, bb);
                 }
             }
-            const RingTerms t = ring_terms(nside, iz, r);
+            const RingTerms t = ring_terms(nside, iz, r, g.rtab);
             if (any) {
                 q.lo = (int)a;
                 len = (int)(bb - a + 1);
@@ -893,7 +915,7 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
                 // the centre pixel, as the norm pass computes it
                 k = cur.ck;
                 const HpRec &r = rec[list[c0 + j]];
-                d = hp_dist(ring_terms(nside, iz, r), k);
+                d = hp_dist(ring_terms(nside, iz, r, g.rtab), k);
             }
             float W, A;
             hp_weights<K>(d, cur.theta, cur.inv_theta, ang, inv_ang, W, A);
@@ -948,13 +970,13 @@ void launch_hp_units(const int64_t *ubase, int64_t n, HpRec *rec, int2 *units, h
 }
 
 void launch_hp_norm(int kernel, const int2 *units, int64_t nu, const HpRec *rec, int64_t nside,
-                    double4 *part, hipStream_t s)
+                    const double4 *rtab, double4 *part, hipStream_t s)
 {
     if (nu == 0)
         return;
     const dim3 g((unsigned)wave_blocks(nu, 256)), blk(256);
     S2G_HP_KERNEL_SWITCH(kernel, hipLaunchKernelGGL(k_hp_norm<KK>, g, blk, 0, s, units, nu, rec,
-                                                    nside, part));
+                                                    nside, rtab, part));
 }
 
 void launch_hp_finish(const int64_t *ubase, int64_t n, const double4 *part, HpRec *rec,

@@ -48,7 +48,7 @@ struct s2g_ctx {
     DevBuf md_P, md_bbox, md_S, md_tab;
     PinnedBuf h_tab, h_bounds, h_tiles;
     // all-sky deposit
-    DevBuf hp_rec, hp_nunits, hp_units, hp_part, hp_ring, hp_list;
+    DevBuf hp_rec, hp_nunits, hp_units, hp_part, hp_ring, hp_list, hp_rtab;
     // TSC: brick offsets, sorted grid coordinates + quantity, packed cells
     DevBuf tsc_off, tsc_gq, tsc_cell;
     // outputs
@@ -1432,6 +1432,7 @@ void s2g_destroy(s2g_ctx *c)
     c->hp_units.release();
     c->hp_part.release();
     c->hp_ring.release();
+    c->hp_rtab.release();
     c->hp_list.release();
     c->tsc_off.release();
     c->tsc_gq.release();
@@ -1600,8 +1601,12 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *const *maps,
             S2G_HIP(hipMemsetAsync(maps[k], 0, sizeof(double) * npx, s));
             S2G_HIP(hipMemsetAsync(wmaps[k], 0, sizeof(double) * npx, s));
         }
+    // per-ring half-angle terms, once per call
+    double4 *rtab = (double4 *)c->hp_rtab.get(sizeof(double4) * 4 * nside);
+    launch_hp_ring_table(nside, rtab, s);
     auto group = [&](int m0) {
         HpGroup g;
+        g.rtab = rtab;
         g.nm = a.nmaps - m0 < 2 ? a.nmaps - m0 : 2;
         for (int k = 0; k < 2; ++k) {
             const int mk = m0 + (k < g.nm ? k : 0);
@@ -1627,7 +1632,7 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *const *maps,
         double4 *part = (double4 *)c->hp_part.get(sizeof(double4) * (nu ? nu : 1));
         launch_hp_units(nun, n, rec, units, s);
         S2G_HIP(hipEventRecord(E[2], s));
-        launch_hp_norm(a.kernel, units, nu, rec, nside, part, s);
+        launch_hp_norm(a.kernel, units, nu, rec, nside, rtab, part, s);
         launch_hp_finish(nun, n, part, rec, s);
         S2G_HIP(hipEventRecord(E[3], s));
         if (rings) {

@@ -114,3 +114,39 @@ def test_plan_argument_errors(s2g):
             s2g.BatchPlan(_descs(maps, "o"), headroom=bad)
     with pytest.raises(_lib.LibraryError, match="at least one map"):
         s2g.BatchPlan([])
+
+
+def test_plan_with_several_passes_and_dtypes(s2g, orc):
+    """Maps of different kernels, quantity counts and accumulator flags make
+    several passes in one graph; FP64 inputs; flat (unreduced) outputs."""
+    import torch
+    dev = torch.device("cuda:0")
+    r = np.random.default_rng(31)
+    specs = [("cubic", 1, 96, 900, False), ("wendlandc6", 3, 128, 1500, False),
+             ("wendlandc4", 2, 80, 700, True), ("cubic", 1, 64, 400, False)]
+    kid = {"cubic": 0, "wendlandc4": 3, "wendlandc6": 4}
+    maps, ref = [], []
+    for kname, nq, npix, n, fp32 in specs:
+        x = r.uniform(-0.5, 0.5, (n, 3))
+        h = np.exp(r.uniform(np.log(0.3 / npix), np.log(60.0 / npix), n))
+        m, rho, w = (r.uniform(0.5, 2.0, n) for _ in range(3))
+        q = r.uniform(0.1, 5.0, (n, nq))
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+        d = dict(pos=T(x), hsml=T(h), m=T(m), rho=T(rho), bin_q=T(q), weights=T(w),
+                 n=n, nq=nq, npix=npix, len2pix=float(npix), kernel=kid[kname],
+                 calc_mean=True, fp32_accum=fp32)
+        maps.append(d)
+        ref.append(orc.cic_mapping_2D(x, h, m, rho, q, w, npix, float(npix), kname, 2, True))
+    eager = [dict(d, out=torch.empty(d["npix"] ** 2 * (d["nq"] + 1), dtype=torch.float64,
+                                     device=dev)) for d in maps]
+    planned = [dict(d, out=torch.empty_like(e["out"])) for d, e in zip(maps, eager)]
+    s2g.deposit_batch_device(eager, dtype="f64")
+    with s2g.BatchPlan(planned, dtype="f64") as plan:
+        for rep in range(2):
+            plan.run()
+            torch.cuda.synchronize()
+            for k, (e, p, rf) in enumerate(zip(eager, planned, ref)):
+                assert _max_rel(p["out"], e["out"]) <= 1e-12, (rep, k)
+                got = p["out"].cpu().numpy().reshape((-1, specs[k][1] + 1), order="F")
+                for q in range(got.shape[1]):
+                    assert rel_l1(got[:, q], rf[:, q]) <= 1e-5, (rep, k, q)
