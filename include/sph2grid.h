@@ -347,7 +347,10 @@ void s2g_plan_destroy(s2g_plan *plan);
  *              fallback of calculate_weights, cic_2D.jl:49-69, cic_3D.jl:55-75)
  *   tile_off   ntiles + 1 int64 (ntiles = ceil(npix/64)^2 in 2D, ceil(npix/16)^3
  *              in 3D, row-major): the large particles whose box meets tile t are
- *              pairs[tile_off[t] .. tile_off[t+1]), in particle order
+ *              pairs[tile_off[t] .. tile_off[t+1]), in the order f_j =
+ *              e_{(j r) mod L} of the large particles e_0 .. e_{L-1} (particle
+ *              order), r = the integer nearest to L / phi (golden ratio) raised
+ *              to the first value coprime to L
  *   pairs      the pair lists, copied when their total is <= pair_cap
  *   small_off  2D: ceil(npix/32)^2 + 1 int64, small_order[small_off[t] ..
  *              small_off[t+1]) are the small particles whose box starts in

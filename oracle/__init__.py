@@ -32,6 +32,7 @@ form or brute force.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 import threading
@@ -353,16 +354,34 @@ def footprints(dims, pos, hsml, bin_q, npix, len2pix, kernel, calc_mean=True,
     return box, cls, fb
 
 
+def large_stride(L):
+    """Stride of the order in which the tile lists take the L large
+    particles: the integer nearest to L / phi (golden ratio, L * 0.618... in
+    double precision, rounded half up), raised to the first value coprime to
+    L; 1 for L <= 2."""
+    if L <= 2:
+        return 1
+    r = int(L * 0.6180339887498949 + 0.5)
+    while math.gcd(r, L) != 1:
+        r += 1
+    return r
+
+
 def tile_lists(dims, box, cls, npix):
     """The deposit's (tile, particle) lists, restated from their definition:
-    tiles of 64^2 pixels (2D) or 16^3 voxels (3D) in row-major order; tile t
-    lists, in particle order, every large particle (class 2 or 3) whose box
-    meets it.  Returns (tile_off, pairs) as CSR arrays."""
+    tiles of 64^2 pixels (2D) or 16^3 voxels (3D) in row-major order; the
+    large particles (class 2 or 3) e_0 .. e_{L-1} in particle order are taken
+    as f_j = e_{(j r) mod L}, r = large_stride(L), and tile t lists, in that
+    order, every one whose box meets it.  Returns (tile_off, pairs) as CSR
+    arrays."""
     T = 64 if dims == 2 else 16
     nt = (npix + T - 1) // T
     ntiles = nt ** dims
     lists = [[] for _ in range(ntiles)]
-    for p in np.flatnonzero((cls == CLS_LARGE_CLOSED) | (cls == CLS_LARGE_EXPLICIT)):
+    large = np.flatnonzero((cls == CLS_LARGE_CLOSED) | (cls == CLS_LARGE_EXPLICIT))
+    L = large.size
+    r = large_stride(L)
+    for p in large[(np.arange(L, dtype=np.int64) * r) % max(L, 1)]:
         r = [range(box[p, 2 * d] // T, box[p, 2 * d + 1] // T + 1) for d in range(dims)]
         if dims == 2:
             for a in r[0]:

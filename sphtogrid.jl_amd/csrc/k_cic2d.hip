@@ -11,7 +11,7 @@
 //                    (closed form minus the border-cut part when smaller)
 //   k_pair_count     counting sort of (tile, particle) pairs: per-chunk counts
 //   (scan)           tile-major exclusive scan -> deterministic list offsets
-//   k_pair_fill      one wave per chunk fills the per-tile lists in particle order
+//   k_pair_fill      one wave per chunk fills the per-tile lists in list order
 //   k_item_*         split every tile list into work items of <= G pairs
 //   k2d_gather       per-tile gather, FP64 register accumulation per pixel
 //                    (k2d_gather_wide for three or four planes)

@@ -361,7 +361,9 @@ __global__ __launch_bounds__(256) void k3d_norm(const int32_t *__restrict__ list
 // pixel, and a per-particle set-up of one FP64 multiplier per plane.
 // ---------------------------------------------------------------------------
 constexpr int kGBatch3 = 32;
-// sub-bricks per uniform test (1, 2 or 4: one bi per test)
+// sub-bricks per uniform test (1, 2 or 4: one bi per test; 4 was slower on
+// the 2^24 cluster set at 256^3, 64.3 -> 69.6 ms, more sub-bricks outside
+// the footprint evaluated)
 constexpr int kG3 = 2;
 
 template <typename T, int K, int NP>

@@ -39,21 +39,23 @@ struct TileMap {
     int32_t nch;
     int32_t csz;  // entries per chunk when the chunk ranges are taken on the device
 };
-// chunk k of map `map`: large-list entries [l0, l1)
+// chunk k of map `map`: list entries lbase[map] + k csz .. (clipped to the
+// map's list), with lbase the per-map list starts on the device
 struct PairChunk {
-    int32_t map, k, l0, l1;
+    int32_t map, k;
 };
 
 // fill == false: count + scan into cnt (ncnt int64, the last is the total)
 // fill == true : fill pairs (particle indices) from the scanned cnt
-// lbase (null: the chunk table's fixed ranges): per-map large-list starts on
-// the device, chunk k of map m then covering lbase[m] + k csz .. (clipped);
-// cap: entries of pairs (slots past it are not written)
+// lbase: per-map large-list starts on the device (nmaps + 1 entries); a map's
+// segment is taken in the order f_j = e_{(j r) mod L} (L its length, r the
+// integer nearest to L / phi made coprime to L), so every tile list follows
+// that order; cap: entries of pairs (slots past it are not written)
 template <typename R>
 void launch_pairs(const int32_t *large, const R *rec, const PairChunk *chunks, int nchunks,
                   const TileMap *tmaps, int max_tiles, int64_t *cnt, int64_t ncnt,
                   int64_t *scan_tmp, int32_t *pairs_or_null, bool fill, hipStream_t s,
-                  const int64_t *lbase = nullptr, int64_t cap = INT64_MAX);
+                  const int64_t *lbase, int64_t cap = INT64_MAX);
 // small-footprint buckets as per-tile lists (fill == false: counts per
 // (tile, chunk) + scan into cnt; fill == true: out = the particles, tile t of
 // map m at cnt[cbase + t nch] ..), chunk ranges from lbase as for the pairs

@@ -164,14 +164,15 @@ struct TilePlan {
     int64_t *tile_items = nullptr;
 };
 
-// tm: tile grid of every map of the pass (cbase / nch filled here from the
-// per-map large-list bounds lbase[0..nmaps])
+// tm: tile grid of every map of the pass (cbase / nch / csz filled here from
+// the per-map large-list bounds lbase[0..nmaps]; d_lbase: the same on the
+// device)
 // Work items of <= G pairs, G = max(gmin, pairs / (ncu * per_cu)): enough
 // items to fill every CU several times.
 TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, bool rec3,
                           std::vector<TileMap> &tm, const std::vector<int64_t> &lbase,
-                          int ntiles, hipStream_t s, double *ms_sort, int64_t gmin = 64,
-                          int64_t per_cu = 16)
+                          const int64_t *d_lbase, int ntiles, hipStream_t s, double *ms_sort,
+                          int64_t gmin = 64, int64_t per_cu = 16)
 {
     hipEvent_t *E = c->stage;
     TilePlan tp;
@@ -196,12 +197,10 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
         const int nch = (int)((l1 - l0 + csz - 1) / csz);
         tm[m].cbase = cb;
         tm[m].nch = nch;
+        tm[m].csz = (int32_t)csz;
         cb += (int64_t)tm[m].ntiles * nch;
-        for (int k = 0; k < nch; ++k) {
-            const int64_t a = l0 + (int64_t)k * csz;
-            const int64_t e = a + csz < l1 ? a + csz : l1;
-            chunks.push_back(PairChunk{m, k, (int32_t)a, (int32_t)e});
-        }
+        for (int k = 0; k < nch; ++k)
+            chunks.push_back(PairChunk{m, k});
     }
     const int nchunks = (int)chunks.size();
     const int64_t ncnt = cb + 1;
@@ -223,10 +222,10 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     S2G_HIP(hipEventRecord(E[4], s));
     if (rec3)
         launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
-                           tmp, nullptr, false, s);
+                           tmp, nullptr, false, s, d_lbase);
     else
         launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
-                           tmp, nullptr, false, s);
+                           tmp, nullptr, false, s, d_lbase);
     // item counts from the scanned pair counts, the item size from the pair
     // total on the device: one read-back gives both totals
     const int64_t target = (int64_t)c->ncu * per_cu;
@@ -247,10 +246,10 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     S2G_HIP(hipEventRecord(E[6], s));
     if (rec3)
         launch_pairs<Rec3>(large, (const Rec3 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
-                           tmp, tp.pairs, true, s);
+                           tmp, tp.pairs, true, s, d_lbase);
     else
         launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
-                           tmp, tp.pairs, true, s);
+                           tmp, tp.pairs, true, s, d_lbase);
     launch_items(cnt, d_tm, nmaps, ntiles, gmin, target, d_npairs, nit, tmp, tp.items,
                  tp.tile_items, true, s);
     S2G_HIP(hipEventRecord(E[7], s));
@@ -263,6 +262,7 @@ struct ClassLists {
     int64_t ns = 0, nl = 0, ne = 0;
     int32_t *l0 = nullptr, *l1 = nullptr, *l2 = nullptr;
     std::vector<int64_t> lbase;  // per-map first large-list entry (+ the total)
+    const int64_t *d_lbase = nullptr;  // the same on the device
 };
 
 // stable class partition of the pass's particles and one read-back of the
@@ -288,6 +288,7 @@ ClassLists partition_classes(s2g_ctx *c, const int32_t *cls, int64_t n, const in
     cl.nl = h[2] - h[1];
     cl.ne = h[3] - h[2];
     cl.lbase.assign(h + 4, h + 5 + nmaps);
+    cl.d_lbase = bounds + 4;
     return cl;
 }
 
@@ -445,8 +446,8 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     int64_t launches = 0;
     TilePlan tp;
     if (cl.nl > 0) {
-        tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, ntiles, s, &ms_sort,
-                              c->tune.gather_item_min, c->tune.gather_items_per_cu);
+        tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, cl.d_lbase, ntiles, s,
+                              &ms_sort, c->tune.gather_item_min, c->tune.gather_items_per_cu);
         if (st)
             st->n_pairs = tp.npairs;
     }
@@ -589,8 +590,8 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     int64_t launches = 0;
     if (cl.nl > 0) {
         std::vector<TileMap> tm{TileMap{0, 0, ntiles, nb, nb, 0, 0}};
-        const TilePlan tp = build_tile_lists(c, cl.l1, rec, true, tm, cl.lbase, ntiles, s,
-                                             &ms_sort);
+        const TilePlan tp = build_tile_lists(c, cl.l1, rec, true, tm, cl.lbase, cl.d_lbase,
+                                             ntiles, s, &ms_sort);
         if (st)
             st->n_pairs = tp.npairs;
         Gather3Args g;
@@ -1139,7 +1140,7 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
         tm[m].csz = (int32_t)csz;
         cb += (int64_t)tm[m].ntiles * nch;
         for (int k = 0; k < nch; ++k)
-            chunks.push_back(PairChunk{m, k, 0, 0});
+            chunks.push_back(PairChunk{m, k});
     }
     pp.nchunks = (int)chunks.size();
     pp.ncnt = cb + 1;
@@ -1173,7 +1174,7 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
         scb += ntl * nch;
         stot += ntl;
         for (int k = 0; k < nch; ++k)
-            schunks.push_back(PairChunk{m, k, 0, 0});
+            schunks.push_back(PairChunk{m, k});
     }
     pp.nschunks = (int)schunks.size();
     pp.sncnt = scb + 1;
@@ -1193,7 +1194,8 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     // creation run of the pair count: the pair capacity
     launch_pairs<Rec2>((const int32_t *)pp.l1.p, (const Rec2 *)pp.rec.p, pp.d_ch, pp.nchunks,
                        pp.d_tm, pp.max_tiles, (int64_t *)pp.cnt.p, pp.ncnt,
-                       (int64_t *)pp.scan_tmp.p, nullptr, false, s, (const int64_t *)pp.bounds.p + 4);
+                       (int64_t *)pp.scan_tmp.p, nullptr, false, s,
+                       (const int64_t *)pp.bounds.p + 4);
     int64_t npairs = 0;
     S2G_HIP(hipMemcpyAsync(&npairs, (int64_t *)pp.cnt.p + pp.ncnt - 1, sizeof(int64_t),
                            hipMemcpyDeviceToHost, s));
@@ -2174,8 +2176,8 @@ int s2g_footprints(s2g_ctx *c, int dims, int dtype, const void *pos, const void 
         std::vector<int64_t> cnt;
         if (cl.nl > 0) {
             double ms = 0.0;
-            const TilePlan tp = build_tile_lists(c, cl.l1, rec, dims == 3, tm, cl.lbase, ntiles,
-                                                 s, &ms);
+            const TilePlan tp = build_tile_lists(c, cl.l1, rec, dims == 3, tm, cl.lbase,
+                                                 cl.d_lbase, ntiles, s, &ms);
             npairs = tp.npairs;
             const int64_t ncnt = (int64_t)ntiles * tm[0].nch + 1;
             cnt.resize(ncnt);

@@ -148,3 +148,25 @@ def test_tile_lists_and_buckets_by_hand(orc):
     assert soff.size == 26 and order.tolist() == [5, 2, 4]
     assert soff[3] == 0 and soff[4] == 1 and soff[8] == 1 and soff[9] == 2 and soff[20] == 2
     assert soff[21] == 3 and soff[25] == 3
+
+
+def test_tile_list_order_by_hand(orc):
+    """The tile lists take the L large particles e_0 .. e_{L-1} as
+    e_{(j r) mod L}: L = 5 gives r = round(5 / phi) = 3 (coprime), so the
+    order is e_0, e_3, e_1, e_4, e_2."""
+    npix = 64
+    box = np.array([[0, 40, 0, 40]] * 6, dtype=np.int32)
+    cls = np.array([2, 3, 1, 2, 2, 3], dtype=np.int32)  # particle 2 is small
+    off, pairs = orc.tile_lists(2, box, cls, npix)
+    assert off.tolist() == [0, 5] and pairs.tolist() == [0, 4, 1, 5, 3]
+
+
+def test_large_stride(orc):
+    # nearest integer to L / phi, raised past common factors with L
+    assert [orc.large_stride(L) for L in (0, 1, 2, 3, 5, 8, 10, 12)] == [1, 1, 1, 2, 3, 5, 7, 7]
+    rng = np.random.default_rng(3)
+    for L in list(rng.integers(3, 5000, 40)) + [2 ** 20, 3 * 5 * 7 * 11 * 13]:
+        L = int(L)
+        r = orc.large_stride(L)
+        assert np.gcd(r, L) == 1 and abs(r - L / ((1 + 5 ** 0.5) / 2)) < 40
+        assert np.unique(np.arange(L, dtype=np.int64) * r % L).size == L
