@@ -124,7 +124,11 @@ typedef struct s2g_stats {
     int64_t n_large_exact;    /* ... of which normalised by an explicit sum */
     int64_t n_pairs;          /* (tile, particle) pairs of the gather path */
     double sum_footprint;     /* reserved (0) */
-    /* device time per stage (HIP events on the deposit stream), ms */
+    /* device time per stage (HIP events), ms.  In the 2D / 3D deposits the
+     * normalisation sums and the small-footprint scatter run on a side stream
+     * beside the pair sort and the gather, so those stages overlap: ms_norm
+     * and ms_small are the spans on the side stream (they include its wait
+     * for a share of the GPU), ms_device the whole call. */
     double ms_setup;          /* per-particle setup + class partition */
     double ms_norm;           /* explicit footprint normalisation sums */
     double ms_sort;           /* pair counting sort + work items */

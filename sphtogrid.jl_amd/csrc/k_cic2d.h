@@ -53,6 +53,7 @@ struct GatherArgs {
     const Map2D *maps;
     double *partial;
     const int64_t *tile_items;  // null: no large particle in the pass
+    int add = 0;  // finalize adds its sums to the image instead of storing them
 };
 
 // setup over all particles of the pass: records and the class array

@@ -15,7 +15,8 @@
 //   k_item_*         split every tile list into work items of <= G pairs
 //   k2d_gather       per-tile gather, FP64 register accumulation per pixel
 //                    (k2d_gather_wide for three or four planes)
-//   k2d_finalize     fixed-order sum of a tile's partials -> image (plain stores)
+//   k2d_finalize     fixed-order sum of a tile's partials -> image (stored, or added
+//                    to the small scatter's sums when that ran first)
 //   k2d_small_*      small footprints: sorted by tile, LDS accumulation per item,
 //                    one atomic add per touched pixel into the image
 //   k2d_reduce_maps  reduce_image_2D: divide by the weight plane + transpose
@@ -499,10 +500,11 @@ __global__ __launch_bounds__(256, 4) void k2d_gather_f32acc(const int4 *__restri
 constexpr int kFinSplit = 16;
 
 template <int NP>
+// add: the image already holds the small scatter's sums (zeroed before it)
 __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ partial,
                                                     const int64_t *__restrict__ tile_items,
                                                     const Map2D *__restrict__ maps, int nmaps,
-                                                    int plane0)
+                                                    int plane0, int add)
 {
     // kFinSplit blocks per tile, 4096 / kFinSplit pixels each, so that a pass
     // of few tiles (a small map) still spreads the sums over the CUs
@@ -527,7 +529,8 @@ __global__ __launch_bounds__(256) void k2d_finalize(const double *__restrict__ p
 #pragma unroll 4
             for (int64_t it = 0; it < k; ++it)
                 s += partial[((b + it) * NP + q) * kGT * kGT + px];
-            image[(int64_t)(plane0 + q) * nd + (int64_t)i * npix + j] = s;
+            double *dst = image + (int64_t)(plane0 + q) * nd + (int64_t)i * npix + j;
+            *dst = add ? *dst + s : s;
         }
     }
 }
@@ -916,10 +919,10 @@ void launch_2d_finalize(const GatherArgs &g, hipStream_t s)
 {
     const dim3 grid((unsigned)(g.ntiles * kFinSplit)), block(256);
     switch (g.np) {
-    case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
-    case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
-    case 3: hipLaunchKernelGGL(k2d_finalize<3>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
-    default: hipLaunchKernelGGL(k2d_finalize<4>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0); break;
+    case 1: hipLaunchKernelGGL(k2d_finalize<1>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0, g.add); break;
+    case 2: hipLaunchKernelGGL(k2d_finalize<2>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0, g.add); break;
+    case 3: hipLaunchKernelGGL(k2d_finalize<3>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0, g.add); break;
+    default: hipLaunchKernelGGL(k2d_finalize<4>, grid, block, 0, s, g.partial, g.tile_items, g.maps, g.nmaps, g.plane0, g.add); break;
     }
 }
 

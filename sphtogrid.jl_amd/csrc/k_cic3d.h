@@ -17,6 +17,7 @@ struct Gather3Args {
     double *partial;
     const int64_t *tile_items;
     double *image;
+    int add = 0;  // finalize adds its sums to the image instead of storing them
 };
 
 struct Small3Args {

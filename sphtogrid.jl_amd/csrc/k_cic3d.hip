@@ -522,10 +522,11 @@ __global__ __launch_bounds__(256, 4) void k3d_gather(const int4 *__restrict__ it
 constexpr int kFinSplit3 = 16;
 
 template <int NP>
+// add: the image already holds the small scatter's sums (zeroed before it)
 __global__ __launch_bounds__(256) void k3d_finalize(const double *__restrict__ partial,
                                                     const int64_t *__restrict__ tile_items,
                                                     int nb, int64_t npix, int plane0,
-                                                    double *__restrict__ image)
+                                                    double *__restrict__ image, int add)
 {
     // kFinSplit3 blocks per brick, so that a grid of few bricks still
     // spreads the sums over the CUs
@@ -544,7 +545,8 @@ __global__ __launch_bounds__(256) void k3d_finalize(const double *__restrict__ p
 #pragma unroll 4
             for (int64_t itm = 0; itm < k; ++itm)
                 s += partial[((b + itm) * NP + q) * (kBT * kBT * kBT) + v];
-            image[(int64_t)(plane0 + q) * nd + ((int64_t)i * npix + j) * npix + kz] = s;
+            double *dst = image + (int64_t)(plane0 + q) * nd + ((int64_t)i * npix + j) * npix + kz;
+            *dst = add ? *dst + s : s;
         }
     }
 }
@@ -762,10 +764,10 @@ void launch_3d_finalize(const Gather3Args &g, hipStream_t s)
     const dim3 grid((unsigned)(g.ntiles * kFinSplit3)), blk(256);
     if (g.np == 1)
         hipLaunchKernelGGL(k3d_finalize<1>, grid, blk, 0, s, g.partial, g.tile_items, g.nb,
-                           g.npix, g.plane0, g.image);
+                           g.npix, g.plane0, g.image, g.add);
     else
         hipLaunchKernelGGL(k3d_finalize<2>, grid, blk, 0, s, g.partial, g.tile_items, g.nb,
-                           g.npix, g.plane0, g.image);
+                           g.npix, g.plane0, g.image, g.add);
 }
 
 template <typename T>
