@@ -530,7 +530,9 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
                                   : nullptr;
         g.tile_items = tp.tile_items;
         g.add = add;
-        const bool timed = ev + 2 < s2g_ctx::kNumStageEvents - 3;
+        // events per plane group: gather start / end, finalize start (after
+        // the join with the side stream) / end
+        const bool timed = ev + 3 < s2g_ctx::kNumStageEvents - 3;
         if (timed)
             S2G_HIP(hipEventRecord(E[ev], s));
         if (tp.nitems > 0) {
@@ -541,10 +543,12 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
             S2G_HIP(hipEventRecord(E[ev + 1], s));
         if (plane0 == 0)
             S2G_HIP(hipStreamWaitEvent(s, c->smalled, 0));
+        if (timed)
+            S2G_HIP(hipEventRecord(E[ev + 2], s));
         launch_2d_finalize(g, s);
         if (timed) {
-            S2G_HIP(hipEventRecord(E[ev + 2], s));
-            ev += 3;
+            S2G_HIP(hipEventRecord(E[ev + 3], s));
+            ev += 4;
         }
     }
 
@@ -554,9 +558,9 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     S2G_HIP(hipEventRecord(E[eS + 2], s));
     S2G_HIP(hipEventSynchronize(E[eS + 2]));
     if (st) {
-        for (int k = 10; k < ev; k += 3) {
+        for (int k = 10; k < ev; k += 4) {
             ms_gather += elapsed(E[k], E[k + 1]);
-            ms_fin += elapsed(E[k + 1], E[k + 2]);
+            ms_fin += elapsed(E[k + 2], E[k + 3]);
         }
         st->ms_device = elapsed(E[0], E[eS + 2]);
         st->ms_setup = elapsed(E[0], E[1]);
@@ -672,8 +676,9 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
         }
         S2G_HIP(hipEventRecord(E[11], s));
         S2G_HIP(hipStreamWaitEvent(s, c->smalled, 0));
-        launch_3d_finalize(g, s);
         S2G_HIP(hipEventRecord(E[12], s));
+        launch_3d_finalize(g, s);
+        S2G_HIP(hipEventRecord(E[13], s));
     }
     S2G_HIP(hipStreamWaitEvent(s, c->smalled, 0));
     S2G_HIP(hipGetLastError());
@@ -682,7 +687,7 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     if (st) {
         if (cl.nl > 0) {
             ms_gather = elapsed(E[10], E[11]);
-            ms_fin = elapsed(E[11], E[12]);
+            ms_fin = elapsed(E[12], E[13]);
         }
         st->ms_device = elapsed(E[0], E[eS + 2]);
         st->ms_setup = elapsed(E[0], E[1]);
@@ -1002,7 +1007,7 @@ struct PlanPass {
     int64_t gmin = 64, gtarget = 1, smin = 256, starget = 1;
     bool any_out = false;
     DevBuf tab, img, rec, cls, counts, scan_tmp, l0, l1, l2, bounds, tiles, cnt, pairs, nitems,
-        items, tile_items, partial, stiles, scnt, sorted, snitems, sitems, stile, flag;
+        items, tile_items, partial, stiles, scnt, sorted, snitems, sitems, stile, flag, sscan_tmp;
     const Map2D *d_maps = nullptr;
     const int64_t *d_pbase = nullptr;
     const TileMap *d_tm = nullptr, *d_stm = nullptr;  // large / small lists' tile grids
@@ -1011,7 +1016,7 @@ struct PlanPass {
     {
         for (DevBuf *b : {&tab, &img, &rec, &cls, &counts, &scan_tmp, &l0, &l1, &l2, &bounds,
                           &tiles, &cnt, &pairs, &nitems, &items, &tile_items, &partial, &stiles,
-                          &scnt, &sorted, &snitems, &sitems, &stile, &flag})
+                          &scnt, &sorted, &snitems, &sitems, &stile, &flag, &sscan_tmp})
             b->release();
     }
 };
@@ -1019,7 +1024,9 @@ struct PlanPass {
 struct s2g_plan {
     s2g_ctx *c = nullptr;
     std::vector<PlanPass> passes;
-    hipStream_t cs = nullptr;  // capture stream
+    hipStream_t cs = nullptr;   // capture stream
+    hipStream_t cs2 = nullptr;  // its side branch (joined back before the capture ends)
+    hipEvent_t fork = nullptr, normed = nullptr, smalled = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t *h_read = nullptr;  // pinned: per pass flag + counts after a run
@@ -1033,21 +1040,30 @@ void plan_release(s2g_plan *pl)
         (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph)
         (void)hipGraphDestroy(pl->graph);
-    if (pl->cs)
-        (void)hipStreamDestroy(pl->cs);
+    for (hipStream_t q : {pl->cs, pl->cs2})
+        if (q)
+            (void)hipStreamDestroy(q);
+    for (hipEvent_t e : {pl->fork, pl->normed, pl->smalled})
+        if (e)
+            (void)hipEventDestroy(e);
     if (pl->h_read)
         (void)hipHostFree(pl->h_read);
     for (PlanPass &pp : pl->passes)
         pp.release();
     pl->exec = nullptr;
     pl->graph = nullptr;
-    pl->cs = nullptr;
+    pl->cs = pl->cs2 = nullptr;
+    pl->fork = pl->normed = pl->smalled = nullptr;
     pl->h_read = nullptr;
 }
 
-// Everything of one pass on `s`, sized by the pass's capacities, with no
-// host read-back (the sequence that is captured).
-void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
+// Everything of one pass on `s` and its side branch `s2` (fork / join by
+// the plan's events), sized by the pass's capacities, with no host read-back
+// (the sequence that is captured).  As in the eager pass, the explicit
+// normalisation and the small-footprint scatter run on the side branch; the
+// images are zeroed there by the finalize kernel with no items (no memset
+// node in the graph), and the finalize adds the large particles' sums.
+void plan_enqueue(s2g_plan *pl, PlanPass &pp, hipStream_t s, hipStream_t s2)
 {
     Rec2 *rec = (Rec2 *)pp.rec.p;
     int32_t *cls = (int32_t *)pp.cls.p, *l0 = (int32_t *)pp.l0.p, *l1 = (int32_t *)pp.l1.p,
@@ -1057,7 +1073,44 @@ void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
     launch_partition(cls, pp.ntot, (int64_t *)pp.counts.p, tmp, l0, l1, l2, s);
     launch_class_bounds((const int64_t *)pp.counts.p, pp.nb, l0, l1, pp.d_pbase, pp.nmaps, bounds,
                         s);
-    launch_2d_norm(pp.kernel, l2, pp.ntot, pp.d_maps, rec, s, bounds);
+    S2G_HIP(hipEventRecord(pl->fork, s));
+    S2G_HIP(hipStreamWaitEvent(s2, pl->fork, 0));
+
+    // side branch: normalisation, zeroed images, small-footprint lists and scatter
+    launch_2d_norm(pp.kernel, l2, pp.ntot, pp.d_maps, rec, s2, bounds);
+    S2G_HIP(hipEventRecord(pl->normed, s2));
+    for (int plane0 = 0; plane0 < pp.nplanes; plane0 += 4) {
+        GatherArgs z;
+        z.np = pp.nplanes - plane0 < 4 ? pp.nplanes - plane0 : 4;
+        z.plane0 = plane0;
+        z.nmaps = pp.nmaps;
+        z.ntiles = pp.ntiles;
+        z.maps = pp.d_maps;
+        z.partial = nullptr;
+        z.tile_items = nullptr;  // no items: every pixel of every tile set to 0
+        launch_2d_finalize(z, s2);
+    }
+    int64_t *stmp = (int64_t *)pp.sscan_tmp.p;
+    int64_t *scnt = (int64_t *)pp.scnt.p, *snit = (int64_t *)pp.snitems.p;
+    // per-tile lists over device-side chunk ranges of each map's small list
+    // (bounds[5 + nmaps + m]), then items (bounds[1] = the small count)
+    const int64_t *sbase = bounds + 5 + pp.nmaps;
+    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
+                        stmp, nullptr, false, sbase, s2);
+    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
+                        stmp, (int32_t *)pp.sorted.p, true, sbase, s2);
+    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit,
+                 stmp, nullptr, nullptr, false, s2);
+    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit,
+                 stmp, (int4 *)pp.sitems.p, (int64_t *)pp.stile.p, true, s2);
+    for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.sstep) {
+        const int np = pp.nplanes - plane0 < pp.sstep ? pp.nplanes - plane0 : pp.sstep;
+        launch_2d_small_tile(pp.dtype, pp.kernel, np, (const int4 *)pp.sitems.p, snit + pp.nstiles,
+                             pp.sitem_cap, (const int32_t *)pp.sorted.p, rec, pp.d_maps, pp.nq,
+                             plane0, s2);
+    }
+    S2G_HIP(hipEventRecord(pl->smalled, s2));
+
     // large particles: pair lists over device-side chunk ranges
     int64_t *cnt = (int64_t *)pp.cnt.p, *nit = (int64_t *)pp.nitems.p;
     const int64_t *lbase = bounds + 4, *d_npairs = cnt + pp.ncnt - 1;
@@ -1070,6 +1123,7 @@ void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
                        (int32_t *)pp.pairs.p, true, s, lbase, pp.pair_cap);
     launch_items(cnt, pp.d_tm, pp.nmaps, pp.ntiles, pp.gmin, pp.gtarget, d_npairs, nit, tmp,
                  (int4 *)pp.items.p, (int64_t *)pp.tile_items.p, true, s);
+    S2G_HIP(hipStreamWaitEvent(s, pl->normed, 0));
     for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.gstep) {
         const int np = pp.nplanes - plane0 < pp.gstep ? pp.nplanes - plane0 : pp.gstep;
         GatherArgs g;
@@ -1090,31 +1144,14 @@ void plan_enqueue(s2g_ctx *c, PlanPass &pp, hipStream_t s)
         g.maps = pp.d_maps;
         g.partial = (double *)pp.partial.p;
         g.tile_items = (const int64_t *)pp.tile_items.p;
+        g.add = 1;
         launch_2d_gather(g, s);
+        if (plane0 == 0)
+            S2G_HIP(hipStreamWaitEvent(s, pl->smalled, 0));
         launch_2d_finalize(g, s);
-    }
-    // small particles: per-tile lists over device-side chunk ranges of each
-    // map's small list (bounds[5 + nmaps + m]), then items (bounds[1] = the
-    // small count)
-    int64_t *scnt = (int64_t *)pp.scnt.p, *snit = (int64_t *)pp.snitems.p;
-    const int64_t *sbase = bounds + 5 + pp.nmaps;
-    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
-                        tmp, nullptr, false, sbase, s);
-    launch_anchor_lists(l0, rec, pp.d_sch, pp.nschunks, pp.d_stm, pp.smax_tiles, scnt, pp.sncnt,
-                        tmp, (int32_t *)pp.sorted.p, true, sbase, s);
-    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
-                 nullptr, nullptr, false, s);
-    launch_items(scnt, pp.d_stm, pp.nmaps, pp.nstiles, pp.smin, pp.starget, bounds + 1, snit, tmp,
-                 (int4 *)pp.sitems.p, (int64_t *)pp.stile.p, true, s);
-    for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.sstep) {
-        const int np = pp.nplanes - plane0 < pp.sstep ? pp.nplanes - plane0 : pp.sstep;
-        launch_2d_small_tile(pp.dtype, pp.kernel, np, (const int4 *)pp.sitems.p, snit + pp.nstiles,
-                             pp.sitem_cap, (const int32_t *)pp.sorted.p, rec, pp.d_maps, pp.nq,
-                             plane0, s);
     }
     if (pp.any_out)
         launch_2d_reduce_maps(pp.d_maps, pp.nmaps, pp.ntiles, pp.nq, s);
-    (void)c;
 }
 
 // Sizes one pass from an eager run of its first stages, allocates it.
@@ -1227,11 +1264,14 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     pp.d_stm = (const TileMap *)ds;
     pp.d_sch = (const PairChunk *)(ds + bstm);
     pp.scnt.get(sizeof(int64_t) * pp.sncnt);
-    // scans: class counts, pair counts, item counts, small lists and items
+    // scans: class counts, pair counts and items on the main branch; small
+    // lists and items on the side branch, with scratch of their own
     int64_t nscan = 3 * pp.nb + 1;
-    for (int64_t v : {pp.ncnt, pp.sncnt, (int64_t)pp.ntiles + 1, (int64_t)pp.nstiles + 1})
+    for (int64_t v : {pp.ncnt, (int64_t)pp.ntiles + 1})
         nscan = v > nscan ? v : nscan;
     pp.scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nscan));
+    const int64_t nsscan = pp.sncnt > (int64_t)pp.nstiles + 1 ? pp.sncnt : pp.nstiles + 1;
+    pp.sscan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nsscan));
     pp.cnt.get(sizeof(int64_t) * pp.ncnt);
     // creation run of the pair count: the pair capacity
     launch_pairs<Rec2>((const int32_t *)pp.l1.p, (const Rec2 *)pp.rec.p, pp.d_ch, pp.nchunks,
@@ -1289,12 +1329,16 @@ void plan_build(s2g_ctx *c, s2g_plan *pl, int dtype, const s2g_map_desc *maps, i
     S2G_HIP(hipStreamSynchronize(s));
     S2G_HIP(hipHostMalloc((void **)&pl->h_read, sizeof(int64_t) * 8 * pl->passes.size(),
                           hipHostMallocDefault));
-    // the replayed sequence, captured on a stream of its own
+    // the replayed sequence, captured on a stream of its own (and its side
+    // branch, default priority like every stream here)
     S2G_HIP(hipStreamCreateWithFlags(&pl->cs, hipStreamNonBlocking));
+    S2G_HIP(hipStreamCreateWithFlags(&pl->cs2, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&pl->fork, &pl->normed, &pl->smalled})
+        S2G_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     S2G_HIP(hipStreamBeginCapture(pl->cs, hipStreamCaptureModeThreadLocal));
     try {
         for (PlanPass &pp : pl->passes)
-            plan_enqueue(c, pp, pl->cs);
+            plan_enqueue(pl, pp, pl->cs, pl->cs2);
         S2G_HIP(hipGetLastError());
     } catch (...) {
         hipGraph_t g = nullptr;
