@@ -650,10 +650,14 @@ constexpr int kHpRingPairs = NM == 1 ? 32 : 28;
 // its centre pixel (constributing_pixels.jl:16-19), which lies outside that
 // range only when the disc query returns no pixel of it.  False for
 // particles that deposit nothing.
+// Every particle kept by the set-up is listed, whatever its scale: the lists
+// are built beside the normalisation pass, which writes the scale (and sets
+// it to 0 only when no pixel of the disc has an area weight; such a particle
+// then adds zeros), so they must not read it.
 __device__ __forceinline__ bool hp_dep_rings(int64_t nside, const HpRec &r, int64_t &cring,
                                              int64_t &ck, bool &cring_extra)
 {
-    if (!r.ok || r.s == 0.0)
+    if (!r.ok)
         return false;
     cring = r.cring;
     ck = r.ck;

@@ -1730,6 +1730,14 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *const *maps,
         int2 *units = (int2 *)c->hp_units.get(sizeof(int2) * (nu ? nu : 1));
         double4 *part = (double4 *)c->hp_part.get(sizeof(double4) * (nu ? nu : 1));
         launch_hp_units(nun, n, rec, units, s);
+        // the per-ring lists (side stream) do not read what the normalisation
+        // writes (the units' partial sums, the records' scale): they are
+        // built beside it, and the deposit waits for both
+        hipStream_t s2 = c->side;
+        if (rings) {
+            S2G_HIP(hipEventRecord(c->fork, s));
+            S2G_HIP(hipStreamWaitEvent(s2, c->fork, 0));
+        }
         S2G_HIP(hipEventRecord(E[2], s));
         launch_hp_norm(a.kernel, units, nu, rec, nside, rtab, part, s);
         launch_hp_finish(nun, n, part, rec, s);
@@ -1743,23 +1751,26 @@ static void healpix_device(s2g_ctx *c, const HpArgs &a, double *const *maps,
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky work units");
             uint32_t *keys = (uint32_t *)c->skeys.get(sizeof(uint32_t) * 2 * ne);
             int32_t *vals = (int32_t *)c->svals.get(sizeof(int32_t) * 2 * ne);
-            launch_hp_entry_keys(units, nu, rec, n, nside, keys, vals, s);
+            launch_hp_entry_keys(units, nu, rec, n, nside, keys, vals, s2);
             const int kbits = hp_entry_key_bits(nside);
             const size_t tb = sort_pairs_tmp_bytes(ne, kbits);
             void *sortmp = c->sort_tmp.get(tb ? tb : 1);
-            sort_pairs(sortmp, tb, keys, keys + ne, vals, vals + ne, ne, kbits, s);
+            sort_pairs(sortmp, tb, keys, keys + ne, vals, vals + ne, ne, kbits, s2);
             int64_t *rb = (int64_t *)c->hp_ring.get(sizeof(int64_t) * 2 * nslots);
             int64_t *eoff = rb, *off = rb + nslots;
-            int64_t *stmp = (int64_t *)c->scan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nslots));
-            launch_hp_entry_offsets(keys + ne, ne, nside, eoff, s);
-            launch_hp_ring_count(keys + ne, eoff, nside, off, s);
-            exclusive_scan_i64(off, nslots, stmp, s);
+            int64_t *stmp =
+                (int64_t *)c->sscan_tmp.get(sizeof(int64_t) * scan_tmp_elems(nslots));
+            launch_hp_entry_offsets(keys + ne, ne, nside, eoff, s2);
+            launch_hp_ring_count(keys + ne, eoff, nside, off, s2);
+            exclusive_scan_i64(off, nslots, stmp, s2);
             int64_t npairs;
-            read_i64(c, s, {off + nslots - 1}, &npairs);
+            read_i64(c, s2, {off + nslots - 1}, &npairs);
             if (npairs >= (int64_t)INT32_MAX)
                 throw Error(S2G_EUNSUPPORTED, "too many all-sky (particle, ring) pairs");
             int32_t *list = (int32_t *)c->hp_list.get(sizeof(int32_t) * (npairs ? npairs : 1));
-            launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, nside, list, s);
+            launch_hp_ring_fill(keys + ne, vals + ne, eoff, off, nside, list, s2);
+            S2G_HIP(hipEventRecord(c->smalled, s2));
+            S2G_HIP(hipStreamWaitEvent(s, c->smalled, 0));
             for (int m0 = 0; m0 < a.nmaps; m0 += 2)
                 launch_hp_ring_deposit(a, group(m0), list, off, rec, s);
         } else {
