@@ -178,6 +178,24 @@ __device__ __forceinline__ bool hp_ring_interval(int64_t nside, int64_t iz, doub
     return true;
 }
 
+// A ring lying wholly inside the disc (polar cap): all nr pixels, taken from
+// the first one at or after phi0 - pi (lo may be negative, as above), so that
+// phi - phi0 stays in [-pi, pi) along the chord and the pixel loops need no
+// wrap of it.  The same set as [0, nr - 1], in another order.
+__device__ __forceinline__ void hp_full_chord(int64_t nr, double shift, double phi0,
+                                              int64_t &lo, int64_t &hi)
+{
+    const double inv2pi = 1.0 / (2.0 * kPi);
+    int64_t a = (int64_t)floor((double)nr * inv2pi * (phi0 - kPi) - shift) + 1;
+    int64_t b = a + nr - 1;
+    if (b >= nr) {
+        a -= nr;
+        b -= nr;
+    }
+    lo = a;
+    hi = b;
+}
+
 // ---------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------
@@ -322,6 +340,20 @@ __device__ __forceinline__ RingTerms ring_terms(int64_t nside, int64_t iz, const
     return t;
 }
 
+// phi_a - phi0 of a chord's first pixel a.  Its true value lies in
+// (-pi, 2 pi / nr] (the chord starts within dphi of phi0 - dphi, dphi <= pi,
+// hp_ring_interval / hp_full_chord), and the index shift of a wrapped chord
+// adds -2 pi or +2 pi; the representative is taken in [-pi - 1e-9, pi / 2),
+// so a start a rounding below -pi keeps its place and the chord's offsets
+// run up to < pi (nr >= 4).
+__device__ __forceinline__ double hp_chord_start(int64_t a, const RingTerms &t)
+{
+    double d0 = (double)a * t.dphi_pix + t.phase;
+    d0 = d0 >= 0.5 * kPi ? d0 - 2.0 * kPi : d0;
+    d0 = d0 < -kPi - 1e-9 ? d0 + 2.0 * kPi : d0;
+    return d0;
+}
+
 // sin(y) for |y| <= pi/2: y (1 + y^2 P(y^2)) with P of degree 3 fitted for
 // the least maximum relative error on the interval (7.6e-9; the FP32
 // evaluation adds ~1e-7, as any FP32 formula does).  One branch-free formula
@@ -442,8 +474,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
             int64_t st, n_r;
             double sh;
             hp_ring_info(nside, iz, st, n_r, sh);
-            a = 0;
-            b = n_r - 1;
+            hp_full_chord(n_r, sh, r.phi0, a, b);
             any = true;
         } else {
             any = hp_ring_interval(nside, iz, r.z0, r.xa, cosr, r.phi0, a, b);
@@ -456,10 +487,7 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
             e.nr = (int)t.nr;
             e.s1 = t.s1;
             e.s2 = t.s2;
-            double d0 = (double)a * t.dphi_pix + t.phase;  // phi_a - phi0
-            d0 = d0 >= kPi ? d0 - 2.0 * kPi : d0;
-            d0 = d0 < -kPi ? d0 + 2.0 * kPi : d0;
-            e.base = (float)d0;
+            e.base = (float)hp_chord_start(a, t);
             e.step = (float)t.dphi_pix;
             if (iz == cring)
                 cov = (a >= 0 && ck >= a && ck <= b) || (a < 0 && (ck <= b || ck >= a + t.nr));
@@ -485,8 +513,9 @@ __device__ __forceinline__ void hp_unit_pixels(int64_t nside, const HpRec &r, in
         const int c = q - cur.off;
         int k = cur.lo + c;
         k = k < 0 ? k + cur.nr : k;
-        float dphi = fmaf((float)c, cur.step, cur.base);
-        dphi = dphi >= (float)kPi ? dphi - (float)(2.0 * kPi) : dphi;
+        // in [-pi, pi] up to FP32 rounding: every chord starts at or after
+        // phi0 - pi and spans less than 2 pi (hp_ring_interval, hp_full_chord)
+        const float dphi = fmaf((float)c, cur.step, cur.base);
         const float d = hp_angle(dphi, cur.s1, cur.s2);
         float W, A;
         hp_weights<K>(d, thh, inv_theta, ang, inv_ang, W, A);
@@ -852,8 +881,7 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
             bool any = false;
             if (iz >= r.ra && iz <= r.rb) {
                 if (iz < r.irmin || iz > r.irmax) {  // a ring fully inside a polar cap
-                    a = 0;
-                    bb = nr - 1;
+                    hp_full_chord(nr, shift, r.phi0, a, bb);
                     any = true;
                 } else {
                     any = hp_ring_interval(nside, iz, r.z0, r.xa, r.cosr, r.phi0, a, bb);
@@ -863,10 +891,7 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
             if (any) {
                 q.lo = (int)a;
                 len = (int)(bb - a + 1);
-                double d0 = (double)a * t.dphi_pix + t.phase;  // phi_a - phi0
-                d0 = d0 >= kPi ? d0 - 2.0 * kPi : d0;
-                d0 = d0 < -kPi ? d0 + 2.0 * kPi : d0;
-                q.base = (float)d0;
+                q.base = (float)hp_chord_start(a, t);
                 q.step = (float)t.dphi_pix;
             }
             q.s1 = t.s1;
@@ -912,9 +937,8 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
             if (c < clen) {
                 k = cur.lo + c;
                 k = k < 0 ? k + (int)nr : k;
-                float dphi = fmaf((float)c, cur.step, cur.base);
-                dphi = dphi >= (float)kPi ? dphi - (float)(2.0 * kPi) : dphi;
-                d = hp_angle(dphi, cur.s1, cur.s2);
+                // in [-pi, pi] up to FP32 rounding, as in hp_unit_pixels
+                d = hp_angle(fmaf((float)c, cur.step, cur.base), cur.s1, cur.s2);
             } else {
                 // the centre pixel, as the norm pass computes it
                 k = cur.ck;
