@@ -157,7 +157,10 @@ typedef struct s2g_tuning {
     int32_t gather_items_per_cu; /* gather work items per CU aimed at; 1..256, default 16 */
     int32_t gather_planes;       /* image planes per gather launch; 1..4, default 4 */
     int32_t small_planes;        /* image planes per small-footprint launch; 1..4, default 4 */
-    int32_t reserved;            /* 0 */
+    int32_t gather_item_max;     /* most pairs per gather work item, so that a large pass
+                                    is cut into more items than gather_items_per_cu asks
+                                    (a shorter tail of the launch); 0 (no bound) or
+                                    gather_item_min..2^30, default 4096 */
 } s2g_tuning;
 
 /* Context: owns the device buffers (grown monotonically) and one blocking

@@ -69,12 +69,14 @@ void launch_plan_check(const int64_t *lbase, int nmaps, const TileMap *tmaps,
                        const int64_t *pair_total, int64_t pair_cap, int *flag, hipStream_t s);
 // fill == false: nitems (ntiles+1, last = total) ; fill == true: items + tile_items.
 // items[i] = (global tile, first pair, count, map).  A tile's list is cut into
-// items of at most G = max(gmin, ceil(*total / target)) entries (G = gmin if
-// total is null); *total is read on the device.
+// items of at most G entries, G = ceil(*total / target) capped at gmax (if
+// gmax > 0), raised to ceil(*total / tmax) (if tmax > 0; then at most
+// ntiles + tmax items) and to gmin; G = gmin if total is null.  *total is
+// read on the device.
 void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles,
                   int64_t gmin, int64_t target, const int64_t *total, int64_t *nitems,
                   int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items, bool fill,
-                  hipStream_t s);
+                  hipStream_t s, int64_t gmax = 0, int64_t tmax = 0);
 
 // index of the last table entry whose key is <= v (keys ascending, key[0] <= v)
 template <typename E, typename K, typename F>

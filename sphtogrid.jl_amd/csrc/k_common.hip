@@ -439,28 +439,39 @@ __device__ __forceinline__ void tile_list(const int64_t *__restrict__ off,
     e = off[tm.cbase + (tl + 1) * tm.nch];
 }
 
-// item size: max(gmin, ceil(total / target)) with the total read on the
-// device (so no host round trip is needed between the count and the items),
-// or gmin when total is null
-__device__ __forceinline__ int64_t item_size(int64_t gmin, int64_t target,
-                                             const int64_t *__restrict__ total)
+// item size with the total read on the device (so no host round trip is
+// needed between the count and the items), or gmin when total is null:
+// ceil(total / target), at most gmax (gmax > 0: a large pass is cut into more
+// items than target, which shortens the launch's tail), at least
+// ceil(total / tmax) (tmax > 0: the items allocated, ntiles + tmax, suffice
+// whatever the total) and at least gmin
+__device__ __forceinline__ int64_t item_size(int64_t gmin, int64_t target, int64_t gmax,
+                                             int64_t tmax, const int64_t *__restrict__ total)
 {
     if (!total)
         return gmin;
-    const int64_t g = (*total + target - 1) / target;
+    const int64_t t = *total;
+    int64_t g = (t + target - 1) / target;
+    if (gmax > 0 && g > gmax)
+        g = gmax;
+    if (tmax > 0) {
+        const int64_t gb = (t + tmax - 1) / tmax;
+        g = g > gb ? g : gb;
+    }
     return g > gmin ? g : gmin;
 }
 
 __global__ void k_item_count(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
-                             int nmaps, int ntiles, int64_t gmin, int64_t target,
-                             const int64_t *__restrict__ total, int64_t *__restrict__ nitems)
+                             int nmaps, int ntiles, int64_t gmin, int64_t target, int64_t gmax,
+                             int64_t tmax, const int64_t *__restrict__ total,
+                             int64_t *__restrict__ nitems)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == ntiles)
         nitems[t] = 0;  // becomes the item total after the scan
     if (t >= ntiles)
         return;
-    const int64_t G = item_size(gmin, target, total);
+    const int64_t G = item_size(gmin, target, gmax, tmax, total);
     int m;
     int64_t s, e;
     tile_list(off, tmaps, nmaps, t, m, s, e);
@@ -468,14 +479,15 @@ __global__ void k_item_count(const int64_t *__restrict__ off, const TileMap *__r
 }
 
 __global__ void k_item_fill(const int64_t *__restrict__ off, const TileMap *__restrict__ tmaps,
-                            int nmaps, int ntiles, int64_t gmin, int64_t target,
-                            const int64_t *__restrict__ total, const int64_t *__restrict__ ibase,
+                            int nmaps, int ntiles, int64_t gmin, int64_t target, int64_t gmax,
+                            int64_t tmax, const int64_t *__restrict__ total,
+                            const int64_t *__restrict__ ibase,
                             int4 *__restrict__ items, int64_t *__restrict__ tile_items)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles)
         return;
-    const int64_t G = item_size(gmin, target, total);
+    const int64_t G = item_size(gmin, target, gmax, tmax, total);
     int m;
     int64_t s, e;
     tile_list(off, tmaps, nmaps, t, m, s, e);
@@ -492,16 +504,16 @@ __global__ void k_item_fill(const int64_t *__restrict__ off, const TileMap *__re
 void launch_items(const int64_t *off, const TileMap *tmaps, int nmaps, int ntiles,
                   int64_t gmin, int64_t target, const int64_t *total, int64_t *nitems,
                   int64_t *scan_tmp, int4 *items_or_null, int64_t *tile_items, bool fill,
-                  hipStream_t s)
+                  hipStream_t s, int64_t gmax, int64_t tmax)
 {
     const unsigned nb = (unsigned)((ntiles + 1 + 255) / 256);
     if (!fill) {
         hipLaunchKernelGGL(k_item_count, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles,
-                           gmin, target, total, nitems);
+                           gmin, target, gmax, tmax, total, nitems);
         exclusive_scan_i64(nitems, ntiles + 1, scan_tmp, s);
     } else {
         hipLaunchKernelGGL(k_item_fill, dim3(nb), dim3(256), 0, s, off, tmaps, nmaps, ntiles,
-                           gmin, target, total, nitems, items_or_null, tile_items);
+                           gmin, target, gmax, tmax, total, nitems, items_or_null, tile_items);
     }
 }
 

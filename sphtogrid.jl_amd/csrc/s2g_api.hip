@@ -25,7 +25,7 @@ using namespace s2g;
 constexpr size_t kScratchBytes = (size_t)80 << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
-constexpr s2g_tuning kDefaultTuning = {kSmallSide2D, 256, 8, 64, 16, 4, 4, 0};
+constexpr s2g_tuning kDefaultTuning = {kSmallSide2D, 256, 8, 64, 16, 4, 4, 4096};
 
 struct s2g_ctx {
     int device = 0;
@@ -172,12 +172,15 @@ struct TilePlan {
 // tm: tile grid of every map of the pass (cbase / nch / csz filled here from
 // the per-map large-list bounds lbase[0..nmaps]; d_lbase: the same on the
 // device)
-// Work items of <= G pairs, G = max(gmin, pairs / (ncu * per_cu)): enough
-// items to fill every CU several times.
+// Work items of <= G pairs, G = pairs / (ncu * per_cu) capped at gmax (> 0)
+// and at least gmin: enough items to fill every CU several times, and for a
+// large pass items of at most gmax pairs, so that the launch's tail is short
+// (2^24 cluster set on 4096^2: 16 items per CU, 64k pairs each, gather 272.6
+// ms; 4096-pair items, about 260 per CU, 246.2 ms).
 TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, bool rec3,
                           std::vector<TileMap> &tm, const std::vector<int64_t> &lbase,
                           const int64_t *d_lbase, int ntiles, hipStream_t s, double *ms_sort,
-                          int64_t gmin = 64, int64_t per_cu = 16)
+                          int64_t gmin = 64, int64_t per_cu = 16, int64_t gmax = 0)
 {
     hipEvent_t *E = c->stage;
     TilePlan tp;
@@ -237,7 +240,7 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
     const int64_t *d_npairs = cnt + ncnt - 1;
     int64_t *nit = (int64_t *)c->nitems.get(sizeof(int64_t) * (ntiles + 1));
     launch_items(cnt, d_tm, nmaps, ntiles, gmin, target, d_npairs, nit, tmp, nullptr, nullptr,
-                 false, s);
+                 false, s, gmax);
     S2G_HIP(hipEventRecord(E[5], s));
     int64_t tot[2];
     read_i64(c, s, {d_npairs, nit + ntiles}, tot);
@@ -256,7 +259,7 @@ TilePlan build_tile_lists(s2g_ctx *c, const int32_t *large, const void *rec, boo
         launch_pairs<Rec2>(large, (const Rec2 *)rec, d_ch, nchunks, d_tm, max_tiles, cnt, ncnt,
                            tmp, tp.pairs, true, s, d_lbase);
     launch_items(cnt, d_tm, nmaps, ntiles, gmin, target, d_npairs, nit, tmp, tp.items,
-                 tp.tile_items, true, s);
+                 tp.tile_items, true, s, gmax);
     S2G_HIP(hipEventRecord(E[7], s));
     S2G_HIP(hipEventSynchronize(E[7]));
     *ms_sort = elapsed(E[4], E[5]) + elapsed(E[6], E[7]);
@@ -502,7 +505,8 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     TilePlan tp;
     if (cl.nl > 0) {
         tp = build_tile_lists(c, cl.l1, rec, false, tm, cl.lbase, cl.d_lbase, ntiles, s,
-                              &ms_sort, c->tune.gather_item_min, c->tune.gather_items_per_cu);
+                              &ms_sort, c->tune.gather_item_min, c->tune.gather_items_per_cu,
+                              c->tune.gather_item_max);
         if (st)
             st->n_pairs = tp.npairs;
     }
@@ -1004,7 +1008,7 @@ struct PlanPass {
     int64_t ntot = 0, nb = 0, ncnt = 0, sncnt = 0, pair_cap = 0, item_cap = 0, sitem_cap = 0;
     int ntiles = 0, nstiles = 0, max_tiles = 1, smax_tiles = 1, nchunks = 0, nschunks = 0;
     int gstep = 4, sstep = 4;
-    int64_t gmin = 64, gtarget = 1, smin = 256, starget = 1;
+    int64_t gmin = 64, gtarget = 1, gmax = 0, smin = 256, starget = 1;
     bool any_out = false;
     DevBuf tab, img, rec, cls, counts, scan_tmp, l0, l1, l2, bounds, tiles, cnt, pairs, nitems,
         items, tile_items, partial, stiles, scnt, sorted, snitems, sitems, stile, flag, sscan_tmp;
@@ -1117,12 +1121,13 @@ void plan_enqueue(s2g_plan *pl, PlanPass &pp, hipStream_t s, hipStream_t s2)
     launch_pairs<Rec2>(l1, rec, pp.d_ch, pp.nchunks, pp.d_tm, pp.max_tiles, cnt, pp.ncnt, tmp,
                        nullptr, false, s, lbase);
     launch_items(cnt, pp.d_tm, pp.nmaps, pp.ntiles, pp.gmin, pp.gtarget, d_npairs, nit, tmp,
-                 nullptr, nullptr, false, s);
+                 nullptr, nullptr, false, s, pp.gmax, pp.item_cap - pp.ntiles - 1);
     launch_plan_check(lbase, pp.nmaps, pp.d_tm, d_npairs, pp.pair_cap, (int *)pp.flag.p, s);
     launch_pairs<Rec2>(l1, rec, pp.d_ch, pp.nchunks, pp.d_tm, pp.max_tiles, cnt, pp.ncnt, tmp,
                        (int32_t *)pp.pairs.p, true, s, lbase, pp.pair_cap);
     launch_items(cnt, pp.d_tm, pp.nmaps, pp.ntiles, pp.gmin, pp.gtarget, d_npairs, nit, tmp,
-                 (int4 *)pp.items.p, (int64_t *)pp.tile_items.p, true, s);
+                 (int4 *)pp.items.p, (int64_t *)pp.tile_items.p, true, s, pp.gmax,
+                 pp.item_cap - pp.ntiles - 1);
     S2G_HIP(hipStreamWaitEvent(s, pl->normed, 0));
     for (int plane0 = 0; plane0 < pp.nplanes; plane0 += pp.gstep) {
         const int np = pp.nplanes - plane0 < pp.gstep ? pp.nplanes - plane0 : pp.gstep;
@@ -1288,8 +1293,16 @@ void plan_pass_build(s2g_ctx *c, PlanPass &pp, double headroom, hipStream_t s)
     pp.pairs.get(sizeof(int32_t) * pp.pair_cap);
     pp.gmin = c->tune.gather_item_min;
     pp.gtarget = (int64_t)c->ncu * c->tune.gather_items_per_cu;
-    // items <= sum over tiles of (count / G + 1) <= target + ntiles
-    pp.item_cap = pp.ntiles + pp.gtarget + 1;
+    pp.gmax = c->tune.gather_item_max;
+    // items <= sum over tiles of (count / G + 1) <= total / G + ntiles; the
+    // budget B = max(target, pair_cap / gmax) bounds total / G at B whatever
+    // the replay's total (G >= total / B, item_size)
+    int64_t budget = pp.gtarget;
+    if (pp.gmax > 0) {
+        const int64_t bg = (pp.pair_cap + pp.gmax - 1) / pp.gmax;
+        budget = bg > budget ? bg : budget;
+    }
+    pp.item_cap = pp.ntiles + budget + 1;
     pp.nitems.get(sizeof(int64_t) * (pp.ntiles + 1));
     pp.items.get(sizeof(int4) * pp.item_cap);
     pp.tile_items.get(sizeof(int64_t) * 2 * pp.ntiles);
@@ -1426,8 +1439,10 @@ int s2g_set_tuning(s2g_ctx *c, const s2g_tuning *t)
                                     "[8, 4096] / [1, 256]");
         if (!in(t->gather_planes, 1, 4) || !in(t->small_planes, 1, 4))
             throw Error(S2G_EINVAL, "gather_planes / small_planes out of range [1, 4]");
-        if (t->reserved != 0)
-            throw Error(S2G_EINVAL, "reserved must be 0");
+        if (t->gather_item_max != 0 &&
+            !in(t->gather_item_max, t->gather_item_min, (int32_t)1 << 30))
+            throw Error(S2G_EINVAL, "gather_item_max out of range (0, or gather_item_min "
+                                    "to 2^30)");
         c->tune = *t;
         return S2G_OK;
     });

@@ -74,16 +74,16 @@ class Tuning(C.Structure):
     _fields_ = [("small_side", C.c_int32), ("small_item_min", C.c_int32),
                 ("small_items_per_cu", C.c_int32), ("gather_item_min", C.c_int32),
                 ("gather_items_per_cu", C.c_int32), ("gather_planes", C.c_int32),
-                ("small_planes", C.c_int32), ("reserved", C.c_int32)]
+                ("small_planes", C.c_int32), ("gather_item_max", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # the library's defaults (kDefaultTuning in csrc/s2g_api.hip)
 DEFAULT_TUNING = dict(small_side=34, small_item_min=256, small_items_per_cu=8,
                       gather_item_min=64, gather_items_per_cu=16, gather_planes=4,
-                      small_planes=4)
+                      small_planes=4, gather_item_max=4096)
 
 
 class MapDesc(C.Structure):

@@ -40,6 +40,7 @@ SPACE = {
     "gather_items_per_cu": [8, 16, 32, 64],
     "gather_planes": [2, 4],
     "small_planes": [2, 4],
+    "gather_item_max": [0, 1024, 4096, 16384],
 }
 
 

@@ -86,4 +86,4 @@ def test_defaults_and_layout_match_the_library():
     src = open(os.path.join(ROOT, "sphtogrid.jl_amd", "csrc", "s2g_api.hip")).read()
     m = re.search(r"kDefaultTuning = \{kSmallSide2D, ([\d, ]+)\};", src)
     vals = [34] + [int(v) for v in m.group(1).split(",")]
-    assert dict(zip(fields, vals)) == dict(_lib.DEFAULT_TUNING, reserved=0)
+    assert dict(zip(fields, vals)) == _lib.DEFAULT_TUNING

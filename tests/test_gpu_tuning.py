@@ -17,6 +17,8 @@ SETTINGS = [
          gather_items_per_cu=256),
     dict(gather_planes=1, small_planes=1),
     dict(gather_planes=2, small_planes=3, small_side=24, small_item_min=4096),
+    dict(gather_item_max=0),  # no cap: items from gather_items_per_cu alone
+    dict(gather_item_min=8, gather_item_max=16, gather_items_per_cu=4),
 ]
 
 
@@ -67,11 +69,12 @@ def test_settings_match_default_and_oracle(s2g, orc, ctx, setting):
 def test_set_tuning_rejects_out_of_range(s2g, ctx):
     from sph2grid import _lib
     for bad in (dict(small_side=35), dict(small_side=4), dict(small_item_min=8),
-                dict(gather_planes=5), dict(small_planes=0), dict(gather_items_per_cu=0)):
+                dict(gather_planes=5), dict(small_planes=0), dict(gather_items_per_cu=0),
+                dict(gather_item_max=32), dict(gather_item_max=-1)):
         with pytest.raises(_lib.LibraryError, match="out of range"):
             ctx.set_tuning(bad)
     t = _lib.Tuning(**_lib.DEFAULT_TUNING)
-    t.reserved = 1
+    t.gather_item_max = (1 << 30) + 1
     rc = ctx.lib.s2g_set_tuning(ctx.handle, __import__("ctypes").byref(t))
     assert rc == _lib.S2G_EINVAL
     assert ctx.get_tuning() == _lib.DEFAULT_TUNING  # unchanged by the rejected calls
