@@ -649,8 +649,12 @@ void deposit3d_device(s2g_ctx *c, const Map3DArgs &a, double *d_image, hipStream
     int64_t launches = 0;
     if (cl.nl > 0) {
         std::vector<TileMap> tm{TileMap{0, 0, ntiles, nb, nb, 0, 0}};
+        // brick items of at most 4096 pairs (2^24 cluster set on 256^3: 16
+        // items per CU of about 20k pairs, gather 69.0 ms; capped at 1024 /
+        // 4096 / 8192 pairs: 60.2 / 60.9 / 63.9 ms, 1024 with a longer
+        // finalize)
         const TilePlan tp = build_tile_lists(c, cl.l1, rec, true, tm, cl.lbase, cl.d_lbase,
-                                             ntiles, s, &ms_sort);
+                                             ntiles, s, &ms_sort, 64, 16, 4096);
         if (st)
             st->n_pairs = tp.npairs;
         Gather3Args g;
