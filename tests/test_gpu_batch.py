@@ -239,3 +239,69 @@ def test_inputs_written_on_the_null_stream_just_before(s2g, orc):
         e = rel_l1(got, want)
         print(f"rep {rep}: device entry point rel_l1 vs oracle {e:.2e}", flush=True)
         assert e <= 1e-5
+
+
+def test_inputs_written_on_a_caller_stream_just_before(s2g, orc):
+    """Ordering with a caller's own (non-blocking) stream: torch queues a long
+    run of work and then the writes of the inputs on a torch.cuda.Stream, and
+    the deposits are enqueued on that same stream with no host
+    synchronisation in between.  The deposits fork their side stream (the
+    normalisation and the small-footprint scatter) from the caller's stream
+    and join it back, so every stage must see the written inputs: the 2D
+    deposit + reduce and the 3D deposit must give the oracle's result on
+    three repeats (the 3D one against the inputs' 2D oracle's total weight:
+    both conserve it)."""
+    import torch
+    from sph2grid import _lib
+    dev = torch.device("cuda:0")
+    c = _calls(s2g, np.float32)[4]
+    npix, nq = 200, 3
+    pm_args = [c[k] for k in ("Pos", "HSML", "M", "Rho", "Bin_Quant", "Weights")]
+    from sph2grid.mapping import _prepare
+    pm = _prepare(*[a.copy() for a in pm_args], param=c["param"], kernel=c["kernel"],
+                  show_progress=False, reduce_image=True, calc_mean=True)
+    src = [torch.from_numpy(a).to(dev) for a in pm.arrays]
+    want = _oracle_result(orc, c)
+    ctx = s2g.default_context()
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    user = torch.cuda.Stream(device=dev)  # default priority, non-blocking
+    big = torch.randn(3072, 3072, device=dev)
+    npix3 = 48
+    par3 = _lib.Params(npixels=npix3, len2pix=float(npix3))
+    for rep in range(3):
+        dst = [torch.zeros_like(a) for a in src]  # wrong values first
+        img = torch.empty(npix * npix * (nq + 1), dtype=torch.float64, device=dev)
+        red = torch.empty(npix * npix * nq, dtype=torch.float64, device=dev)
+        img3 = torch.empty(2 * npix3 ** 3, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(user):
+            for _ in range(6):
+                big = (big @ big).mul_(1e-3)  # keeps the caller's stream busy
+            for d, a in zip(dst, src):
+                d.copy_(a)  # queued behind the matmuls, on the caller's stream
+            sp = C.c_void_p(user.cuda_stream)
+            ctx.check(ctx.lib.s2g_deposit_device(
+                ctx.handle, 2, _lib.S2G_F32, vp(dst[0]), vp(dst[1]), vp(dst[2]), vp(dst[3]),
+                vp(dst[4]), nq, vp(dst[5]), pm.n, C.byref(pm.params), pm.kid, 2,
+                _lib.S2G_FLAG_CALC_MEAN, vp(img), sp, None), "s2g_deposit_device")
+            ctx.check(ctx.lib.s2g_reduce_device(ctx.handle, 2, npix, nq, _lib.S2G_FLAG_REDUCE,
+                                                vp(img), vp(red), sp), "s2g_reduce_device")
+            # 3D deposit of the same particles (first quantity), same stream
+            q0 = dst[4].view(-1, nq)[:, 0].contiguous()
+            ctx.check(ctx.lib.s2g_deposit_device(
+                ctx.handle, 3, _lib.S2G_F32, vp(dst[0]), vp(dst[1]), vp(dst[2]), vp(dst[3]),
+                vp(q0), 1, vp(dst[5]), pm.n, C.byref(par3), pm.kid, 3, 0, vp(img3), sp, None),
+                "s2g_deposit_device 3D")
+        user.synchronize()
+        got = red.cpu().numpy().reshape((npix, npix, nq), order="F")
+        e = rel_l1(got, want)
+        # 3D: the weight plane's total is sum(w m / rho) len2pix^3 * len2pix over the
+        # particles whose footprint meets the grid (all of them here: centres inside)
+        x = dst[0].double().view(-1, 3).cpu().numpy()
+        w, m, rho = (t.double().cpu().numpy() for t in (dst[5], dst[2], dst[3]))
+        want3 = float((w * m / rho).sum()) * npix3 ** 4
+        w3 = float(img3.view(2, -1)[1].sum().item())
+        print(f"rep {rep}: 2D rel_l1 vs oracle {e:.2e}; 3D weight total rel err "
+              f"{abs(w3 / want3 - 1):.2e} ({x.shape[0]} particles)", flush=True)
+        assert e <= 1e-5
+        assert abs(w3 / want3 - 1) <= 1e-6
