@@ -452,8 +452,8 @@ void deposit2d_pass(s2g_ctx *c, const MapIn *mi, int nmaps, hipStream_t s, s2g_s
     // for the normalisation (it reads the records' scale); the small scatter
     // adds into images the side stream zeroed first, and the first finalize
     // waits for it and adds the large particles' sums (2^24 cluster set on
-    // 4096^2: the 5.7 ms normalisation and 6.4 ms small scatter leave the
-    // step's critical path).
+    // 4096^2: the 5.7 ms normalisation runs beside the 5.7 ms pair lists and
+    // the 6.4 ms small scatter beside the gather; 288 -> 284 ms per step).
     hipStream_t s2 = c->side;
     S2G_HIP(hipEventRecord(c->fork, s));
     S2G_HIP(hipStreamWaitEvent(s2, c->fork, 0));
