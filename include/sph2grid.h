@@ -151,7 +151,7 @@ typedef struct s2g_tuning {
     int32_t small_item_min;      /* least particles per small-footprint work item;
                                     16..4096, default 256 */
     int32_t small_items_per_cu;  /* small-footprint work items per CU aimed at;
-                                    1..64, default 8 */
+                                    1..64, default 16 */
     int32_t gather_item_min;     /* least (tile, particle) pairs per gather work item;
                                     8..4096, default 64 */
     int32_t gather_items_per_cu; /* gather work items per CU aimed at; 1..256, default 16 */

@@ -76,7 +76,7 @@ struct S2GTuning
     small_planes::Int32
     gather_item_max::Int32
 end
-S2GTuning(; small_side=34, small_item_min=256, small_items_per_cu=8, gather_item_min=64,
+S2GTuning(; small_side=34, small_item_min=256, small_items_per_cu=16, gather_item_min=64,
           gather_items_per_cu=16, gather_planes=4, small_planes=4, gather_item_max=4096) =
     S2GTuning(small_side, small_item_min, small_items_per_cu, gather_item_min,
               gather_items_per_cu, gather_planes, small_planes, gather_item_max)

@@ -81,7 +81,7 @@ class Tuning(C.Structure):
 
 
 # the library's defaults (kDefaultTuning in csrc/s2g_api.hip)
-DEFAULT_TUNING = dict(small_side=34, small_item_min=256, small_items_per_cu=8,
+DEFAULT_TUNING = dict(small_side=34, small_item_min=256, small_items_per_cu=16,
                       gather_item_min=64, gather_items_per_cu=16, gather_planes=4,
                       small_planes=4, gather_item_max=4096)
 
