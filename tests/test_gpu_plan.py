@@ -150,3 +150,32 @@ def test_plan_with_several_passes_and_dtypes(s2g, orc):
                 got = p["out"].cpu().numpy().reshape((-1, specs[k][1] + 1), order="F")
                 for q in range(got.shape[1]):
                     assert rel_l1(got[:, q], rf[:, q]) <= 1e-5, (rep, k, q)
+
+
+def test_plan_item_budget_with_capped_items(s2g):
+    """A plan built with tiny gather items (gather_item_max 64, one item per
+    CU aimed at, so the items come from the cap): its item allocation is the
+    budget pair_cap / 64, and a replay whose inputs grew within the plan's
+    headroom (hsml 5 % larger: more pairs than at creation) must stay within
+    it and match the eager batch run with the same split."""
+    ctx = s2g.default_context()
+    maps = _device_maps(s2g, nmaps=3, scale=1.0 / 800)
+    try:
+        ctx.set_tuning(dict(gather_item_min=8, gather_item_max=64, gather_items_per_cu=1))
+        st = {}
+        with s2g.BatchPlan(_descs(maps, "o_plan"), dtype="f32", headroom=1.5) as plan:
+            plan.run(stats=st)
+            pairs0 = st["n_pairs"]
+            for m in maps:
+                m["t"][1].mul_(1.05)
+            eager = _descs(maps, "o_eager")
+            for rep in range(2):
+                s2g.deposit_batch_device(eager, dtype="f32")
+                plan.run(stats=st)
+                errs = [_max_rel(m["o_plan"], m["o_eager"]) for m in maps]
+                print(f"replay {rep}: pairs {pairs0} -> {st['n_pairs']}, max rel diff "
+                      f"{['%.1e' % e for e in errs]}", flush=True)
+                assert st["n_pairs"] > pairs0
+                assert max(errs) <= 1e-12
+    finally:
+        ctx.set_tuning(None)
