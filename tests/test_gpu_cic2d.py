@@ -431,3 +431,38 @@ def test_fp32_accumulation(s2g, orc, nq):
     _check_flat(got, ref, f"fp32acc/nq{nq}")
     for k in range(nq + 1):
         assert rel_l1(got[:, k], f64[:, k]) <= 1e-5
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_particle_order_does_not_change_the_map(s2g, dims):
+    """A deposit is a sum over particles: the cluster-like set in its
+    generated order, sorted in space (by the 64-pixel tile, or 16-voxel
+    brick, of the centre: the order of a Peano-Hilbert-sorted snapshot, which
+    the tile lists' stride order exists for) and randomly permuted must give
+    the same image up to the order of the FP64 sums (1e-12 relative)."""
+    from sph2grid import workloads
+    npix = 512 if dims == 2 else 96
+    d = workloads.cluster(n=1 << 16, npix=npix)
+    par = s2g.mappingParameters(x_lim=d["lims"][0], y_lim=d["lims"][1], z_lim=d["lims"][2],
+                                Npixels=npix)
+    pos = d["Pos"]
+    n = pos.shape[1]
+    T = 64 if dims == 2 else 16
+    t = np.clip(((pos - np.array(d["lims"])[:, :1]) /
+                 (np.array(d["lims"])[:, 1:] - np.array(d["lims"])[:, :1]) * npix // T),
+                0, npix // T - 1).astype(np.int64)
+    key = (t[0] * (npix // T) + t[1]) * (npix // T) + t[2]
+    orders = {"generated": np.arange(n), "sorted": np.argsort(key, kind="stable"),
+              "permuted": np.random.default_rng(3).permutation(n)}
+    q = d["Bin_Quant"]
+    imgs = {}
+    for name, o in orders.items():
+        imgs[name] = s2g.sphMapping(pos[:, o].copy(), d["HSML"][o], d["M"][o], d["Rho"][o],
+                                    q[o], d["Weights"][o], param=par,
+                                    kernel=s2g.WendlandC6(dims), show_progress=False,
+                                    return_both_maps=True, dimensions=dims)
+    ref = imgs["generated"]
+    for name in ("sorted", "permuted"):
+        e = max_rel(imgs[name], ref)
+        print(f"{dims}D {name} vs generated: max rel diff {e:.2e}", flush=True)
+        assert e <= 1e-12, name
