@@ -23,8 +23,12 @@
 //                     the ball of radius T0;
 //                  3. every particle in the cube at distance <= T goes to a
 //                     per-wave LDS buffer (T shrinks as the buffer is cut back
-//                     to the k nearest); the exact k-th smallest distance is
-//                     an 8-bit radix select on the FP64 bit patterns;
+//                     to the k nearest), the cells within T0/2 first: the
+//                     k-th nearest of those gives T1, and of the other cells
+//                     only those within T1 are read; the search works on
+//                     squared distances, and the exact k-th smallest is a
+//                     radix select on their FP64 bit patterns (its square
+//                     root is hsml);
 //                  4. the kernel sum over the buffer and the bias correction
 //                     of Dehnen & Aly (2012, eqs. 18-19).
 // The cube holds the whole ball of radius T0 and T0 is an upper bound, so the
@@ -164,12 +168,16 @@ __device__ __forceinline__ double md_comp(double a, double b, double box, int pe
     return x;
 }
 
-__device__ __forceinline__ double md_dist(const double4 &a, const double4 &b, const MdGrid &g)
+// squared distance, rounded as the reference rounds it before the square
+// root (no contraction).  The search selects on these: the k-th smallest
+// squared distance is the square of the k-th smallest distance, because a
+// correctly rounded sqrt is monotone, so sqrt of it is the reference's hsml.
+__device__ __forceinline__ double md_dist2(const double4 &a, const double4 &b, const MdGrid &g)
 {
     const double dx = md_comp(a.x, b.x, g.box[0], g.periodic);
     const double dy = md_comp(a.y, b.y, g.box[1], g.periodic);
     const double dz = md_comp(a.z, b.z, g.box[2], g.periodic);
-    return sqrt(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)));
+    return add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz));
 }
 
 // exclusive prefix sum over the lanes of a wave
@@ -184,20 +192,82 @@ __device__ __forceinline__ int wave_excl_scan(int v, int ln)
     return x - v;
 }
 
-// k-th smallest (1-based) of the non-negative doubles buf[0..cnt), exactly:
-// most-significant-byte-first radix select on the bit patterns
+// value of another lane of the same 16-lane row (DPP lane pattern)
+template <int kCtrl>
+__device__ __forceinline__ double dpp_get(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// minimum and maximum over the 64 lanes of a fully active wave, on every lane
+__device__ __forceinline__ void wave_min_max(double &mn, double &mx)
+{
+    mn = fmin(mn, dpp_get<0xB1>(mn));  // quad_perm [1, 0, 3, 2]
+    mx = fmax(mx, dpp_get<0xB1>(mx));
+    mn = fmin(mn, dpp_get<0x4E>(mn));  // quad_perm [2, 3, 0, 1]
+    mx = fmax(mx, dpp_get<0x4E>(mx));
+    mn = fmin(mn, dpp_get<0x141>(mn));  // row_half_mirror
+    mx = fmax(mx, dpp_get<0x141>(mx));
+    mn = fmin(mn, dpp_get<0x140>(mn));  // row_mirror
+    mx = fmax(mx, dpp_get<0x140>(mx));
+    double a = mn, b = mx;
+#pragma unroll
+    for (int r = 16; r < kWave; r += 16) {
+        a = fmin(a, __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mn), r),
+                                     __builtin_amdgcn_readlane(__double2loint(mn), r)));
+        b = fmax(b, __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), r),
+                                     __builtin_amdgcn_readlane(__double2loint(mx), r)));
+    }
+    mn = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a), 0),
+                          __builtin_amdgcn_readlane(__double2loint(a), 0));
+    mx = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(b), 0),
+                          __builtin_amdgcn_readlane(__double2loint(b), 0));
+}
+
+// k-th smallest (1-based) of the doubles buf[0..cnt) (each +0 or positive),
+// exactly: most-significant-bits-first radix select on the bit patterns.
+// Zeros (the particle itself, coincident particles) are counted apart; the
+// bits above the highest one in which the smallest and largest positive
+// entries differ are common to all of them, so the first 8-bit pass starts
+// there and spreads the entries over many bins (a pass from the top byte
+// sends them all to one bin, and its LDS atomics serialise).  Once the chosen
+// bin holds a single entry, that entry is the k-th: one scan finds it.
 __device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln)
 {
-    uint64_t prefix = 0;
-    for (int shift = 56; shift >= 0; shift -= 8) {
+    double mn = __builtin_inf(), mx = 0.0;
+    int nz = 0;  // uniform
+    for (int e0 = 0; e0 < cnt; e0 += kWave) {
+        const int e = e0 + ln;
+        const double v = e < cnt ? buf[e] : -1.0;
+        nz += (int)__popcll(__ballot(v == 0.0));
+        if (v > 0.0) {
+            mn = fmin(mn, v);
+            mx = fmax(mx, v);
+        }
+    }
+    if (k <= nz)
+        return 0.0;
+    k -= nz;
+    wave_min_max(mn, mx);
+    const uint64_t lo = (uint64_t)__double_as_longlong(mn);
+    const uint64_t hi = (uint64_t)__double_as_longlong(mx);
+    if (lo == hi)
+        return mn;
+    int top = 64 - __clzll((long long)(lo ^ hi));  // bits from `top` up are common (top < 64)
+    uint64_t prefix = lo >> top << top;
+    for (;;) {
+        const int w = top < 8 ? top : 8;
+        const int shift = top - w;
+        const uint64_t mask = (1ull << w) - 1;
         for (int b = ln; b < 256; b += kWave)
             hist[b] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         for (int e = ln; e < cnt; e += kWave) {
             const uint64_t key = (uint64_t)__double_as_longlong(buf[e]);
-            const bool match = shift == 56 || (key >> (shift + 8)) == (prefix >> (shift + 8));
-            if (match)
-                atomicAdd(&hist[(key >> shift) & 255u], 1);
+            if (key != 0 && (key >> top) == (prefix >> top))
+                atomicAdd(&hist[(key >> shift) & mask], 1);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // lane l owns bins 4l .. 4l+3
@@ -210,13 +280,14 @@ __device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln
         const bool mine = before < k && k <= before + loc;
         const uint64_t owner = __ballot(mine);
         const int src = __ffsll((unsigned long long)owner) - 1;
-        int bin = 0, below = 0;
+        int bin = 0, below = 0, nbin = 0;
         if (mine) {
             int acc = before;
             for (int j = 0; j < 4; ++j) {
                 if (k <= acc + c4[j]) {
                     bin = 4 * ln + j;
                     below = acc;
+                    nbin = c4[j];
                     break;
                 }
                 acc += c4[j];
@@ -224,11 +295,25 @@ __device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln
         }
         bin = __shfl(bin, src, kWave);
         below = __shfl(below, src, kWave);
+        nbin = __shfl(nbin, src, kWave);
         k -= below;
         prefix |= (uint64_t)bin << shift;
+        top = shift;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (top == 0)
+            return __longlong_as_double((long long)prefix);
+        if (nbin == 1) {
+            // the one positive entry whose bits from `top` up equal the prefix's
+            for (int e0 = 0; e0 < cnt; e0 += kWave) {
+                const int e = e0 + ln;
+                const uint64_t key = e < cnt ? (uint64_t)__double_as_longlong(buf[e]) : 0;
+                const uint64_t bh = __ballot(key != 0 && (key >> top) == (prefix >> top));
+                if (bh)  // wave-uniform
+                    return __shfl(__longlong_as_double((long long)key),
+                                  __ffsll((unsigned long long)bh) - 1, kWave);
+            }
+        }
     }
-    return __longlong_as_double((long long)prefix);
 }
 
 // keep the entries below T and, of those equal to T, as many as make k in
@@ -267,8 +352,9 @@ __device__ int md_compact(double *buf, int *jb, int cnt, double T, int k, int ln
     return out;
 }
 
-// Append the candidates (lane's j, distance d, valid flag) with d <= T to the
-// buffer; when it is about to overflow, cut it back to the k nearest.
+// Append the candidates (lane's j, squared distance d, valid flag) with
+// d <= T to the buffer; when it is about to overflow, cut it back to the k
+// nearest.
 __device__ __forceinline__ void md_offer(bool valid, double d, int j, double &T, int &cnt,
                                          double *buf, int *jb, int *hist, int k1, int ln,
                                          uint64_t below)
@@ -312,19 +398,19 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
         const double4 me = S[q];
         // 1. first bound: the k-th nearest of the particles next in key order
         int cnt = 0;
-        double T = 1e300;
+        double T = 1e300;  // squared distances from here on
         {
             const int64_t a = q - k1 > 0 ? q - k1 : 0;
             const int64_t b = q + k1 < n - 1 ? q + k1 : n - 1;
             for (int64_t t0 = a; t0 <= b; t0 += kWave) {
                 const int64_t t = t0 + ln;
                 const bool valid = t <= b;
-                const double d = valid ? md_dist(me, S[t], g) : 0.0;
+                const double d = valid ? md_dist2(me, S[t], g) : 0.0;
                 md_offer(valid, d, (int)t, T, cnt, buf, jb, hist, k1, ln, below);
             }
             T = md_select(buf, cnt, k1, hist, ln);
         }
-        const double T0 = T;
+        const double T20 = T, T0 = sqrt(T20);
         // 2. level with cells in [T0/2, T0) along the narrowest axis; per
         //    axis the cells that meet [x - T0, x + T0], found with the
         //    particles' own quantisation widened by one quantum: the span is
@@ -361,87 +447,126 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
         // quanta per axis) hold no candidate
         const double T0m = T0 * (1.0 + 1e-12);
         const int ncube = cl[0] * cl[1] * cl[2];
-        // 3. the cube's particles; cells are ranges of the sorted order
-        int C = 0;  // candidates so far (uniform)
-        for (int t0 = 0; t0 < ncube; t0 += kWave) {
-            const int t = t0 + ln;
-            int s = 0, e = 0;
-            bool near = false;
-            uint32_t cc[3];
-            if (t < ncube) {
-                int r = t;
-                double dd = 0.0;
-                for (int d = 0; d < 3; ++d) {
-                    int o = ca[d] + r % cl[d];
-                    r /= cl[d];
-                    if (cl[d] < (int)ncl || !g.periodic) {
-                        // unwrapped cell o spans [o, o + 1) * cw around x
-                        const double cw = g.ext[d] / (double)ncl;
-                        const double eps = 2.0 * g.ext[d] / (double)(1u << kMdBits);
-                        const double xr = mv[d] - g.lo[d];
-                        const double a = (double)o * cw - xr - eps;
-                        const double b = xr - (double)(o + 1) * cw - eps;
-                        const double dx = a > 0.0 ? a : (b > 0.0 ? b : 0.0);
-                        dd += dx * dx;
+        // 3. the cube's particles; cells are ranges of the sorted order.  The
+        //    cells whose nearest point lies within T0/2 go first: the k-th
+        //    nearest of their particles is a bound T1 close to the true k-th
+        //    distance (T0 is often half as large again), and of the other
+        //    cells only those within T1 are read
+        // per axis: unwrapped cell o spans [o, o + 1) * cw around x; the
+        // distance to it is taken eps = two quanta short
+        double cw[3], xlo[3], xhi[3];
+        float rcl[3];
+        for (int d = 0; d < 3; ++d) {
+            const double eps = 2.0 * g.ext[d] / (double)(1u << kMdBits);
+            const double xr = mv[d] - g.lo[d];
+            cw[d] = g.ext[d] / (double)ncl;
+            xlo[d] = xr + eps;
+            xhi[d] = xr - eps;
+            rcl[d] = 1.0f / (float)cl[d];
+        }
+        auto gather_cells = [&](double lo2, double hi2) -> int {
+            // the cells with lo2 < (nearest distance)^2 <= hi2 into
+            // cstart/cpre; returns their particle count (uniform)
+            int C = 0;
+            for (int t0 = 0; t0 < ncube; t0 += kWave) {
+                const int t = t0 + ln;
+                int s = 0, e = 0;
+                bool near = false;
+                uint32_t cc[3];
+                if (t < ncube) {
+                    int r = t;
+                    double dd = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        // r = q cl + rem with r < 216, cl <= 6: the float
+                        // quotient of r + 1/2 is at least 1/12 from an integer
+                        const int qd = d < 2 ? (int)(((float)r + 0.5f) * rcl[d]) : 0;
+                        int o = ca[d] + (r - qd * cl[d]);
+                        r = qd;
+                        if (cl[d] < (int)ncl || !g.periodic) {
+                            const double a = (double)o * cw[d] - xlo[d];
+                            const double b = xhi[d] - (double)(o + 1) * cw[d];
+                            const double dx = a > 0.0 ? a : (b > 0.0 ? b : 0.0);
+                            dd += dx * dx;
+                        }
+                        o = o < 0 ? o + (int)ncl : (o >= (int)ncl ? o - (int)ncl : o);
+                        cc[d] = (uint32_t)o;
                     }
-                    o = o < 0 ? o + (int)ncl : (o >= (int)ncl ? o - (int)ncl : o);
-                    cc[d] = (uint32_t)o;
+                    near = dd > lo2 && dd <= hi2;
                 }
-                near = dd <= T0m * T0m;
-            }
-            if (near) {
-                const uint64_t pre = md_morton(cc[0], cc[1], cc[2]);  // 3*lev bits
-                if (lev <= lt) {
-                    s = tab[pre << (3 * (lt - lev))];
-                    e = tab[(pre + 1) << (3 * (lt - lev))];
-                } else {
-                    const uint64_t par = pre >> (3 * (lev - lt));
-                    const int64_t ps = tab[par], pe = tab[par + 1];
-                    const int ksh = 3 * (kMdBits - lev);
-                    s = (int)md_lower_bound(SK, ps, pe, pre << ksh);
-                    e = (int)md_lower_bound(SK, s, pe, (pre + 1) << ksh);
+                if (near) {
+                    const uint64_t pre = md_morton(cc[0], cc[1], cc[2]);  // 3*lev bits
+                    if (lev <= lt) {
+                        s = tab[pre << (3 * (lt - lev))];
+                        e = tab[(pre + 1) << (3 * (lt - lev))];
+                    } else {
+                        const uint64_t par = pre >> (3 * (lev - lt));
+                        const int64_t ps = tab[par], pe = tab[par + 1];
+                        const int ksh = 3 * (kMdBits - lev);
+                        s = (int)md_lower_bound(SK, ps, pe, pre << ksh);
+                        e = (int)md_lower_bound(SK, s, pe, (pre + 1) << ksh);
+                    }
                 }
+                const int len = e - s;
+                const int ex = wave_excl_scan(len, ln);
+                if (t < ncube) {
+                    cstart[t] = s;
+                    cpre[t] = C + ex;
+                }
+                C += __shfl(ex + len, kWave - 1, kWave);
             }
-            const int len = e - s;
-            const int ex = wave_excl_scan(len, ln);
-            if (t < ncube) {
-                cstart[t] = s;
-                cpre[t] = C + ex;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            return C;
+        };
+        auto offer_cells = [&](int C) {
+            for (int u0 = 0; u0 < C; u0 += kWave) {
+                const int u = u0 + ln;
+                bool valid = u < C;
+                double d = 0.0;
+                int j = 0;
+                if (valid) {
+                    // cell of candidate u: last c with cpre[c] <= u
+                    int lo = 0, hi = ncube - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (cpre[mid] <= u)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    j = cstart[lo] + (u - cpre[lo]);
+                    d = md_dist2(me, S[j], g);
+                }
+                md_offer(valid, d, j, T, cnt, buf, jb, hist, k1, ln, below);
             }
-            C += __shfl(ex + len, kWave - 1, kWave);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // the next pass rewrites cstart/cpre
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        };
         cnt = 0;  // fresh buffer, threshold T0 (inclusive)
-        T = T0;
-        for (int u0 = 0; u0 < C; u0 += kWave) {
-            const int u = u0 + ln;
-            bool valid = u < C;
-            double d = 0.0;
-            int j = 0;
-            if (valid) {
-                // cell of candidate u: last c with cpre[c] <= u
-                int lo = 0, hi = ncube - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (cpre[mid] <= u)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                j = cstart[lo] + (u - cpre[lo]);
-                d = md_dist(me, S[j], g);
+        T = T20;
+        const double inner2 = 0.25 * T0m * T0m;
+        for (int pass = 0; pass < 2; ++pass) {
+            double lo2 = -1.0, hi2 = inner2;
+            if (pass == 1) {
+                const double T1m = sqrt(T) * (1.0 + 1e-12);
+                lo2 = inner2;
+                hi2 = T1m * T1m;
             }
-            md_offer(valid, d, j, T, cnt, buf, jb, hist, k1, ln, below);
+            offer_cells(gather_cells(lo2, hi2));
+            // after the inner cells: T1 <= T0; at the end the buffer holds
+            // every particle nearer than the k-th distance, each once
+            if (cnt >= k1)
+                T = md_select(buf, cnt, k1, hist, ln);
         }
-        T = md_select(buf, cnt, k1, hist, ln);
-        cnt = md_compact(buf, jb, cnt, T, k1, ln);
         // 4. mass_density.jl:70-86: kernel sum over the neighbours, bias correction
-        const double h = T, h_inv = 1.0 / h;
+        const double h = sqrt(T), h_inv = 1.0 / h;
         double rho = 0.0;
         for (int e = ln; e < cnt; e += kWave) {
-            const double d = buf[e];
-            if (d < h)
-                rho += S[jb[e]].w * shape_of<K, double>(d * h_inv);
+            const double d2 = buf[e];
+            if (d2 <= T) {
+                const double d = sqrt(d2);
+                if (d < h)
+                    rho += S[jb[e]].w * shape_of<K, double>(d * h_inv);
+            }
         }
         rho = wave_sum(rho);
         double hd = h_inv;

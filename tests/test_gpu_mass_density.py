@@ -75,6 +75,20 @@ def test_lattice_and_kernel_dim_2(s2g, orc):
            m, s2g.WendlandC6(2), "wendlandc6", 20, None)
 
 
+@pytest.mark.parametrize("k", [64, 400])
+def test_coincident_particles_and_buffer_cutbacks(s2g, orc, k):
+    """40 copies of one particle and 500 pairs of coincident ones among 3000;
+    400 neighbours make the candidate buffer overflow and be cut back while
+    the cells nearest to a particle are read, before the outer ones."""
+    r = np.random.default_rng(17)
+    n = 3000
+    pos, m = r.random((3, n)) * 6.0, r.uniform(0.5, 2.0, n)
+    pos[:, 1:40] = pos[:, :1]
+    pos[:, 2000:2500] = pos[:, 1000:1500]
+    for box in ((6.0,) * 3, None):
+        _check(s2g, orc, pos, m, s2g.WendlandC4(3), "wendlandc4", k, box)
+
+
 def test_errors(s2g):
     r = np.random.default_rng(0)
     pos, m = r.random((3, 20)), np.ones(20)
