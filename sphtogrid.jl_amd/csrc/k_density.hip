@@ -234,6 +234,13 @@ __device__ __forceinline__ void wave_min_max(double &mn, double &mx)
 // there and spreads the entries over many bins (a pass from the top byte
 // sends them all to one bin, and its LDS atomics serialise).  Once the chosen
 // bin holds a single entry, that entry is the k-th: one scan finds it.
+// kExact == false: an upper bound of the k-th smallest instead, for
+// thresholds that only have to keep at least k entries: the upper edge of
+// the bin that holds it once the bins are narrower than 1/16 of a binade
+// (the exponent and four mantissa bits chosen; entries that straddle a power
+// of two where many exponent bits flip can need a second pass for that), or
+// the largest entry if smaller.
+template <bool kExact>
 __device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln)
 {
     double mn = __builtin_inf(), mx = 0.0;
@@ -300,6 +307,8 @@ __device__ double md_select(const double *buf, int cnt, int k, int *hist, int ln
         prefix |= (uint64_t)bin << shift;
         top = shift;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (!kExact && top <= 48)
+            return fmin(__longlong_as_double((long long)(prefix | ((1ull << top) - 1))), mx);
         if (top == 0)
             return __longlong_as_double((long long)prefix);
         if (nbin == 1) {
@@ -369,7 +378,7 @@ __device__ __forceinline__ void md_offer(bool valid, double d, int j, double &T,
     cnt += (int)__popcll(bt);
     if (cnt > kMdCap - kWave) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        T = md_select(buf, cnt, k1, hist, ln);
+        T = md_select<true>(buf, cnt, k1, hist, ln);
         cnt = md_compact(buf, jb, cnt, T, k1, ln);
     }
 }
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
                 const double d = valid ? md_dist2(me, S[t], g) : 0.0;
                 md_offer(valid, d, (int)t, T, cnt, buf, jb, hist, k1, ln, below);
             }
-            T = md_select(buf, cnt, k1, hist, ln);
+            T = md_select<false>(buf, cnt, k1, hist, ln);  // >= the k-th of these
         }
         const double T20 = T, T0 = sqrt(T20);
         // 2. level with cells in [T0/2, T0) along the narrowest axis; per
@@ -464,10 +473,13 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
             xhi[d] = xr - eps;
             rcl[d] = 1.0f / (float)cl[d];
         }
+        int ncells = 0;  // uniform
         auto gather_cells = [&](double lo2, double hi2) -> int {
-            // the cells with lo2 < (nearest distance)^2 <= hi2 into
-            // cstart/cpre; returns their particle count (uniform)
-            int C = 0;
+            // the non-empty cells with lo2 < (nearest distance)^2 <= hi2, in
+            // cube order, into cstart/cpre (first particle, number of the
+            // first candidate) and ncells; returns their particle count
+            // (uniform)
+            int C = 0, nc = 0;
             for (int t0 = 0; t0 < ncube; t0 += kWave) {
                 const int t = t0 + ln;
                 int s = 0, e = 0;
@@ -508,34 +520,48 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
                 }
                 const int len = e - s;
                 const int ex = wave_excl_scan(len, ln);
-                if (t < ncube) {
-                    cstart[t] = s;
-                    cpre[t] = C + ex;
+                const uint64_t bne = __ballot(len > 0);
+                if (len > 0) {
+                    const int at = nc + (int)__popcll(bne & below);
+                    cstart[at] = s;
+                    cpre[at] = C + ex;
                 }
+                nc += (int)__popcll(bne);
                 C += __shfl(ex + len, kWave - 1, kWave);
             }
+            ncells = nc;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             return C;
         };
         auto offer_cells = [&](int C) {
+            // candidate u lies in the last cell c with cpre[c] <= u.  Of a
+            // chunk of 64 candidates from u0, the cells after cp (the cell
+            // of u0 - 1) that start in it are the next ones in the list:
+            // their starts are flagged (in the histogram's first 64 ints,
+            // free here), and a lane's cell is cp plus the flags at or
+            // below its position
+            int cp = -1;  // uniform
             for (int u0 = 0; u0 < C; u0 += kWave) {
+                hist[ln] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const int ci = cp + 1 + ln;
+                if (ci < ncells) {
+                    const int st = cpre[ci] - u0;  // >= 0
+                    if (st < kWave)
+                        hist[st] = 1;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const uint64_t bf = __ballot(hist[ln] != 0);
                 const int u = u0 + ln;
-                bool valid = u < C;
+                const bool valid = u < C;
                 double d = 0.0;
                 int j = 0;
                 if (valid) {
-                    // cell of candidate u: last c with cpre[c] <= u
-                    int lo = 0, hi = ncube - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (cpre[mid] <= u)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
-                    }
-                    j = cstart[lo] + (u - cpre[lo]);
+                    const int c = cp + (int)__popcll(bf & (below | (1ull << ln)));
+                    j = cstart[c] + (u - cpre[c]);
                     d = md_dist2(me, S[j], g);
                 }
+                cp += (int)__popcll(bf);
                 md_offer(valid, d, j, T, cnt, buf, jb, hist, k1, ln, below);
             }
             // the next pass rewrites cstart/cpre
@@ -552,10 +578,12 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
                 hi2 = T1m * T1m;
             }
             offer_cells(gather_cells(lo2, hi2));
-            // after the inner cells: T1 <= T0; at the end the buffer holds
-            // every particle nearer than the k-th distance, each once
+            // after the inner cells a bound T1 <= T0 of the k-th (at least
+            // k entries are within it); at the end the exact k-th, with
+            // every particle nearer than it in the buffer, each once
             if (cnt >= k1)
-                T = md_select(buf, cnt, k1, hist, ln);
+                T = pass == 0 ? md_select<false>(buf, cnt, k1, hist, ln)
+                              : md_select<true>(buf, cnt, k1, hist, ln);
         }
         // 4. mass_density.jl:70-86: kernel sum over the neighbours, bias correction
         const double h = sqrt(T), h_inv = 1.0 / h;
