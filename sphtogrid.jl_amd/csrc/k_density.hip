@@ -405,7 +405,10 @@ __global__ __launch_bounds__(kMdWaves * kWave, 4) void k_md_knn(
     const int64_t nw = (int64_t)gridDim.x * kMdWaves;
     for (int64_t q = (int64_t)blockIdx.x * kMdWaves + wv; q < n; q += nw) {
         const double4 me = S[q];
-        // 1. first bound: the k-th nearest of the particles next in key order
+        // 1. first bound: the k-th nearest of the particles next in key
+        //    order, k on each side (where the particles on one side are far,
+        //    as at the edge of a cluster in key order, the other side alone
+        //    still holds k)
         int cnt = 0;
         double T = 1e300;  // squared distances from here on
         {
