@@ -835,6 +835,23 @@ struct HpPair {
     double sw[NM], bq[NM];   // per map: pixel scale and quantity
 };
 
+// Ring of block i.  The hardware hands consecutive blocks to the eight XCDs
+// in turn (block i to XCD i mod 8), each with its own L2; here every XCD
+// takes runs of kHpXcdRun consecutive rings, so the blocks resident on it
+// read the particle records of neighbouring rings (a particle spans many)
+// through one L2, and the runs go to the XCDs in turn, which keeps their
+// work balanced (one band per XCD would give the polar ones a quarter of
+// the pixels).  The last G mod (8 kHpXcdRun) blocks keep their own index.
+constexpr int kXcds = 8, kHpXcdRun = 16;
+__device__ __forceinline__ int64_t hp_xcd_ring(int64_t i, int64_t G)
+{
+    const int64_t span = (int64_t)kXcds * kHpXcdRun;
+    if (i >= G / span * span)
+        return i;
+    const int64_t x = i % kXcds, t = i / kXcds;
+    return t / kHpXcdRun * span + x * kHpXcdRun + t % kHpXcdRun;
+}
+
 template <typename T, int K, int NM>
 __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
     const int32_t *__restrict__ list, const int64_t *__restrict__ off,
@@ -843,7 +860,7 @@ __global__ __launch_bounds__(kHpRingThreads<NM>, 4) void k_hp_ring_deposit(
     constexpr int kThreads = kHpRingThreads<NM>, kPairs = kHpRingPairs<NM>;
     __shared__ double s_m[NM][kHpRingLds], s_w[NM][kHpRingLds];
     __shared__ HpPair<NM> s_tab[kThreads / kWave][kPairs];
-    const int64_t iz = (int64_t)blockIdx.x + 1;  // ring, 1-based
+    const int64_t iz = hp_xcd_ring(blockIdx.x, gridDim.x) + 1;  // ring, 1-based
     int64_t start, nr;
     double shift;
     hp_ring_info(nside, iz, start, nr, shift);
