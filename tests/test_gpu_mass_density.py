@@ -89,6 +89,22 @@ def test_coincident_particles_and_buffer_cutbacks(s2g, orc, k):
         _check(s2g, orc, pos, m, s2g.WendlandC4(3), "wendlandc4", k, box)
 
 
+@pytest.mark.parametrize("shape", ["plane", "line", "fewest"])
+def test_degenerate_sets(s2g, orc, shape):
+    """Particles in one plane (zero z extent) or on one line, and the fewest
+    particles allowed (Nneighbors + 1, every particle a neighbour of every
+    other), without a box."""
+    r = np.random.default_rng(23)
+    n, k = {"plane": (2000, 32), "line": (600, 20), "fewest": (33, 32)}[shape]
+    pos, m = r.random((3, n)) * 5.0, r.uniform(0.5, 2.0, n)
+    if shape == "plane":
+        pos[2] = 1.5
+    elif shape == "line":
+        pos[1] = 2.0
+        pos[2] = -1.0
+    _check(s2g, orc, pos, m, s2g.WendlandC4(3), "wendlandc4", k, None)
+
+
 def test_errors(s2g):
     r = np.random.default_rng(0)
     pos, m = r.random((3, 20)), np.ones(20)
