@@ -78,7 +78,7 @@ __global__ __launch_bounds__(256) void k_md_prep(const T *__restrict__ pos,
     }
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int)threadIdx.x < s)
             for (int d = 0; d < 3; ++d) {
                 s_lo[d][threadIdx.x] = fmin(s_lo[d][threadIdx.x], s_lo[d][threadIdx.x + s]);
                 s_hi[d][threadIdx.x] = fmax(s_hi[d][threadIdx.x], s_hi[d][threadIdx.x + s]);
