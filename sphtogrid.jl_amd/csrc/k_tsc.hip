@@ -257,8 +257,6 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
     for (int d = 0; d < 3; ++d)
         b0[d] = bi[d] * side[d];
     const int ln = threadIdx.x;
-    for (int e = ln; e < 2 * kTX * kTY * kTZ; e += kWave)
-        (&acc[0][0])[e] = 0.0;
     // neighbour brick of lane ln, if it exists and is not a repeat of an
     // earlier offset on some axis (periodic grids of one or two bricks)
     int len = 0;
@@ -294,6 +292,21 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
         incl += ln >= sh ? t : 0;
     }
     const int total = __shfl(incl, 26);
+    const int64_t nd = npix * npix * npix;
+    // lane ln writes cells (t, ln / kTZ, ln % kTZ) of the brick, t < kTX
+    static_assert(kTY * kTZ == kWave, "one x plane of a brick per wave");
+    const int64_t j = b0[1] + ln / kTZ, k = b0[2] + ln % kTZ;
+    if (total == 0) {  // wave-uniform: no particle reaches the brick
+        for (int64_t i = b0[0]; i < b0[0] + kTX; ++i)
+            if (i < npix && j < npix && k < npix) {
+                const int64_t idx = (i * npix + j) * npix + k;
+                grid[idx] = 0.0;
+                grid[nd + idx] = 0.0;
+            }
+        return;
+    }
+    for (int e = ln; e < 2 * kTX * kTY * kTZ; e += kWave)
+        (&acc[0][0])[e] = 0.0;
     if (ln < 27) {
         s_start[ln] = start;
         s_base[ln] = incl - len;
@@ -345,14 +358,12 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
     if (tail > head)
         tsc_step(queue, head, tail - head, ln, b0, npix, periodic, acc);
     __syncthreads();
-    const int64_t nd = npix * npix * npix;
-    for (int e = ln; e < kTX * kTY * kTZ; e += kWave) {
-        const int64_t i = b0[0] + e / (kTY * kTZ), j = b0[1] + (e / kTZ) % kTY,
-                      k = b0[2] + e % kTZ;
+    for (int t = 0; t < kTX; ++t) {
+        const int64_t i = b0[0] + t;
         if (i < npix && j < npix && k < npix) {
             const int64_t idx = (i * npix + j) * npix + k;
-            grid[idx] = acc[0][e];
-            grid[nd + idx] = acc[1][e];
+            grid[idx] = acc[0][t * kWave + ln];
+            grid[nd + idx] = acc[1][t * kWave + ln];
         }
     }
 }
