@@ -179,24 +179,30 @@ __device__ __forceinline__ int tsc_reach(int64_t c, int64_t b0, int64_t side, in
     return m;
 }
 
-// Deposit queue of the gather: the candidates whose stencil reaches the
-// brick, in scan order (a ring of LDS slots: grid coordinates + quantity).
-constexpr int kTscQueue = 128;
 // particles per deposit step: 9 lanes each, one stencil row (o0, o1) per
 // lane and its three cells along z
 constexpr int kTscPerStep = kWave / 9;
+// Deposit queue of the gather: the candidates whose stencil reaches the
+// brick, in scan order (a ring of LDS slots holding sorted particle
+// indices; the deposit step loads their records).  It holds at most
+// kTscPerStep - 1 leftovers plus one chunk of kWave candidates.  The LDS of
+// a block, not the VGPRs, bounds the waves per CU: with 32-byte records in
+// 128 slots 19 waves, in 72 slots 23, with 4-byte indices 33.
+constexpr int kTscQueue = 72;
+static_assert(kTscQueue >= kTscPerStep - 1 + kWave, "queue holds one chunk and the leftovers");
 
 // One deposit step: particles slots [h, h + cnt) of the queue.  The
 // weights are formed as the oracle forms them: per axis W(g - (c + o) - 1/2)
 // with c = floor(g), and the product (wx * wy) * wz.
-__device__ __forceinline__ void tsc_step(const double4 *queue, int h, int cnt, int ln,
+__device__ __forceinline__ void tsc_step(const double4 *__restrict__ gq, const int32_t *queue,
+                                         int h, int cnt, int ln,
                                          const int64_t b0[3], int64_t npix, int periodic,
                                          double (*acc)[kTX * kTY * kTZ])
 {
     const int slot = ln / 9, row = ln - 9 * slot;
     if (slot >= cnt)
         return;
-    const double4 e = queue[(h + slot) % kTscQueue];
+    const double4 e = gq[queue[(h + slot) % kTscQueue]];
     const double g[3] = {e.x, e.y, e.z};
     int64_t c[3];
     for (int d = 0; d < 3; ++d)
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
                                                       double *__restrict__ grid)
 {
     __shared__ double acc[2][kTX * kTY * kTZ];
-    __shared__ double4 queue[kTscQueue];
+    __shared__ int32_t queue[kTscQueue];
     __shared__ int64_t s_start[32];
     __shared__ int s_base[32];
     int64_t nb[3];
@@ -346,17 +352,17 @@ __global__ __launch_bounds__(kWave) void k_tsc_gather(const double4 *__restrict_
         }
         const uint64_t mask = __ballot(use);
         if (use)
-            queue[(tail + (int)__popcll(mask & below)) % kTscQueue] = gq[kuse];
+            queue[(tail + (int)__popcll(mask & below)) % kTscQueue] = (int32_t)kuse;
         tail += (int)__popcll(mask);
         __builtin_amdgcn_wave_barrier();
         while (tail - head >= kTscPerStep) {
-            tsc_step(queue, head, kTscPerStep, ln, b0, npix, periodic, acc);
+            tsc_step(gq, queue, head, kTscPerStep, ln, b0, npix, periodic, acc);
             head += kTscPerStep;
         }
         __builtin_amdgcn_wave_barrier();
     }
     if (tail > head)
-        tsc_step(queue, head, tail - head, ln, b0, npix, periodic, acc);
+        tsc_step(gq, queue, head, tail - head, ln, b0, npix, periodic, acc);
     __syncthreads();
     for (int t = 0; t < kTX; ++t) {
         const int64_t i = b0[0] + t;
